@@ -1,0 +1,200 @@
+"""Benchmark: FDV time steps/s at 2.1M hex8 on one MI355X.
+
+Workload: the 3D hypersonic 15-degree compression ramp, structured hex8
+256x128x64 (2,097,152 elements, 2,154,945 nodes, 5 equations), M=6,
+Re=5e5, adiabatic wall, CFL 0.5, synthetic initial state (uniform
+freestream, wall at rest).  A step is one full FDV time step on the GPU:
+CFL, element build (FDVParam + domain + boundary + ApplyBC, FP64 arithmetic),
+GMRES to a 1e-8 relative residual, update and RMS monitor.
+
+Two paths are timed from the same initial state:
+  fp64  : FP64 blocks, FP64 restarted GMRES(m)
+  mixed : FP32 blocks with FP32 accumulation inside FP64 iterative
+          refinement (outer residual with FP64 blocks)
+``value`` is the mixed path's steps/s; the FP64 path and the ratio are
+reported beside it.  The CPU baseline times the CPU oracle (a port of the
+reference algorithm) on a bounded random sample of the same mesh and
+extrapolates to one full step.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--size", default="256x128x64")
+    p.add_argument("--cfl", type=float, default=0.5)
+    p.add_argument("--m", type=int, default=30)
+    p.add_argument("--rtol", type=float, default=1e-8)
+    p.add_argument("--inner-m", type=int, default=30)
+    p.add_argument("--inner-rtol", type=float, default=1e-3)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-sample", type=int, default=32768)
+    return p.parse_args()
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def run_steps(ctx, U0, mode, n, args):
+    ctx.set_state(U0)
+    out = []
+    for i in range(n):
+        st = ctx.step(args.cfl, mode=mode, restarts=60 if mode == "fp64" else 20, m=args.m,
+                      rtol=args.rtol, inner_m=args.inner_m, inner_rtol=args.inner_rtol,
+                      inner_restarts=3)
+        out.append(st)
+        log(f"  {mode} step {i}: build {st.ms_build:.1f} ms solve {st.ms_solve:.1f} ms "
+            f"mv64 {st.solve.matvecs64} mv32 {st.solve.matvecs32} rel {st.solve.rel_resid:.2e}")
+    return out
+
+
+def cpu_baseline(mesh, th, U, n_matvec, sample, seed=7):
+    """Times the CPU oracle on a random sample of elements; returns s/step."""
+    from oracle import oracle as O
+    from cfdv_amd.mesh import Mesh
+    rng = np.random.default_rng(seed)
+    sel = np.sort(rng.choice(mesh.n_elem, size=min(sample, mesh.n_elem), replace=False))
+    conn = mesh.conn[sel]
+    nodes, inv = np.unique(conn, return_inverse=True)
+    sub = Mesh(mesh.ndim, mesh.nnod, mesh.neqn, inv.reshape(conn.shape).astype(np.int32),
+               mesh.x[nodes], mesh.bc[sel], mesh.dirichlet[nodes])
+    Us = U[nodes]
+    dt = 1e-4
+    t0 = time.perf_counter()
+    R, rhs, _ = O.build(th, sub, Us, dt)
+    t_build = time.perf_counter() - t0
+    x = rng.standard_normal(sub.n_node * sub.neqn)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        O.matvec(sub, R, x)
+    t_mv = (time.perf_counter() - t0) / reps
+    scale = mesh.n_elem / sub.n_elem
+    step_s = scale * (t_build + n_matvec * t_mv)
+    return step_s, {"t_build_sample_s": t_build, "t_matvec_sample_s": t_mv,
+                    "sample_elements": int(sub.n_elem)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the FDV path "
+              "is built and measured on a single MI355X", file=sys.stderr)
+        sys.exit(2)
+    from cfdv_amd import mesh as M
+    from cfdv_amd.fdv import FDVContext
+
+    nx, ny, nz = (int(v) for v in args.size.split("x"))
+    t0 = time.time()
+    mesh, th, U0 = M.compression_ramp(nx, ny, nz)
+    log(f"mesh {nx}x{ny}x{nz}: {mesh.n_elem} elements, {mesh.n_node} nodes ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    ctx = FDVContext(mesh, th, U0, blocks=("fp64", "fp32"))
+    info = ctx.info()
+    log(f"context: {info} ({time.time() - t0:.1f}s)")
+
+    results = {}
+    for mode in ("fp64", "mixed"):
+        log(f"{mode}: warmup {args.warmup}")
+        run_steps(ctx, U0, mode, args.warmup, args)
+        sync()
+        t0 = time.perf_counter()
+        sts = run_steps(ctx, U0, mode, args.steps, args)
+        sync()
+        el = time.perf_counter() - t0
+        results[mode] = (el, sts)
+
+    el_mx, st_mx = results["mixed"]
+    el_64, st_64 = results["fp64"]
+    value = args.steps / el_mx
+    fp64_sps = args.steps / el_64
+
+    ne, nn = mesh.n_elem, mesh.n_node
+    b32 = ne * 1600 * 4 + nn * 5 * 2 * 4
+    b64 = ne * 1600 * 8 + nn * 5 * 2 * 8
+    ms32 = sum(s.ms_matvec32 for s in st_mx)
+    n32 = sum(s.n_matvec32 for s in st_mx)
+    ms64 = sum(s.ms_matvec64 for s in st_64)
+    n64 = sum(s.n_matvec64 for s in st_64)
+    ach32 = b32 * n32 / (ms32 * 1e-3) / 1e9 if ms32 > 0 else None
+    ach64 = b64 * n64 / (ms64 * 1e-3) / 1e9 if ms64 > 0 else None
+
+    cpu = None
+    if not args.no_cpu:
+        mv_per_step = max(1, int(round(n64 / max(1, args.steps))))
+        log(f"cpu baseline: oracle on a {args.cpu_sample}-element sample, {mv_per_step} matvecs/step")
+        step_s, det = cpu_baseline(mesh, th, U0, mv_per_step, args.cpu_sample)
+        cpu = {"value": 1.0 / step_s, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": (f"CPU oracle (FP64 port of the reference loop order), build of "
+                          f"{det['sample_elements']} randomly sampled elements "
+                          f"({det['t_build_sample_s']:.2f} s) and EBE matvec over them "
+                          f"({det['t_matvec_sample_s'] * 1e3:.1f} ms), scaled to {ne} elements "
+                          f"with {mv_per_step} matvecs per step (the FP64 GPU path's count)")}
+
+    line = {
+        "metric": "FDV steps/s at 2.1M hex8: FP32+FP64-refine vs FP64 GPU, equal 1e-8 residual",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / value,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 blocks/matvec + f64 build, residual, Givens, update",
+        "data": "synthetic (generated ramp mesh and freestream initial state)",
+        "config": {"workload": f"3D hypersonic 15-deg compression ramp hex8 {nx}x{ny}x{nz} "
+                               f"({ne} elements, {nn} nodes), M=6, Re=5e5, adiabatic, "
+                               f"CFL {args.cfl}, GMRES({args.m}) to rel. residual {args.rtol}",
+                   "elements": ne, "nodes": nn, "colors": info["ncolors"]},
+        "fp64_steps_per_s": fp64_sps,
+        "mixed_over_fp64": value / fp64_sps,
+        "rel_resid_mixed": max(s.solve.rel_resid for s in st_mx),
+        "rel_resid_fp64": max(s.solve.rel_resid for s in st_64),
+        "ms_build": float(np.mean([s.ms_build for s in st_mx])),
+        "ms_solve_mixed": float(np.mean([s.ms_solve for s in st_mx])),
+        "ms_solve_fp64": float(np.mean([s.ms_solve for s in st_64])),
+        "matvecs_per_step_mixed": {"fp32": n32 / args.steps,
+                                   "fp64": sum(s.n_matvec64 for s in st_mx) / args.steps},
+        "matvecs_per_step_fp64": n64 / args.steps,
+        "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (ach32 / HBM_PEAK_GBS) if ach32 else None, "traffic": None,
+                     "kernel": "matvec_kernel<float,float,float,8,5> (EBE matvec, FP32 blocks), "
+                               f"{b32} algorithmic bytes per full matvec "
+                               f"({info['ncolors']} color launches)"},
+        "roofline_fp64_matvec": {"achieved": ach64, "frac": (ach64 / HBM_PEAK_GBS) if ach64 else None,
+                                 "bytes_per_matvec": b64},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1358 @@
+// FDV element build on gfx950: FDVParam + domain + boundary + ApplyBC in
+// one pass, blocks written structure-of-arrays to HBM.
+//
+// Restates FDVGalerkin::operator() (src/FDV/FDVGalerkin.h:121-154) applied
+// by std::for_each over all elements (src/main.cpp:185-186) followed by
+// ApplyBC (src/FDV/Function/ApplyBC.h:14-133, src/main.cpp:194).
+//
+// Mapping: one 64-lane wavefront per hex8 element; lane (N, M) = (lane/8,
+// lane%8) owns the 5x5 sub-block R(5N+r, 5M+s).  Every lane evaluates the
+// Gauss-point state; only the node-pair contraction differs per lane.  The
+// quad4 kernel packs four elements per wavefront (16 lanes each).
+//
+// Block layout (both precisions): blk_offset() in fdv_device.h -- chunks of
+// 64 elements, 16-byte entry groups interleaved across the chunk, so that a
+// thread-per-element matvec reads one 16-byte vector per lane from 1 KB of
+// contiguous memory.
+#include "cfdv_internal.h"
+#include "fdv_device.h"
+
+namespace cfdv {
+
+__constant__ GaussTables c_gauss;
+
+void upload_gauss_tables(const GaussTables& g) {
+    HIP_CHECK_THROW(hipMemcpyToSymbol(HIP_SYMBOL(c_gauss), &g, sizeof(g)));
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that
+// consecutive logical blocks (neighbouring elements, i.e. the same 64-element
+// output chunk) execute on one XCD and share its L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int q = nb / 8, r = nb % 8;
+    const int x = b % 8, i = b / 8;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+// ---------------------------------------------------------------------------
+// FDVParam (src/FDV/Function/FDVParam.h:15-111).  The r == 0.01 tie
+// (quirk Q11) leaves s1/s3 as the previous element's value in the
+// reference; here the tie is reported through *tie and s keeps 0.
+// ---------------------------------------------------------------------------
+template <int NDIM, int NNOD>
+CFDV_HD void fdvparam(const DevThermo& th, const double vn[][NDIM], const double* Tn,
+                                         const double* rhon, double s[4], int* tie) {
+#pragma clang fp contract(off)
+    double M_min = 0, M_max = 0, Re_min = 0, Re_max = 0;
+#pragma unroll
+    for (int beta = 0; beta < NNOD; beta++) {
+        double dd = 0;
+#pragma unroll
+        for (int i = 0; i < NDIM; i++) dd += vn[beta][i] * vn[beta][i];
+        const double V = sqrt(dd);
+        double T = Tn[beta];
+        if (T < 1.0E-12) T = 0;
+        const double a = sqrt(th.gamma * th.cgas * T);
+        double M = 0;
+        if (!(a < 1.0E-12)) M = V / a;
+        const double mu = calc_mu(th, T);
+        const double Re = (rhon[beta] * V) / mu;
+        if (beta == 0) { M_max = M; M_min = M; Re_max = Re; Re_min = Re; }
+        M_max = (M_max < M) ? M : M_max;
+        M_min = (M < M_min) ? M : M_min;
+        Re_max = (Re_max < Re) ? Re : Re_max;
+        Re_min = (Re < Re_min) ? Re : Re_min;
+    }
+    const double eps = 0.0000001;
+    double r = sqrt(M_max * M_max - M_min * M_min) / (M_min + eps);
+    *tie = 0;
+    s[0] = 0.0;
+    if (r > 0.01) s[0] = (1.0 < r) ? 1.0 : r;
+    else if (r < 0.01 && M_min >= 10E-10) s[0] = 0;
+    else if (M_min < 10E-10) s[0] = 1.0;
+    else *tie = 1;
+    s[1] = 0.5 * (1 + pow(s[0], 0.2));
+    r = sqrt(Re_max * Re_max - Re_min * Re_min) / (Re_min + eps);
+    s[2] = 0.0;
+    if (r > 0.01) s[2] = (1.0 < r) ? 1.0 : r;
+    else if (r < 0.01 && Re_min >= 10E-10) s[2] = 0.0;
+    else if (Re_min <= 10E-10) s[2] = 1.0;
+    else *tie = 1;
+    s[3] = 0.5 * (1 + pow(s[2], 0.2));
+}
+
+// NodeUnpack (src/FDV/Function/NodeUnpack.h:124-144) for one node
+template <int NDIM>
+CFDV_HD void unpack(const DevThermo& th, const double* U, double& rho, double v[NDIM],
+                                       double& E, double& T) {
+    rho = U[0];
+    double vsqr = 0;
+#pragma unroll
+    for (int i = 0; i < NDIM; i++) {
+        v[i] = U[1 + i] / U[0];
+        vsqr += v[i] * v[i];
+    }
+    E = U[NDIM + 1] / U[0];
+    const double e = E - 0.5 * vsqr;
+    T = e / th.Cv;
+}
+
+// ---------------------------------------------------------------------------
+// eval2dboundary (src/FDV/Function/eval2dboundary.h:14-371): bc edits of
+// the face Gauss-point state.  Returns false for an unknown code.
+// ---------------------------------------------------------------------------
+CFDV_HD bool eval2dboundary(const DevThermo& th, int bc, GPState<3>& f, const double en[3]) {
+    switch (bc) {
+    case 2: case 12:
+        for (int i = 0; i < 3; i++) f.dV[i][2] = 0;
+        f.dT[2] = 0;
+        return true;                                  // tau stays 0
+    case 22:
+        for (int i = 0; i < 3; i++) f.dV[i][1] = 0;
+        f.dT[1] = 0;
+        calc_tau<3>(f);
+        f.tau[0][1] = 0.0; f.tau[1][1] = 0.0; f.tau[2][1] = 0.0;
+        f.tau[1][0] = 0.0; f.tau[1][2] = 0.0;
+        return true;
+    case 1: case 51:
+        if (th.adiabatic) f.dT[2] = 0.0;
+        calc_tau<3>(f);
+        return true;
+    case 21: case 31:
+        if (th.adiabatic) f.dT[1] = 0.0;
+        calc_tau<3>(f);
+        return true;
+    case 11:
+        if (th.adiabatic) f.dT[0] = 0.0;
+        calc_tau<3>(f);
+        return true;
+    case -1:
+        return true;                                  // returns before tau (Q6)
+    case 6:
+        f.dT[0] = 0.0;
+        calc_tau<3>(f);
+        for (int i = 0; i < 3; i++) f.tau[i][0] = 0;
+        return true;
+    case 7:
+        f.dT[2] = 0.0;
+        calc_tau<3>(f);
+        for (int i = 0; i < 3; i++) f.tau[i][2] = 0;
+        // no break in the source: continues into 77/32 (Q6)
+        f.dT[1] = 0.0;
+        calc_tau<3>(f);
+        for (int i = 0; i < 3; i++) f.tau[i][1] = 0;
+        return true;
+    case 77: case 32:
+        f.dT[1] = 0.0;
+        calc_tau<3>(f);
+        for (int i = 0; i < 3; i++) f.tau[i][1] = 0;
+        return true;
+    case 4: {
+        // projection with the unnormalised en; tau not recomputed; only the
+        // first three equations' gradients are projected (loop bound jmax)
+        f.v[0] = f.v[1] = f.v[2] = 0;
+        const double dudot = f.dV[0][0] * en[0] + f.dV[0][1] * en[1] + f.dV[0][2] * en[2];
+        const double dvdot = f.dV[1][0] * en[0] + f.dV[1][1] * en[1] + f.dV[1][2] * en[2];
+        const double dtdot = f.dT[0] * en[0] + f.dT[1] * en[1] + f.dT[2] * en[2];
+        for (int j = 0; j < 3; j++) {
+            f.dV[0][j] -= dudot * en[j];
+            f.dV[1][j] -= dvdot * en[j];
+            f.dT[j] -= dtdot * en[j];
+        }
+        double dUdot[3];
+        for (int i = 0; i < 3; i++) dUdot[i] = f.dU[i][0] * en[0] + f.dU[i][1] * en[1] + f.dU[i][2] * en[2];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) f.dU[i][j] -= dUdot[i] * en[j];
+        return true;
+    }
+    case 8:
+        calc_tau<3>(f);
+        return true;
+    case -12:
+        f.dV[0][2] = 0.0; f.dV[1][2] = 0.0; f.dV[2][2] = 0.0;
+        f.dT[2] = 0.0;
+        f.v[2] = 0.0;
+        return true;
+    default:
+        return false;
+    }
+}
+
+// eval1dboundary (src/FDV/Function/eval1dboundary.h:9-251)
+CFDV_HD bool eval1dboundary(int bc, GPState<2>& f, const double x0[2], const double x1[2],
+                               const double en[2]) {
+    const double len = sqrt(en[0] * en[0] + en[1] * en[1]);
+    const double n[2] = {en[0] / len, en[1] / len};
+    switch (bc) {
+    case 2: case 12: {
+        double d = f.v[0] * n[0] + f.v[1] * n[1];
+        f.v[0] -= d * n[0];
+        d = f.v[0] * n[0] + f.v[1] * n[1];
+        f.v[1] -= d * n[1];
+        d = f.dT[0] * n[0] + f.dT[1] * n[1];
+        f.dT[0] -= d * n[0];
+        d = f.dT[0] * n[0] + f.dT[1] * n[1];
+        f.dT[1] -= d * n[1];
+        break;
+    }
+    case 1: case 51: {
+        double d = f.dT[0] * n[0] + f.dT[1] * n[1];
+        f.dT[0] -= d * n[0];
+        d = f.dT[0] * n[0] + f.dT[1] * n[1];
+        f.dT[1] -= d * n[1];
+        break;
+    }
+    case -1:
+        f.dV[0][0] = 0.0; f.dV[1][0] = 0.0; f.dV[0][1] = 0.0; f.dV[1][1] = 0.0;
+        f.dT[0] = 0.0;
+        break;
+    case 6:
+        f.dV[0][0] = 0.0; f.dV[1][0] = 0.0; f.dV[0][1] = 0.0; f.dV[1][1] = 0.0;
+        f.dT[0] = 0.0;
+        for (int i = 0; i < 4; i++) f.dU[i][0] = 0.0;
+        break;
+    case -2: case 7:
+        f.dV[0][1] = 0.0; f.dV[1][1] = 0.0;
+        f.dT[1] = 0.0;
+        for (int i = 0; i < 4; i++) f.dU[i][1] = 0.0;
+        break;
+    case 4: case 8: {
+        double nn[2];
+        if (bc == 4) {
+            f.v[0] = f.v[1] = 0;
+            nn[0] = en[0]; nn[1] = en[1];
+        } else {
+            const double alpha = 10 * (3.14159 / 180.0);
+            const double dl = fabs(x1[1] - x0[1]);
+            nn[0] = cos(alpha) * dl;
+            nn[1] = sin(alpha) * dl;
+        }
+        const double dudot = f.dV[0][0] * nn[0] + f.dV[0][1] * nn[1];
+        const double dvdot = f.dV[1][0] * nn[0] + f.dV[1][1] * nn[1];
+        const double dtdot = f.dT[0] * nn[0] + f.dT[1] * nn[1];
+        for (int j = 0; j < 2; j++) {
+            f.dV[0][j] -= dudot * nn[j];
+            f.dV[1][j] -= dvdot * nn[j];
+            f.dT[j] -= dtdot * nn[j];
+        }
+        double dUdot[4];
+        for (int i = 0; i < 4; i++) dUdot[i] = f.dU[i][0] * nn[0] + f.dU[i][1] * nn[1];
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 2; j++) f.dU[i][j] -= dUdot[i] * nn[j];
+        break;
+    }
+    case -12:
+        f.dV[0][1] = 0.0; f.dV[1][1] = 0.0;
+        f.dT[1] = 0.0;
+        f.v[1] = 0.0;
+        for (int i = 0; i < 4; i++) f.dU[i][1] = 0.0;
+        break;
+    case 11: case 21:
+        f.v[0] = f.v[1] = 0;
+        f.dV[0][0] = 0.0; f.dV[1][0] = 0.0;
+        f.dT[1] = 0.0;
+        for (int i = 0; i < 4; i++) f.dU[i][1] = 0.0;
+        break;
+    case 0:
+        break;
+    default:
+        return false;
+    }
+    calc_tau<2>(f);
+    if (bc == 2 || bc == 12 || bc == 7) { f.tau[0][1] = 0.0; f.tau[1][0] = 0.0; }
+    if (bc == 6) { f.tau[0][0] = 0.; f.tau[1][0] = 0.; }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Node-pair contraction shared by domain and boundary terms (see header of
+// fdv_device.h for the factorised form).
+// ---------------------------------------------------------------------------
+template <int NDIM, int NEQ>
+CFDV_HD void contract(double R[NEQ][NEQ], double cI, const double wg[NDIM],
+                                         const double wa[NDIM], const double ws[NDIM], double cA,
+                                         const double a[NDIM][NEQ][NEQ], const double b[NDIM][NEQ][NEQ],
+                                         double s1, double s2, double s3, double s4) {
+    double Ah[NEQ][NEQ], Sh[NEQ][NEQ];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) {
+            double acc = (r == s) ? cI : 0.0;
+            double ah = 0.0, sh = 0.0;
+#pragma unroll
+            for (int i = 0; i < NDIM; i++) {
+                acc += wg[i] * (s1 * a[i][r][s] + s3 * b[i][r][s]);
+                ah += wa[i] * (a[i][r][s] + b[i][r][s]);
+                sh += ws[i] * (s2 * a[i][r][s] + s4 * b[i][r][s]);
+            }
+            R[r][s] += acc;
+            Ah[r][s] = ah;
+            Sh[r][s] = sh;
+        }
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) {
+            double p = 0.0;
+#pragma unroll
+            for (int t = 0; t < NEQ; t++) p += Ah[r][t] * Sh[t][s];
+            R[r][s] += cA * p;
+        }
+}
+
+template <typename T>
+CFDV_HD size_t blk_index(long e, int q, int KK) {
+    return blk_offset<T>(e, q, KK);
+}
+
+// ---------------------------------------------------------------------------
+// hex8 build kernel
+// ---------------------------------------------------------------------------
+// per-wave node data in LDS: x(3), U(5), v(3), E, T, rho
+#define ND_X 0
+#define ND_U 3
+#define ND_V 8
+#define ND_E 11
+#define ND_T 12
+#define ND_RHO 13
+#define ND_STRIDE 14
+
+CFDV_HD void load_node3d(const BuildArgs& A, int g, double* row) {
+    double Ul[5];
+    for (int i = 0; i < 3; i++) row[ND_X + i] = A.x[(size_t)g * 3 + i];
+    for (int q = 0; q < 5; q++) {
+        Ul[q] = A.U[(size_t)g * 5 + q];
+        row[ND_U + q] = Ul[q];
+    }
+    double rho, v[3], E, T;
+    unpack<3>(A.th, Ul, rho, v, E, T);
+    for (int i = 0; i < 3; i++) row[ND_V + i] = v[i];
+    row[ND_E] = E;
+    row[ND_T] = T;
+    row[ND_RHO] = rho;
+}
+
+// Accumulates, for one Gauss point of the domain or of a face, the
+// direction-streamed factorised contraction into R and the rhs pieces Q.
+// wa/ws/wg: node-pair weights; A_i = a_i + b_i is also returned through
+// AQ[r] += wr_i * sum_s A_i(r,s) D(s) for the rhs.
+CFDV_HD void contract3(const DevThermo& th, const GPState<3>& f, double R[5][5],
+                                          double cI, const double wg[3], const double wa[3],
+                                          const double ws[3], double cA, const double wr[3],
+                                          const double D[5], double AQ[5], double s1, double s2,
+                                          double s3, double s4) {
+    double Ah[5][5], Sh[5][5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        AQ[r] = 0.0;
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+            Ah[r][s] = 0.0;
+            Sh[r][s] = 0.0;
+        }
+        R[r][r] += cI;
+    }
+    double a[5][5], b[5][5];
+#define DIR(I)                                                                      \
+    {                                                                               \
+        calc_a3_dir<I>(th, f, a);                                                   \
+        calc_b3_dir<I>(th, f, b);                                                   \
+        _Pragma("unroll") for (int r = 0; r < 5; r++) {                             \
+            double q = 0.0;                                                         \
+            _Pragma("unroll") for (int s = 0; s < 5; s++) {                         \
+                const double A_ = a[r][s] + b[r][s];                                \
+                R[r][s] += wg[I] * (s1 * a[r][s] + s3 * b[r][s]);                   \
+                Ah[r][s] += wa[I] * A_;                                             \
+                Sh[r][s] += ws[I] * (s2 * a[r][s] + s4 * b[r][s]);                  \
+                q += A_ * D[s];                                                     \
+            }                                                                       \
+            AQ[r] += wr[I] * q;                                                     \
+        }                                                                           \
+    }
+    DIR(0)
+    DIR(1)
+    DIR(2)
+#undef DIR
+#pragma unroll
+    for (int r = 0; r < 5; r++)
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+            double p = 0.0;
+#pragma unroll
+            for (int t = 0; t < 5; t++) p += Ah[r][t] * Sh[t][s];
+            R[r][s] += cA * p;
+        }
+}
+
+// One hex8 element seen from lane (N, M) = (lane/8, lane%8): FDVParam,
+// the 8-point domain integral, the face integrals and ApplyBC for the 5x5
+// sub-block R(5N+r, 5M+s) and (for M == 0) rhs(5N+r).  nd holds the element's
+// node data (x, U, v, E, T, rho per node).
+CFDV_HD void build3d_elem(const BuildArgs& A, const GaussTables& G, const double (*nd)[ND_STRIDE], long e,
+                          int lane, double R[5][5], double rhs[5], double sp[4], int* tie, int* status) {
+    constexpr int NNOD = 8, NEQ = 5;
+    const DevThermo th = A.th;
+    const int N = lane >> 3, M = lane & 7;
+    {
+        double vn[NNOD][3], Tn[NNOD], rhon[NNOD];
+#pragma unroll
+        for (int n = 0; n < NNOD; n++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) vn[n][i] = nd[n][ND_V + i];
+            Tn[n] = nd[n][ND_T];
+            rhon[n] = nd[n][ND_RHO];
+        }
+        fdvparam<3, 8>(th, vn, Tn, rhon, sp, tie);
+    }
+    const double s1 = sp[0], s2 = sp[1], s3 = sp[2], s4 = sp[3];
+    const double dt = A.dt, dtsq_2 = (dt * dt) / 2.0;
+    const double reinv = 1.0 / th.creyn;
+
+#pragma unroll
+    for (int r = 0; r < NEQ; r++) {
+        rhs[r] = 0.0;
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) R[r][s] = 0.0;
+    }
+
+    // ---------------- domain integral: 8 Gauss points ----------------
+    for (int gp = 0; gp < 8; gp++) {
+        double jac[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int n = 0; n < 8; n++) acc += G.dpxi3[gp][n][j] * nd[n][ND_X + i];
+                jac[i][j] = acc;
+            }
+        const double J = jac[0][0] * jac[1][1] * jac[2][2] + jac[0][1] * jac[1][2] * jac[2][0] +
+                         jac[0][2] * jac[1][0] * jac[2][1] - jac[0][2] * jac[1][1] * jac[2][0] -
+                         jac[0][1] * jac[1][0] * jac[2][2] - jac[0][0] * jac[1][2] * jac[2][1];
+        double ji[3][3];
+        ji[0][0] = jac[1][1] * jac[2][2] - jac[1][2] * jac[2][1];
+        ji[0][1] = -(jac[1][0] * jac[2][2] - jac[1][2] * jac[2][0]);
+        ji[0][2] = jac[1][0] * jac[2][1] - jac[1][1] * jac[2][0];
+        ji[1][0] = -(jac[0][1] * jac[2][2] - jac[0][2] * jac[2][1]);
+        ji[1][1] = jac[0][0] * jac[2][2] - jac[0][2] * jac[2][0];
+        ji[1][2] = -(jac[0][0] * jac[2][1] - jac[0][1] * jac[2][0]);
+        ji[2][0] = jac[0][1] * jac[1][2] - jac[0][2] * jac[1][1];
+        ji[2][1] = -(jac[0][0] * jac[1][2] - jac[0][2] * jac[1][0]);
+        ji[2][2] = jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0];
+        const double Jinv = 1.0 / J;
+        // Gauss-point state (FDVGalerkin::sum, :761-797) and my node pair
+        GPState<3> f;
+        f.E = 0; f.T = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            f.v[i] = 0; f.dT[i] = 0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) f.dV[i][j] = 0;
+        }
+#pragma unroll
+        for (int q = 0; q < NEQ; q++) {
+            f.U[q] = 0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) f.dU[q][j] = 0;
+        }
+        double SPhi = 0.0, SD[3] = {0, 0, 0};
+        double PN = 0, PM = 0, DN[3] = {0, 0, 0}, DM[3] = {0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            double dphi[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * G.dpxi3[gp][n][j]);
+                dphi[i] = acc;
+            }
+            const double ph = G.phi3[gp][n];
+            if (n == N) { PN = ph; DN[0] = dphi[0]; DN[1] = dphi[1]; DN[2] = dphi[2]; }
+            if (n == M) { PM = ph; DM[0] = dphi[0]; DM[1] = dphi[1]; DM[2] = dphi[2]; }
+            SPhi += ph;
+            f.E += ph * nd[n][ND_E];
+            const double Tnn = nd[n][ND_T];
+            f.T += ph * Tnn;
+#pragma unroll
+            for (int q = 0; q < NEQ; q++) {
+                const double u_ = nd[n][ND_U + q];
+                f.U[q] += ph * u_;
+#pragma unroll
+                for (int j = 0; j < 3; j++) f.dU[q][j] += dphi[j] * u_;
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const double v_ = nd[n][ND_V + i];
+                SD[i] += dphi[i];
+                f.v[i] += ph * v_;
+                f.dT[i] += dphi[i] * Tnn;
+#pragma unroll
+                for (int j = 0; j < 3; j++) f.dV[i][j] += dphi[j] * v_;
+            }
+        }
+        f.mu = calc_mu(th, f.T);
+        f.k = calc_k(th, f.mu);
+        calc_tau<3>(f);
+        const double wJ = G.w3[gp] * J;
+
+        // rhs pieces: P(i,r) = dt sum_b Phi_b G_b(i,r) and D(s) = sum_jb dPhi_bj
+        // G_b(j,s), G_b from the Gauss-point tau/dT/k with node velocities
+        // (sumFGB :801-811, calc_G NavierStokes3D.h:41-64; F = 0, quirk Q1)
+        double D[5];
+        D[0] = 0.0;
+#pragma unroll
+        for (int s = 1; s < 4; s++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) acc += (-f.tau[j][s - 1] * reinv) * SD[j];
+            D[s] = acc;
+        }
+        {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                double td = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) td += f.tau[j][k] * f.dV[k][j];
+                acc += (-td - f.k * f.dT[j] * SD[j]) * reinv;
+            }
+            D[4] = acc;
+        }
+        double wg[3], wr[3], W[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            wg[i] = -dt * wJ * PM * DN[i];
+            wr[i] = wJ * DN[i];
+#pragma unroll
+            for (int j = 0; j < 3; j++) W[i][j] = -dt * s3 * wJ * DN[i] * DM[j];
+        }
+        double AQ[5];
+        contract3(th, f, R, wJ * PN * PM, wg, DN, DM, dtsq_2 * wJ, wr, D, AQ, s1, s2, s3, s4);
+        accum_c3(th, f, W, R);
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) {
+            double acc = 0.0;
+            if (r > 0) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    double P;
+                    if (r < 4) P = dt * (-f.tau[i][r - 1] * reinv) * SPhi;
+                    else {
+                        double tv = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 3; k++) tv += f.tau[i][k] * f.v[k];
+                        P = dt * ((-tv - f.k * f.dT[i] * SPhi) * reinv);
+                    }
+                    acc += wr[i] * P;
+                }
+            }
+            rhs[r] += acc - dtsq_2 * AQ[r];
+        }
+    }
+
+    // ---------------- boundary integral: faces with 0 < bc < 100 ----------------
+    const int* bcs = A.bc + e * 6;
+    int anyface = 0;
+#pragma unroll
+    for (int fc = 0; fc < 6; fc++) anyface |= (bcs[fc] != 0 && bcs[fc] < 100);
+    if (anyface) {
+        const int FN[6][4] = {{3, 2, 1, 0}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {6, 7, 4, 5}};
+        for (int fgp = 0; fgp < 4; fgp++) {
+            const double w0 = G.wf[fgp][0], w1 = G.wf[fgp][1];
+            const double z0 = G.zf[fgp][0], z1 = G.zf[fgp][1];
+            for (int fc = 0; fc < 6; fc++) {
+                const int bc = bcs[fc];
+                if (!(bc != 0 && bc < 100)) continue;
+                int fl[4];
+                double nl[4][3];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    fl[t] = FN[fc][t];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) nl[t][i] = nd[fl[t]][ND_X + i];
+                }
+                // TFNorm3D (calc_TF_norm3D.h:143-301), including the Lambda/Mu
+                // magnitude recomputed after each component (:208-212)
+                double phib[4];
+                phib[0] = 0.25 * (1 - z0) * (1 - z1);
+                phib[1] = 0.25 * (1 + z0) * (1 - z1);
+                phib[2] = 0.25 * (1 + z0) * (1 + z1);
+                phib[3] = 0.25 * (1 - z0) * (1 + z1);
+                double Lm[3], Mu[3];
+#pragma unroll
+                for (int i = 0; i < 3; i++) { Lm[i] = nl[1][i] - nl[0][i]; Mu[i] = nl[3][i] - nl[0][i]; }
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    Lm[j] /= sqrt(Lm[0] * Lm[0] + Lm[1] * Lm[1] + Lm[2] * Lm[2]);
+                    Mu[j] /= sqrt(Mu[0] * Mu[0] + Mu[1] * Mu[1] + Mu[2] * Mu[2]);
+                }
+                double en_[3], ga[3];
+                en_[0] = Lm[1] * Mu[2] - Lm[2] * Mu[1];
+                en_[1] = Lm[2] * Mu[0] - Lm[0] * Mu[2];
+                en_[2] = Lm[0] * Mu[1] - Lm[1] * Mu[0];
+                ga[0] = en_[1] * Lm[2] - en_[2] * Lm[1];
+                ga[1] = en_[2] * Lm[0] - en_[0] * Lm[2];
+                ga[2] = en_[0] * Lm[1] - en_[1] * Lm[0];
+                double dp[4][2];
+                dp[0][0] = -0.25 * (1 - z1); dp[1][0] = 0.25 * (1 - z1);
+                dp[2][0] = 0.25 * (1 + z1);  dp[3][0] = -0.25 * (1 + z1);
+                dp[0][1] = -0.25 * (1 - z0); dp[1][1] = -0.25 * (1 + z0);
+                dp[2][1] = 0.25 * (1 + z0);  dp[3][1] = 0.25 * (1 - z0);
+                double dxd[3][2];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int t = 0; t < 4; t++) acc += dp[t][j] * nl[t][i];
+                        dxd[i][j] = acc;
+                    }
+                double dtr[2][2];
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    dtr[0][j] = Lm[0] * dxd[0][j] + Lm[1] * dxd[1][j] + Lm[2] * dxd[2][j];
+                    dtr[1][j] = ga[0] * dxd[0][j] + ga[1] * dxd[1][j] + ga[2] * dxd[2][j];
+                }
+                const double detjac = dtr[0][0] * dtr[1][1] - dtr[1][0] * dtr[0][1];
+                const double djinv = 1. / detjac;
+                if (detjac < 1.0E-12) *status |= 2;
+                double en[3];
+#pragma unroll
+                for (int i = 0; i < 3; i++) en[i] = en_[i] * w0 * w1 * detjac;
+
+                // face Gauss-point state; raw (unedited) sums kept for G
+                GPState<3> f;
+                f.E = 0; f.T = 0;
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    f.v[i] = 0; f.dT[i] = 0;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) { f.dV[i][j] = 0; f.tau[i][j] = 0; }
+                }
+#pragma unroll
+                for (int q = 0; q < NEQ; q++) {
+                    f.U[q] = 0;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) f.dU[q][j] = 0;
+                }
+                double Sph = 0.0, Sdp[3] = {0, 0, 0};
+                int fN = -1, fM = -1;
+                double phN = 0, phM = 0, dpM[3] = {0, 0, 0};
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int n = fl[t];
+                    double dpx[3];
+                    const double d0 = (dtr[1][1] * dp[t][0] - dtr[1][0] * dp[t][1]) * djinv;
+                    const double d1 = (-dtr[0][1] * dp[t][0] + dtr[0][0] * dp[t][1]) * djinv;
+#pragma unroll
+                    for (int i = 0; i < 3; i++) dpx[i] = Lm[i] * d0 + ga[i] * d1;
+                    const double ph = phib[t];
+                    if (n == N) { fN = t; phN = ph; }
+                    if (n == M) { fM = t; phM = ph; dpM[0] = dpx[0]; dpM[1] = dpx[1]; dpM[2] = dpx[2]; }
+                    Sph += ph;
+                    f.E += ph * nd[n][ND_E];
+                    const double Tt = nd[n][ND_T];
+                    f.T += ph * Tt;
+#pragma unroll
+                    for (int q = 0; q < NEQ; q++) {
+                        const double u_ = nd[n][ND_U + q];
+                        f.U[q] += ph * u_;
+#pragma unroll
+                        for (int j = 0; j < 3; j++) f.dU[q][j] += dpx[j] * u_;
+                    }
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        const double v_ = nd[n][ND_V + i];
+                        Sdp[i] += dpx[i];
+                        f.v[i] += ph * v_;
+                        f.dT[i] += dpx[i] * Tt;
+#pragma unroll
+                        for (int j = 0; j < 3; j++) f.dV[i][j] += dpx[j] * v_;
+                    }
+                }
+                double vph[3], dVp[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    vph[i] = f.v[i];
+#pragma unroll
+                    for (int j = 0; j < 3; j++) dVp[i][j] = f.dV[i][j];
+                }
+                f.mu = calc_mu(th, f.T);
+                f.k = calc_k(th, f.mu);
+                if (!eval2dboundary(th, bc, f, en)) {
+                    *status |= 1;
+                    continue;
+                }
+                if (fN < 0) continue;   // this lane's row is not on the face
+                double D[5];
+                D[0] = 0.0;
+#pragma unroll
+                for (int s = 1; s < 4; s++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) acc += (-f.tau[j][s - 1] * reinv) * Sdp[j];
+                    D[s] = acc;
+                }
+                {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        double td = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 3; k++) td += f.tau[j][k] * dVp[k][j];
+                        acc += (-td - f.k * f.dT[j] * Sdp[j]) * reinv;
+                    }
+                    D[4] = acc;
+                }
+                const double onM = (fM >= 0) ? 1.0 : 0.0;
+                double wg[3], wr[3], W[3][3], wa[3];
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    wg[i] = dt * phN * phM * en[i] * onM;
+                    wr[i] = phN * en[i];
+                    wa[i] = en[i] * onM;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) W[i][j] = dt * s3 * phN * en[i] * dpM[j] * onM;
+                }
+                double AQ[5];
+                contract3(th, f, R, 0.0, wg, wa, dpM, -dtsq_2 * phN, wr, D, AQ, s1, s2, s3, s4);
+                accum_c3(th, f, W, R);
+                if (M == 0) {
+#pragma unroll
+                    for (int r = 0; r < NEQ; r++) {
+                        double acc = 0.0;
+                        if (r > 0) {
+#pragma unroll
+                            for (int i = 0; i < 3; i++) {
+                                double P;
+                                if (r < 4) P = dt * (-f.tau[i][r - 1] * reinv) * Sph;
+                                else {
+                                    double tv = 0.0;
+#pragma unroll
+                                    for (int k = 0; k < 3; k++) tv += f.tau[i][k] * vph[k];
+                                    P = dt * ((-tv - f.k * f.dT[i] * Sph) * reinv);
+                                }
+                                acc += wr[i] * P;
+                            }
+                        }
+                        rhs[r] += -acc + dtsq_2 * AQ[r];
+                    }
+                }
+            }
+        }
+    }
+
+    // ---------------- ApplyBC: constrained rows ----------------
+    const unsigned long long cm = A.cmask[e];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++) {
+        if ((cm >> (NEQ * N + r)) & 1ULL) {
+            rhs[r] = 0.0;
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) R[r][s] = (N == M && r == s) ? 0.5 : 0.0;
+        }
+    }
+
+}
+
+template <bool W64, bool W32>
+__global__ __launch_bounds__(256) void build3d_kernel(BuildArgs A) {
+    constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600;
+    __shared__ double s_nd[4][NNOD][ND_STRIDE];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const long e = (long)xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
+    const DevThermo th = A.th;
+    const bool in_range = e < A.n_elem_pad;
+    const int* cn = A.conn + (in_range ? e : 0) * NNOD;
+    const bool real = in_range && cn[0] >= 0;
+    double (*nd)[ND_STRIDE] = s_nd[wv];
+    if (real && lane < NNOD) load_node3d(A, cn[lane], nd[lane]);
+    __syncthreads();
+    if (!in_range) return;
+    if (!real) {   // padding slot of a color range: zero block
+        if (W64)
+            for (int q = lane; q < KK; q += 64) A.R64[blk_index<double>(e, q, KK)] = 0.0;
+        if (W32)
+            for (int q = lane; q < KK; q += 64) A.R32[blk_index<float>(e, q, KK)] = 0.0f;
+        if (lane < K) A.rhs[e * K + lane] = 0.0;
+        return;
+    }
+    const int N = lane >> 3, M = lane & 7;
+    double R[NEQ][NEQ], rhs[NEQ], sp[4];
+    int tie = 0, status = 0;
+    build3d_elem(A, c_gauss, nd, e, lane, R, rhs, sp, &tie, &status);
+    if (status && lane == 0) atomicOr(A.status, status);
+    // ---------------- write ----------------
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) {
+            const int q = (NEQ * N + r) * K + NEQ * M + s;
+            if (W64) A.R64[blk_index<double>(e, q, KK)] = R[r][s];
+            if (W32) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
+        }
+    if (M == 0) {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
+    }
+    if (lane == 0) {
+        if (A.svals) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) A.svals[e * 4 + i] = sp[i];
+        }
+        if (tie) atomicAdd(A.ties, 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// quad4 build kernel: four elements per wavefront, 16 lanes each
+// ---------------------------------------------------------------------------
+// One quad4 element seen from lane (N, M) = (l16/4, l16%4).
+CFDV_HD void build2d_elem(const BuildArgs& A, const GaussTables& G, long e, int l16, double R[4][4],
+                          double rhs[4], double sp[4], int* tie, int* status) {
+    // No FMA contraction here: the discontinuity-capturing branch
+    // (FDVGalerkin.h:349-379) tests gamma = |v||A| == 0, and on elements with
+    // a uniform velocity along a side A is an exact zero only when the
+    // Jacobian, dPhi/dx and dV sums round as in the reference's non-FMA
+    // arithmetic; a contracted residue flips tau_b from 0 to tau.
+#pragma clang fp contract(off)
+    constexpr int NEQ = 4;
+    const int N = l16 >> 2, M = l16 & 3;
+    const DevThermo th = A.th;
+    const int* cn = A.conn + e * 4;
+    double xn[4][2], Un[4][4], vn[4][2], En[4], Tn[4], rhon[4], pn[4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const int g = cn[n];
+        xn[n][0] = A.x[(size_t)g * 2 + 0];
+        xn[n][1] = A.x[(size_t)g * 2 + 1];
+#pragma unroll
+        for (int q = 0; q < 4; q++) Un[n][q] = A.U[(size_t)g * 4 + q];
+        unpack<2>(th, Un[n], rhon[n], vn[n], En[n], Tn[n]);
+        const double vs = vn[n][0] * vn[n][0] + vn[n][1] * vn[n][1];
+        pn[n] = rhon[n] * (th.gamma - 1.0) * (En[n] - 0.5 * vs);
+    }
+    fdvparam<2, 4>(th, vn, Tn, rhon, sp, tie);
+    const double s1 = sp[0], s2 = sp[1], s3 = sp[2], s4 = sp[3];
+    const double dt = A.dt, dtsq_2 = (dt * dt) / 2.0;
+    const double reinv = 1.0 / th.creyn;
+
+    // node fluxes F (NavierStokes2D::calc_F, :11-29) are state-only
+    double Fn[4][2][4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const double rho = Un[n][0], l = Un[n][1], m = Un[n][2], en = Un[n][3], p = pn[n];
+        Fn[n][0][0] = l; Fn[n][0][1] = p + (l * l) / rho; Fn[n][0][2] = (l * m) / rho; Fn[n][0][3] = (p + en) * (l / rho);
+        Fn[n][1][0] = m; Fn[n][1][1] = (l * m) / rho; Fn[n][1][2] = p + (m * m) / rho; Fn[n][1][3] = (p + en) * (m / rho);
+    }
+    // element sizes for the numerical diffusion (FDVGalerkin.h:284-297, Q9)
+    const double h_xi = (0.5 * xn[1][0] + 0.5 * xn[2][0]) - (0.5 * xn[0][0] + 0.5 * xn[3][0]);
+    const double h_eta = (0.5 * xn[0][1] + 0.5 * xn[1][1]) - (0.5 * xn[2][1] + 0.5 * xn[3][1]);
+
+#pragma unroll
+    for (int r = 0; r < NEQ; r++) {
+        rhs[r] = 0.0;
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) R[r][s] = 0.0;
+    }
+    for (int gp = 0; gp < 4; gp++) {
+        double jac[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int n = 0; n < 4; n++) acc += G.dpxi2[gp][n][j] * xn[n][i];
+                jac[i][j] = acc;
+            }
+        const double J = jac[0][0] * jac[1][1] - jac[1][0] * jac[0][1];
+        const double Jinv = 1.0 / J;
+        double dphi[4][2];
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const double d0 = G.dpxi2[gp][n][0], d1 = G.dpxi2[gp][n][1];
+            dphi[n][0] = Jinv * (d0 * jac[1][1] - jac[1][0] * d1);
+            dphi[n][1] = Jinv * (jac[0][0] * d1 - d0 * jac[0][1]);
+        }
+        GPState<2> f;
+        f.E = 0; f.T = 0;
+#pragma unroll
+        for (int i = 0; i < 2; i++) { f.v[i] = 0; f.dT[i] = 0; f.dV[i][0] = f.dV[i][1] = 0; }
+#pragma unroll
+        for (int q = 0; q < 4; q++) { f.U[q] = 0; f.dU[q][0] = f.dU[q][1] = 0; }
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const double ph = G.phi2[gp][n];
+            f.E += ph * En[n];
+            f.T += ph * Tn[n];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                f.U[q] += ph * Un[n][q];
+#pragma unroll
+                for (int j = 0; j < 2; j++) f.dU[q][j] += dphi[n][j] * Un[n][q];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                f.v[i] += ph * vn[n][i];
+                f.dT[i] += dphi[n][i] * Tn[n];
+#pragma unroll
+                for (int j = 0; j < 2; j++) f.dV[i][j] += dphi[n][j] * vn[n][i];
+            }
+        }
+        f.mu = calc_mu(th, f.T);
+        f.k = calc_k(th, f.mu);
+        calc_tau<2>(f);
+        // numerical diffusion and discontinuity capturing (:225-387)
+        double Psi[4];
+        {
+            const double J_xi = jac[0][0] * jac[0][0] + jac[1][0] * jac[1][0];
+            const double J_eta = jac[0][1] * jac[0][1] + jac[1][1] * jac[1][1];
+            const double ixi = 1. / pow(J_xi, 0.5), ieta = 1. / pow(J_eta, 0.5);
+            const double v_xi = f.v[0] * (ixi * jac[0][0]) + f.v[1] * (ixi * jac[1][0]);
+            const double v_eta = f.v[0] * (ieta * jac[0][1]) + f.v[1] * (ieta * jac[1][1]);
+            const double S = f.v[0] * f.v[0] + f.v[1] * f.v[1];
+            const double R_xi = f.U[0] * v_xi * h_xi / f.mu;
+            const double R_eta = f.U[0] * v_eta * h_eta / f.mu;
+            const double al_xi = (R_xi == 0) ? 0.0 : cosh(R_xi / 2.) / sinh(R_xi / 2.) - 2. / R_xi;
+            const double al_eta = (R_eta == 0) ? 0.0 : cosh(R_eta / 2.) / sinh(R_eta / 2.) - 2. / R_eta;
+            const double taun = (S == 0) ? 0.0 : (1. / 4.) * (al_xi * h_xi * v_xi + al_eta * h_eta * v_eta) / S;
+            double Av[2] = {0, 0};
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) Av[j] += f.dV[j][k] * f.v[k];
+            const double gam = sqrt(f.v[0] * f.v[0] + f.v[1] * f.v[1]) * sqrt(Av[0] * Av[0] + Av[1] * Av[1]);
+            double tau_b = 0;
+#pragma unroll
+            for (int j = 0; j < 2; j++) tau_b += taun * f.v[j] * Av[j] / gam;
+            if ((tau_b - taun) < 0) tau_b = taun;
+            if (gam == 0) tau_b = 0;
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                double ps = 0.0;
+#pragma unroll
+                for (int i = 0; i < 2; i++) ps += taun * f.v[i] * dphi[n][i];
+#pragma unroll
+                for (int i = 0; i < 2; i++) ps += tau_b * f.v[i] * dphi[n][i];
+                Psi[n] = ps;
+            }
+        }
+        double a[2][4][4], b[2][4][4];
+        calc_a2(th, f, a);
+        calc_b2(th, f, b);
+        double PN = 0, PM = 0, DN[2] = {0, 0}, DM[2] = {0, 0}, PsiN = 0;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const double ph = G.phi2[gp][n];
+            if (n == N) { PN = ph; DN[0] = dphi[n][0]; DN[1] = dphi[n][1]; PsiN = Psi[n]; }
+            if (n == M) { PM = ph; DM[0] = dphi[n][0]; DM[1] = dphi[n][1]; }
+        }
+        const double wJ = G.w2[gp] * J;
+        // rhs with F + G at nodes
+        {
+            double P[2][4], D[4];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) P[i][r] = 0.0;
+#pragma unroll
+            for (int s = 0; s < 4; s++) D[s] = 0.0;
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                double Gv[2][4];
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    Gv[i][0] = 0.0;
+                    Gv[i][1] = -f.tau[i][0] * reinv;
+                    Gv[i][2] = -f.tau[i][1] * reinv;
+                    Gv[i][3] = (-f.tau[i][0] * vn[n][0] - f.tau[i][1] * vn[n][1] - f.k * f.dT[i]) * reinv;
+                }
+                const double ph = G.phi2[gp][n];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const double fg = Fn[n][i][r] + Gv[i][r];
+                        P[i][r] += dt * fg * ph;
+                        D[r] += dphi[n][i] * fg;
+                    }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    double Q = 0.0;
+#pragma unroll
+                    for (int s = 0; s < 4; s++) Q += (a[i][r][s] + b[i][r][s]) * D[s];
+                    acc += DN[i] * (P[i][r] - dtsq_2 * Q);
+                }
+                rhs[r] += wJ * acc;
+            }
+        }
+        {
+            double wg[2], W[2][2];
+            const double sdm = DM[0] + DM[1];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                wg[i] = -dt * wJ * PM * DN[i] + dt * wJ * PsiN * sdm;   // g and the nd term (Q10)
+#pragma unroll
+                for (int j = 0; j < 2; j++) W[i][j] = -dt * s3 * wJ * DN[i] * DM[j];
+            }
+            contract<2, 4>(R, wJ * PN * PM, wg, DN, DM, dtsq_2 * wJ, a, b, s1, s2, s3, s4);
+            accum_c2(th, f, W, R);
+        }
+    }
+    // boundary edges
+    const int* bcs = A.bc + e * 4;
+    for (int egp = 0; egp < 2; egp++) {
+        const double w0 = G.we[egp], z0 = G.ze[egp];
+        for (int fc = 0; fc < 4; fc++) {
+            const int bc = bcs[fc];
+            if (!(bc != 0 && bc < 100)) continue;
+            const int n0 = fc, n1 = (fc + 1) & 3;
+            double x0[2], x1[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                double a0 = 0, a1 = 0;
+#pragma unroll
+                for (int n = 0; n < 4; n++) { a0 = (n == n0) ? xn[n][i] : a0; a1 = (n == n1) ? xn[n][i] : a1; }
+                x0[i] = a0; x1[i] = a1;
+            }
+            const double phib[2] = {0.5 * (1 - z0), 0.5 * (1 + z0)};
+            const double dxds = -0.5 * x0[0] + 0.5 * x1[0];
+            const double dyds = -0.5 * x0[1] + 0.5 * x1[1];
+            const double en[2] = {dyds * w0, -dxds * w0};
+            const double delx = x1[0] - x0[0], dely = x1[1] - x0[1];
+            double dpidx[2][2] = {{0, 0}, {0, 0}};
+            if (!(fabs(delx) < 1.0E-12)) { dpidx[0][0] = -1.0 / delx; dpidx[1][0] = 1.0 / delx; }
+            if (!(fabs(dely) < 1.0E-12)) { dpidx[0][1] = -1.0 / dely; dpidx[1][1] = 1.0 / dely; }
+            const int fln[2] = {n0, n1};
+            GPState<2> f;
+            f.E = 0; f.T = 0;
+#pragma unroll
+            for (int i = 0; i < 2; i++) { f.v[i] = 0; f.dT[i] = 0; f.dV[i][0] = f.dV[i][1] = 0; f.tau[i][0] = f.tau[i][1] = 0; }
+#pragma unroll
+            for (int q = 0; q < 4; q++) { f.U[q] = 0; f.dU[q][0] = f.dU[q][1] = 0; }
+            double vb[2][2], Ub[2][4], Fb[2][2][4];
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) Fb[t][i][r] = 0.0;
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                double Et = 0, Tt = 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) Ub[t][q] = 0;
+                vb[t][0] = vb[t][1] = 0;
+#pragma unroll
+                for (int n = 0; n < 4; n++) {
+                    const bool sel = (n == fln[t]);
+                    Et = sel ? En[n] : Et;
+                    Tt = sel ? Tn[n] : Tt;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) Ub[t][q] = sel ? Un[n][q] : Ub[t][q];
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        vb[t][i] = sel ? vn[n][i] : vb[t][i];
+#pragma unroll
+                        for (int r = 0; r < 4; r++) Fb[t][i][r] = sel ? Fn[n][i][r] : Fb[t][i][r];
+                    }
+                }
+                const double ph = phib[t];
+                f.E += ph * Et;
+                f.T += ph * Tt;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    f.U[q] += ph * Ub[t][q];
+#pragma unroll
+                    for (int j = 0; j < 2; j++) f.dU[q][j] += dpidx[t][j] * Ub[t][q];
+                }
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    f.v[i] += ph * vb[t][i];
+                    f.dT[i] += dpidx[t][i] * Tt;
+#pragma unroll
+                    for (int j = 0; j < 2; j++) f.dV[i][j] += dpidx[t][j] * vb[t][i];
+                }
+            }
+            f.mu = calc_mu(th, f.T);
+            f.k = calc_k(th, f.mu);
+            if (!eval1dboundary(bc, f, x0, x1, en)) {
+                *status |= 1;
+                continue;
+            }
+            double a[2][4][4], b[2][4][4];
+            calc_a2(th, f, a);
+            calc_b2(th, f, b);
+            int fN = (N == n0) ? 0 : ((N == n1) ? 1 : -1);
+            int fM = (M == n0) ? 0 : ((M == n1) ? 1 : -1);
+            const double phN = (fN == 0) ? phib[0] : ((fN == 1) ? phib[1] : 0.0);
+            const double phM = (fM == 0) ? phib[0] : ((fM == 1) ? phib[1] : 0.0);
+            double dpM[2] = {0, 0};
+            if (fM >= 0) { dpM[0] = dpidx[fM][0]; dpM[1] = dpidx[fM][1]; }
+            if (fN >= 0 && M == 0) {
+                double P[2][4], D[4];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) P[i][r] = 0.0;
+#pragma unroll
+                for (int s = 0; s < 4; s++) D[s] = 0.0;
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        double Gv[4];
+                        Gv[0] = 0.0;
+                        Gv[1] = -f.tau[i][0] * reinv;
+                        Gv[2] = -f.tau[i][1] * reinv;
+                        Gv[3] = (-f.tau[i][0] * vb[t][0] - f.tau[i][1] * vb[t][1] - f.k * f.dT[i]) * reinv;
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const double fg = Fb[t][i][r] + Gv[r];
+                            P[i][r] += dt * fg * phib[t];
+                            D[r] += dpidx[t][i] * fg;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        double Q = 0.0;
+#pragma unroll
+                        for (int s = 0; s < 4; s++) Q += (a[i][r][s] + b[i][r][s]) * D[s];
+                        acc += en[i] * (-P[i][r] + dtsq_2 * Q);
+                    }
+                    rhs[r] += phN * acc;
+                }
+            }
+            if (fN >= 0 && fM >= 0) {
+                double wg[2], W[2][2];
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    wg[i] = dt * phN * phM * en[i];
+#pragma unroll
+                    for (int j = 0; j < 2; j++) W[i][j] = dt * s3 * phN * en[i] * dpM[j];
+                }
+                contract<2, 4>(R, 0.0, wg, en, dpM, -dtsq_2 * phN, a, b, s1, s2, s3, s4);
+                accum_c2(th, f, W, R);
+            }
+        }
+    }
+    const unsigned long long cm = A.cmask[e];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++) {
+        if ((cm >> (NEQ * N + r)) & 1ULL) {
+            rhs[r] = 0.0;
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) R[r][s] = (N == M && r == s) ? 0.5 : 0.0;
+        }
+    }
+}
+
+template <bool W64, bool W32>
+__global__ __launch_bounds__(256) void build2d_kernel(BuildArgs A) {
+    constexpr int NNOD = 4, NEQ = 4, K = 16, KK = 256;
+    const int lane = threadIdx.x & 63;
+    const long e = ((long)blockIdx.x * 256 + threadIdx.x) >> 4;
+    if (e >= A.n_elem_pad) return;
+    const int l16 = lane & 15;
+    const int N = l16 >> 2, M = l16 & 3;
+    const DevThermo th = A.th;
+    const int* cn = A.conn + e * NNOD;
+    if (cn[0] < 0) {
+        for (int q = l16; q < KK; q += 16) {
+            if (W64) A.R64[blk_index<double>(e, q, KK)] = 0.0;
+            if (W32) A.R32[blk_index<float>(e, q, KK)] = 0.0f;
+        }
+        A.rhs[e * K + l16] = 0.0;
+        return;
+    }
+    double R[NEQ][NEQ], rhs[NEQ], sp[4];
+    int tie = 0, status = 0;
+    build2d_elem(A, c_gauss, e, l16, R, rhs, sp, &tie, &status);
+    if (status) atomicOr(A.status, status);
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) {
+            const int q = (NEQ * N + r) * K + NEQ * M + s;
+            if (W64) A.R64[blk_index<double>(e, q, KK)] = R[r][s];
+            if (W32) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
+        }
+    if (M == 0) {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
+    }
+    if (l16 == 0) {
+        if (A.svals)
+            for (int i = 0; i < 4; i++) A.svals[e * 4 + i] = sp[i];
+        if (tie) atomicAdd(A.ties, 1);
+    }
+}
+
+void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st) {
+    if (a.ndim == 3) {
+        const int nb = (int)((a.n_elem_pad + 3) / 4);
+        if (w64 && w32) hipLaunchKernelGGL((build3d_kernel<true, true>), dim3(nb), dim3(256), 0, st, a);
+        else if (w64) hipLaunchKernelGGL((build3d_kernel<true, false>), dim3(nb), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3(nb), dim3(256), 0, st, a);
+    } else {
+        const int nb = (int)((a.n_elem_pad * 16 + 255) / 256);
+        if (w64 && w32) hipLaunchKernelGGL((build2d_kernel<true, true>), dim3(nb), dim3(256), 0, st, a);
+        else if (w64) hipLaunchKernelGGL((build2d_kernel<true, false>), dim3(nb), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((build2d_kernel<false, true>), dim3(nb), dim3(256), 0, st, a);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Refinement flags: AdaptiveRefineList's s1 > 0.8 test
+// (src/adap/RefineListAdaptive.h:28-38), thread per element, no FMA
+// contraction so that the flags are bit-identical to the CPU restatement.
+// ---------------------------------------------------------------------------
+template <int NDIM, int NNOD>
+__global__ void flags_kernel(const DevThermo th, const int* __restrict__ conn, const double* __restrict__ U,
+                             const int* __restrict__ perm_inv, long n_elem, uint8_t* __restrict__ flags,
+                             double* __restrict__ s1_out, int* ties) {
+#pragma clang fp contract(off)
+    const long eo = (long)blockIdx.x * blockDim.x + threadIdx.x;   // original element id
+    if (eo >= n_elem) return;
+    const long e = perm_inv[eo];
+    const int* cn = conn + e * NNOD;
+    double vn[NNOD][NDIM], Tn[NNOD], rhon[NNOD];
+    for (int n = 0; n < NNOD; n++) {
+        double Ul[NDIM + 2], E;
+        const int g = cn[n];
+        for (int q = 0; q < NDIM + 2; q++) Ul[q] = U[(size_t)g * (NDIM + 2) + q];
+        rhon[n] = Ul[0];
+        double vsqr = 0;
+        for (int i = 0; i < NDIM; i++) {
+            vn[n][i] = Ul[1 + i] / Ul[0];
+            vsqr += (vn[n][i] * vn[n][i]);
+        }
+        E = Ul[NDIM + 1] / Ul[0];
+        const double ee = E - 0.5 * vsqr;
+        Tn[n] = ee / th.Cv;
+    }
+    double s[4];
+    int tie;
+    fdvparam<NDIM, NNOD>(th, vn, Tn, rhon, s, &tie);
+    flags[eo] = (s[0] > 0.8) ? 1 : 0;
+    if (s1_out) s1_out[eo] = s[0];
+    if (tie) atomicAdd(ties, 1);
+}
+
+void launch_flags(int ndim, const DevThermo& th, const int* conn, const double* U, const int* perm_inv,
+                  long n_elem, uint8_t* flags, double* s1, int* ties, hipStream_t st) {
+    const int nb = (int)((n_elem + 255) / 256);
+    if (ndim == 3)
+        hipLaunchKernelGGL((flags_kernel<3, 8>), dim3(nb), dim3(256), 0, st, th, conn, U, perm_inv, n_elem, flags, s1, ties);
+    else
+        hipLaunchKernelGGL((flags_kernel<2, 4>), dim3(nb), dim3(256), 0, st, th, conn, U, perm_inv, n_elem, flags, s1, ties);
+}
+
+// ---------------------------------------------------------------------------
+// CFL (src/FDV/Function/MPI_CFL.h:170-217): per-element dxele, then a
+// block min; min is order-independent, so the result is exact.
+// ---------------------------------------------------------------------------
+template <int NDIM, int NNOD>
+__global__ void cfl_kernel(const DevThermo th, const int* __restrict__ conn, const double* __restrict__ U,
+                           const double* __restrict__ cleng, long n_elem_pad, double* __restrict__ part) {
+    __shared__ double sm[256];
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dxele = 1000;
+    if (e < n_elem_pad && conn[e * NNOD] >= 0) {
+        for (int n = 0; n < NNOD; n++) {
+            const int g = conn[e * NNOD + n];
+            double v[NDIM], rho, E, T;
+            double Ul[NDIM + 2];
+            for (int q = 0; q < NDIM + 2; q++) Ul[q] = U[(size_t)g * (NDIM + 2) + q];
+            unpack<NDIM>(th, Ul, rho, v, E, T);
+            const double Temp = (T > 10E-10) ? T : 0.0;
+            double vv = 0;
+            for (int i = 0; i < NDIM; i++) vv += v[i] * v[i];
+            const double vm = sqrt(vv);
+            const double a = sqrt(1.4 * th.cgas * Temp);
+            if (vm + a > 10E-10) {
+                const double dxnode = cleng[e] / (vm + a);
+                if (dxnode < dxele) dxele = dxnode;
+            }
+        }
+    }
+    sm[threadIdx.x] = dxele;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+void launch_cfl(int ndim, const DevThermo& th, const int* conn, const double* U, const double* cleng,
+                long n_elem_pad, double* part, int nblocks, hipStream_t st) {
+    if (ndim == 3)
+        hipLaunchKernelGGL((cfl_kernel<3, 8>), dim3(nblocks), dim3(256), 0, st, th, conn, U, cleng, n_elem_pad, part);
+    else
+        hipLaunchKernelGGL((cfl_kernel<2, 4>), dim3(nblocks), dim3(256), 0, st, th, conn, U, cleng, n_elem_pad, part);
+}
+
+// CalcLength (src/FDV/Function/CalcLength.h:256-332): area by 2-point Gauss,
+// cleng = area^(1/ndim)
+template <int NDIM, int NNOD>
+__global__ void length_kernel(const int* __restrict__ conn, const double* __restrict__ x, long n_elem_pad,
+                              double* __restrict__ area, double* __restrict__ cleng) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_elem_pad) return;
+    if (conn[e * NNOD] < 0) { area[e] = 0; cleng[e] = 0; return; }
+    double xn[NNOD][NDIM];
+    for (int n = 0; n < NNOD; n++)
+        for (int i = 0; i < NDIM; i++) xn[n][i] = x[(size_t)conn[e * NNOD + n] * NDIM + i];
+    double ar = 0;
+    if (NDIM == 3) {
+        for (int gp = 0; gp < 8; gp++) {
+            double j[3][3];
+            for (int i = 0; i < 3; i++)
+                for (int k = 0; k < 3; k++) {
+                    double acc = 0;
+                    for (int n = 0; n < 8; n++) acc += c_gauss.dpxi3[gp][n][k] * xn[n][i];
+                    j[i][k] = acc;
+                }
+            const double J = j[0][0] * j[1][1] * j[2][2] + j[0][1] * j[1][2] * j[2][0] + j[0][2] * j[1][0] * j[2][1] -
+                             j[0][2] * j[1][1] * j[2][0] - j[0][1] * j[1][0] * j[2][2] - j[0][0] * j[1][2] * j[2][1];
+            ar += c_gauss.w3[gp] * J;
+        }
+    } else {
+        for (int gp = 0; gp < 4; gp++) {
+            double j[2][2];
+            for (int i = 0; i < 2; i++)
+                for (int k = 0; k < 2; k++) {
+                    double acc = 0;
+                    for (int n = 0; n < 4; n++) acc += c_gauss.dpxi2[gp][n][k] * xn[n][i];
+                    j[i][k] = acc;
+                }
+            ar += c_gauss.w2[gp] * (j[0][0] * j[1][1] - j[1][0] * j[0][1]);
+        }
+    }
+    area[e] = ar;
+    cleng[e] = pow(ar, 1.0 / (double)NDIM);
+}
+
+void launch_length(int ndim, const int* conn, const double* x, long n_elem_pad, double* area, double* cleng,
+                   hipStream_t st) {
+    const int nb = (int)((n_elem_pad + 255) / 256);
+    if (ndim == 3) hipLaunchKernelGGL((length_kernel<3, 8>), dim3(nb), dim3(256), 0, st, conn, x, n_elem_pad, area, cleng);
+    else hipLaunchKernelGGL((length_kernel<2, 4>), dim3(nb), dim3(256), 0, st, conn, x, n_elem_pad, area, cleng);
+}
+
+}  // namespace cfdv

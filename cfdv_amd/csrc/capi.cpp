@@ -1,0 +1,1093 @@
+// Host runtime and C ABI of libcfdv_amd (see include/cfdv.h).
+//
+// Owns the device-resident mesh, state, element blocks and Krylov space, and
+// drives the kernels of build.hip / solve.hip on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/cfdv.h"
+#include "cfdv_internal.h"
+
+using namespace cfdv;
+
+namespace {
+
+constexpr int NBLK_RED = 512;   // blocks of the reduction kernels
+
+// 2-point Gauss-Legendre by Newton iteration, the rule of
+// GaussQuad::get_gauss (src/FiniteElement/GaussQuad.h:258-301).
+void gauss2(double w[2], double z[2]) {
+    const unsigned N = 2;
+    const double eps = 3.E-14;
+    double p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+    double zz = std::cos(3.141592654 * (1 - 0.25) / (N + 0.5));
+    double z1 = zz + 10;
+    while (std::fabs(zz - z1) > eps) {
+        p1 = 1.0;
+        p2 = 0.0;
+        for (unsigned j = 1; j <= N; j++) {
+            p3 = p2;
+            p2 = p1;
+            p1 = ((2.0 * j - 1.0) * zz * p2 - (j - 1.) * p3) / j;
+        }
+        p0 = N * (zz * p1 - p2) / (zz * zz - 1.0);
+        z1 = zz;
+        zz = z1 - p1 / p0;
+    }
+    z[0] = -zz;
+    z[1] = zz;
+    w[0] = 2.0 / ((1.0 - zz * zz) * p0 * p0);
+    w[1] = w[0];
+}
+
+GaussTables make_tables() {
+    GaussTables g;
+    std::memset(&g, 0, sizeof(g));
+    double w[2], z[2];
+    gauss2(w, z);
+    // 3D domain points, last index fastest (GaussQuad::three, :231-254)
+    int gp = 0;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            for (int k = 0; k < 2; k++, gp++) {
+                const double zt[3] = {z[i], z[j], z[k]};
+                g.w3[gp] = 1.0 * w[i] * w[j] * w[k];
+                // testfunction3d / dtestfunction3d (TestFunctions3D.h:7-56)
+                const double sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+                const double sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+                const double sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+                for (int n = 0; n < 8; n++) {
+                    g.phi3[gp][n] = 0.125 * (1 + sx[n] * zt[0]) * (1 + sy[n] * zt[1]) * (1 + sz[n] * zt[2]);
+                    g.dpxi3[gp][n][0] = 0.125 * sx[n] * (1 + sy[n] * zt[1]) * (1 + sz[n] * zt[2]);
+                    g.dpxi3[gp][n][1] = 0.125 * sy[n] * (1 + sx[n] * zt[0]) * (1 + sz[n] * zt[2]);
+                    g.dpxi3[gp][n][2] = 0.125 * sz[n] * (1 + sx[n] * zt[0]) * (1 + sy[n] * zt[1]);
+                }
+            }
+    // face points (GaussQuad::two)
+    gp = 0;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++, gp++) {
+            g.wf[gp][0] = w[i];
+            g.wf[gp][1] = w[j];
+            g.zf[gp][0] = z[i];
+            g.zf[gp][1] = z[j];
+        }
+    // 2D domain (testfunction / dtestfunction, TestFunctions2D.h:7-32)
+    gp = 0;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++, gp++) {
+            const double zt[2] = {z[i], z[j]};
+            g.w2[gp] = 1.0 * w[i] * w[j];
+            g.z2[gp][0] = zt[0];
+            g.z2[gp][1] = zt[1];
+            const double sx[4] = {-1, 1, 1, -1};
+            const double sy[4] = {-1, -1, 1, 1};
+            for (int n = 0; n < 4; n++) {
+                g.phi2[gp][n] = 0.25 * (1 + sx[n] * zt[0]) * (1 + sy[n] * zt[1]);
+                g.dpxi2[gp][n][0] = 0.25 * sx[n] * (1 + sy[n] * zt[1]);
+                g.dpxi2[gp][n][1] = 0.25 * sy[n] * (1 + sx[n] * zt[0]);
+            }
+        }
+    for (int i = 0; i < 2; i++) {
+        g.we[i] = w[i];
+        g.ze[i] = z[i];
+    }
+    return g;
+}
+
+const int FACE3[6][4] = {{3, 2, 1, 0}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {6, 7, 4, 5}};
+
+template <typename T>
+T* dmalloc(size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    HIP_CHECK_THROW(hipMalloc(&p, n * sizeof(T)));
+    return (T*)p;
+}
+
+}  // namespace
+
+struct cfdv_ctx {
+    std::string err;
+    int ndim = 0, nnod = 0, neqn = 0, nface = 0, nbnod = 0, k = 0;
+    long n_elem = 0, n_node = 0, n_pad = 0, ndof = 0;
+    int ncolor = 0;
+    std::vector<long> cstart;       // color ranges in slots, size ncolor+1
+    std::vector<int> perm;          // slot -> original element (-1 padding)
+    std::vector<int> perm_inv;      // original -> slot
+    std::vector<int> color_orig;    // color per original element
+    DevThermo th{};
+    hipStream_t st = nullptr;
+    int block_flags = 0;
+    bool built = false, have_du = false;
+
+    int *d_conn = nullptr, *d_bc = nullptr, *d_perm_inv = nullptr;
+    unsigned long long* d_cmask = nullptr;
+    double *d_x = nullptr, *d_U = nullptr, *d_U0 = nullptr, *d_area = nullptr, *d_cleng = nullptr;
+    double *d_R64 = nullptr;
+    float* d_R32 = nullptr;
+    double *d_rhs = nullptr, *d_s = nullptr;
+    int *d_node_ptr = nullptr, *d_node_ent = nullptr;
+    double *d_b = nullptr, *d_du = nullptr, *d_r = nullptr, *d_w = nullptr, *d_y = nullptr;
+    double* d_V = nullptr;
+    int vcap = 0;
+    float *d_V32 = nullptr, *d_w32 = nullptr;
+    int vcap32 = 0;
+    double *d_d64 = nullptr;
+    double *d_part = nullptr, *d_small = nullptr, *d_hcol = nullptr, *d_cflpart = nullptr;
+    double* h_pin = nullptr;   // pinned staging for small host-to-device arrays
+    int *d_counters = nullptr;     // [0] ties, [1] status
+    hipEvent_t ev[8];
+    std::vector<void*> allocs;
+    // matvec timing: event pairs recorded around every full matvec while a
+    // step runs; summed once the step has synchronised
+    bool timing = false;
+    std::vector<hipEvent_t> evpool;
+    std::vector<std::pair<int, int>> mv_marks;   // (event index, precision)
+    int ev_used = 0;
+
+    ~cfdv_ctx() {
+        for (auto& e : evpool) (void)hipEventDestroy(e);
+        for (void* p : allocs) (void)hipFree(p);
+        if (h_pin) (void)hipHostFree(h_pin);
+        if (st) {
+            for (auto& e : ev) (void)hipEventDestroy(e);
+            (void)hipStreamDestroy(st);
+        }
+    }
+    template <typename T>
+    T* alloc(size_t n) {
+        T* p = dmalloc<T>(n);
+        allocs.push_back(p);
+        return p;
+    }
+    void sync() { HIP_CHECK_THROW(hipStreamSynchronize(st)); }
+    void check_launch() { HIP_CHECK_THROW(hipGetLastError()); }
+
+    // ---- EBE matvec over all colors: y = K x (or K^T x) ----
+    hipEvent_t next_event() {
+        if (ev_used == (int)evpool.size()) {
+            hipEvent_t e;
+            HIP_CHECK_THROW(hipEventCreate(&e));
+            evpool.push_back(e);
+        }
+        return evpool[ev_used++];
+    }
+    void matvec(const void* x, void* y, int prec, bool tr) {
+        int mark = -1;
+        if (timing) {
+            mark = ev_used;
+            HIP_CHECK_THROW(hipEventRecord(next_event(), st));
+        }
+        if (prec == 32) vec_fill_f((float*)y, 0.0f, ndof, st);
+        else vec_fill((double*)y, 0.0, ndof, st);
+        for (int c = 0; c < ncolor; c++) {
+            MatvecArgs a{ndim, nnod, neqn, cstart[c], cstart[c + 1], d_conn};
+            launch_matvec_color(a, d_R64, d_R32, x, y, prec, tr, st);
+        }
+        check_launch();
+        if (timing) {
+            HIP_CHECK_THROW(hipEventRecord(next_event(), st));
+            mv_marks.push_back({mark, prec});
+        }
+    }
+    // sum of the recorded matvec intervals (ms) per precision; call after a sync
+    void collect_matvec_times(double* ms64, double* ms32, int* n64, int* n32) {
+        *ms64 = *ms32 = 0;
+        *n64 = *n32 = 0;
+        for (auto& m : mv_marks) {
+            float t = 0;
+            HIP_CHECK_THROW(hipEventElapsedTime(&t, evpool[m.first], evpool[m.first + 1]));
+            if (m.second == 32) { *ms32 += t; (*n32)++; }
+            else { *ms64 += t; (*n64)++; }
+        }
+        mv_marks.clear();
+        ev_used = 0;
+    }
+    // dot products: out[j] = <V_j, w> (device), optionally copied to host
+    template <typename T>
+    void multidot(const T* V, int nv, const T* w, double* out_dev) {
+        vec_multidot<T>(V, ndof, nv, w, ndof, d_part, out_dev, NBLK_RED, st);
+    }
+    double norm(const double* v) {
+        vec_multidot<double>(v, ndof, 1, v, ndof, d_part, d_small, NBLK_RED, st);
+        double h;
+        HIP_CHECK_THROW(hipMemcpyAsync(&h, d_small, sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+        return std::sqrt(h);
+    }
+    void ensure_krylov(int m) {
+        if (vcap >= m + 1) return;
+        if (d_V) {
+            auto it = std::find(allocs.begin(), allocs.end(), (void*)d_V);
+            if (it != allocs.end()) allocs.erase(it);
+            (void)hipFree(d_V);
+        }
+        d_V = alloc<double>((size_t)(m + 1) * ndof);
+        vcap = m + 1;
+    }
+    void ensure_krylov32(int m) {
+        if (vcap32 >= m + 1) return;
+        if (d_V32) {
+            auto it = std::find(allocs.begin(), allocs.end(), (void*)d_V32);
+            if (it != allocs.end()) allocs.erase(it);
+            (void)hipFree(d_V32);
+        }
+        d_V32 = alloc<float>((size_t)(m + 1) * ndof);
+        vcap32 = m + 1;
+    }
+};
+
+namespace {
+
+int fail(cfdv_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+DevThermo make_thermo(const cfdv_thermo* t) {
+    // Thermo::set (src/FDV/Thermo.h:12-33)
+    DevThermo th{};
+    th.gamma = t->gamma;
+    th.gamm1 = (t->gamma - 1.0);
+    th.gm1d2 = ((t->gamma - 1.0) / 2.0);
+    th.gamm3 = (t->gamma - 3.0);
+    th.gm3d2 = ((t->gamma - 3.0) / 2.0);
+    th.Pr = t->prandtl;
+    th.csuth = 110.0 / t->tinf;
+    th.cmach = t->mach;
+    th.Cv = 1. / t->gamma / (t->gamma - 1.0) / (t->mach * t->mach);
+    th.cgas = 1.0 / t->gamma / (t->mach * t->mach);
+    th.creyn = t->reynolds;
+    th.Twall = t->twall;
+    th.adiabatic = t->adiabatic;
+    return th;
+}
+
+void setup(cfdv_ctx* c, const cfdv_mesh* m) {
+    const int nn = c->nnod, nf = c->nface, nq = c->neqn;
+    const long ne = c->n_elem, nnode = c->n_node;
+    for (long i = 0; i < ne * nn; i++)
+        if (m->conn[i] < 0 || m->conn[i] >= nnode) throw std::invalid_argument("connectivity out of range");
+    // node -> element CSR (element order), used by coloring and compose
+    std::vector<int> ptr(nnode + 1, 0), ent((size_t)ne * nn), elem((size_t)ne * nn);
+    for (long i = 0; i < ne * nn; i++) ptr[m->conn[i] + 1]++;
+    for (long i = 0; i < nnode; i++) ptr[i + 1] += ptr[i];
+    {
+        std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+        for (long e = 0; e < ne; e++)
+            for (int l = 0; l < nn; l++) {
+                const int g = m->conn[e * nn + l];
+                elem[pos[g]] = (int)e;
+                ent[pos[g]++] = (int)(e * nn + l);   // original ids, fixed below
+            }
+    }
+    // ---- coloring on the device (Jones-Plassmann == greedy in priority order) ----
+    std::vector<unsigned long long> prio(ne);
+    // priority = element index: the coloring equals sequential first-fit
+    // greedy in element order (8 colors on a structured hex mesh); rounds
+    // grow with the mesh extent (~nx + 2ny + 4nz), a one-time setup cost
+    for (long e = 0; e < ne; e++) prio[e] = (unsigned long long)e;
+    {
+        int* d_conn_o = dmalloc<int>((size_t)ne * nn);
+        int* d_ptr = dmalloc<int>(nnode + 1);
+        int* d_elem = dmalloc<int>((size_t)ne * nn);
+        unsigned long long* d_prio = dmalloc<unsigned long long>(ne);
+        int* d_c0 = dmalloc<int>(ne);
+        int* d_c1 = dmalloc<int>(ne);
+        int* d_chg = dmalloc<int>(1);
+        HIP_CHECK_THROW(hipMemcpy(d_conn_o, m->conn, sizeof(int) * ne * nn, hipMemcpyHostToDevice));
+        HIP_CHECK_THROW(hipMemcpy(d_ptr, ptr.data(), sizeof(int) * (nnode + 1), hipMemcpyHostToDevice));
+        HIP_CHECK_THROW(hipMemcpy(d_elem, elem.data(), sizeof(int) * ne * nn, hipMemcpyHostToDevice));
+        HIP_CHECK_THROW(hipMemcpy(d_prio, prio.data(), sizeof(unsigned long long) * ne, hipMemcpyHostToDevice));
+        std::vector<int> init(ne, -1);
+        HIP_CHECK_THROW(hipMemcpy(d_c0, init.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+        int rounds = 0;
+        for (;;) {
+            int zero = 0, chg = 0;
+            HIP_CHECK_THROW(hipMemcpyAsync(d_chg, &zero, sizeof(int), hipMemcpyHostToDevice, c->st));
+            launch_color_round(d_conn_o, nn, d_ptr, d_elem, ne, d_prio, d_c0, d_c1, d_chg, c->st);
+            HIP_CHECK_THROW(hipGetLastError());
+            HIP_CHECK_THROW(hipMemcpyAsync(&chg, d_chg, sizeof(int), hipMemcpyDeviceToHost, c->st));
+            HIP_CHECK_THROW(hipStreamSynchronize(c->st));
+            std::swap(d_c0, d_c1);
+            if (!chg) break;
+            if (++rounds > 100000) throw std::runtime_error("coloring did not converge");
+        }
+        c->color_orig.resize(ne);
+        HIP_CHECK_THROW(hipMemcpy(c->color_orig.data(), d_c0, sizeof(int) * ne, hipMemcpyDeviceToHost));
+        for (void* p : {(void*)d_conn_o, (void*)d_ptr, (void*)d_elem, (void*)d_prio, (void*)d_c0, (void*)d_c1,
+                        (void*)d_chg})
+            (void)hipFree(p);
+    }
+    int ncol = 0;
+    for (long e = 0; e < ne; e++) ncol = std::max(ncol, c->color_orig[e] + 1);
+    c->ncolor = ncol;
+    // ---- slots: elements sorted by (color, id), each color padded to 64 ----
+    std::vector<long> cnt(ncol, 0);
+    for (long e = 0; e < ne; e++) cnt[c->color_orig[e]]++;
+    c->cstart.assign(ncol + 1, 0);
+    for (int k = 0; k < ncol; k++) c->cstart[k + 1] = c->cstart[k] + (cnt[k] + 63) / 64 * 64;
+    c->n_pad = c->cstart[ncol];
+    c->perm.assign(c->n_pad, -1);
+    c->perm_inv.assign(ne, -1);
+    {
+        std::vector<long> pos(c->cstart.begin(), c->cstart.end() - 1);
+        for (long e = 0; e < ne; e++) {
+            const long s = pos[c->color_orig[e]]++;
+            c->perm[s] = (int)e;
+            c->perm_inv[e] = (int)s;
+        }
+    }
+    // ---- sorted element arrays ----
+    const long np = c->n_pad;
+    std::vector<int> conn_s((size_t)np * nn, -1), bc_s((size_t)np * nf, 0);
+    std::vector<unsigned long long> cmask(np, 0);
+    for (long s = 0; s < np; s++) {
+        const int e = c->perm[s];
+        if (e < 0) continue;
+        for (int l = 0; l < nn; l++) conn_s[s * nn + l] = m->conn[(long)e * nn + l];
+        for (int f = 0; f < nf; f++) bc_s[s * nf + f] = m->bc[(long)e * nf + f];
+        // ApplyBC (ApplyBC.h:14-133): dof (ja, jeqn) of a face with bc != 0 is
+        // constrained when the node is Dirichlet for jeqn, except energy on an
+        // isothermal bc == 1 face (quirk Q5)
+        unsigned long long mk = 0;
+        for (int f = 0; f < nf; f++) {
+            const int bc = m->bc[(long)e * nf + f];
+            if (bc == 0) continue;
+            for (int t = 0; t < c->nbnod; t++) {
+                const int ja = (c->ndim == 3) ? FACE3[f][t] : ((t == 0) ? f : ((f + 1) & 3));
+                const int g = m->conn[(long)e * nn + ja];
+                for (int q = 0; q < nq; q++) {
+                    if (!m->dirichlet[(long)g * nq + q]) continue;
+                    if (bc == 1 && q == nq - 1 && !c->th.adiabatic) continue;
+                    mk |= 1ULL << (nq * ja + q);
+                }
+            }
+        }
+        cmask[s] = mk;
+    }
+    // compose CSR entries point at slots; lists stay in original element order
+    for (size_t i = 0; i < ent.size(); i++) {
+        const int e = ent[i] / nn, l = ent[i] % nn;
+        ent[i] = c->perm_inv[e] * nn + l;
+    }
+    c->d_conn = c->alloc<int>((size_t)np * nn);
+    c->d_bc = c->alloc<int>((size_t)np * nf);
+    c->d_cmask = c->alloc<unsigned long long>(np);
+    c->d_x = c->alloc<double>((size_t)nnode * c->ndim);
+    c->d_U = c->alloc<double>(c->ndof);
+    c->d_U0 = c->alloc<double>(c->ndof);
+    c->d_area = c->alloc<double>(np);
+    c->d_cleng = c->alloc<double>(np);
+    c->d_node_ptr = c->alloc<int>(nnode + 1);
+    c->d_node_ent = c->alloc<int>((size_t)ne * nn);
+    c->d_perm_inv = c->alloc<int>(ne);
+    HIP_CHECK_THROW(hipMemcpy(c->d_conn, conn_s.data(), sizeof(int) * np * nn, hipMemcpyHostToDevice));
+    HIP_CHECK_THROW(hipMemcpy(c->d_bc, bc_s.data(), sizeof(int) * np * nf, hipMemcpyHostToDevice));
+    HIP_CHECK_THROW(hipMemcpy(c->d_cmask, cmask.data(), sizeof(unsigned long long) * np, hipMemcpyHostToDevice));
+    HIP_CHECK_THROW(hipMemcpy(c->d_x, m->x, sizeof(double) * nnode * c->ndim, hipMemcpyHostToDevice));
+    HIP_CHECK_THROW(hipMemcpy(c->d_node_ptr, ptr.data(), sizeof(int) * (nnode + 1), hipMemcpyHostToDevice));
+    HIP_CHECK_THROW(hipMemcpy(c->d_node_ent, ent.data(), sizeof(int) * ne * nn, hipMemcpyHostToDevice));
+    HIP_CHECK_THROW(hipMemcpy(c->d_perm_inv, c->perm_inv.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+    HIP_CHECK_THROW(hipMemsetAsync(c->d_U0, 0, sizeof(double) * c->ndof, c->st));
+    // block storage
+    const size_t kk = (size_t)c->k * c->k;
+    if (c->block_flags & CFDV_BLOCKS_FP64) c->d_R64 = c->alloc<double>((size_t)np * kk);
+    if (c->block_flags & CFDV_BLOCKS_FP32) c->d_R32 = c->alloc<float>((size_t)np * kk);
+    c->d_rhs = c->alloc<double>((size_t)np * c->k);
+    c->d_s = c->alloc<double>((size_t)np * 4);
+    c->d_b = c->alloc<double>(c->ndof);
+    c->d_du = c->alloc<double>(c->ndof);
+    c->d_r = c->alloc<double>(c->ndof);
+    c->d_w = c->alloc<double>(c->ndof);
+    c->d_y = c->alloc<double>(c->ndof);
+    c->d_d64 = c->alloc<double>(c->ndof);
+    c->d_w32 = c->alloc<float>(c->ndof);
+    c->d_part = c->alloc<double>((size_t)NBLK_RED * 16 + 64);
+    c->d_small = c->alloc<double>(1024);
+    c->d_hcol = c->alloc<double>(2048);
+    c->d_cflpart = c->alloc<double>((np + 255) / 256);
+    HIP_CHECK_THROW(hipHostMalloc((void**)&c->h_pin, sizeof(double) * 1024, hipHostMallocDefault));
+    c->d_counters = c->alloc<int>(4);
+    // CalcLength once (main.cpp:120-121)
+    launch_length(c->ndim, c->d_conn, c->d_x, np, c->d_area, c->d_cleng, c->st);
+    c->check_launch();
+    c->sync();
+}
+
+// ---------------------------------------------------------------------------
+// GMRES variants
+// ---------------------------------------------------------------------------
+// MPI_GMRES::iterate at one rank (MPI_GMRES.h:125-220), quirks Q2-Q4.
+int solve_reference(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
+    const int m_ = o->m;
+    const long n = c->ndof;
+    c->ensure_krylov(m_);
+    double* V = c->d_V;
+    std::vector<double> h((size_t)(m_ + 1) * m_), g(m_ + 1), y(m_);
+    auto H = [&](int i, int j) -> double& { return h[i + (size_t)(m_ + 1) * j]; };
+    vec_fill(c->d_du, 0.0, n, c->st);
+    launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
+    const double bnorm = c->norm(c->d_b);
+    const double tol = bnorm * 1.0E-9;
+    s->bnorm = bnorm;
+    for (int gloop = 0; gloop < o->restarts; gloop++) {
+        int mm = m_;
+        std::fill(h.begin(), h.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        std::fill(y.begin(), y.end(), 0.0);
+        c->matvec(c->d_du, c->d_y, 64, false);
+        s->matvecs64++;
+        vec_sub(c->d_r, c->d_b, c->d_y, n, c->st);
+        const double mag_r = c->norm(c->d_r);
+        g[0] = mag_r;
+        if (!(mag_r > 0)) s->breakdown = 1;
+        // v_0 = r / mag_r
+        vec_set(c->d_small + 8, mag_r, c->st);
+        vec_scale_inv<double>(V, c->d_r, c->d_small + 8, n, c->st);
+        for (int j = 0; j < mm; j++) {
+            double* w = c->d_w;
+            c->matvec(V + (size_t)j * n, w, 64, false);
+            s->matvecs64++;
+            s->iters++;
+            // modified Gram-Schmidt, one inner product at a time (:166-174)
+            for (int i = 0; i <= j; i++) {
+                c->multidot<double>(V + (size_t)i * n, 1, w, c->d_hcol + i);
+                vec_axpy(w, V + (size_t)i * n, c->d_hcol + i, -1.0, n, c->st);
+            }
+            c->multidot<double>(w, 1, w, c->d_hcol + j + 1);
+            std::vector<double> col(j + 2);
+            HIP_CHECK_THROW(hipMemcpyAsync(col.data(), c->d_hcol, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, c->st));
+            c->sync();
+            for (int i = 0; i <= j; i++) H(i, j) = col[i];
+            H(j + 1, j) = std::sqrt(col[j + 1]);
+            if (!(H(j + 1, j) > 0)) s->breakdown = 1;
+            vec_set(c->d_small + 9, H(j + 1, j), c->st);
+            vec_scale_inv<double>(V + (size_t)(j + 1) * n, w, c->d_small + 9, n, c->st);
+            // rotate_and_check (:696-730): c, s from the unrotated h (Q3)
+            const int nn = j + 1;
+            std::vector<double> hh((size_t)(nn + 1) * nn), gg(nn + 1, 0.0);
+            gg[0] = g[0];
+            for (int i = 0; i < nn + 1; i++)
+                for (int jj = 0; jj < nn; jj++) hh[i + (size_t)(nn + 1) * jj] = H(i, jj);
+            for (int i = 0; i < nn; i++) {
+                const double mag = std::sqrt(H(i, i) * H(i, i) + H(i + 1, i) * H(i + 1, i));
+                const double cc = H(i, i) / mag, ss = H(i + 1, i) / mag;
+                for (int jj = 0; jj < nn; jj++) {
+                    const double h0 = hh[i + (size_t)(nn + 1) * jj], h1 = hh[i + 1 + (size_t)(nn + 1) * jj];
+                    hh[i + (size_t)(nn + 1) * jj] = cc * h0 + ss * h1;
+                    hh[i + 1 + (size_t)(nn + 1) * jj] = -ss * h0 + cc * h1;
+                }
+                gg[i + 1] = -ss * gg[i];
+                gg[i] = cc * gg[i];
+            }
+            if (std::fabs(gg[nn]) < tol) {
+                mm = j + 1;
+                break;
+            }
+        }
+        // rotate (:733-752)
+        for (int i = 0; i < mm; i++) {
+            const double mag = std::sqrt(H(i, i) * H(i, i) + H(i + 1, i) * H(i + 1, i));
+            const double cc = H(i, i) / mag, ss = H(i + 1, i) / mag;
+            for (int jj = 0; jj < mm; jj++) {
+                const double h0 = H(i, jj), h1 = H(i + 1, jj);
+                H(i, jj) = cc * h0 + ss * h1;
+                H(i + 1, jj) = -ss * h0 + cc * h1;
+            }
+            g[i + 1] = -ss * g[i];
+            g[i] = cc * g[i];
+        }
+        s->est_resid = std::fabs(g[mm]);
+        // solve (:756-780): the back substitution reads h transposed (Q2)
+        for (int j = mm - 1; j >= 0; j--) {
+            y[j] = g[j];
+            for (int i = mm - 1; i >= 0; i--)
+                if (i != j) y[j] -= H(i, j) * y[i];
+            y[j] /= H(j, j);
+        }
+        std::memcpy(c->h_pin, y.data(), sizeof(double) * mm);
+        HIP_CHECK_THROW(hipMemcpyAsync(c->d_small + 16, c->h_pin, sizeof(double) * mm, hipMemcpyHostToDevice, c->st));
+        vec_combine<double>(c->d_du, V, n, mm, c->d_small + 16, n, c->st);
+        s->restarts++;   // Q4: no early exit from the restart loop
+    }
+    // final true residual for reporting
+    c->matvec(c->d_du, c->d_y, 64, false);
+    vec_sub(c->d_r, c->d_b, c->d_y, n, c->st);
+    s->resid = c->norm(c->d_r);
+    s->rel_resid = bnorm > 0 ? s->resid / bnorm : 0.0;
+    c->have_du = true;
+    if (s->breakdown) return CFDV_ERR_BREAKDOWN;
+    if (!std::isfinite(s->resid)) return CFDV_ERR_NAN;
+    return CFDV_OK;
+}
+
+// Restarted GMRES(m) with CGS2 orthogonalisation and Givens rotations.
+// Solves K x = rhs_dev (or K^T x) into x_dev, starting from x_dev = 0.
+// prec: 64 (FP64 blocks/vectors) or 32 (FP32 blocks and basis; dots and
+// Hessenberg in FP64).  Returns Arnoldi iterations; *conv set when the
+// Givens estimate met tol_abs.
+template <typename T>
+int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, int maxrestart, double tol_abs,
+               double rnorm0, cfdv_solve_stats* s, bool* conv, T* V, T* w) {
+    const long n = c->ndof;
+    std::vector<double> h((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m);
+    auto H = [&](int i, int j) -> double& { return h[i + (size_t)(m + 1) * j]; };
+    int iters = 0;
+    *conv = false;
+    double beta = rnorm0;
+    // r0 = rhs (x = 0 on entry); later restarts recompute r = rhs - K x
+    for (int rs = 0; rs < maxrestart; rs++) {
+        if (rs > 0) {
+            // r = rhs - K x, in the working precision of the basis
+            if (prec == 64) {
+                c->matvec(x, c->d_y, 64, tr);
+                s->matvecs64++;
+                vec_sub(c->d_r, (const double*)rhs, c->d_y, n, c->st);
+                beta = c->norm(c->d_r);
+                if (beta <= tol_abs) { *conv = true; break; }
+                vec_set(c->d_small + 8, beta, c->st);
+                vec_scale_inv<double>((double*)V, c->d_r, c->d_small + 8, n, c->st);
+            } else {
+                // FP32 inner: r = rhs - K32 x with x held in FP64 (d_d64)
+                c->matvec(x, c->d_y, 3, tr);
+                s->matvecs32++;
+                // rhs is FP32 here; form r in FP64 then round into V_0
+                vec_fill(c->d_r, 0.0, n, c->st);
+                vec_add_f2d(c->d_r, (const float*)rhs, 1.0, n, c->st);
+                vec_axpy(c->d_r, c->d_y, c->d_small + 10, -1.0, n, c->st);   // d_small[10] == 1
+                beta = c->norm(c->d_r);
+                if (beta <= tol_abs) { *conv = true; break; }
+                vec_d2f_scaled((float*)V, c->d_r, 1.0 / beta, n, c->st);
+            }
+        } else {
+            vec_set(c->d_small + 8, beta, c->st);
+            vec_scale_inv<T>(V, rhs, c->d_small + 8, n, c->st);
+        }
+        std::fill(h.begin(), h.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int jdone = 0;
+        bool brk = false;
+        for (int j = 0; j < m; j++) {
+            c->matvec(V + (size_t)j * n, w, prec, tr);
+            if (prec == 64) s->matvecs64++; else s->matvecs32++;
+            iters++;
+            // CGS2: two classical Gram-Schmidt passes
+            c->multidot<T>(V, j + 1, w, c->d_hcol);
+            vec_multi_axpy<T>(w, V, n, j + 1, c->d_hcol, 1.0, n, c->st);
+            c->multidot<T>(V, j + 1, w, c->d_hcol + 1024);
+            vec_multi_axpy<T>(w, V, n, j + 1, c->d_hcol + 1024, 1.0, n, c->st);
+            c->multidot<T>(w, 1, w, c->d_hcol + 2040);
+            std::vector<double> col(j + 1), col2(j + 1);
+            double nrm2;
+            HIP_CHECK_THROW(hipMemcpyAsync(col.data(), c->d_hcol, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->st));
+            HIP_CHECK_THROW(hipMemcpyAsync(col2.data(), c->d_hcol + 1024, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->st));
+            HIP_CHECK_THROW(hipMemcpyAsync(&nrm2, c->d_hcol + 2040, sizeof(double), hipMemcpyDeviceToHost, c->st));
+            c->sync();
+            for (int i = 0; i <= j; i++) H(i, j) = col[i] + col2[i];
+            const double hn = std::sqrt(nrm2);
+            H(j + 1, j) = hn;
+            if (!std::isfinite(hn)) { s->breakdown = 1; brk = true; }
+            if (hn > 0 && !brk) {
+                vec_set(c->d_small + 9, H(j + 1, j), c->st);
+                vec_scale_inv<T>(V + (size_t)(j + 1) * n, w, c->d_small + 9, n, c->st);
+            } else {
+                brk = true;   // happy breakdown or failure: the space is invariant
+            }
+            for (int i = 0; i < j; i++) {
+                const double h0 = H(i, j), h1 = H(i + 1, j);
+                H(i, j) = cs[i] * h0 + sn[i] * h1;
+                H(i + 1, j) = -sn[i] * h0 + cs[i] * h1;
+            }
+            const double mag = std::sqrt(H(j, j) * H(j, j) + H(j + 1, j) * H(j + 1, j));
+            cs[j] = H(j, j) / mag;
+            sn[j] = H(j + 1, j) / mag;
+            H(j, j) = mag;
+            H(j + 1, j) = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            jdone = j + 1;
+            s->est_resid = std::fabs(g[j + 1]);
+            if (s->est_resid <= tol_abs || brk) break;
+        }
+        for (int j = jdone - 1; j >= 0; j--) {
+            y[j] = g[j];
+            for (int i = j + 1; i < jdone; i++) y[j] -= H(j, i) * y[i];
+            y[j] /= H(j, j);
+        }
+        std::memcpy(c->h_pin, y.data(), sizeof(double) * jdone);
+        HIP_CHECK_THROW(hipMemcpyAsync(c->d_small + 16, c->h_pin, sizeof(double) * jdone, hipMemcpyHostToDevice, c->st));
+        c->sync();
+        vec_combine<T>(x, V, n, jdone, c->d_small + 16, n, c->st);
+        s->restarts++;
+        if (s->est_resid <= tol_abs) { *conv = true; break; }
+        if (brk) break;
+    }
+    return iters;
+}
+
+int solve_fp64(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
+    const long n = c->ndof;
+    if (!c->d_R64) return fail(c, CFDV_ERR_ARG, "FP64 solve needs CFDV_BLOCKS_FP64");
+    c->ensure_krylov(o->m);
+    launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
+    vec_fill(c->d_du, 0.0, n, c->st);
+    const double bnorm = c->norm(c->d_b);
+    s->bnorm = bnorm;
+    bool conv = false;
+    if (bnorm > 0)
+        s->iters += gmres_core<double>(c, 64, o->transpose != 0, c->d_b, c->d_du, o->m, o->restarts, o->rtol * bnorm,
+                                       bnorm, s, &conv, c->d_V, c->d_w);
+    c->matvec(c->d_du, c->d_y, 64, o->transpose != 0);
+    vec_sub(c->d_r, c->d_b, c->d_y, n, c->st);
+    s->resid = c->norm(c->d_r);
+    s->rel_resid = bnorm > 0 ? s->resid / bnorm : 0.0;
+    c->have_du = true;
+    if (!std::isfinite(s->resid)) return fail(c, CFDV_ERR_NAN, "NaN in the FP64 solve");
+    if (s->rel_resid > o->rtol * 10 && bnorm > 0) return fail(c, CFDV_ERR_NOCONV, "FP64 GMRES did not converge");
+    return CFDV_OK;
+}
+
+// FP64 iterative refinement around an FP32 GMRES: r = b - K64 du (FP64),
+// solve K32 d = r/||r|| in FP32, du += ||r|| d (FP64).
+int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
+    const long n = c->ndof;
+    if (!c->d_R32) return fail(c, CFDV_ERR_ARG, "mixed solve needs CFDV_BLOCKS_FP32");
+    const int outer_prec = c->d_R64 ? 64 : 3;   // FP64 residual with FP64 (or widened FP32) blocks
+    const bool tr = o->transpose != 0;
+    c->ensure_krylov32(o->inner_m);
+    launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
+    vec_fill(c->d_du, 0.0, n, c->st);
+    const double one = 1.0;
+    vec_set(c->d_small + 10, one, c->st);
+    const double bnorm = c->norm(c->d_b);
+    s->bnorm = bnorm;
+    vec_copy(c->d_r, c->d_b, n, c->st);
+    double rnorm = bnorm;
+    int sweeps = 0;
+    while (rnorm > o->rtol * bnorm && sweeps < o->restarts) {
+        // inner right-hand side: FP32 copy of r / ||r||
+        vec_d2f_scaled(c->d_w32, c->d_r, 1.0 / rnorm, n, c->st);
+        vec_fill(c->d_d64, 0.0, n, c->st);
+        bool conv = false;
+        cfdv_solve_stats si{};
+        // the FP32 right-hand side stays in d_w32; the last basis slot is the
+        // Arnoldi work vector
+        float* wbuf = c->d_V32 + (size_t)o->inner_m * n;   // last slot as the Arnoldi work vector
+        s->iters += gmres_core<float>(c, 32, tr, c->d_w32, c->d_d64, o->inner_m - 1, o->inner_restarts,
+                                      o->inner_rtol, 1.0, &si, &conv, c->d_V32, wbuf);
+        s->matvecs32 += si.matvecs32;
+        s->matvecs64 += si.matvecs64;
+        // du += ||r|| d  (FP64)
+        vec_set(c->d_small + 11, rnorm, c->st);
+        vec_axpy(c->d_du, c->d_d64, c->d_small + 11, 1.0, n, c->st);
+        // r = b - K du in FP64
+        c->matvec(c->d_du, c->d_y, outer_prec, tr);
+        if (outer_prec == 64) s->matvecs64++; else s->matvecs32++;
+        vec_sub(c->d_r, c->d_b, c->d_y, n, c->st);
+        rnorm = c->norm(c->d_r);
+        sweeps++;
+        if (!std::isfinite(rnorm)) break;
+    }
+    s->restarts = sweeps;
+    s->resid = rnorm;
+    s->rel_resid = bnorm > 0 ? rnorm / bnorm : 0.0;
+    c->have_du = true;
+    if (!std::isfinite(rnorm)) return fail(c, CFDV_ERR_NAN, "NaN in the mixed solve");
+    if (s->rel_resid > o->rtol && bnorm > 0) return fail(c, CFDV_ERR_NOCONV, "mixed refinement did not converge");
+    return CFDV_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+#define GUARD_BEGIN try {
+#define GUARD_END                                                    \
+    }                                                                \
+    catch (const std::exception& ex) {                               \
+        return fail(ctx, CFDV_ERR_HIP, ex.what());                   \
+    }
+
+extern "C" {
+
+int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flags, cfdv_ctx** out) {
+    if (!out || !mesh || !thermo) return CFDV_ERR_ARG;
+    *out = nullptr;
+    if (!((mesh->ndim == 3 && mesh->nnod == 8 && mesh->neqn == 5) ||
+          (mesh->ndim == 2 && mesh->nnod == 4 && mesh->neqn == 4)))
+        return CFDV_ERR_ARG;
+    if (mesh->n_elem <= 0 || mesh->n_node <= 0 || !mesh->conn || !mesh->x || !mesh->bc || !mesh->dirichlet)
+        return CFDV_ERR_ARG;
+    if (!(block_flags & (CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32))) return CFDV_ERR_ARG;
+    cfdv_ctx* ctx = new (std::nothrow) cfdv_ctx();
+    if (!ctx) return CFDV_ERR_HIP;
+    try {
+        ctx->ndim = mesh->ndim;
+        ctx->nnod = mesh->nnod;
+        ctx->neqn = mesh->neqn;
+        ctx->nface = (mesh->ndim == 3) ? 6 : 4;
+        ctx->nbnod = (mesh->ndim == 3) ? 4 : 2;
+        ctx->k = ctx->nnod * ctx->neqn;
+        ctx->n_elem = mesh->n_elem;
+        ctx->n_node = mesh->n_node;
+        ctx->ndof = mesh->n_node * mesh->neqn;
+        ctx->block_flags = block_flags;
+        ctx->th = make_thermo(thermo);
+        HIP_CHECK_THROW(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
+        for (auto& e : ctx->ev) HIP_CHECK_THROW(hipEventCreate(&e));
+        upload_gauss_tables(make_tables());
+        setup(ctx, mesh);
+    } catch (const std::invalid_argument& ex) {
+        delete ctx;
+        return CFDV_ERR_ARG;
+    } catch (const std::exception& ex) {
+        std::fprintf(stderr, "cfdv_create: %s\n", ex.what());
+        delete ctx;
+        return CFDV_ERR_HIP;
+    }
+    *out = ctx;
+    return CFDV_OK;
+}
+
+void cfdv_destroy(cfdv_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->st);
+    delete ctx;
+}
+
+const char* cfdv_last_error(const cfdv_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cfdv_mesh_info(const cfdv_ctx* ctx, int64_t* n_elem, int64_t* n_node, int* ncolors, int64_t* n_slots) {
+    if (!ctx) return CFDV_ERR_ARG;
+    if (n_elem) *n_elem = ctx->n_elem;
+    if (n_node) *n_node = ctx->n_node;
+    if (ncolors) *ncolors = ctx->ncolor;
+    if (n_slots) *n_slots = ctx->n_pad;
+    return CFDV_OK;
+}
+
+int cfdv_set_state(cfdv_ctx* ctx, const double* U) {
+    if (!ctx || !U) return CFDV_ERR_ARG;
+    GUARD_BEGIN
+    HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_U, U, sizeof(double) * ctx->ndof, hipMemcpyHostToDevice, ctx->st));
+    ctx->sync();
+    ctx->built = false;
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_get_state(cfdv_ctx* ctx, double* U, double* U0) {
+    if (!ctx) return CFDV_ERR_ARG;
+    GUARD_BEGIN
+    if (U) HIP_CHECK_THROW(hipMemcpyAsync(U, ctx->d_U, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    if (U0) HIP_CHECK_THROW(hipMemcpyAsync(U0, ctx->d_U0, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_cfl(cfdv_ctx* ctx, double cfl, double* dt) {
+    if (!ctx || !dt) return CFDV_ERR_ARG;
+    GUARD_BEGIN
+    const int nb = (int)((ctx->n_pad + 255) / 256);
+    launch_cfl(ctx->ndim, ctx->th, ctx->d_conn, ctx->d_U, ctx->d_cleng, ctx->n_pad, ctx->d_cflpart, nb, ctx->st);
+    min_reduce(ctx->d_cflpart, nb, ctx->d_small, ctx->st);
+    ctx->check_launch();
+    double mn;
+    HIP_CHECK_THROW(hipMemcpyAsync(&mn, ctx->d_small, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    if (!(mn < 1000)) return fail(ctx, CFDV_ERR_CFL, "no element set dt (CFL)");
+    *dt = cfl * mn;
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_build(cfdv_ctx* ctx, double dt) {
+    if (!ctx) return CFDV_ERR_ARG;
+    GUARD_BEGIN
+    HIP_CHECK_THROW(hipMemsetAsync(ctx->d_counters, 0, sizeof(int) * 4, ctx->st));
+    BuildArgs a{};
+    a.th = ctx->th;
+    a.ndim = ctx->ndim;
+    a.n_elem_pad = ctx->n_pad;
+    a.conn = ctx->d_conn;
+    a.bc = ctx->d_bc;
+    a.cmask = ctx->d_cmask;
+    a.x = ctx->d_x;
+    a.U = ctx->d_U;
+    a.dt = dt;
+    a.R64 = ctx->d_R64;
+    a.R32 = ctx->d_R32;
+    a.rhs = ctx->d_rhs;
+    a.svals = ctx->d_s;
+    a.ties = ctx->d_counters;
+    a.status = ctx->d_counters + 1;
+    launch_build(a, ctx->d_R64 != nullptr, ctx->d_R32 != nullptr, ctx->st);
+    ctx->check_launch();
+    int cnt[2];
+    HIP_CHECK_THROW(hipMemcpyAsync(cnt, ctx->d_counters, sizeof(int) * 2, hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    ctx->built = true;
+    if (cnt[1] & 1) return fail(ctx, CFDV_ERR_BC, "unknown boundary-condition code");
+    if (cnt[1] & 2) return fail(ctx, CFDV_ERR_GEOM, "face surface Jacobian below 1e-12");
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_get_blocks(cfdv_ctx* ctx, double* R, double* rhs, double* s) {
+    if (!ctx || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_get_blocks before cfdv_build");
+    GUARD_BEGIN
+    const int k = ctx->k;
+    const size_t kk = (size_t)k * k;
+    if (R) {
+        if (!ctx->d_R64) return fail(ctx, CFDV_ERR_ARG, "no FP64 blocks");
+        std::vector<double> h((size_t)ctx->n_pad * kk);
+        HIP_CHECK_THROW(hipMemcpyAsync(h.data(), ctx->d_R64, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->st));
+        ctx->sync();
+        for (long e = 0; e < ctx->n_elem; e++) {
+            const long sl = ctx->perm_inv[e];
+            for (int i = 0; i < k; i++)
+                for (int j = 0; j < k; j++)
+                    R[(size_t)e * kk + i + (size_t)k * j] = h[blk_offset<double>(sl, i * k + j, (int)kk)];
+        }
+    }
+    if (rhs || s) {
+        std::vector<double> hr((size_t)ctx->n_pad * k), hs((size_t)ctx->n_pad * 4);
+        HIP_CHECK_THROW(hipMemcpyAsync(hr.data(), ctx->d_rhs, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, ctx->st));
+        HIP_CHECK_THROW(hipMemcpyAsync(hs.data(), ctx->d_s, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->st));
+        ctx->sync();
+        for (long e = 0; e < ctx->n_elem; e++) {
+            const long sl = ctx->perm_inv[e];
+            if (rhs) std::memcpy(rhs + (size_t)e * k, hr.data() + (size_t)sl * k, sizeof(double) * k);
+            if (s) std::memcpy(s + (size_t)e * 4, hs.data() + (size_t)sl * 4, sizeof(double) * 4);
+        }
+    }
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_get_blocks32(cfdv_ctx* ctx, float* R) {
+    if (!ctx || !ctx->built || !R || !ctx->d_R32) return fail(ctx, CFDV_ERR_ARG, "no FP32 blocks built");
+    GUARD_BEGIN
+    const int k = ctx->k;
+    const size_t kk = (size_t)k * k;
+    std::vector<float> h((size_t)ctx->n_pad * kk);
+    HIP_CHECK_THROW(hipMemcpyAsync(h.data(), ctx->d_R32, sizeof(float) * h.size(), hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    for (long e = 0; e < ctx->n_elem; e++) {
+        const long sl = ctx->perm_inv[e];
+        for (int i = 0; i < k; i++)
+            for (int j = 0; j < k; j++)
+                R[(size_t)e * kk + i + (size_t)k * j] = h[blk_offset<float>(sl, i * k + j, (int)kk)];
+    }
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_get_rhs_vector(cfdv_ctx* ctx, double* b) {
+    if (!ctx || !b || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_get_rhs_vector before cfdv_build");
+    GUARD_BEGIN
+    launch_compose(ctx->d_node_ptr, ctx->d_node_ent, ctx->d_rhs, ctx->k, ctx->neqn, ctx->n_node, ctx->d_b, ctx->st);
+    ctx->check_launch();
+    HIP_CHECK_THROW(hipMemcpyAsync(b, ctx->d_b, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_solve(cfdv_ctx* ctx, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
+    if (!ctx || !o || !s) return CFDV_ERR_ARG;
+    if (!ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_solve before cfdv_build");
+    if (o->m < 1 || o->m > 500 || o->restarts < 1) return fail(ctx, CFDV_ERR_ARG, "bad Krylov dimension/restarts");
+    std::memset(s, 0, sizeof(*s));
+    GUARD_BEGIN
+    switch (o->mode) {
+    case CFDV_SOLVE_REFERENCE:
+        if (!ctx->d_R64) return fail(ctx, CFDV_ERR_ARG, "reference solve needs FP64 blocks");
+        if (o->transpose) return fail(ctx, CFDV_ERR_ARG, "reference solve has no transpose");
+        return solve_reference(ctx, o, s);
+    case CFDV_SOLVE_FP64:
+        return solve_fp64(ctx, o, s);
+    case CFDV_SOLVE_MIXED:
+        if (o->inner_m < 2 || o->inner_m > 500 || o->inner_restarts < 1)
+            return fail(ctx, CFDV_ERR_ARG, "bad inner Krylov parameters");
+        return solve_mixed(ctx, o, s);
+    default:
+        return fail(ctx, CFDV_ERR_ARG, "unknown solve mode");
+    }
+    GUARD_END
+}
+
+int cfdv_get_du(cfdv_ctx* ctx, double* du) {
+    if (!ctx || !du || !ctx->have_du) return fail(ctx, CFDV_ERR_ARG, "no solution");
+    GUARD_BEGIN
+    HIP_CHECK_THROW(hipMemcpyAsync(du, ctx->d_du, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_update(cfdv_ctx* ctx) {
+    if (!ctx || !ctx->have_du) return fail(ctx, CFDV_ERR_ARG, "cfdv_update before cfdv_solve");
+    GUARD_BEGIN
+    vec_update_state(ctx->d_U, ctx->d_U0, ctx->d_du, ctx->ndof, ctx->st);
+    ctx->check_launch();
+    ctx->sync();
+    ctx->built = false;
+    ctx->have_du = false;
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_rms(cfdv_ctx* ctx, double* urms) {
+    if (!ctx || !urms) return CFDV_ERR_ARG;
+    GUARD_BEGIN
+    double su[5], su0[5];
+    vec_sumsq_eqn(ctx->d_U, ctx->neqn, ctx->n_node, ctx->d_part, ctx->d_small, NBLK_RED, ctx->st);
+    vec_sumsq_eqn(ctx->d_U0, ctx->neqn, ctx->n_node, ctx->d_part + NBLK_RED * 5, ctx->d_small + 5, NBLK_RED, ctx->st);
+    ctx->check_launch();
+    HIP_CHECK_THROW(hipMemcpyAsync(su, ctx->d_small, sizeof(double) * ctx->neqn, hipMemcpyDeviceToHost, ctx->st));
+    HIP_CHECK_THROW(hipMemcpyAsync(su0, ctx->d_small + 5, sizeof(double) * ctx->neqn, hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    for (int i = 0; i < ctx->neqn; i++) urms[i] = (su0[i] == 0) ? 1 : std::sqrt(su[i] / su0[i]) - 1.0;
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_refine_flags(cfdv_ctx* ctx, uint8_t* flags, double* s1, int32_t* nflagged, int32_t* nties) {
+    if (!ctx || !flags) return CFDV_ERR_ARG;
+    GUARD_BEGIN
+    uint8_t* d_f = ctx->alloc<uint8_t>(ctx->n_elem);
+    double* d_s1 = s1 ? ctx->alloc<double>(ctx->n_elem) : nullptr;
+    HIP_CHECK_THROW(hipMemsetAsync(ctx->d_counters, 0, sizeof(int) * 4, ctx->st));
+    launch_flags(ctx->ndim, ctx->th, ctx->d_conn, ctx->d_U, ctx->d_perm_inv, ctx->n_elem, d_f, d_s1, ctx->d_counters,
+                 ctx->st);
+    ctx->check_launch();
+    int ties;
+    HIP_CHECK_THROW(hipMemcpyAsync(flags, d_f, ctx->n_elem, hipMemcpyDeviceToHost, ctx->st));
+    if (s1) HIP_CHECK_THROW(hipMemcpyAsync(s1, d_s1, sizeof(double) * ctx->n_elem, hipMemcpyDeviceToHost, ctx->st));
+    HIP_CHECK_THROW(hipMemcpyAsync(&ties, ctx->d_counters, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    if (d_s1) { ctx->allocs.pop_back(); (void)hipFree(d_s1); }
+    ctx->allocs.pop_back();
+    (void)hipFree(d_f);
+    if (nflagged) {
+        long cnt = 0;
+        for (long e = 0; e < ctx->n_elem; e++) cnt += flags[e];
+        *nflagged = (int32_t)cnt;
+    }
+    if (nties) *nties = ties;
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_color(cfdv_ctx* ctx, int32_t* color, int32_t* ncolors) {
+    if (!ctx) return CFDV_ERR_ARG;
+    if (color) std::memcpy(color, ctx->color_orig.data(), sizeof(int32_t) * ctx->n_elem);
+    if (ncolors) *ncolors = ctx->ncolor;
+    return CFDV_OK;
+}
+
+int cfdv_matvec(cfdv_ctx* ctx, const double* x, double* y, int precision, int transpose) {
+    if (!ctx || !x || !y || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_matvec before cfdv_build");
+    if (precision == 64 && !ctx->d_R64) return fail(ctx, CFDV_ERR_ARG, "no FP64 blocks");
+    if ((precision == 32 || precision == 3) && !ctx->d_R32) return fail(ctx, CFDV_ERR_ARG, "no FP32 blocks");
+    if (precision != 64 && precision != 32 && precision != 3) return fail(ctx, CFDV_ERR_ARG, "bad precision");
+    GUARD_BEGIN
+    const long n = ctx->ndof;
+    HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_r, x, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    if (precision == 32) {
+        vec_d2f_scaled(ctx->d_w32, ctx->d_r, 1.0, n, ctx->st);
+        float* yf = ctx->alloc<float>(n);
+        ctx->matvec(ctx->d_w32, yf, 32, transpose != 0);
+        vec_fill(ctx->d_y, 0.0, n, ctx->st);
+        vec_add_f2d(ctx->d_y, yf, 1.0, n, ctx->st);
+        ctx->sync();
+        ctx->allocs.pop_back();
+        (void)hipFree(yf);
+    } else {
+        ctx->matvec(ctx->d_r, ctx->d_y, precision, transpose != 0);
+    }
+    HIP_CHECK_THROW(hipMemcpyAsync(y, ctx->d_y, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_profile_matvec(cfdv_ctx* ctx, int precision, int reps, double* ms) {
+    if (!ctx || !ms || reps < 1 || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "profile before build");
+    if (precision == 64 && !ctx->d_R64) return fail(ctx, CFDV_ERR_ARG, "no FP64 blocks");
+    if (precision == 32 && !ctx->d_R32) return fail(ctx, CFDV_ERR_ARG, "no FP32 blocks");
+    GUARD_BEGIN
+    void* x = precision == 32 ? (void*)ctx->d_w32 : (void*)ctx->d_r;
+    void* y = precision == 32 ? (void*)(ctx->d_V32 ? ctx->d_V32 : nullptr) : (void*)ctx->d_y;
+    float* ytmp = nullptr;
+    if (precision == 32) {
+        vec_fill_f(ctx->d_w32, 1.0f, ctx->ndof, ctx->st);
+        if (!y) { ytmp = ctx->alloc<float>(ctx->ndof); y = ytmp; }
+    } else {
+        vec_fill(ctx->d_r, 1.0, ctx->ndof, ctx->st);
+    }
+    ctx->matvec(x, y, precision, false);   // warm
+    HIP_CHECK_THROW(hipEventRecord(ctx->ev[0], ctx->st));
+    for (int i = 0; i < reps; i++) ctx->matvec(x, y, precision, false);
+    HIP_CHECK_THROW(hipEventRecord(ctx->ev[1], ctx->st));
+    HIP_CHECK_THROW(hipEventSynchronize(ctx->ev[1]));
+    float t = 0;
+    HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
+    *ms = t / reps;
+    if (ytmp) { ctx->allocs.pop_back(); (void)hipFree(ytmp); }
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_step(cfdv_ctx* ctx, double cfl, const cfdv_solve_opts* o, cfdv_step_stats* st) {
+    if (!ctx || !o || !st) return CFDV_ERR_ARG;
+    std::memset(st, 0, sizeof(*st));
+    GUARD_BEGIN
+    ctx->timing = true;
+    ctx->mv_marks.clear();
+    ctx->ev_used = 0;
+    HIP_CHECK_THROW(hipEventRecord(ctx->ev[0], ctx->st));
+    int rc = cfdv_cfl(ctx, cfl, &st->dt);
+    if (rc) return rc;
+    HIP_CHECK_THROW(hipEventRecord(ctx->ev[1], ctx->st));
+    rc = cfdv_build(ctx, st->dt);
+    if (rc) return rc;
+    HIP_CHECK_THROW(hipEventRecord(ctx->ev[2], ctx->st));
+    rc = cfdv_solve(ctx, o, &st->solve);
+    ctx->timing = false;
+    if (rc) return rc;
+    HIP_CHECK_THROW(hipEventRecord(ctx->ev[3], ctx->st));
+    rc = cfdv_update(ctx);
+    if (rc) return rc;
+    rc = cfdv_rms(ctx, st->urms);
+    if (rc) return rc;
+    HIP_CHECK_THROW(hipEventRecord(ctx->ev[4], ctx->st));
+    HIP_CHECK_THROW(hipEventSynchronize(ctx->ev[4]));
+    float t;
+    HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
+    st->ms_cfl = t;
+    HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[1], ctx->ev[2]));
+    st->ms_build = t;
+    HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
+    st->ms_solve = t;
+    HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[3], ctx->ev[4]));
+    st->ms_update = t;
+    ctx->collect_matvec_times(&st->ms_matvec64, &st->ms_matvec32, &st->n_matvec64, &st->n_matvec32);
+    return CFDV_OK;
+    GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Internal declarations shared by the HIP translation units of libcfdv_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "fdv_device.h"
+
+#define HIP_CHECK_THROW(x)                                                               \
+    do {                                                                                 \
+        hipError_t err_ = (x);                                                           \
+        if (err_ != hipSuccess)                                                          \
+            throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(err_)); \
+    } while (0)
+
+namespace cfdv {
+
+struct BuildArgs {
+    DevThermo th;
+    int ndim;
+    long n_elem_pad;          // color-sorted, padded element count
+    const int* conn;          // [n_elem_pad][nnod], -1 in padding slots
+    const int* bc;            // [n_elem_pad][nface]
+    const unsigned long long* cmask;  // [n_elem_pad] constrained (Dirichlet) dofs
+    const double* x;          // [n_node][ndim]
+    const double* U;          // [n_node][neqn]
+    double dt;
+    double* R64;              // chunked SoA blocks or nullptr
+    float* R32;
+    double* rhs;              // [n_elem_pad][k]
+    double* svals;            // [n_elem_pad][4] or nullptr
+    int* ties;                // count of FDVParam r == 0.01 ties
+    int* status;              // bit 0: unknown bc code, bit 1: bad face detjac
+};
+
+void upload_gauss_tables(const GaussTables& g);
+void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st);
+void launch_flags(int ndim, const DevThermo& th, const int* conn, const double* U, const int* perm_inv,
+                  long n_elem, uint8_t* flags, double* s1, int* ties, hipStream_t st);
+void launch_cfl(int ndim, const DevThermo& th, const int* conn, const double* U, const double* cleng,
+                long n_elem_pad, double* part, int nblocks, hipStream_t st);
+void launch_length(int ndim, const int* conn, const double* x, long n_elem_pad, double* area, double* cleng,
+                   hipStream_t st);
+
+// ---- solve kernels (solve.hip) ----
+struct MatvecArgs {
+    int ndim, nnod, neqn;
+    long e_begin, e_end;      // color range (multiple of 64)
+    const int* conn;          // [n_elem_pad][nnod]
+};
+// y += sum over elements of one color of R_e x_e (or R_e^T x_e)
+void launch_matvec_color(const MatvecArgs& a, const double* R64, const float* R32, const void* x, void* y,
+                         int prec, bool transpose, hipStream_t st);
+void launch_compose(const int* node_ptr, const int* node_ent, const double* rhs, int k, int neqn, long n_node,
+                    double* b, hipStream_t st);
+
+// vector kernels; all reductions write per-block partials in a fixed order
+// and finish with a single-block reduction, so results are deterministic.
+void vec_fill(double* y, double v, long n, hipStream_t st);
+void vec_set(double* p, double v, hipStream_t st);   // *p = v, stream-ordered
+void vec_fill_f(float* y, float v, long n, hipStream_t st);
+void vec_copy(double* y, const double* x, long n, hipStream_t st);
+void vec_axpy(double* y, const double* x, const double* alpha_dev, double scale, long n, hipStream_t st);
+void vec_sub(double* r, const double* b, const double* y, long n, hipStream_t st);  // r = b - y
+void vec_scale_to(double* y, const double* x, const double* denom_dev, long n, hipStream_t st);  // y = x / *d
+void vec_d2f_scaled(float* y, const double* x, double s, long n, hipStream_t st);
+void vec_add_f2d(double* y, const float* x, double s, long n, hipStream_t st);
+void vec_update_state(double* U, double* U0, const double* du, long n, hipStream_t st);
+// dots: out[j] = <V_j, w> for j < nv (V rows of length n, stride ldv)
+template <typename T>
+void vec_multidot(const T* V, long ldv, int nv, const T* w, long n, double* part, double* out, int nblk,
+                  hipStream_t st);
+template <typename T>
+void vec_multi_axpy(T* w, const T* V, long ldv, int nv, const double* h, double sign, long n, hipStream_t st);
+template <typename T>
+void vec_scale_inv(T* y, const T* x, const double* denom_dev, long n, hipStream_t st);
+template <typename T>
+void vec_combine(double* x, const T* V, long ldv, int nv, const double* y, long n, hipStream_t st);
+void vec_sumsq_eqn(const double* U, int neqn, long n_node, double* part, double* out, int nblk, hipStream_t st);
+void min_reduce(const double* part, int n, double* out, hipStream_t st);
+
+// ---- coloring (color.hip) ----
+void launch_color_round(const int* conn, int nnod, const int* node_ptr, const int* node_elem, long n_elem,
+                        const unsigned long long* prio, const int* color_in, int* color_out, int* changed,
+                        hipStream_t st);
+
+}  // namespace cfdv

@@ -1,0 +1,424 @@
+// EBE matvec and Krylov vector kernels (gfx950).
+//
+// The EBE matvec restates MPI_GMRES::sum_Av / sum_Ax at one rank
+// (src/Utility/Solvers/MPI_GMRES.h:442-693): y[gx*neqn+xx] +=
+// R(x*neqn+xx, y*neqn+yy) v[gy*neqn+yy].  Elements are sorted by color; a
+// launch covers one color, in which no two elements share a node, so the
+// scatter into y needs no atomics and the result is deterministic.
+#include "cfdv_internal.h"
+
+namespace cfdv {
+
+// ---------------------------------------------------------------------------
+// EBE matvec, one thread per element.  Each lane gathers its element's k
+// inputs, streams the k*k block as 16-byte vectors (blk_offset layout: one
+// 1 KB contiguous read per wavefront per vector group), accumulates all k
+// outputs in registers and scatters them once.  The loop is fully unrolled
+// so every register index is static and loads can be issued far ahead of
+// their use.
+// ---------------------------------------------------------------------------
+typedef double v2d_t __attribute__((ext_vector_type(2)));
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { typedef v2d_t type; };
+template <> struct Vec16<float> { typedef v4f_t type; };
+
+template <typename TB, typename TV, typename TA, int NNOD, int NEQ, bool TR>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) void matvec_kernel(long e0, long e1, const int* __restrict__ conn,
+                                                     const TB* __restrict__ R, const TV* __restrict__ x,
+                                                     TV* __restrict__ y) {
+    constexpr int K = NNOD * NEQ, KK = K * K, V = 16 / (int)sizeof(TB), GPR = K / V;   // vector groups per row
+    typedef typename Vec16<TB>::type vec_t;
+    const long e = e0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    const int* ce = conn + e * NNOD;
+    if (ce[0] < 0) return;
+    const vec_t* Rb = reinterpret_cast<const vec_t*>(R + (size_t)(e >> 6) * 64 * KK) + (e & 63);
+    if (!TR) {
+        // y_block(n) = sum_j R(5n+r, j) x_j : all k inputs in registers, one
+        // node's NEQ rows (NEQ*GPR independent 16-byte loads) per iteration
+        TA xv[K];
+#pragma unroll
+        for (int m = 0; m < NNOD; m++) {
+            const int gm = ce[m];
+#pragma unroll
+            for (int q = 0; q < NEQ; q++) xv[m * NEQ + q] = (TA)x[(size_t)gm * NEQ + q];
+        }
+        for (int n = 0; n < NNOD; n++) {
+            vec_t rv[NEQ][GPR];
+#pragma unroll
+            for (int r = 0; r < NEQ; r++)
+#pragma unroll
+                for (int c = 0; c < GPR; c++)
+                    rv[r][c] = __builtin_nontemporal_load(Rb + (size_t)((n * NEQ + r) * GPR + c) * 64);
+            const int gn = ce[n];
+#pragma unroll
+            for (int r = 0; r < NEQ; r++) {
+                TA acc = 0;
+#pragma unroll
+                for (int c = 0; c < GPR; c++)
+#pragma unroll
+                    for (int v = 0; v < V; v++) acc += (TA)rv[r][c][v] * xv[c * V + v];
+                TV* yp = y + (size_t)gn * NEQ + r;
+                *yp = (TV)((TA)*yp + acc);
+            }
+        }
+    } else {
+        // y_j += sum_i R(i, j) x_i : all k outputs in registers
+        TA yv[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) yv[j] = 0;
+        for (int n = 0; n < NNOD; n++) {
+            vec_t rv[NEQ][GPR];
+#pragma unroll
+            for (int r = 0; r < NEQ; r++)
+#pragma unroll
+                for (int c = 0; c < GPR; c++)
+                    rv[r][c] = __builtin_nontemporal_load(Rb + (size_t)((n * NEQ + r) * GPR + c) * 64);
+            const int gn = ce[n];
+            TA xi[NEQ];
+#pragma unroll
+            for (int r = 0; r < NEQ; r++) xi[r] = (TA)x[(size_t)gn * NEQ + r];
+#pragma unroll
+            for (int r = 0; r < NEQ; r++)
+#pragma unroll
+                for (int c = 0; c < GPR; c++)
+#pragma unroll
+                    for (int v = 0; v < V; v++) yv[c * V + v] += (TA)rv[r][c][v] * xi[r];
+        }
+#pragma unroll
+        for (int m = 0; m < NNOD; m++) {
+            const int gm = ce[m];
+#pragma unroll
+            for (int q = 0; q < NEQ; q++) {
+                TV* yp = y + (size_t)gm * NEQ + q;
+                *yp = (TV)((TA)*yp + yv[m * NEQ + q]);
+            }
+        }
+    }
+}
+
+void launch_matvec_color(const MatvecArgs& a, const double* R64, const float* R32, const void* x, void* y,
+                         int prec, bool tr, hipStream_t st) {
+    const long n = a.e_end - a.e_begin;
+    if (n <= 0) return;
+    const int bs = 128;
+    const int nb = (int)((n + bs - 1) / bs);
+#define L(TB, TV, TA, NN, NQ, T)                                                                      \
+    hipLaunchKernelGGL((matvec_kernel<TB, TV, TA, NN, NQ, T>), dim3(nb), dim3(bs), 0, st, a.e_begin, \
+                       a.e_end, a.conn, (const TB*)(prec == 64 ? (const void*)R64 : (const void*)R32),  \
+                       (const TV*)x, (TV*)y)
+    if (a.ndim == 3) {
+        if (prec == 64) { if (tr) L(double, double, double, 8, 5, true); else L(double, double, double, 8, 5, false); }
+        else if (prec == 32) { if (tr) L(float, float, float, 8, 5, true); else L(float, float, float, 8, 5, false); }
+        else { if (tr) L(float, double, double, 8, 5, true); else L(float, double, double, 8, 5, false); }
+    } else {
+        if (prec == 64) { if (tr) L(double, double, double, 4, 4, true); else L(double, double, double, 4, 4, false); }
+        else if (prec == 32) { if (tr) L(float, float, float, 4, 4, true); else L(float, float, float, 4, 4, false); }
+        else { if (tr) L(float, double, double, 4, 4, true); else L(float, double, double, 4, 4, false); }
+    }
+#undef L
+}
+
+// ---------------------------------------------------------------------------
+// compose_rhs (MPI_GMRES.h:369-389) as a node-centric gather: for each node
+// the contributing (element, local node) pairs are listed in element order,
+// so b is summed in the same order as the reference's element loop.
+// ---------------------------------------------------------------------------
+__global__ void compose_kernel(const int* __restrict__ node_ptr, const int* __restrict__ node_ent,
+                               const double* __restrict__ rhs, int k, int neqn, long n_node,
+                               double* __restrict__ b) {
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_node) return;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int p = node_ptr[n]; p < node_ptr[n + 1]; p++) {
+        const long ent = node_ent[p];           // e * nnod + local
+        const int nnod = k / neqn;
+        const long e = ent / nnod;
+        const int l = (int)(ent % nnod);
+        for (int q = 0; q < neqn; q++) acc[q] += rhs[e * k + l * neqn + q];
+    }
+    for (int q = 0; q < neqn; q++) b[n * neqn + q] = acc[q];
+}
+
+void launch_compose(const int* node_ptr, const int* node_ent, const double* rhs, int k, int neqn, long n_node,
+                    double* b, hipStream_t st) {
+    const int nb = (int)((n_node + 255) / 256);
+    hipLaunchKernelGGL(compose_kernel, dim3(nb), dim3(256), 0, st, node_ptr, node_ent, rhs, k, neqn, n_node, b);
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------
+static inline int grid_for(long n, int bs = 256, int cap = 8192) {
+    long nb = (n + bs - 1) / bs;
+    if (nb > cap) nb = cap;
+    if (nb < 1) nb = 1;
+    return (int)nb;
+}
+
+__global__ void fill_kernel(double* y, double v, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = v;
+}
+__global__ void fill_f_kernel(float* y, float v, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = v;
+}
+__global__ void copy_kernel(double* y, const double* x, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = x[i];
+}
+__global__ void axpy_kernel(double* y, const double* x, const double* alpha, double scale, long n) {
+    const double a = *alpha * scale;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] += a * x[i];
+}
+__global__ void sub_kernel(double* r, const double* b, const double* y, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        r[i] = b[i] - y[i];
+}
+__global__ void scale_to_kernel(double* y, const double* x, const double* d, long n) {
+    const double dd = *d;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = x[i] / dd;
+}
+__global__ void d2f_kernel(float* y, const double* x, double s, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = (float)(x[i] * s);
+}
+__global__ void add_f2d_kernel(double* y, const float* x, double s, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] += s * (double)x[i];
+}
+__global__ void update_kernel(double* U, double* U0, const double* du, long n) {
+    // MPI_GMRES::update at one rank (MPI_GMRES.h:246-253)
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double u = U[i];
+        U0[i] = u;
+        U[i] = u + du[i];
+    }
+}
+
+__global__ void set_kernel(double* p, double v) { *p = v; }
+void vec_set(double* p, double v, hipStream_t st) { hipLaunchKernelGGL(set_kernel, dim3(1), dim3(1), 0, st, p, v); }
+void vec_fill(double* y, double v, long n, hipStream_t st) {
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, v, n);
+}
+void vec_fill_f(float* y, float v, long n, hipStream_t st) {
+    hipLaunchKernelGGL(fill_f_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, v, n);
+}
+void vec_copy(double* y, const double* x, long n, hipStream_t st) {
+    hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, x, n);
+}
+void vec_axpy(double* y, const double* x, const double* alpha, double scale, long n, hipStream_t st) {
+    hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, x, alpha, scale, n);
+}
+void vec_sub(double* r, const double* b, const double* y, long n, hipStream_t st) {
+    hipLaunchKernelGGL(sub_kernel, dim3(grid_for(n)), dim3(256), 0, st, r, b, y, n);
+}
+void vec_scale_to(double* y, const double* x, const double* d, long n, hipStream_t st) {
+    hipLaunchKernelGGL(scale_to_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, x, d, n);
+}
+void vec_d2f_scaled(float* y, const double* x, double s, long n, hipStream_t st) {
+    hipLaunchKernelGGL(d2f_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, x, s, n);
+}
+void vec_add_f2d(double* y, const float* x, double s, long n, hipStream_t st) {
+    hipLaunchKernelGGL(add_f2d_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, x, s, n);
+}
+void vec_update_state(double* U, double* U0, const double* du, long n, hipStream_t st) {
+    hipLaunchKernelGGL(update_kernel, dim3(grid_for(n)), dim3(256), 0, st, U, U0, du, n);
+}
+
+// multi-dot: part[blk*nv + j] = sum over this block's slice of V_j . w
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void multidot_kernel(const T* __restrict__ V, long ldv, int nv,
+                                                       const T* __restrict__ w, long n, double* __restrict__ part) {
+    __shared__ double sm[NV][256 / 64];
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) acc[j] = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double wi = (double)w[i];
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            if (j < nv) acc[j] += (double)V[(size_t)j * ldv + i] * wi;
+    }
+    // wavefront reduction (64 lanes) then across the 4 waves of the block
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        double v = acc[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) sm[j][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV && (int)threadIdx.x < nv) {
+        const int j = threadIdx.x;
+        part[(size_t)blockIdx.x * nv + j] = ((sm[j][0] + sm[j][1]) + sm[j][2]) + sm[j][3];
+    }
+}
+
+// out[j] = sum over blocks of part[blk*nv + j], fixed order
+__global__ void finish_dot_kernel(const double* __restrict__ part, int nblk, int nv, double* __restrict__ out) {
+    __shared__ double sm[256];
+    for (int j = 0; j < nv; j++) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += 256) acc += part[(size_t)b * nv + j];
+        sm[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[j] = sm[0];
+        __syncthreads();
+    }
+}
+
+template <typename T>
+void vec_multidot(const T* V, long ldv, int nv, const T* w, long n, double* part, double* out, int nblk,
+                  hipStream_t st) {
+    for (int j0 = 0; j0 < nv; j0 += 16) {
+        const int c = (nv - j0 < 16) ? nv - j0 : 16;
+        if (c <= 1)
+            hipLaunchKernelGGL((multidot_kernel<T, 1>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part);
+        else if (c <= 4)
+            hipLaunchKernelGGL((multidot_kernel<T, 4>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part);
+        else
+            hipLaunchKernelGGL((multidot_kernel<T, 16>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part);
+        hipLaunchKernelGGL(finish_dot_kernel, dim3(1), dim3(256), 0, st, part, nblk, c, out + j0);
+    }
+}
+template void vec_multidot<double>(const double*, long, int, const double*, long, double*, double*, int, hipStream_t);
+template void vec_multidot<float>(const float*, long, int, const float*, long, double*, double*, int, hipStream_t);
+
+// w -= sign * sum_j h[j] V_j  (h on device)
+template <typename T>
+__global__ __launch_bounds__(256) void multi_axpy_kernel(T* __restrict__ w, const T* __restrict__ V, long ldv, int nv,
+                                                         const double* __restrict__ h, double sign, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int j = 0; j < nv; j++) acc += h[j] * (double)V[(size_t)j * ldv + i];
+        w[i] = (T)((double)w[i] - sign * acc);
+    }
+}
+template <typename T>
+void vec_multi_axpy(T* w, const T* V, long ldv, int nv, const double* h, double sign, long n, hipStream_t st) {
+    hipLaunchKernelGGL((multi_axpy_kernel<T>), dim3(grid_for(n)), dim3(256), 0, st, w, V, ldv, nv, h, sign, n);
+}
+template void vec_multi_axpy<double>(double*, const double*, long, int, const double*, double, long, hipStream_t);
+template void vec_multi_axpy<float>(float*, const float*, long, int, const double*, double, long, hipStream_t);
+
+template <typename T>
+__global__ void scale_inv_kernel(T* y, const T* x, const double* d, long n) {
+    const double dd = *d;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = (T)((double)x[i] / dd);
+}
+template <typename T>
+void vec_scale_inv(T* y, const T* x, const double* d, long n, hipStream_t st) {
+    hipLaunchKernelGGL((scale_inv_kernel<T>), dim3(grid_for(n)), dim3(256), 0, st, y, x, d, n);
+}
+template void vec_scale_inv<double>(double*, const double*, const double*, long, hipStream_t);
+template void vec_scale_inv<float>(float*, const float*, const double*, long, hipStream_t);
+
+// x += sum_j y[j] V_j
+template <typename T>
+__global__ void combine_kernel(double* __restrict__ x, const T* __restrict__ V, long ldv, int nv,
+                               const double* __restrict__ y, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int j = 0; j < nv; j++) acc += y[j] * (double)V[(size_t)j * ldv + i];
+        x[i] += acc;
+    }
+}
+template <typename T>
+void vec_combine(double* x, const T* V, long ldv, int nv, const double* y, long n, hipStream_t st) {
+    hipLaunchKernelGGL((combine_kernel<T>), dim3(grid_for(n)), dim3(256), 0, st, x, V, ldv, nv, y, n);
+}
+template void vec_combine<double>(double*, const double*, long, int, const double*, long, hipStream_t);
+template void vec_combine<float>(double*, const float*, long, int, const double*, long, hipStream_t);
+
+// per-equation sums of squares of a node array (MPI_RMSError.h:19-26)
+__global__ void sumsq_eqn_kernel(const double* __restrict__ U, int neqn, long n_node, double* __restrict__ part) {
+    __shared__ double sm[5][256];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n_node; i += (long)gridDim.x * blockDim.x)
+        for (int q = 0; q < neqn; q++) {
+            const double u = U[i * neqn + q];
+            acc[q] += u * u;
+        }
+    for (int q = 0; q < 5; q++) sm[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 5; q++) sm[q][threadIdx.x] += sm[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < (unsigned)neqn) part[(size_t)blockIdx.x * neqn + threadIdx.x] = sm[threadIdx.x][0];
+}
+void vec_sumsq_eqn(const double* U, int neqn, long n_node, double* part, double* out, int nblk, hipStream_t st) {
+    hipLaunchKernelGGL(sumsq_eqn_kernel, dim3(nblk), dim3(256), 0, st, U, neqn, n_node, part);
+    hipLaunchKernelGGL(finish_dot_kernel, dim3(1), dim3(256), 0, st, part, nblk, neqn, out);
+}
+
+__global__ void min_reduce_kernel(const double* part, int n, double* out) {
+    __shared__ double sm[256];
+    double m = 1000.0;
+    for (int i = threadIdx.x; i < n; i += 256) m = fmin(m, part[i]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sm[0];
+}
+void min_reduce(const double* part, int n, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(min_reduce_kernel, dim3(1), dim3(256), 0, st, part, n, out);
+}
+
+// ---------------------------------------------------------------------------
+// Greedy coloring, Jones-Plassmann form: an element is colored once every
+// neighbour (element sharing a node) of higher priority is colored, with
+// the smallest color none of those neighbours uses.  This is exactly the
+// sequential first-fit greedy coloring visiting elements in priority order.
+// ---------------------------------------------------------------------------
+__global__ void color_round_kernel(const int* __restrict__ conn, int nnod, const int* __restrict__ node_ptr,
+                                   const int* __restrict__ node_elem, long n_elem,
+                                   const unsigned long long* __restrict__ prio, const int* __restrict__ cin,
+                                   int* __restrict__ cout, int* changed) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_elem) return;
+    if (cin[e] >= 0) { cout[e] = cin[e]; return; }
+    const unsigned long long pe = prio[e];
+    unsigned long long used0 = 0, used1 = 0;
+    bool ready = true;
+    for (int n = 0; n < nnod && ready; n++) {
+        const int g = conn[e * nnod + n];
+        for (int p = node_ptr[g]; p < node_ptr[g + 1]; p++) {
+            const int f = node_elem[p];
+            if (f == e) continue;
+            if (prio[f] < pe) {
+                const int c = cin[f];
+                if (c < 0) { ready = false; break; }
+                if (c < 64) used0 |= 1ULL << c;
+                else used1 |= 1ULL << (c - 64);
+            }
+        }
+    }
+    if (!ready) { cout[e] = -1; *changed = 1; return; }   // still waiting
+    int c = 0;
+    while (c < 64 && ((used0 >> c) & 1ULL)) c++;
+    if (c == 64)
+        while (c < 128 && ((used1 >> (c - 64)) & 1ULL)) c++;
+    cout[e] = c;
+}
+
+void launch_color_round(const int* conn, int nnod, const int* node_ptr, const int* node_elem, long n_elem,
+                        const unsigned long long* prio, const int* cin, int* cout, int* changed, hipStream_t st) {
+    const int nb = (int)((n_elem + 255) / 256);
+    hipLaunchKernelGGL(color_round_kernel, dim3(nb), dim3(256), 0, st, conn, nnod, node_ptr, node_elem, n_elem, prio,
+                       cin, cout, changed);
+}
+
+}  // namespace cfdv

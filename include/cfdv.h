@@ -1,0 +1,168 @@
+/*
+ * C ABI of libcfdv_amd: the MI355X (gfx950) FDV time step of hahnpv/CFDV.
+ *
+ * Plain pointers and sizes only.  The caller owns every host array; the
+ * context owns every device buffer.  No pointer is retained past a call.
+ * Every function returns CFDV_OK (0) or a negative error code and never
+ * exits, throws across the ABI or reads stdin; cfdv_last_error() gives the
+ * message.  A context is not thread-safe; it runs on one HIP stream and every
+ * call is synchronous on return unless stated otherwise.
+ *
+ * Reference interfaces replaced (paths relative to the reference src/):
+ *   cfdv_create/destroy  Thermo::set (FDV/Thermo.h:22-33) + the mesh held by
+ *                        LoadBinaryData (Utility/IO/ASCIItoBinary/LoadBinary.h:
+ *                        156-318: nodes, elements, faces, Dirichlet flags)
+ *                        and CalcLength (FDV/Function/CalcLength.h:256-288,
+ *                        main.cpp:120-121)
+ *   cfdv_set_state       node U after LoadBinary::loadNodes (LoadBinary.h:
+ *                        174-200) or any later state
+ *   cfdv_cfl             CFL (FDV/Function/MPI_CFL.h:170-217, main.cpp:179)
+ *   cfdv_build           for_each(ClearElement), for_each(NodeUnpack),
+ *                        for_each(FDVGalerkin), for_each(ApplyBC)
+ *                        (main.cpp:169-194; FDV/FDVGalerkin.h:121-154,
+ *                        FDV/Function/ApplyBC.h:14-133)
+ *   cfdv_get_blocks      Element::R, Element::rhs (FDV/Element.h:227-228)
+ *   cfdv_solve           GMRESbase::iterate (Utility/Solvers/GMRESbase.h:
+ *                        71-72) as implemented by MPI_GMRES::iterate
+ *                        (Utility/Solvers/MPI_GMRES.h:125-220)
+ *   cfdv_update          GMRESbase::update / MPI_GMRES::update
+ *                        (MPI_GMRES.h:224-301)
+ *   cfdv_rms             MPI_RMSError (FDV/Function/MPI_RMSError.h:19-74)
+ *   cfdv_refine_flags    the flag pass of AdaptiveRefineList
+ *                        (adap/RefineListAdaptive.h:28-38)
+ *   cfdv_step            one iteration of the time loop (main.cpp:162-206)
+ *                        without Save/Output/Sensor
+ *   cfdv_color, cfdv_matvec, cfdv_profile_matvec: no reference counterpart
+ *                        (element coloring for the atomic-free EBE matvec,
+ *                        and test/measurement hooks).
+ */
+#ifndef CFDV_H
+#define CFDV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFDV_OK 0
+#define CFDV_ERR_ARG (-1)        /* bad argument or call order               */
+#define CFDV_ERR_HIP (-2)        /* HIP runtime error (incl. out of memory)  */
+#define CFDV_ERR_BC (-3)         /* unknown boundary-condition code          */
+#define CFDV_ERR_GEOM (-4)       /* non-positive element or face Jacobian    */
+#define CFDV_ERR_NAN (-5)        /* NaN/Inf in the solve                     */
+#define CFDV_ERR_BREAKDOWN (-6)  /* Krylov breakdown (zero Arnoldi vector)   */
+#define CFDV_ERR_CFL (-7)        /* no element set dt                        */
+#define CFDV_ERR_NOCONV (-8)     /* tolerance not met within the limits      */
+
+typedef struct cfdv_ctx cfdv_ctx;
+
+typedef struct {
+    int ndim;              /* 3 (hex8) or 2 (quad4)                               */
+    int nnod;              /* nodes per element: 8 or 4                           */
+    int neqn;              /* equations per node: 5 or 4                          */
+    int64_t n_elem;
+    int64_t n_node;
+    const int32_t *conn;   /* [n_elem][nnod] node ids (0-based)                   */
+    const double *x;       /* [n_node][ndim] coordinates                          */
+    const int32_t *bc;     /* [n_elem][nface] face bc codes, 0 = interior; face
+                              numbering of get_nodes_from_face (FiniteElement/
+                              get_face.h:80-165)                                  */
+    const uint8_t *dirichlet; /* [n_node][neqn] Node::dirichlet (LoadBinary.h:
+                              277-316)                                            */
+} cfdv_mesh;
+
+typedef struct {          /* the aero deck keys (MPIBinaryConfiguration.h:54-63) */
+    double gamma, mach, reynolds, prandtl, tinf, twall;
+    int adiabatic;
+} cfdv_thermo;
+
+/* block storage requested at create time */
+#define CFDV_BLOCKS_FP64 1
+#define CFDV_BLOCKS_FP32 2
+
+/* solver modes */
+#define CFDV_SOLVE_REFERENCE 0   /* MPI_GMRES::iterate as written: MGS Arnoldi,
+                                    fixed number of restarts, early-break
+                                    estimate and back substitution exactly as
+                                    the reference (quirks Q2-Q4)               */
+#define CFDV_SOLVE_FP64 1        /* restarted GMRES(m), FP64 blocks and vectors,
+                                    exit at ||b - K du|| <= rtol ||b||         */
+#define CFDV_SOLVE_MIXED 2       /* FP64 iterative refinement around an FP32
+                                    GMRES(inner_m) on FP32 blocks; outer
+                                    residual, Hessenberg/Givens and update in
+                                    FP64                                        */
+
+typedef struct {
+    int mode;          /* CFDV_SOLVE_*                                            */
+    int restarts;      /* reference: gmresrestart; FP64: max restarts; mixed: max
+                          outer refinement sweeps                                */
+    int m;             /* Krylov dimension (gmresiter)                            */
+    double rtol;       /* FP64/mixed: relative residual target                   */
+    int inner_m;       /* mixed: inner Krylov dimension                           */
+    double inner_rtol; /* mixed: inner relative residual target                  */
+    int inner_restarts;/* mixed: max inner restarts per sweep                    */
+    int transpose;     /* solve K^T du = b (adjoint); FP64 and mixed modes only   */
+} cfdv_solve_opts;
+
+typedef struct {
+    double bnorm;        /* ||b||                                                 */
+    double resid;        /* final ||b - K du|| recomputed in FP64                  */
+    double rel_resid;    /* resid / bnorm                                          */
+    double est_resid;    /* last Givens residual estimate                          */
+    int matvecs64;       /* EBE matvecs with FP64 blocks                           */
+    int matvecs32;       /* EBE matvecs with FP32 blocks                           */
+    int iters;           /* Arnoldi iterations (all restarts / sweeps)             */
+    int restarts;        /* restarts (FP64, reference) or refinement sweeps        */
+    int breakdown;
+} cfdv_solve_stats;
+
+typedef struct {
+    double dt;
+    double urms[5];
+    double ms_cfl, ms_build, ms_solve, ms_update;   /* HIP-event timings */
+    double ms_matvec64, ms_matvec32;   /* summed EBE matvec time (events around
+                                          each full matvec: zero-fill + one
+                                          launch per color)                  */
+    int n_matvec64, n_matvec32;
+    cfdv_solve_stats solve;
+} cfdv_step_stats;
+
+int cfdv_create(const cfdv_mesh *mesh, const cfdv_thermo *thermo, int block_flags,
+                cfdv_ctx **out);
+void cfdv_destroy(cfdv_ctx *ctx);
+const char *cfdv_last_error(const cfdv_ctx *ctx);
+int cfdv_mesh_info(const cfdv_ctx *ctx, int64_t *n_elem, int64_t *n_node, int *ncolors,
+                   int64_t *n_slots);
+
+int cfdv_set_state(cfdv_ctx *ctx, const double *U);            /* [n_node][neqn] */
+int cfdv_get_state(cfdv_ctx *ctx, double *U, double *U0);      /* either may be NULL */
+int cfdv_cfl(cfdv_ctx *ctx, double cfl, double *dt);
+int cfdv_build(cfdv_ctx *ctx, double dt);
+/* R: [n_elem][k][k] with R(i,j) at [e][j][i] (the reference Tensor's
+   column-major order), rhs: [n_elem][k], s: [n_elem][4] (s1..s4).  Any may
+   be NULL. */
+int cfdv_get_blocks(cfdv_ctx *ctx, double *R, double *rhs, double *s);
+int cfdv_get_blocks32(cfdv_ctx *ctx, float *R);
+int cfdv_get_rhs_vector(cfdv_ctx *ctx, double *b);             /* assembled b */
+int cfdv_solve(cfdv_ctx *ctx, const cfdv_solve_opts *opts, cfdv_solve_stats *st);
+int cfdv_get_du(cfdv_ctx *ctx, double *du);
+int cfdv_update(cfdv_ctx *ctx);
+int cfdv_rms(cfdv_ctx *ctx, double *urms);
+/* flags [n_elem] = s1 > 0.8; ties = elements where r == 0.01 exactly left
+   s1/s3 unassigned in the reference (quirk Q11) */
+int cfdv_refine_flags(cfdv_ctx *ctx, uint8_t *flags, double *s1, int32_t *nflagged,
+                      int32_t *nties);
+int cfdv_color(cfdv_ctx *ctx, int32_t *color, int32_t *ncolors);
+/* y = K x (or K^T x) with the built blocks; precision 64, 32 (FP32 blocks and
+   vectors) or 3 (FP32 blocks, FP64 vectors); host arrays [n_node*neqn] */
+int cfdv_matvec(cfdv_ctx *ctx, const double *x, double *y, int precision, int transpose);
+/* average time of one full EBE matvec (all colors), measured with HIP events
+   on the context stream */
+int cfdv_profile_matvec(cfdv_ctx *ctx, int precision, int reps, double *ms);
+int cfdv_step(cfdv_ctx *ctx, double cfl, const cfdv_solve_opts *opts, cfdv_step_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

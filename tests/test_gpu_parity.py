@@ -1,0 +1,161 @@
+"""GPU parity of the HIP FDV path against the CPU oracle (through the C ABI).
+
+Tolerances (FP64 path): element blocks within 1e-11 relative Frobenius per
+element (the GPU contracts with FMA and in factorised order, the oracle in
+the reference's loop order without FMA); one reference-mode GMRES step
+within 1e-10 relative on du; flags, colors and connectivity bit-exact.
+Mixed path: converged fields within 1e-6 of the FP64 path.
+"""
+import numpy as np
+import pytest
+
+from cfdv_amd import mesh as M
+from cfdv_amd.fdv import FDVContext, FDVGalerkin, HipGMRES
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    if name == "box3d":
+        return M.mixed_bc_box()
+    if name == "box3d_adiabatic":
+        return M.mixed_bc_box(adiabatic=True, seed=11)
+    if name == "plate":
+        return M.flat_plate(12, 6, 5)
+    if name == "ramp":
+        return M.compression_ramp(12, 5, 6)
+    if name == "lid":
+        return M.lid_cavity(8)
+    raise KeyError(name)
+
+
+def _rel_fro(a, b):
+    num = np.linalg.norm((a - b).reshape(a.shape[0], -1), axis=1)
+    den = np.linalg.norm(b.reshape(b.shape[0], -1), axis=1)
+    return num / np.maximum(den, 1e-300)
+
+
+@pytest.mark.parametrize("case", ["box3d", "box3d_adiabatic", "plate", "ramp", "lid"])
+def test_build_parity(gpu_available, case):
+    m, th, U = _case(case)
+    ctx = FDVContext(m, th, U, blocks=("fp64", "fp32"))
+    dt = ctx.cfl(0.5)
+    assert dt == pytest.approx(O.cfl(th, m, U, 0.5), rel=1e-14)
+    FDVGalerkin(ctx, dt)()
+    R, rhs, s = ctx.blocks()
+    Ro, rhso, so = O.build(th, m, U, dt)
+    print(case, "n_elem", m.n_elem, "colors", ctx.info()["ncolors"])
+    assert np.allclose(s, so, rtol=1e-13, atol=0)
+    err = _rel_fro(R, Ro)
+    print("R rel err max", err.max())
+    assert err.max() < 1e-11
+    rerr = np.linalg.norm(rhs - rhso) / max(np.linalg.norm(rhso), 1e-300)
+    print("rhs rel err", rerr)
+    assert rerr < 1e-10
+    # FP32 blocks are the FP64 blocks rounded once
+    R32 = ctx.blocks32()
+    assert np.array_equal(R32, R.astype(np.float32))
+    assert _rel_fro(R32.astype(np.float64), Ro).max() < 1e-7
+    b = ctx.rhs_vector()
+    bo = O.compose_rhs(m, rhso)
+    assert np.linalg.norm(b - bo) <= 1e-10 * max(np.linalg.norm(bo), 1e-300)
+
+
+@pytest.mark.parametrize("case", ["box3d", "plate", "lid"])
+def test_coloring_and_matvec(gpu_available, case):
+    m, th, U = _case(case)
+    ctx = FDVContext(m, th, U)
+    col, nc = ctx.color()
+    colo, nco = O.color_greedy(m)
+    assert nc == nco
+    assert np.array_equal(col, colo)
+    # validity: no two elements of one color share a node
+    for c in range(nc):
+        nodes = m.conn[col == c].ravel()
+        assert len(np.unique(nodes)) == len(nodes)
+    dt = ctx.cfl(0.5)
+    ctx.build(dt)
+    R, _, _ = ctx.blocks()
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(m.n_node * m.neqn)
+    y = ctx.matvec(x)
+    yo = O.matvec(m, R, x)
+    assert np.linalg.norm(y - yo) <= 1e-13 * np.linalg.norm(yo)
+    yt = ctx.matvec(x, transpose=True)
+    yto = O.matvec(m, R, x, transpose=True)
+    assert np.linalg.norm(yt - yto) <= 1e-13 * np.linalg.norm(yto)
+    z = rng.standard_normal(m.n_node * m.neqn)
+    assert abs(np.dot(ctx.matvec(x), z) - np.dot(x, ctx.matvec(z, transpose=True))) \
+        <= 1e-13 * np.linalg.norm(y) * np.linalg.norm(z)
+
+
+@pytest.mark.parametrize("case,restarts,mdim", [("box3d", 1, 20), ("box3d", 2, 6),
+                                                ("lid", 1, 20), ("plate", 1, 30)])
+def test_reference_gmres_step(gpu_available, case, restarts, mdim):
+    m, th, U = _case(case)
+    ctx = FDVContext(m, th, U)
+    st = ctx.step(0.5, mode="reference", restarts=restarts, m=mdim)
+    U1, U0 = ctx.get_state()
+    Uo, U0o, sto = O.step(th, m, U, cfl=0.5, restart=restarts, mdim=mdim, mode=0)
+    assert st.dt == pytest.approx(sto.dt, rel=1e-14)
+    du = (U1 - U0).ravel()
+    duo = (Uo - U0o).ravel()
+    err = np.linalg.norm(du - duo) / np.linalg.norm(duo)
+    print(case, "du rel err", err, "iters", st.solve.iters, sto.gm.iters)
+    assert st.solve.iters == sto.gm.iters
+    assert err < 1e-10
+    assert np.allclose(st.urms[:m.neqn], np.array(sto.urms)[:m.neqn], rtol=1e-8, atol=1e-14)
+
+
+@pytest.mark.parametrize("case", ["box3d", "plate"])
+def test_fp64_and_mixed_solve(gpu_available, case):
+    m, th, U = _case(case)
+    ctx = FDVContext(m, th, U, blocks=("fp64", "fp32"))
+    dt = ctx.cfl(0.5)
+    ctx.build(dt)
+    R, rhs, _ = ctx.blocks()
+    b = O.compose_rhs(m, rhs)
+    st, _ = ctx.solve("fp64", restarts=40, m=30, rtol=1e-10)
+    assert st.rel_resid <= 1e-10
+    du64 = ctx.du()
+    dense = np.zeros((m.n_node * m.neqn,) * 2)
+    if dense.shape[0] <= 3000:
+        for e in range(m.n_elem):
+            dofs = (m.conn[e][:, None] * m.neqn + np.arange(m.neqn)).ravel()
+            dense[np.ix_(dofs, dofs)] += R[e]
+        xs = np.linalg.solve(dense, b)
+        assert np.linalg.norm(du64 - xs) <= 1e-8 * np.linalg.norm(xs)
+    st2, _ = ctx.solve("mixed", restarts=20, m=30, rtol=1e-10, inner_m=20, inner_rtol=1e-4,
+                       inner_restarts=3)
+    assert st2.rel_resid <= 1e-10
+    assert st2.matvecs32 > 0
+    dmx = ctx.du()
+    assert np.linalg.norm(dmx - du64) <= 1e-8 * np.linalg.norm(du64)
+
+
+def test_hip_gmres_interface(gpu_available):
+    m, th, U = _case("box3d")
+    ctx = FDVContext(m, th, U)
+    dt = ctx.cfl(0.5)
+    FDVGalerkin(ctx, dt)()
+    g = HipGMRES(ctx, restart=1, m=20)
+    st = g.iterate()
+    g.update()
+    U1, U0 = ctx.get_state()
+    assert np.array_equal(U0, U)
+    Uo, _, _ = O.step(th, m, U, cfl=0.5, restart=1, mdim=20)
+    assert np.linalg.norm(U1 - Uo) <= 1e-10 * np.linalg.norm(Uo - U)
+    assert st.iters > 0
+
+
+@pytest.mark.parametrize("case", ["box3d", "ramp", "plate", "lid"])
+def test_refine_flags_bit_exact(gpu_available, case):
+    m, th, U = _case(case)
+    ctx = FDVContext(m, th, U)
+    f, s1, ties = ctx.refine_flags()
+    fo = O.refine_flags(th, m, U)
+    so, _ = O.fdvparam_all(th, m, U)
+    assert ties == 0
+    assert np.array_equal(f, fo)
+    assert np.array_equal(s1, so[:, 0])
