@@ -1,0 +1,67 @@
+"""Stand-alone run of the element build and the EBE matvec, for profilers.
+
+    python -m cfdv_amd.profile_matvec --size 256x128x64 --reps 20
+
+Builds the compression-ramp mesh of the given size, runs one element build,
+then ``--reps`` full EBE matvecs in FP32 and in FP64 blocks, and prints one
+JSON line with the event-timed milliseconds and algorithmic bytes of each.
+With ``--copy-mib N`` it first runs one plain device copy of N MiB (an
+elementwise ``mul(src, 1)`` through torch) whose HBM bytes are known, so that
+hardware byte counters can be scaled against it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+from . import mesh as M
+from .fdv import FDVContext
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser()
+    p.add_argument("--size", default="256x128x64")
+    p.add_argument("--reps", type=int, default=20)
+    p.add_argument("--cfl", type=float, default=0.5)
+    p.add_argument("--copy-mib", type=int, default=0)
+    p.add_argument("--precision", choices=["32", "64", "both"], default="both")
+    a = p.parse_args(argv)
+    nx, ny, nz = (int(v) for v in a.size.split("x"))
+    out = {"size": a.size}
+    if a.copy_mib:
+        import torch
+        n = a.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        out["copy_bytes_read"] = n * 4
+        out["copy_bytes_written"] = n * 4
+        del src, dst
+    mesh, th, U = M.compression_ramp(nx, ny, nz)
+    ctx = FDVContext(mesh, th, U, blocks=("fp64", "fp32"))
+    dt = ctx.cfl(a.cfl)
+    t0 = time.perf_counter()
+    ctx.build(dt)
+    ctx.rhs_vector()
+    out["build_wall_ms"] = (time.perf_counter() - t0) * 1e3
+    ne, nn = mesh.n_elem, mesh.n_node
+    k = mesh.k
+    for prec in ((32, 64) if a.precision == "both" else (int(a.precision),)):
+        ms = ctx.profile_matvec(prec, a.reps)
+        w = prec // 8
+        nbytes = ne * k * k * w + nn * mesh.neqn * 2 * w
+        out[f"matvec{prec}_ms"] = ms
+        out[f"matvec{prec}_bytes"] = nbytes
+        out[f"matvec{prec}_GBps"] = nbytes / (ms * 1e-3) / 1e9
+    out["elements"] = ne
+    out["nodes"] = nn
+    out["colors"] = ctx.info()["ncolors"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,105 @@
+"""Writes the golden fixtures under tests/golden/.
+
+1. Copies three of the reference's example decks (data files only: control,
+   aero, binarynodes, binaryelements; plus the ASCII nodes/elements/faces of
+   the small 3D deck, which pin the binary record layout and the face node
+   tables) from the reference checkout into tests/golden/decks/:
+     3dtestlagrange  324 hex8, bc -1 1 2 22 6 7 32, isothermal (Twall 2.8)
+     3dtestramp_new  5814 hex8, bc -1 1 2 4 8 21 22, adiabatic
+     lid50           Lid/50x50, 2401 quad4, bc -1 1
+2. Runs the CPU oracle on each deck and stores, per deck, an .npz with
+   s1..s4 per element, the ApplyBC'd element blocks of a selection of
+   elements (every bc code, interior elements) and the Frobenius norm of
+   every block, the element rhs, dt at the deck's CFL, the composed b, and
+   two reference-mode GMRES steps with the deck's gmresiter/gmresrestart
+   (du, U after each step, RMS rows).
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from cfdv_amd import mesh as M  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+SRC = "/root/reference/example"
+DECKS = {
+    "3dtestlagrange": ("3dtestlagrange", True),
+    "3dtestramp_new": ("3dtestramp_new", False),
+    "lid50": ("Lid/50x50", False),
+}
+BIN_FILES = ("control", "aero", "binarynodes", "binaryelements")
+ASCII_FILES = ("nodes", "elements", "faces")
+
+
+def copy_decks():
+    for name, (rel, with_ascii) in DECKS.items():
+        dst = os.path.join(HERE, "decks", name)
+        os.makedirs(dst, exist_ok=True)
+        files = BIN_FILES + (ASCII_FILES if with_ascii else ())
+        for f in files:
+            shutil.copyfile(os.path.join(SRC, rel, f), os.path.join(dst, f))
+            os.chmod(os.path.join(dst, f), 0o644)
+
+
+def select_elements(mesh, per_code=3, n_interior=8):
+    sel = []
+    for code in sorted(set(mesh.bc.ravel().tolist()) - {0}):
+        idx = np.nonzero((mesh.bc == code).any(axis=1))[0]
+        sel.extend(idx[:per_code].tolist())
+        sel.extend(idx[-1:].tolist())
+    interior = np.nonzero((mesh.bc == 0).all(axis=1))[0]
+    if len(interior):
+        sel.extend(interior[np.linspace(0, len(interior) - 1, n_interior).astype(int)].tolist())
+    return np.array(sorted(set(sel)), dtype=np.int64)
+
+
+def make(name):
+    mesh, th, U, ctl = M.read_deck(os.path.join(HERE, "decks", name))
+    cfl = float(ctl["cfl"])
+    mdim = int(float(ctl["gmresiter"]))
+    restart = int(float(ctl["gmresrestart"]))
+    dt = O.cfl(th, mesh, U, cfl)
+    s, ties = O.fdvparam_all(th, mesh, U)
+    R, rhs, _ = O.build(th, mesh, U, dt)
+    sel = select_elements(mesh)
+    b = O.compose_rhs(mesh, rhs)
+    steps_U, steps_du, steps_rms, steps_dt, steps_it = [], [], [], [], []
+    Uc = U.copy()
+    for _ in range(2):
+        U1, U0, st = O.step(th, mesh, Uc, cfl=cfl, restart=restart, mdim=mdim, mode=0)
+        steps_U.append(U1)
+        steps_du.append((U1 - U0).ravel())
+        steps_rms.append(np.array(st.urms)[:mesh.neqn])
+        steps_dt.append(st.dt)
+        steps_it.append(st.gm.iters)
+        Uc = U1
+    out = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(
+        out, cfl=cfl, mdim=mdim, restart=restart, dt=dt, s=s, ties=ties,
+        sel=sel, R_sel=R[sel], R_fro=np.linalg.norm(R.reshape(len(R), -1), axis=1),
+        rhs=rhs, b=b, U0=U, U_steps=np.array(steps_U), du_steps=np.array(steps_du),
+        rms_steps=np.array(steps_rms), dt_steps=np.array(steps_dt),
+        iters_steps=np.array(steps_it))
+    print(f"{name}: {mesh.n_elem} elements, {mesh.n_node} nodes, dt {dt:.6e}, "
+          f"{len(sel)} selected blocks, ties {ties}, -> {os.path.getsize(out)} bytes")
+
+
+def main():
+    if os.path.isdir(SRC):
+        copy_decks()
+    for name in DECKS:
+        make(name)
+
+
+if __name__ == "__main__":
+    main()

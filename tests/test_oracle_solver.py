@@ -1,0 +1,277 @@
+"""Pins of the oracle's mesh metrics, FDVParam, EBE products, GMRES variants,
+coloring and refinement flags.
+
+Each check is an independent statement of what the reference routine must
+produce: closed forms on boxes (CalcLength.h:256-332, MPI_CFL.h:170-217),
+hand-computed FDVParam values (FDVParam.h:15-111), the dense assembled
+operator for the element-by-element products (GMRES.h:460-530), a dense
+solve for textbook GMRES, the one-step minimal-residual closed form and the
+diagonal back substitution of the reference's GMRES (MPI_GMRES.h:125-220,
+756-780), and brute-force first-fit coloring.
+"""
+import numpy as np
+import pytest
+
+from cfdv_amd import mesh as M
+from oracle import oracle as O
+
+
+def _assemble(mesh, R):
+    n = mesh.n_node * mesh.neqn
+    A = np.zeros((n, n))
+    for e in range(mesh.n_elem):
+        dofs = (mesh.conn[e][:, None] * mesh.neqn + np.arange(mesh.neqn)).ravel()
+        A[np.ix_(dofs, dofs)] += R[e]
+    return A
+
+
+def _random_blocks(mesh, seed, diag=4.0):
+    rng = np.random.default_rng(seed)
+    k = mesh.k
+    R = rng.standard_normal((mesh.n_elem, k, k)) * 0.3
+    R += diag * np.eye(k)[None]
+    return R
+
+
+def _plain_box(nx, ny, nz, hs):
+    x, conn = M.box_3d(nx, ny, nz, lx=nx * hs[0], ly=ny * hs[1], lz=nz * hs[2])
+    bc = np.zeros((conn.shape[0], 6), np.int32)
+    return M.Mesh(3, 8, 5, conn, x, bc, np.zeros((x.shape[0], 5), np.uint8))
+
+
+# ---------------------------------------------------------------- metrics
+
+def test_calc_length_box_and_square():
+    hs = (0.3, 0.2, 0.25)
+    m = _plain_box(3, 2, 2, hs)
+    area, cleng = O.calc_length(m)
+    vol = hs[0] * hs[1] * hs[2]
+    assert np.allclose(area, vol, rtol=1e-14)
+    assert np.allclose(cleng, vol ** (1 / 3), rtol=1e-14)
+    m2, _, _ = M.lid_cavity(5)
+    a2, c2 = O.calc_length(m2)
+    assert np.allclose(a2, 1 / 25, rtol=1e-14)
+    assert np.allclose(c2, 1 / 5, rtol=1e-14)
+
+
+def test_calc_length_ramp_volume_sums_to_domain():
+    m, _, _ = M.compression_ramp(10, 3, 4)
+    area, _ = O.calc_length(m)
+    # domain: x in [0,1], y in [0,0.5], z from the ramp surface to 0.5;
+    # linear bottom in each x cell, so trilinear elements are exact
+    xs = np.linspace(0, 1, 11)
+    zb = np.where(xs > 0.5, (xs - 0.5) * np.tan(np.radians(15)), 0.0)
+    height = 0.5 - zb
+    vol = 0.5 * np.sum(0.5 * (height[1:] + height[:-1]) * np.diff(xs))
+    assert abs(area.sum() - vol) < 1e-13
+
+
+def test_cfl_closed_form_and_literal_gamma():
+    """dt = CFL * min_e cleng_e / (|v| + sqrt(1.4 cgas T)); the sound speed
+    uses the literal 1.4 even when gamma differs (MPI_CFL.h:197)."""
+    hs = (0.3, 0.2, 0.25)
+    m = _plain_box(3, 2, 2, hs)
+    for gamma in (1.4, 1.3):
+        th = M.Thermo(gamma=gamma, mach=2.0, reynolds=100.0)
+        rng = np.random.default_rng(1)
+        T = 1 + rng.random(m.n_node)
+        v = rng.standard_normal((m.n_node, 3)) * 0.5
+        U = M.conservative(np.ones(m.n_node), v, T, th.Cv)
+        dt = O.cfl(th, m, U, 0.7)
+        p = O.prim_arrays(O.unpack(th, m, U), 3)
+        speed = np.linalg.norm(p["v"], axis=1) + np.sqrt(1.4 * th.cgas * p["T"])
+        cl = (hs[0] * hs[1] * hs[2]) ** (1 / 3)
+        expect = 0.7 * min(cl / speed[m.conn[e]].max() for e in range(m.n_elem))
+        assert dt == pytest.approx(expect, rel=1e-14)
+        assert np.allclose(p["T"], T, rtol=1e-13)
+
+
+# ---------------------------------------------------------------- FDVParam
+
+def _one_hex(speeds, T=1.0, rho=1.0, gamma=1.4):
+    x, conn = M.box_3d(1, 1, 1)
+    m = M.Mesh(3, 8, 5, conn, x, np.zeros((1, 6), np.int32), np.zeros((8, 5), np.uint8))
+    th = M.Thermo(gamma=gamma, mach=2.0, reynolds=1000.0)
+    v = np.zeros((8, 3))
+    v[:, 0] = speeds
+    U = M.conservative(np.full(8, rho), v, np.full(8, T), th.Cv)
+    return m, th, U
+
+
+def test_fdvparam_hand_values():
+    # T = 1: a = sqrt(gamma cgas) and mu = 1, so M = V/a and Re = rho V
+    m, th, U = _one_hex(np.linspace(1.0, 1.1, 8))
+    s, ties = O.fdvparam_all(th, m, U)
+    a = np.sqrt(th.gamma * th.cgas)
+    Mn = np.linspace(1.0, 1.1, 8) / a
+    r = np.sqrt(Mn.max() ** 2 - Mn.min() ** 2) / (Mn.min() + 1e-7)
+    rr = np.sqrt(1.1 ** 2 - 1.0) / (1.0 + 1e-7)
+    assert ties == 0
+    assert s[0, 0] == pytest.approx(min(r, 1.0), rel=1e-12)
+    assert s[0, 1] == pytest.approx(0.5 * (1 + min(r, 1.0) ** 0.2), rel=1e-12)
+    assert s[0, 2] == pytest.approx(min(rr, 1.0), rel=1e-12)
+    assert s[0, 3] == pytest.approx(0.5 * (1 + min(rr, 1.0) ** 0.2), rel=1e-12)
+
+
+def test_fdvparam_special_cases():
+    # uniform flow: r = 0 < 0.01 with M_min > 0 -> s1 = s3 = 0, s2 = s4 = 0.5
+    m, th, U = _one_hex(np.full(8, 0.9))
+    s, _ = O.fdvparam_all(th, m, U)
+    assert np.array_equal(s[0], [0.0, 0.5, 0.0, 0.5])
+    # fluid at rest: M_min = Re_min = 0 -> s1 = s3 = 1, s2 = s4 = 1
+    m, th, U = _one_hex(np.zeros(8))
+    s, _ = O.fdvparam_all(th, m, U)
+    assert np.array_equal(s[0], [1.0, 1.0, 1.0, 1.0])
+    # one node at rest (a wall node): r = M_max / 1e-7 -> clipped to 1
+    sp = np.full(8, 0.5)
+    sp[0] = 0.0
+    m, th, U = _one_hex(sp)
+    s, _ = O.fdvparam_all(th, m, U)
+    assert np.array_equal(s[0], [1.0, 1.0, 1.0, 1.0])
+
+
+def test_refine_flags_are_s1_threshold():
+    m, th, _ = M.mixed_bc_box(5, 4, 3, seed=3)
+    rng = np.random.default_rng(3)
+    # speeds spread so that s1 lands on both sides of 0.8
+    v = np.zeros((m.n_node, 3))
+    v[:, 0] = 1 + 0.35 * rng.random(m.n_node)
+    U = M.conservative(1 + 0.1 * rng.random(m.n_node), v, np.ones(m.n_node), th.Cv)
+    s, _ = O.fdvparam_all(th, m, U)
+    f = O.refine_flags(th, m, U)
+    assert np.array_equal(f.astype(bool), s[:, 0] > 0.8)
+    assert 0 < f.sum() < m.n_elem
+
+
+# ---------------------------------------------------------------- EBE products
+
+@pytest.mark.parametrize("case", ["hex", "quad"])
+def test_matvec_equals_dense_assembly(case):
+    if case == "hex":
+        m, _, _ = M.mixed_bc_box(3, 3, 2)
+    else:
+        m, _, _ = M.lid_cavity(4)
+    R = _random_blocks(m, 2)
+    A = _assemble(m, R)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(A.shape[0])
+    assert np.allclose(O.matvec(m, R, x), A @ x, rtol=0, atol=1e-13 * np.abs(A @ x).max())
+    assert np.allclose(O.matvec(m, R, x, transpose=True), A.T @ x, rtol=0,
+                       atol=1e-13 * np.abs(A.T @ x).max())
+    rhs = rng.standard_normal((m.n_elem, m.k))
+    b = np.zeros(A.shape[0])
+    for e in range(m.n_elem):
+        dofs = (m.conn[e][:, None] * m.neqn + np.arange(m.neqn)).ravel()
+        b[dofs] += rhs[e]
+    assert np.allclose(O.compose_rhs(m, rhs), b, rtol=1e-15, atol=1e-15)
+
+
+@pytest.mark.parametrize("transpose", [False, True])
+def test_gmres_true_matches_dense_solve(transpose):
+    m, _, _ = M.mixed_bc_box(3, 2, 2)
+    R = _random_blocks(m, 4)
+    A = _assemble(m, R)
+    b = np.random.default_rng(5).standard_normal(A.shape[0])
+    du, st = O.gmres_true(m, R, b, 25, 60, 1e-12, transpose=transpose)
+    xs = np.linalg.solve(A.T if transpose else A, b)
+    assert np.linalg.norm(du - xs) <= 1e-9 * np.linalg.norm(xs)
+    assert st.iters > 0
+
+
+def test_gmres_quirk_one_step_is_minimal_residual():
+    m, _, _ = M.mixed_bc_box(3, 2, 2)
+    R = _random_blocks(m, 6)
+    A = _assemble(m, R)
+    rhs = np.random.default_rng(7).standard_normal((m.n_elem, m.k))
+    b = O.compose_rhs(m, rhs)
+    du, st = O.gmres_quirk(m, R, rhs, 1, 1)
+    Ab = A @ b
+    assert np.allclose(du, (b @ Ab) / (Ab @ Ab) * b, rtol=1e-12, atol=0)
+    assert st.iters == 1
+
+
+def test_gmres_quirk_diagonal_back_substitution():
+    """One cycle of m Arnoldi steps; the reference's back substitution reads
+    the rotated Hessenberg transposed, so only the diagonal survives:
+    y_j = g_j / R_jj with (R, g) the QR-rotated least-squares system."""
+    m, _, _ = M.mixed_bc_box(3, 2, 2)
+    R = _random_blocks(m, 8, diag=2.0)
+    A = _assemble(m, R)
+    rhs = np.random.default_rng(8).standard_normal((m.n_elem, m.k))
+    b = O.compose_rhs(m, rhs)
+    mdim = 6
+    du, st = O.gmres_quirk(m, R, rhs, 1, mdim)
+    assert st.iters == mdim
+    # textbook Arnoldi (modified Gram-Schmidt) on the dense operator
+    n = len(b)
+    V = np.zeros((n, mdim + 1))
+    H = np.zeros((mdim + 1, mdim))
+    beta = np.linalg.norm(b)
+    V[:, 0] = b / beta
+    for j in range(mdim):
+        w = A @ V[:, j]
+        for i in range(j + 1):
+            H[i, j] = w @ V[:, i]
+            w -= H[i, j] * V[:, i]
+        H[j + 1, j] = np.linalg.norm(w)
+        V[:, j + 1] = w / H[j + 1, j]
+    Q, Rq = np.linalg.qr(H, mode="complete")
+    g = Q.T @ (beta * np.eye(mdim + 1)[0])
+    sgn = np.sign(np.diag(Rq))
+    y = (sgn * g[:mdim]) / (sgn * np.diag(Rq))
+    expect = V[:, :mdim] @ y
+    assert np.linalg.norm(du - expect) <= 1e-9 * np.linalg.norm(expect)
+    # and it is not the textbook GMRES iterate
+    ytrue = np.linalg.lstsq(H, beta * np.eye(mdim + 1)[0], rcond=None)[0]
+    assert np.linalg.norm(du - V[:, :mdim] @ ytrue) > 1e-3 * np.linalg.norm(expect)
+
+
+def test_update_and_rms():
+    rng = np.random.default_rng(2)
+    U = 1 + rng.random((7, 5))
+    du = 0.01 * rng.standard_normal(35)
+    U1 = U.ravel().copy()
+    U0 = np.zeros(35)
+    O.lib().orc_update(35, O._p(U1), O._p(U0), O._p(du))
+    assert np.array_equal(U0, U.ravel())
+    assert np.array_equal(U1, U.ravel() + du)
+    urms = np.zeros(5)
+    O.lib().orc_rms(7, 5, O._p(U1), O._p(U0), O._p(urms))
+    expect = np.sqrt((U1.reshape(7, 5) ** 2).sum(0) / (U.reshape(7, 5) ** 2).sum(0)) - 1
+    assert np.allclose(urms, expect, rtol=1e-12, atol=1e-15)
+
+
+# ---------------------------------------------------------------- coloring
+
+def _first_fit(mesh, order):
+    col = np.full(mesh.n_elem, -1)
+    node_elems = {}
+    for e in range(mesh.n_elem):
+        for g in mesh.conn[e]:
+            node_elems.setdefault(int(g), []).append(e)
+    for e in order:
+        used = {col[f] for g in mesh.conn[e] for f in node_elems[int(g)] if col[f] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        col[e] = c
+    return col
+
+
+@pytest.mark.parametrize("case", ["box", "lid", "ramp"])
+def test_greedy_coloring_brute_force(case):
+    if case == "box":
+        m, _, _ = M.mixed_bc_box(4, 3, 3)
+    elif case == "lid":
+        m, _, _ = M.lid_cavity(6)
+    else:
+        m, _, _ = M.compression_ramp(6, 4, 3)
+    col, nc = O.color_greedy(m)
+    assert np.array_equal(col, _first_fit(m, range(m.n_elem)))
+    assert nc == (8 if m.ndim == 3 else 4)
+    keys = O.jp_keys(m.n_elem)
+    colk, _ = O.color_greedy(m, keys)
+    assert np.array_equal(colk, _first_fit(m, np.argsort(keys, kind="stable")))
+    for c in range(nc):
+        nodes = m.conn[col == c].ravel()
+        assert len(np.unique(nodes)) == len(nodes)
