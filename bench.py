@@ -5,7 +5,8 @@ Workload: the 3D hypersonic 15-degree compression ramp, structured hex8
 Re=5e5, adiabatic wall, CFL 0.5, synthetic initial state (uniform
 freestream, wall at rest).  A step is one full FDV time step on the GPU:
 CFL, element build (FDVParam + domain + boundary + ApplyBC, FP64 arithmetic),
-GMRES to a 1e-8 relative residual, update and RMS monitor.
+GMRES to a 1e-8 relative true residual (both paths right-preconditioned by
+node-block Jacobi unless --precond none), update and RMS monitor.
 
 Two paths are timed from the same initial state:
   fp64  : FP64 blocks, FP64 restarted GMRES(m)
@@ -31,6 +32,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per full EBE matvec (8 color launches) measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE on MI355X, each scaled by the counter's reading for
+# a plain 1 GiB copy kernel (fetch x2.0, write x1.0); summary and commands in
+# profiles/ebe_matvec_ramp_256x128x64_hbm_counters.txt.  Only the default
+# workload size has a measurement.
+MEASURED_TRAFFIC = {"256x128x64": {"fp32": 14.607e9, "fp64": 29.121e9}}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -47,6 +55,7 @@ def parse():
     p.add_argument("--rtol", type=float, default=1e-8)
     p.add_argument("--inner-m", type=int, default=30)
     p.add_argument("--inner-rtol", type=float, default=1e-3)
+    p.add_argument("--precond", choices=["nodeblock", "none"], default="nodeblock")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=32768)
     return p.parse_args()
@@ -67,7 +76,7 @@ def run_steps(ctx, U0, mode, n, args):
     for i in range(n):
         st = ctx.step(args.cfl, mode=mode, restarts=60 if mode == "fp64" else 20, m=args.m,
                       rtol=args.rtol, inner_m=args.inner_m, inner_rtol=args.inner_rtol,
-                      inner_restarts=3)
+                      inner_restarts=3, precond=args.precond)
         out.append(st)
         log(f"  {mode} step {i}: build {st.ms_build:.1f} ms solve {st.ms_solve:.1f} ms "
             f"mv64 {st.solve.matvecs64} mv32 {st.solve.matvecs32} rel {st.solve.rel_resid:.2e}")
@@ -157,6 +166,7 @@ def main():
                           f"({det['t_matvec_sample_s'] * 1e3:.1f} ms), scaled to {ne} elements "
                           f"with {mv_per_step} matvecs per step (the FP64 GPU path's count)")}
 
+    traffic = MEASURED_TRAFFIC.get(args.size, {})
     line = {
         "metric": "FDV steps/s at 2.1M hex8: FP32+FP64-refine vs FP64 GPU, equal 1e-8 residual",
         "value": value,
@@ -172,8 +182,10 @@ def main():
         "data": "synthetic (generated ramp mesh and freestream initial state)",
         "config": {"workload": f"3D hypersonic 15-deg compression ramp hex8 {nx}x{ny}x{nz} "
                                f"({ne} elements, {nn} nodes), M=6, Re=5e5, adiabatic, "
-                               f"CFL {args.cfl}, GMRES({args.m}) to rel. residual {args.rtol}",
-                   "elements": ne, "nodes": nn, "colors": info["ncolors"]},
+                               f"CFL {args.cfl}, GMRES({args.m}) to rel. residual {args.rtol}, "
+                               f"{args.precond} right preconditioner on both paths",
+                   "elements": ne, "nodes": nn, "colors": info["ncolors"],
+                   "precond": args.precond},
         "fp64_steps_per_s": fp64_sps,
         "mixed_over_fp64": value / fp64_sps,
         "rel_resid_mixed": max(s.solve.rel_resid for s in st_mx),
@@ -185,12 +197,13 @@ def main():
                                    "fp64": sum(s.n_matvec64 for s in st_mx) / args.steps},
         "matvecs_per_step_fp64": n64 / args.steps,
         "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (ach32 / HBM_PEAK_GBS) if ach32 else None, "traffic": None,
+                     "frac": (ach32 / HBM_PEAK_GBS) if ach32 else None,
+                     "traffic": traffic.get("fp32"),
                      "kernel": "matvec_kernel<float,float,float,8,5> (EBE matvec, FP32 blocks), "
                                f"{b32} algorithmic bytes per full matvec "
                                f"({info['ncolors']} color launches)"},
         "roofline_fp64_matvec": {"achieved": ach64, "frac": (ach64 / HBM_PEAK_GBS) if ach64 else None,
-                                 "bytes_per_matvec": b64},
+                                 "bytes_per_matvec": b64, "traffic": traffic.get("fp64")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

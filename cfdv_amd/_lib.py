@@ -28,6 +28,9 @@ CFDV_SOLVE_REFERENCE = 0
 CFDV_SOLVE_FP64 = 1
 CFDV_SOLVE_MIXED = 2
 
+CFDV_PRECOND_NONE = 0
+CFDV_PRECOND_NODE_BLOCK = 1
+
 # every symbol declared in include/cfdv.h
 EXPORTS = ["cfdv_create", "cfdv_destroy", "cfdv_last_error", "cfdv_mesh_info", "cfdv_set_state",
            "cfdv_get_state", "cfdv_cfl", "cfdv_build", "cfdv_get_blocks", "cfdv_get_blocks32",
@@ -51,7 +54,7 @@ class ThermoS(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("restarts", C.c_int), ("m", C.c_int), ("rtol", C.c_double),
                 ("inner_m", C.c_int), ("inner_rtol", C.c_double), ("inner_restarts", C.c_int),
-                ("transpose", C.c_int)]
+                ("transpose", C.c_int), ("precond", C.c_int)]
 
 
 class SolveStats(C.Structure):

@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcfdv_amd.so")
 SOURCES = ["build.hip", "solve.hip", "capi.cpp"]
+ROOT = os.path.dirname(HERE)
+RUNNER = os.path.join(HERE, "cfdv_run")
 ARCH = os.environ.get("CFDV_OFFLOAD_ARCH", "gfx950")
 
 
@@ -50,9 +52,24 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    build_runner()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", RUNNER)
     return LIB
+
+
+def build_runner() -> str:
+    """The C++ host driver (host/cfdv_run.cpp) over the C ABI, built with g++."""
+    cxx = shutil.which("g++") or "c++"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(HERE, "host"), os.path.join(HERE, "host", "cfdv_run.cpp"),
+           "-L", HERE, "-lcfdv_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib",
+           "-o", RUNNER + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for cfdv_run:\n{r.stderr[-4000:]}")
+    os.replace(RUNNER + ".tmp", RUNNER)
+    return RUNNER
 
 
 if __name__ == "__main__":

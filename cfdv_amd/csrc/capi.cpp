@@ -141,9 +141,13 @@ struct cfdv_ctx {
     float *d_V32 = nullptr, *d_w32 = nullptr;
     int vcap32 = 0;
     double *d_d64 = nullptr;
+    // node-block Jacobi preconditioner (built lazily after each build)
+    double *d_P64 = nullptr, *d_z64 = nullptr, *d_t64 = nullptr;
+    float *d_P32 = nullptr, *d_z32 = nullptr;
+    bool pinv_valid = false;
     double *d_part = nullptr, *d_small = nullptr, *d_hcol = nullptr, *d_cflpart = nullptr;
     double* h_pin = nullptr;   // pinned staging for small host-to-device arrays
-    int *d_counters = nullptr;     // [0] ties, [1] status
+    int *d_counters = nullptr;     // [0] ties, [1] status, [2] singular node block
     hipEvent_t ev[8];
     std::vector<void*> allocs;
     // matvec timing: event pairs recorded around every full matvec while a
@@ -242,6 +246,25 @@ struct cfdv_ctx {
         }
         d_V32 = alloc<float>((size_t)(m + 1) * ndof);
         vcap32 = m + 1;
+    }
+    // node-block inverses from the current blocks (FP64 if held, else FP32)
+    void ensure_precond() {
+        if (!d_P64) {
+            d_P64 = alloc<double>((size_t)n_node * neqn * neqn);
+            d_P32 = alloc<float>((size_t)n_node * neqn * neqn);
+            d_z64 = alloc<double>((size_t)ndof);
+            d_z32 = alloc<float>((size_t)ndof);
+            d_t64 = alloc<double>((size_t)ndof);
+        }
+        if (pinv_valid) return;
+        HIP_CHECK_THROW(hipMemsetAsync(d_counters + 2, 0, sizeof(int), st));
+        launch_nodeblock(d_node_ptr, d_node_ent, d_R64, d_R32, nnod, neqn, n_node, d_P64, d_P32, d_counters + 2, st);
+        check_launch();
+        int singular = 0;
+        HIP_CHECK_THROW(hipMemcpyAsync(&singular, d_counters + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+        sync();
+        if (singular) throw std::runtime_error("singular node block in the node-block Jacobi preconditioner");
+        pinv_valid = true;
     }
 };
 
@@ -533,11 +556,14 @@ int solve_reference(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) 
 // Restarted GMRES(m) with CGS2 orthogonalisation and Givens rotations.
 // Solves K x = rhs_dev (or K^T x) into x_dev, starting from x_dev = 0.
 // prec: 64 (FP64 blocks/vectors) or 32 (FP32 blocks and basis; dots and
-// Hessenberg in FP64).  Returns Arnoldi iterations; *conv set when the
-// Givens estimate met tol_abs.
+// Hessenberg in FP64).  With Pinv (node-block inverses in T) the system is
+// right-preconditioned: K Pinv u = rhs, x = Pinv u, so the Givens estimate
+// is still the true residual; z is a work vector for Pinv V_j.  Returns
+// Arnoldi iterations; *conv set when the Givens estimate met tol_abs.
 template <typename T>
 int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, int maxrestart, double tol_abs,
-               double rnorm0, cfdv_solve_stats* s, bool* conv, T* V, T* w) {
+               double rnorm0, cfdv_solve_stats* s, bool* conv, T* V, T* w, const T* Pinv = nullptr,
+               T* z = nullptr) {
     const long n = c->ndof;
     std::vector<double> h((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m);
     auto H = [&](int i, int j) -> double& { return h[i + (size_t)(m + 1) * j]; };
@@ -578,7 +604,12 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
         int jdone = 0;
         bool brk = false;
         for (int j = 0; j < m; j++) {
-            c->matvec(V + (size_t)j * n, w, prec, tr);
+            if (Pinv) {
+                vec_nodeblock_apply<T, T>(Pinv, V + (size_t)j * n, z, c->neqn, c->n_node, false, tr, c->st);
+                c->matvec(z, w, prec, tr);
+            } else {
+                c->matvec(V + (size_t)j * n, w, prec, tr);
+            }
             if (prec == 64) s->matvecs64++; else s->matvecs32++;
             iters++;
             // CGS2: two classical Gram-Schmidt passes
@@ -627,7 +658,14 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
         std::memcpy(c->h_pin, y.data(), sizeof(double) * jdone);
         HIP_CHECK_THROW(hipMemcpyAsync(c->d_small + 16, c->h_pin, sizeof(double) * jdone, hipMemcpyHostToDevice, c->st));
         c->sync();
-        vec_combine<T>(x, V, n, jdone, c->d_small + 16, n, c->st);
+        if (Pinv) {
+            // x += Pinv (V y), the combination formed in FP64
+            vec_fill(c->d_t64, 0.0, n, c->st);
+            vec_combine<T>(c->d_t64, V, n, jdone, c->d_small + 16, n, c->st);
+            vec_nodeblock_apply<double, T>(Pinv, c->d_t64, x, c->neqn, c->n_node, true, tr, c->st);
+        } else {
+            vec_combine<T>(x, V, n, jdone, c->d_small + 16, n, c->st);
+        }
         s->restarts++;
         if (s->est_resid <= tol_abs) { *conv = true; break; }
         if (brk) break;
@@ -644,9 +682,12 @@ int solve_fp64(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
     const double bnorm = c->norm(c->d_b);
     s->bnorm = bnorm;
     bool conv = false;
+    const bool pc = o->precond == CFDV_PRECOND_NODE_BLOCK;
+    if (pc) c->ensure_precond();
     if (bnorm > 0)
         s->iters += gmres_core<double>(c, 64, o->transpose != 0, c->d_b, c->d_du, o->m, o->restarts, o->rtol * bnorm,
-                                       bnorm, s, &conv, c->d_V, c->d_w);
+                                       bnorm, s, &conv, c->d_V, c->d_w, pc ? c->d_P64 : nullptr,
+                                       pc ? c->d_z64 : nullptr);
     c->matvec(c->d_du, c->d_y, 64, o->transpose != 0);
     vec_sub(c->d_r, c->d_b, c->d_y, n, c->st);
     s->resid = c->norm(c->d_r);
@@ -665,6 +706,8 @@ int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
     const int outer_prec = c->d_R64 ? 64 : 3;   // FP64 residual with FP64 (or widened FP32) blocks
     const bool tr = o->transpose != 0;
     c->ensure_krylov32(o->inner_m);
+    const bool pc = o->precond == CFDV_PRECOND_NODE_BLOCK;
+    if (pc) c->ensure_precond();
     launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
     vec_fill(c->d_du, 0.0, n, c->st);
     const double one = 1.0;
@@ -684,7 +727,8 @@ int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
         // Arnoldi work vector
         float* wbuf = c->d_V32 + (size_t)o->inner_m * n;   // last slot as the Arnoldi work vector
         s->iters += gmres_core<float>(c, 32, tr, c->d_w32, c->d_d64, o->inner_m - 1, o->inner_restarts,
-                                      o->inner_rtol, 1.0, &si, &conv, c->d_V32, wbuf);
+                                      o->inner_rtol, 1.0, &si, &conv, c->d_V32, wbuf, pc ? c->d_P32 : nullptr,
+                                      pc ? c->d_z32 : nullptr);
         s->matvecs32 += si.matvecs32;
         s->matvecs64 += si.matvecs64;
         // du += ||r|| d  (FP64)
@@ -783,6 +827,7 @@ int cfdv_set_state(cfdv_ctx* ctx, const double* U) {
     HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_U, U, sizeof(double) * ctx->ndof, hipMemcpyHostToDevice, ctx->st));
     ctx->sync();
     ctx->built = false;
+    ctx->pinv_valid = false;
     return CFDV_OK;
     GUARD_END
 }
@@ -910,6 +955,10 @@ int cfdv_solve(cfdv_ctx* ctx, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
     if (!ctx || !o || !s) return CFDV_ERR_ARG;
     if (!ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_solve before cfdv_build");
     if (o->m < 1 || o->m > 500 || o->restarts < 1) return fail(ctx, CFDV_ERR_ARG, "bad Krylov dimension/restarts");
+    if (o->precond != CFDV_PRECOND_NONE && o->precond != CFDV_PRECOND_NODE_BLOCK)
+        return fail(ctx, CFDV_ERR_ARG, "unknown preconditioner");
+    if (o->precond != CFDV_PRECOND_NONE && o->mode == CFDV_SOLVE_REFERENCE)
+        return fail(ctx, CFDV_ERR_ARG, "the reference solve is unpreconditioned");
     std::memset(s, 0, sizeof(*s));
     GUARD_BEGIN
     switch (o->mode) {
@@ -945,6 +994,7 @@ int cfdv_update(cfdv_ctx* ctx) {
     ctx->check_launch();
     ctx->sync();
     ctx->built = false;
+    ctx->pinv_valid = false;
     ctx->have_du = false;
     return CFDV_OK;
     GUARD_END

@@ -78,6 +78,14 @@ template <typename T>
 void vec_scale_inv(T* y, const T* x, const double* denom_dev, long n, hipStream_t st);
 template <typename T>
 void vec_combine(double* x, const T* V, long ldv, int nv, const double* y, long n, hipStream_t st);
+// node-block Jacobi: Pinv per node, row-major neqn x neqn, FP64 and FP32
+// copies; *status set to 1 when a node block is singular
+void launch_nodeblock(const int* node_ptr, const int* node_ent, const double* R64, const float* R32, int nnod,
+                      int neqn, long n_node, double* P64, float* P32, int* status, hipStream_t st);
+// z = Pinv v (accumulate: z += Pinv v; trans: Pinv^T)
+template <typename TV, typename TP>
+void vec_nodeblock_apply(const TP* P, const TV* v, TV* z, int neqn, long n_node, bool accumulate, bool trans,
+                         hipStream_t st);
 void vec_sumsq_eqn(const double* U, int neqn, long n_node, double* part, double* out, int nblk, hipStream_t st);
 void min_reduce(const double* part, int n, double* out, hipStream_t st);
 

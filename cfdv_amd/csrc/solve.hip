@@ -338,6 +338,133 @@ void vec_combine(double* x, const T* V, long ldv, int nv, const double* y, long 
 template void vec_combine<double>(double*, const double*, long, int, const double*, long, hipStream_t);
 template void vec_combine<float>(double*, const float*, long, int, const double*, long, hipStream_t);
 
+// ---------------------------------------------------------------------------
+// Node-block Jacobi preconditioner.  P_n = sum over the (element, local node)
+// pairs at node n of the element's diagonal block R_e(l, l) (neqn x neqn),
+// summed in element order; Pinv_n by Gauss-Jordan with partial pivoting in
+// FP64.  One thread per node.  The same node CSR as compose_kernel.
+// ---------------------------------------------------------------------------
+template <typename TB, int NEQ>
+__global__ __launch_bounds__(256) void nodeblock_kernel(const int* __restrict__ node_ptr,
+                                                        const int* __restrict__ node_ent,
+                                                        const TB* __restrict__ R, int nnod, long n_node,
+                                                        double* __restrict__ P64, float* __restrict__ P32,
+                                                        int* __restrict__ status) {
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_node) return;
+    const int k = nnod * NEQ, KK = k * k;
+    double A[NEQ][NEQ], B[NEQ][NEQ];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int c = 0; c < NEQ; c++) {
+            A[r][c] = 0.0;
+            B[r][c] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int p = node_ptr[n]; p < node_ptr[n + 1]; p++) {
+        const long ent = node_ent[p];   // slot * nnod + local node
+        const long e = ent / nnod;
+        const int l = (int)(ent % nnod);
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int c = 0; c < NEQ; c++) {
+                const int q = (l * NEQ + r) * k + (l * NEQ + c);   // R(i, j) at i k + j
+                A[r][c] += (double)R[blk_offset<TB>(e, q, KK)];
+            }
+    }
+    bool singular = false;
+#pragma unroll
+    for (int c = 0; c < NEQ; c++) {
+        // pivot: largest |A[r][c]| for r >= c (rows swapped by value)
+        int piv = c;
+        double best = fabs(A[c][c]);
+#pragma unroll
+        for (int r = c + 1; r < NEQ; r++)
+            if (fabs(A[r][c]) > best) { best = fabs(A[r][c]); piv = r; }
+        if (!(best > 0.0)) singular = true;
+#pragma unroll
+        for (int r = c + 1; r < NEQ; r++)
+            if (r == piv) {
+#pragma unroll
+                for (int t = 0; t < NEQ; t++) {
+                    double a = A[c][t]; A[c][t] = A[r][t]; A[r][t] = a;
+                    double b = B[c][t]; B[c][t] = B[r][t]; B[r][t] = b;
+                }
+            }
+        const double inv = 1.0 / A[c][c];
+#pragma unroll
+        for (int t = 0; t < NEQ; t++) { A[c][t] *= inv; B[c][t] *= inv; }
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+            if (r != c) {
+                const double f = A[r][c];
+#pragma unroll
+                for (int t = 0; t < NEQ; t++) { A[r][t] -= f * A[c][t]; B[r][t] -= f * B[c][t]; }
+            }
+    }
+    if (singular) *status = 1;
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int c = 0; c < NEQ; c++) {
+            const double v = singular ? ((r == c) ? 1.0 : 0.0) : B[r][c];
+            P64[(size_t)n * NEQ * NEQ + r * NEQ + c] = v;
+            P32[(size_t)n * NEQ * NEQ + r * NEQ + c] = (float)v;
+        }
+}
+
+void launch_nodeblock(const int* node_ptr, const int* node_ent, const double* R64, const float* R32, int nnod,
+                      int neqn, long n_node, double* P64, float* P32, int* status, hipStream_t st) {
+    const int nb = (int)((n_node + 255) / 256);
+    if (neqn == 5) {
+        if (R64) hipLaunchKernelGGL((nodeblock_kernel<double, 5>), dim3(nb), dim3(256), 0, st, node_ptr, node_ent, R64, nnod, n_node, P64, P32, status);
+        else hipLaunchKernelGGL((nodeblock_kernel<float, 5>), dim3(nb), dim3(256), 0, st, node_ptr, node_ent, R32, nnod, n_node, P64, P32, status);
+    } else {
+        if (R64) hipLaunchKernelGGL((nodeblock_kernel<double, 4>), dim3(nb), dim3(256), 0, st, node_ptr, node_ent, R64, nnod, n_node, P64, P32, status);
+        else hipLaunchKernelGGL((nodeblock_kernel<float, 4>), dim3(nb), dim3(256), 0, st, node_ptr, node_ent, R32, nnod, n_node, P64, P32, status);
+    }
+}
+
+// z[n] = Pinv_n v[n] (or z[n] += Pinv_n v[n]; Pinv_n^T with trans);
+// arithmetic in the type of z
+template <typename TV, typename TP, int NEQ>
+__global__ __launch_bounds__(256) void nodeblock_apply_kernel(const TP* __restrict__ P, const TV* __restrict__ v,
+                                                              TV* __restrict__ z, long n_node, int accumulate,
+                                                              int trans) {
+    for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < n_node; n += (long)gridDim.x * blockDim.x) {
+        TV x[NEQ];
+#pragma unroll
+        for (int c = 0; c < NEQ; c++) x[c] = v[n * NEQ + c];
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) {
+            TV acc = accumulate ? z[n * NEQ + r] : (TV)0;
+#pragma unroll
+            for (int c = 0; c < NEQ; c++)
+                acc += (TV)P[(size_t)n * NEQ * NEQ + (trans ? c * NEQ + r : r * NEQ + c)] * x[c];
+            z[n * NEQ + r] = acc;
+        }
+    }
+}
+
+template <typename TV, typename TP>
+void vec_nodeblock_apply(const TP* P, const TV* v, TV* z, int neqn, long n_node, bool accumulate, bool trans,
+                         hipStream_t st) {
+    const int nb = grid_for(n_node);
+    if (neqn == 5)
+        hipLaunchKernelGGL((nodeblock_apply_kernel<TV, TP, 5>), dim3(nb), dim3(256), 0, st, P, v, z, n_node,
+                           (int)accumulate, (int)trans);
+    else
+        hipLaunchKernelGGL((nodeblock_apply_kernel<TV, TP, 4>), dim3(nb), dim3(256), 0, st, P, v, z, n_node,
+                           (int)accumulate, (int)trans);
+}
+template void vec_nodeblock_apply<double, double>(const double*, const double*, double*, int, long, bool, bool,
+                                                  hipStream_t);
+template void vec_nodeblock_apply<float, float>(const float*, const float*, float*, int, long, bool, bool,
+                                                hipStream_t);
+template void vec_nodeblock_apply<double, float>(const float*, const double*, double*, int, long, bool, bool,
+                                                 hipStream_t);
+
 // per-equation sums of squares of a node array (MPI_RMSError.h:19-26)
 __global__ void sumsq_eqn_kernel(const double* __restrict__ U, int neqn, long n_node, double* __restrict__ part) {
     __shared__ double sm[5][256];

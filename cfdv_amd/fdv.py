@@ -25,6 +25,9 @@ from . import _lib
 from .mesh import Mesh, Thermo
 
 
+
+_PRECOND = {"none": _lib.CFDV_PRECOND_NONE, "nodeblock": _lib.CFDV_PRECOND_NODE_BLOCK}
+
 class CFDVError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"cfdv error {code}: {msg}")
@@ -128,11 +131,13 @@ class FDVContext:
         return b
 
     def solve(self, mode="reference", restarts=1, m=20, rtol=1e-8, inner_m=30,
-              inner_rtol=1e-3, inner_restarts=4, transpose=False, check=True):
+              inner_rtol=1e-3, inner_restarts=4, transpose=False, check=True, precond="none"):
+        """``precond``: "none" or "nodeblock" (node-block Jacobi, right-applied;
+        FP64 and mixed modes)."""
         md = {"reference": _lib.CFDV_SOLVE_REFERENCE, "fp64": _lib.CFDV_SOLVE_FP64,
               "mixed": _lib.CFDV_SOLVE_MIXED}[mode]
         o = _lib.SolveOpts(md, restarts, m, rtol, inner_m, inner_rtol, inner_restarts,
-                           int(transpose))
+                           int(transpose), _PRECOND[precond])
         st = _lib.SolveStats()
         rc = self.L.cfdv_solve(self.h, C.byref(o), C.byref(st))
         if check:
@@ -157,7 +162,7 @@ class FDVContext:
               "mixed": _lib.CFDV_SOLVE_MIXED}[mode]
         o = _lib.SolveOpts(md, kw.get("restarts", 1), kw.get("m", 20), kw.get("rtol", 1e-8),
                            kw.get("inner_m", 30), kw.get("inner_rtol", 1e-3),
-                           kw.get("inner_restarts", 4), 0)
+                           kw.get("inner_restarts", 4), 0, _PRECOND[kw.get("precond", "none")])
         st = _lib.StepStats()
         self._check(self.L.cfdv_step(self.h, cfl, C.byref(o), C.byref(st)))
         return st

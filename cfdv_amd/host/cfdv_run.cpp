@@ -1,0 +1,267 @@
+// cfdv_run: the reference's time loop (main.cpp:162-206) on one MI355X,
+// driven through the C++ adapter over the C ABI.
+//
+//   cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed]
+//            [--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE]
+//            [--flatten FILE]
+//
+// Reads a reference binary deck (control, aero, binarynodes, binaryelements;
+// LoadBinary.h:156-245) into element/node objects shaped like the
+// reference's (Element::node / face, Node::number / x / U / U0 / dirichlet),
+// sets the Dirichlet flags as add_face does (LoadBinary.h:255-318), then per
+// iteration: CFL, FDVGalerkin + ApplyBC, GMRES iterate + update, and appends
+// the RMS line to <out>/RMSError.txt in MPI_RMSError's format
+// (MPI_RMSError.h:55-70: "iter\t" then each Urms followed by a space).
+// --dump writes the final node U as raw doubles [n_node][neqn].
+// --flatten writes the arrays handed to cfdv_create (conn int32, bc int32,
+// x double, dirichlet uint8, U double, in that order) and exits without
+// touching the device.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "cfdv_reference_adapter.hpp"
+
+namespace {
+
+// Tensor<T,1>-like storage with the reference's call syntax
+template <class T>
+struct Vec {
+    std::vector<T> d;
+    explicit Vec(size_t n = 0) : d(n, T()) {}
+    T& operator()(int i) { return d[(size_t)i]; }
+    T operator()(int i) const { return d[(size_t)i]; }
+};
+struct BVec {
+    std::vector<unsigned char> d;
+    explicit BVec(size_t n = 0) : d(n, 0) {}
+    void set(int i) { d[(size_t)i] = 1; }
+    bool operator()(int i) const { return d[(size_t)i] != 0; }
+};
+
+struct Node {
+    Node(int ndim, int neqn) : x(ndim), U(neqn), U0(neqn), dirichlet(neqn) {}
+    unsigned int number = 0;
+    Vec<double> x, U, U0;
+    BVec dirichlet;
+};
+struct Face {
+    int face = 0;
+    int bc = 0;
+};
+struct Element {
+    std::vector<Node*> node;
+    std::vector<Face*> face;
+};
+
+// face -> local node ids (get_face.h:80-165)
+const int FACE3[6][4] = {{3, 2, 1, 0}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {6, 7, 4, 5}};
+const int FACE2[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+
+std::map<std::string, double> read_keys(const std::string& path) {
+    std::map<std::string, double> k;
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("cannot open " + path);
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '!' || line[0] == '#') continue;
+        size_t eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        std::string key = line.substr(0, eq), val = line.substr(eq + 1);
+        key.erase(key.find_last_not_of(" \t") + 1);
+        key.erase(0, key.find_first_not_of(" \t"));
+        k[key] = std::atof(val.c_str());
+    }
+    return k;
+}
+
+struct Deck {
+    int ndim = 0, nnod = 0, neqn = 0;
+    std::map<std::string, double> control, aero;
+    cfdv_thermo th{};
+    std::vector<std::unique_ptr<Node>> node_store;
+    std::vector<std::unique_ptr<Element>> elem_store;
+    std::vector<std::unique_ptr<Face>> face_store;
+    std::vector<Node*> nodes;
+    std::vector<Element*> elements;
+};
+
+Deck load(const std::string& dir) {
+    Deck d;
+    d.control = read_keys(dir + "/control");
+    d.aero = read_keys(dir + "/aero");
+    d.ndim = (int)d.control.at("ndim");
+    d.nnod = (int)d.control.at("nnod");
+    d.neqn = (int)d.control.at("neqn");
+    d.th.gamma = d.aero.at("gamma");
+    d.th.mach = d.aero.at("M");
+    d.th.reynolds = d.aero.at("Re");
+    d.th.prandtl = d.aero.at("Pr");
+    d.th.tinf = d.aero.at("Tinf");
+    d.th.twall = d.aero.at("Twall");
+    d.th.adiabatic = d.aero.at("Adiabatic") != 0;
+    const int ndim = d.ndim, nnod = d.nnod, neqn = d.neqn, nface = ndim == 3 ? 6 : 4;
+    // Thermo::set (Thermo.h:22-33): Cv = 1 / (gamma (gamma-1) M^2)
+    const double g = d.th.gamma, M = d.th.mach;
+    const double Cv = 1.0 / (g * (g - 1.0) * M * M);
+
+    std::ifstream nin(dir + "/binarynodes", std::ios::binary);
+    std::ifstream ein(dir + "/binaryelements", std::ios::binary);
+    if (!nin || !ein) throw std::runtime_error("missing binary deck files in " + dir);
+    const long nn = (long)d.control.at("nodes"), ne = (long)d.control.at("elements");
+    std::vector<double> rec(2 * ndim + 2);
+    for (long i = 0; i < nn; i++) {
+        nin.read((char*)rec.data(), (std::streamsize)(rec.size() * sizeof(double)));
+        if (!nin) throw std::runtime_error("short binarynodes");
+        auto n = std::make_unique<Node>(ndim, neqn);
+        n->number = (unsigned)i;
+        for (int j = 0; j < ndim; j++) n->x(j) = rec[j];
+        // primitive -> conservative (LoadBinary.h:179-196)
+        const double rho = rec[ndim];
+        n->U(0) = rho;
+        for (int j = 0; j < ndim; j++) n->U(1 + j) = rec[ndim + 1 + j];
+        n->U(ndim + 1) = rec[2 * ndim + 1] * Cv;
+        double vs = 0;
+        for (int j = 0; j < ndim; j++) {
+            vs += n->U(j + 1) * n->U(j + 1);
+            n->U(j + 1) *= n->U(0);
+        }
+        n->U(ndim + 1) += 0.5 * vs;
+        n->U(ndim + 1) *= n->U(0);
+        d.nodes.push_back(n.get());
+        d.node_store.push_back(std::move(n));
+    }
+    std::vector<int32_t> erec(nnod + nface);
+    for (long i = 0; i < ne; i++) {
+        ein.read((char*)erec.data(), (std::streamsize)(erec.size() * sizeof(int32_t)));
+        if (!ein) throw std::runtime_error("short binaryelements");
+        auto e = std::make_unique<Element>();
+        for (int j = 0; j < nnod; j++) {
+            if (erec[j] < 0 || erec[j] >= nn) throw std::runtime_error("node id out of range");
+            e->node.push_back(d.nodes[(size_t)erec[j]]);
+        }
+        for (int f = 0; f < nface; f++) {
+            const int bc = erec[nnod + f];
+            if (bc == 0) continue;
+            auto fc = std::make_unique<Face>();
+            fc->face = f;
+            fc->bc = bc;
+            e->face.push_back(fc.get());
+            d.face_store.push_back(std::move(fc));
+            // Dirichlet flags of add_face (LoadBinary.h:276-316)
+            const int nb = ndim == 3 ? 4 : 2;
+            for (int i = 0; i < nb; i++) {
+                Node* nd = e->node[(size_t)(ndim == 3 ? FACE3[f][i] : FACE2[f][i])];
+                if (bc == -1 || bc == -12) {
+                    for (int q = 0; q < neqn; q++) nd->dirichlet.set(q);
+                } else if (bc == 2 || bc == 9) {
+                    nd->dirichlet.set(ndim);
+                } else if (bc == 22) {
+                    nd->dirichlet.set(2);
+                } else if (bc == 1 || bc == 11 || bc == 21 || bc == 31 || bc == 41 || bc == 4) {
+                    for (int q = 0; q < ndim; q++) nd->dirichlet.set(1 + q);
+                    if (!d.th.adiabatic) nd->dirichlet.set(1 + ndim);
+                }
+            }
+        }
+        d.elements.push_back(e.get());
+        d.elem_store.push_back(std::move(e));
+    }
+    return d;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed] "
+                             "[--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE] [--flatten FILE]\n");
+        return 2;
+    }
+    std::string dir = argv[1], out = ".", dump, mode = "reference", flat, precond = "none";
+    int iters = -1;
+    double rtol = 1e-8;
+    for (int i = 2; i + 1 < argc; i += 2) {
+        std::string a = argv[i], v = argv[i + 1];
+        if (a == "--iters") iters = std::atoi(v.c_str());
+        else if (a == "--mode") mode = v;
+        else if (a == "--rtol") rtol = std::atof(v.c_str());
+        else if (a == "--out") out = v;
+        else if (a == "--dump") dump = v;
+        else if (a == "--flatten") flat = v;
+        else if (a == "--precond") precond = v;
+        else {
+            std::fprintf(stderr, "unknown option %s\n", a.c_str());
+            return 2;
+        }
+    }
+    try {
+        Deck d = load(dir);
+        if (!flat.empty()) {
+            cfdv::MeshArrays a = cfdv::flatten(d.elements, d.nodes, d.ndim, d.nnod, d.neqn);
+            std::ofstream f(flat, std::ios::binary);
+            f.write((const char*)a.conn.data(), (std::streamsize)(a.conn.size() * sizeof(int32_t)));
+            f.write((const char*)a.bc.data(), (std::streamsize)(a.bc.size() * sizeof(int32_t)));
+            f.write((const char*)a.x.data(), (std::streamsize)(a.x.size() * sizeof(double)));
+            f.write((const char*)a.dirichlet.data(), (std::streamsize)a.dirichlet.size());
+            for (const Node* n : d.nodes) f.write((const char*)n->U.d.data(), (std::streamsize)(d.neqn * sizeof(double)));
+            return f ? 0 : 1;
+        }
+        const double cfl = d.control.at("cfl");
+        const int restart = (int)d.control.at("gmresrestart");
+        const int m = (int)d.control.at("gmresiter");
+        int iter = (int)d.control.count("iter") ? (int)d.control.at("iter") : 0;
+        const int itermax = iters >= 0 ? iter + iters : (int)d.control.at("itermax");
+        const int md = mode == "fp64" ? CFDV_SOLVE_FP64 : mode == "mixed" ? CFDV_SOLVE_MIXED : CFDV_SOLVE_REFERENCE;
+        const int blocks = md == CFDV_SOLVE_MIXED ? (CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32) : CFDV_BLOCKS_FP64;
+        std::printf("nodes/eles: %zu %zu\n", d.nodes.size(), d.elements.size());
+        cfdv::Context gpu(d.elements, d.nodes, d.ndim, d.nnod, d.neqn, d.th, blocks);
+        cfdv::HipGMRES<Element, Node> gmres(gpu, restart, m, md, rtol);
+        if (md != CFDV_SOLVE_REFERENCE) gmres.options().restarts = std::max(restart, 200);
+        if (precond == "nodeblock") gmres.options().precond = CFDV_PRECOND_NODE_BLOCK;
+        std::ofstream rms(out + "/RMSError.txt", std::ios::app);
+        double t = 0;
+        for (; iter < itermax; iter++) {
+            gpu.set_state(d.nodes);
+            const double dt = gpu.cfl(cfl);
+            cfdv::FDVGalerkin fdv(gpu, dt);
+            fdv();
+            gmres.iterate(d.elements);
+            gmres.update(d.nodes);
+            // MPI_RMSError (MPI_RMSError.h:19-74), from the updated nodes
+            std::vector<double> s(d.neqn, 0.0), s0(d.neqn, 0.0);
+            for (const Node* n : d.nodes)
+                for (int j = 0; j < d.neqn; j++) {
+                    s[j] += n->U(j) * n->U(j);
+                    s0[j] += n->U0(j) * n->U0(j);
+                }
+            rms << iter << "\t";
+            for (int j = 0; j < d.neqn; j++) rms << (s0[j] == 0 ? 1.0 : std::sqrt(s[j] / s0[j]) - 1.0) << " ";
+            rms << "\n";
+            const cfdv_solve_stats& st = gmres.stats();
+            std::printf(" t = %g, iteration %d dt: %g gmres iters %d rel resid %.3e\n", t, iter, dt, st.iters,
+                        st.rel_resid);
+            std::fflush(stdout);
+            t += dt;
+        }
+        if (!dump.empty()) {
+            std::ofstream f(dump, std::ios::binary);
+            for (const Node* n : d.nodes) f.write((const char*)n->U.d.data(), (std::streamsize)(d.neqn * sizeof(double)));
+        }
+    } catch (const cfdv::Error& e) {
+        std::fprintf(stderr, "cfdv error %d: %s\n", e.code(), e.what());
+        return 1;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

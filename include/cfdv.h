@@ -103,7 +103,15 @@ typedef struct {
     double inner_rtol; /* mixed: inner relative residual target                  */
     int inner_restarts;/* mixed: max inner restarts per sweep                    */
     int transpose;     /* solve K^T du = b (adjoint); FP64 and mixed modes only   */
+    int precond;       /* FP64 and mixed modes: CFDV_PRECOND_NONE or
+                          CFDV_PRECOND_NODE_BLOCK (right preconditioner; the
+                          stopping test stays on the true residual)            */
 } cfdv_solve_opts;
+
+#define CFDV_PRECOND_NONE 0
+/* node-block Jacobi: P_n = sum over the elements at node n of their
+   neqn x neqn diagonal block R_e(n, n); applied as K P^-1 z = b, du = P^-1 z */
+#define CFDV_PRECOND_NODE_BLOCK 1
 
 typedef struct {
     double bnorm;        /* ||b||                                                 */

@@ -15,32 +15,42 @@
  *  - pow(s, 0.2), pow(x, 0.5), pow(area, 1/ndim), cosh, sinh, cos call libm
  *    as the reference does.
  *
- * Pinning (what checks each part, tests/test_oracle_*.py):
+ * Pinning (what checks each part; tests/test_oracle_*.py and
+ * tests/test_golden_decks.py):
  *  orc_gauss                 closed form (+-1/sqrt3, w = 1)
- *  orc_shape*, orc_jacobian* partition of unity, affine exactness
+ *  orc_shape*, orc_jacobian* partition of unity, nodal interpolation,
+ *                            affine exactness (det J, dPhi/dx)
  *  orc_tfnorm3d              axis-aligned closed form + the Lambda/Mu
  *                            renormalisation quirk on a skewed face
  *  orc_thermo_set/mu/k       closed forms
  *  orc_node_unpack           round trip with the deck's primitive map
- *  orc_calc_length, orc_cfl  closed forms on uniform meshes
+ *  orc_calc_length, orc_cfl  closed forms on boxes, ramp volume, literal 1.4
  *  orc_fdvparam              special cases + hand-computed r
  *  orc_tau                   symmetry, trace closed form
  *  orc_calc_a                Euler homogeneity A_i U = F_i(U) + central FD
- *  orc_calc_b, orc_calc_c    central FD of the viscous flux G(U, grad U)
+ *  orc_calc_b, orc_calc_c    central FD of the textbook viscous flux
  *  orc_calc_G                closed form
- *  orc_eval2dboundary        per-bc closed-form edits
- *  orc_build_element         column-sum identity, uniform-state closed form
- *                            (mass + dt^2/4 K_ij a_i a_j), axis-permutation
- *                            covariance, conservation of the domain rhs
- *  orc_apply_bc              row structure
- *  orc_matvec*, orc_compose  dense assembled K, <Kx,y> = <x,K^T y>
- *  orc_gmres_true            dense direct solve
- *  orc_gmres_quirk           m = 1 equals true GMRES; y = g/diag property (Q2)
- *  orc_color_greedy          validity + brute force on tiny meshes
- *  orc_refine_flags          pure function of orc_fdvparam
- *  orc_eval1dboundary and the 2D numerical-diffusion branch are pinned only
- *  through the 2D build checks (column-sum identity); their bc-specific
- *  edits are "parity unpinned" beyond that.
+ *  orc_eval2dboundary        per-code edits: which gradients/stress columns
+ *                            vanish, textbook stress where recomputed, the
+ *                            7 -> 77 fall-through, the unnormalised ramp
+ *                            projection of code 4, error on unknown codes
+ *  orc_build_element (3D)    uniform-state closed form (mass + dt^2/4
+ *                            K_ij a_i a_j) and its face term, column-sum
+ *                            identity, rhs conservation, rotation covariance
+ *  orc_build_element (2D)    column-sum identity and rhs conservation with
+ *                            the numerical-diffusion term active
+ *  orc_apply_bc              row structure against the Dirichlet rules
+ *  orc_matvec*, orc_compose  dense assembled K
+ *  orc_gmres_true            dense direct solve (K and K^T)
+ *  orc_gmres_quirk           m = 1 closed form; one cycle equals
+ *                            y_j = g_j / R_jj of a textbook Arnoldi + QR
+ *  orc_color_greedy*         brute-force first fit, validity
+ *  orc_refine_flags          s1 > 0.8 of orc_fdvparam
+ *  orc_update, orc_rms       closed forms
+ * parity unpinned: orc_eval1dboundary (2D face edits) and the 2D
+ *  discontinuity-capturing coefficients are reached only through the 2D
+ *  column-sum/conservation invariants, which they leave unchanged; their
+ *  values follow the source by reading only.
  */
 #include "cfdv_oracle.h"
 

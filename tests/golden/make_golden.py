@@ -12,7 +12,13 @@
    elements (every bc code, interior elements) and the Frobenius norm of
    every block, the element rhs, dt at the deck's CFL, the composed b, and
    two reference-mode GMRES steps with the deck's gmresiter/gmresrestart
-   (du, U after each step, RMS rows).
+   (du, U after each step, RMS rows).  For each step it also stores how far
+   du moves when the element blocks and/or the element rhs are perturbed by
+   1e-16 relative noise: the reference's back substitution (quirk:
+   transposed read) can amplify rounding enormously, and the assembled b
+   cancels heavily where the flow is uniform, so a parity tolerance below
+   that spread would test summation order, not the algorithm.  Finally the textbook-GMRES solution
+   of the first step's system to a 1e-12 relative residual.
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -73,9 +79,20 @@ def make(name):
     R, rhs, _ = O.build(th, mesh, U, dt)
     sel = select_elements(mesh)
     b = O.compose_rhs(mesh, rhs)
-    steps_U, steps_du, steps_rms, steps_dt, steps_it = [], [], [], [], []
+    steps_U, steps_du, steps_rms, steps_dt, steps_it, steps_sens = [], [], [], [], [], []
     Uc = U.copy()
+    rng = np.random.default_rng(20261203)
     for _ in range(2):
+        dtc = O.cfl(th, mesh, Uc, cfl)
+        Rc, rhsc, _ = O.build(th, mesh, Uc, dtc)
+        du_a, _ = O.gmres_quirk(mesh, Rc, rhsc, restart, mdim)
+        spread = 0.0
+        for noise_R, noise_rhs in ((1e-16, 0.0), (0.0, 1e-16), (1e-16, 1e-16)):
+            Rp = Rc * (1 + noise_R * rng.standard_normal(Rc.shape))
+            rp = rhsc * (1 + noise_rhs * rng.standard_normal(rhsc.shape))
+            du_b, _ = O.gmres_quirk(mesh, Rp, rp, restart, mdim)
+            spread = max(spread, np.linalg.norm(du_b - du_a) / np.linalg.norm(du_a))
+        steps_sens.append(spread)
         U1, U0, st = O.step(th, mesh, Uc, cfl=cfl, restart=restart, mdim=mdim, mode=0)
         steps_U.append(U1)
         steps_du.append((U1 - U0).ravel())
@@ -83,15 +100,17 @@ def make(name):
         steps_dt.append(st.dt)
         steps_it.append(st.gm.iters)
         Uc = U1
+    du_true, gst = O.gmres_true(mesh, R, b, 30, 400, 1e-12)
     out = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(
         out, cfl=cfl, mdim=mdim, restart=restart, dt=dt, s=s, ties=ties,
         sel=sel, R_sel=R[sel], R_fro=np.linalg.norm(R.reshape(len(R), -1), axis=1),
         rhs=rhs, b=b, U0=U, U_steps=np.array(steps_U), du_steps=np.array(steps_du),
         rms_steps=np.array(steps_rms), dt_steps=np.array(steps_dt),
-        iters_steps=np.array(steps_it))
+        iters_steps=np.array(steps_it), sens_steps=np.array(steps_sens), du_true=du_true)
     print(f"{name}: {mesh.n_elem} elements, {mesh.n_node} nodes, dt {dt:.6e}, "
-          f"{len(sel)} selected blocks, ties {ties}, -> {os.path.getsize(out)} bytes")
+          f"{len(sel)} selected blocks, ties {ties}, step sensitivity {steps_sens}, "
+          f"true GMRES iters {gst.iters}, -> {os.path.getsize(out)} bytes")
 
 
 def main():

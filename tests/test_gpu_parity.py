@@ -159,3 +159,45 @@ def test_refine_flags_bit_exact(gpu_available, case):
     assert ties == 0
     assert np.array_equal(f, fo)
     assert np.array_equal(s1, so[:, 0])
+
+
+@pytest.mark.parametrize("case", ["box3d", "plate", "lid"])
+def test_node_block_preconditioned_solves(gpu_available, case):
+    """Right node-block Jacobi: same solution (dense solve) and true-residual
+    stopping for FP64, mixed and transpose solves.  (The iteration count is
+    printed, not asserted: on these small, well-scaled boxes it hardly
+    changes; the bench measures it at full size.)"""
+    m, th, U = _case(case)
+    ctx = FDVContext(m, th, U, blocks=("fp64", "fp32"))
+    dt = ctx.cfl(0.5)
+    ctx.build(dt)
+    R, rhs, _ = ctx.blocks()
+    b = O.compose_rhs(m, rhs)
+    n = m.n_node * m.neqn
+    dense = np.zeros((n, n))
+    for e in range(m.n_elem):
+        dofs = (m.conn[e][:, None] * m.neqn + np.arange(m.neqn)).ravel()
+        dense[np.ix_(dofs, dofs)] += R[e]
+    xs = np.linalg.solve(dense, b)
+    xt = np.linalg.solve(dense.T, b)
+    st0, _ = ctx.solve("fp64", restarts=60, m=30, rtol=1e-10)
+    st1, _ = ctx.solve("fp64", restarts=60, m=30, rtol=1e-10, precond="nodeblock")
+    d1 = ctx.du()
+    print(case, "iters plain", st0.iters, "node-block", st1.iters, "rel", st1.rel_resid)
+    assert st1.rel_resid <= 1e-10
+    assert np.linalg.norm(d1 - xs) <= 1e-8 * np.linalg.norm(xs)
+    st2, _ = ctx.solve("mixed", restarts=20, m=30, rtol=1e-10, inner_m=20, inner_rtol=1e-4,
+                       inner_restarts=3, precond="nodeblock")
+    assert st2.rel_resid <= 1e-10 and st2.matvecs32 > 0
+    assert np.linalg.norm(ctx.du() - xs) <= 1e-8 * np.linalg.norm(xs)
+    st3, _ = ctx.solve("fp64", restarts=60, m=30, rtol=1e-10, transpose=True, precond="nodeblock")
+    assert st3.rel_resid <= 1e-10
+    assert np.linalg.norm(ctx.du() - xt) <= 1e-8 * np.linalg.norm(xt)
+
+
+def test_preconditioner_rejected_in_reference_mode(gpu_available):
+    m, th, U = _case("box3d")
+    ctx = FDVContext(m, th, U)
+    ctx.build(ctx.cfl(0.5))
+    _, rc = ctx.solve("reference", restarts=1, m=10, precond="nodeblock", check=False)
+    assert rc == -1

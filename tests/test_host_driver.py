@@ -1,0 +1,85 @@
+"""The C++ host side: the reference-shaped adapter (cfdv_amd/host/
+cfdv_reference_adapter.hpp) and the deck driver cfdv_run built on it.
+
+CPU: the driver's deck loader + the adapter's flattening hand cfdv_create
+exactly the arrays the Python deck reader produces (connectivity, face bc
+codes, coordinates, Dirichlet flags of add_face, conservative U).
+GPU: two reference-mode iterations of the driver on 3dtestramp_new match
+the oracle's golden states, and RMSError.txt has MPI_RMSError's format.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cfdv_amd import build_lib
+from cfdv_amd import mesh as M
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+DECKS = ["3dtestlagrange", "3dtestramp_new", "lid50"]
+
+
+@pytest.fixture(scope="module")
+def runner():
+    if not os.path.exists(build_lib.RUNNER):
+        if not os.path.exists(build_lib.LIB):
+            build_lib.build()
+        else:
+            build_lib.build_runner()
+    return build_lib.RUNNER
+
+
+@pytest.mark.parametrize("name", DECKS)
+def test_driver_flattens_deck_like_the_reader(runner, tmp_path, name):
+    deck = os.path.join(GOLD, "decks", name)
+    out = tmp_path / "flat.bin"
+    subprocess.run([runner, deck, "--flatten", str(out)], check=True, capture_output=True)
+    mesh, th, U, _ = M.read_deck(deck)
+    raw = out.read_bytes()
+    ne, nn, nnod, nf, nd, nq = mesh.n_elem, mesh.n_node, mesh.nnod, mesh.nface, mesh.ndim, mesh.neqn
+    o = 0
+    conn = np.frombuffer(raw, np.int32, ne * nnod, o).reshape(ne, nnod)
+    o += 4 * ne * nnod
+    bc = np.frombuffer(raw, np.int32, ne * nf, o).reshape(ne, nf)
+    o += 4 * ne * nf
+    x = np.frombuffer(raw, np.float64, nn * nd, o).reshape(nn, nd)
+    o += 8 * nn * nd
+    d = np.frombuffer(raw, np.uint8, nn * nq, o).reshape(nn, nq)
+    o += nn * nq
+    Uc = np.frombuffer(raw, np.float64, nn * nq, o).reshape(nn, nq)
+    assert o + 8 * nn * nq == len(raw)
+    assert np.array_equal(conn, mesh.conn)
+    assert np.array_equal(bc, mesh.bc)
+    assert np.array_equal(x, mesh.x)
+    assert np.array_equal(d, mesh.dirichlet)
+    assert np.array_equal(Uc, U)
+
+
+def test_driver_reports_missing_device_or_deck(runner, tmp_path):
+    r = subprocess.run([runner, str(tmp_path / "nope")], capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_two_reference_iterations(gpu_available, runner, tmp_path):
+    name = "3dtestramp_new"
+    g = np.load(os.path.join(GOLD, f"{name}.npz"))
+    dump = tmp_path / "U.bin"
+    r = subprocess.run([runner, os.path.join(GOLD, "decks", name), "--iters", "2",
+                        "--out", str(tmp_path), "--dump", str(dump)],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    U2 = np.fromfile(dump, dtype=np.float64).reshape(g["U0"].shape)
+    ref = g["U_steps"][1]
+    err = np.linalg.norm(U2 - ref) / np.linalg.norm(ref - g["U0"])
+    print("U after 2 iterations, error relative to the change:", err)
+    assert err < 1e-9
+    lines = (tmp_path / "RMSError.txt").read_text().splitlines()
+    assert len(lines) == 2
+    for k, line in enumerate(lines):
+        it, vals = line.split("\t")
+        assert int(it) == k
+        v = np.array([float(t) for t in vals.split()])
+        assert np.allclose(v, g["rms_steps"][k], rtol=1e-5, atol=1e-12)

@@ -206,3 +206,126 @@ def test_apply_bc_rows(adiabatic):
                 assert np.array_equal(R1[e, c], R0[e, c])
                 assert rhs1[e, c] == rhs0[e, c]
     assert n_constrained > 0
+
+
+def test_rotation_covariance_interior_elements():
+    """A cyclic permutation of the axes (a rotation, det +1) applied to node
+    coordinates and velocities permutes the momentum equations and nothing
+    else: R'(Nr', Ms') = R(Nr, Ms), rhs'(Nr') = rhs(Nr).  Any index slip in
+    a_i, b_i, c_ij, G_i or in the geometric factors breaks it."""
+    m, th, U = M.mixed_bc_box(4, 4, 4, seed=9)
+    perm = [1, 2, 0]
+    x2 = np.ascontiguousarray(m.x[:, perm])
+    U2 = U.copy()
+    U2[:, 1:4] = U[:, 1:4][:, perm]
+    m2 = M.Mesh(3, 8, 5, m.conn, x2, np.zeros_like(m.bc), m.dirichlet)
+    m1 = M.Mesh(3, 8, 5, m.conn, m.x, np.zeros_like(m.bc), m.dirichlet)
+    R1, rhs1, s1 = O.build(th, m1, U, 2e-3, apply_bc=False)
+    R2, rhs2, s2 = O.build(th, m2, U2, 2e-3, apply_bc=False)
+    eq = np.array([0, 1 + perm[0], 1 + perm[1], 1 + perm[2], 4])
+    idx = (np.arange(8)[:, None] * 5 + eq[None, :]).ravel()
+    assert np.allclose(s1, s2, rtol=1e-13, atol=1e-15)
+    for e in range(m.n_elem):
+        expect = R1[e][np.ix_(idx, idx)]
+        assert np.allclose(R2[e], expect, rtol=1e-11, atol=1e-13 * np.abs(expect).max())
+        assert np.allclose(rhs2[e], rhs1[e][idx], rtol=1e-10,
+                           atol=1e-12 * np.abs(rhs1[e]).max())
+
+
+def _flow(rng, th):
+    v = rng.standard_normal(3)
+    dV = rng.standard_normal((3, 3))
+    dT = rng.standard_normal(3)
+    Uv = np.concatenate([[1.2], 1.2 * v, [3.0]])
+    f = O.flow_struct(3.0, 1.1, v, np.zeros(3), Uv, rng.standard_normal((5, 3)), dV, dT,
+                      0.7, 0.9)
+    return f
+
+
+def _textbook_tau(mu, dV):
+    return mu * (dV + dV.T) - 2.0 / 3.0 * mu * np.trace(dV) * np.eye(3)
+
+
+@pytest.mark.parametrize("adiabatic", [False, True])
+def test_eval2dboundary_edits(adiabatic):
+    """Face-state edits of eval2dboundary.h:14-371: which gradient components
+    vanish, which stress rows/columns vanish and when the stress is the
+    textbook deviatoric stress of the edited velocity gradient."""
+    import ctypes as C
+    th = M.Thermo(gamma=1.4, mach=2.0, reynolds=100.0, adiabatic=adiabatic)
+    t = O.thermo(th)
+    rng = np.random.default_rng(4)
+    en = np.array([0.0, 0.0, -0.02])
+
+    def run(bc):
+        f = _flow(rng, th)
+        dV0 = np.ctypeslib.as_array(f.dV).copy()
+        dT0 = np.ctypeslib.as_array(f.dT).copy()
+        rc = O.lib().orc_eval2dboundary(C.byref(t), bc, C.byref(f), O._p(en))
+        return rc, f, dV0, dT0
+
+    # symmetry planes: the normal derivative vanishes
+    for bc, d in ((2, 2), (12, 2), (22, 1)):
+        rc, f, dV0, dT0 = run(bc)
+        dV = np.ctypeslib.as_array(f.dV)
+        dT = np.ctypeslib.as_array(f.dT)
+        assert rc == 0
+        assert np.all(dV[:, d] == 0) and dT[d] == 0
+        keep = [j for j in range(3) if j != d]
+        assert np.array_equal(dV[:, keep], dV0[:, keep])
+    # walls: tau is recomputed; the wall-normal dT vanishes only if adiabatic
+    for bc, d in ((1, 2), (51, 2), (21, 1), (31, 1), (11, 0)):
+        rc, f, dV0, dT0 = run(bc)
+        tau = np.ctypeslib.as_array(f.tau)
+        dT = np.ctypeslib.as_array(f.dT)
+        assert np.allclose(tau, _textbook_tau(f.mu, dV0), rtol=1e-14, atol=1e-15)
+        assert (dT[d] == 0) == adiabatic or (not adiabatic and dT[d] == dT0[d])
+    # outflow/top: dT and stress column of the normal direction vanish.
+    # 7 falls through into 77/32 (the source has no break): dT_z and dT_y
+    # vanish, and the stress recomputed in 77 restores column z, so only
+    # column y stays zero
+    for bc, dts, cols in ((6, [0], [0]), (7, [2, 1], [1]), (77, [1], [1]), (32, [1], [1])):
+        rc, f, dV0, dT0 = run(bc)
+        tau = np.ctypeslib.as_array(f.tau)
+        dT = np.ctypeslib.as_array(f.dT)
+        assert all(dT[c] == 0 for c in dts)
+        for c in cols:
+            assert np.all(tau[:, c] == 0)
+        rest = [j for j in range(3) if j not in cols]
+        full = _textbook_tau(f.mu, dV0)
+        assert np.allclose(tau[:, rest], full[:, rest], rtol=1e-14, atol=1e-15)
+    # 8 (exit): stress recomputed, nothing else edited
+    rc, f, dV0, dT0 = run(8)
+    assert np.allclose(np.ctypeslib.as_array(f.tau), _textbook_tau(f.mu, dV0), rtol=1e-14,
+                       atol=1e-15)
+    assert np.array_equal(np.ctypeslib.as_array(f.dT), dT0)
+    # -12 (lid): z-derivatives and w vanish, no stress is formed
+    rc, f, dV0, dT0 = run(-12)
+    assert np.all(np.ctypeslib.as_array(f.dV)[:, 2] == 0) and f.dT[2] == 0 and f.v[2] == 0
+    assert np.all(np.ctypeslib.as_array(f.tau) == 0)
+    # 4 (ramp wall): v = 0 and the gradients of u, v, T and of the first
+    # three equations lose their component along en, which the source uses
+    # unnormalised: with |en| = 1 the projection is exact, otherwise the
+    # normal component is scaled by (1 - |en|^2)
+    for scale in (1.0, 0.3):
+        nrm = np.array([0.6, 0.0, 0.8]) * scale
+        f = _flow(rng, th)
+        dV0 = np.ctypeslib.as_array(f.dV).copy()
+        dT0 = np.ctypeslib.as_array(f.dT).copy()
+        dU0 = np.ctypeslib.as_array(f.dU).copy()
+        assert O.lib().orc_eval2dboundary(C.byref(t), 4, C.byref(f), O._p(nrm)) == 0
+        k = 1 - nrm @ nrm
+        dV = np.ctypeslib.as_array(f.dV)
+        dU = np.ctypeslib.as_array(f.dU)
+        assert np.all(np.ctypeslib.as_array(f.v) == 0)
+        assert np.allclose(dV[:2] @ nrm, k * (dV0[:2] @ nrm), atol=1e-14)
+        assert np.array_equal(dV[2], dV0[2])
+        assert np.allclose(np.ctypeslib.as_array(f.dT) @ nrm, k * (dT0 @ nrm), atol=1e-14)
+        assert np.allclose(dU[:3] @ nrm, k * (dU0[:3] @ nrm), atol=1e-14)
+        assert np.array_equal(dU[3:], dU0[3:])
+    # inflow returns before any stress is formed
+    rc, f, _, _ = run(-1)
+    assert rc == 0 and np.all(np.ctypeslib.as_array(f.tau) == 0)
+    # unknown code is an error
+    rc, _, _, _ = run(99)
+    assert rc == -1
