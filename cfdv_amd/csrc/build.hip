@@ -759,13 +759,271 @@ CFDV_HD void build3d_elem(const BuildArgs& A, const GaussTables& G, const double
 
 }
 
+// ---------------------------------------------------------------------------
+// Interior hex8 elements (no face with a bc code, no constrained dof): the
+// Gauss-point state is the same for all 64 node pairs, so it is evaluated
+// once per point (phase 1: lanes 0..7, one point each) into LDS, and the 64
+// lanes then contract it for their own node pair (phase 2).  Same arithmetic
+// as build3d_elem's domain loop, in the same order.
+// ---------------------------------------------------------------------------
+struct GPShared {
+    double dphi[8][3];   // dPhi_n/dx_j at the point
+    double wJ, SPhi;     // w * det J, sum_n Phi_n
+    double D[5];         // sum_jb dPhi_bj G_b(j, s)
+    double P[3][5];      // P(i, r) for r = 1..4 (r = 0 unused)
+    GPState<3> f;
+};
+
+CFDV_HD void gp_phase1(const DevThermo& th, const GaussTables& G, const double (*nd)[ND_STRIDE], int gp,
+                       double dt, GPShared& o) {
+    constexpr int NEQ = 5;
+    const double reinv = 1.0 / th.creyn;
+    double jac[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int n = 0; n < 8; n++) acc += G.dpxi3[gp][n][j] * nd[n][ND_X + i];
+            jac[i][j] = acc;
+        }
+    const double J = jac[0][0] * jac[1][1] * jac[2][2] + jac[0][1] * jac[1][2] * jac[2][0] +
+                     jac[0][2] * jac[1][0] * jac[2][1] - jac[0][2] * jac[1][1] * jac[2][0] -
+                     jac[0][1] * jac[1][0] * jac[2][2] - jac[0][0] * jac[1][2] * jac[2][1];
+    double ji[3][3];
+    ji[0][0] = jac[1][1] * jac[2][2] - jac[1][2] * jac[2][1];
+    ji[0][1] = -(jac[1][0] * jac[2][2] - jac[1][2] * jac[2][0]);
+    ji[0][2] = jac[1][0] * jac[2][1] - jac[1][1] * jac[2][0];
+    ji[1][0] = -(jac[0][1] * jac[2][2] - jac[0][2] * jac[2][1]);
+    ji[1][1] = jac[0][0] * jac[2][2] - jac[0][2] * jac[2][0];
+    ji[1][2] = -(jac[0][0] * jac[2][1] - jac[0][1] * jac[2][0]);
+    ji[2][0] = jac[0][1] * jac[1][2] - jac[0][2] * jac[1][1];
+    ji[2][1] = -(jac[0][0] * jac[1][2] - jac[0][2] * jac[1][0]);
+    ji[2][2] = jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0];
+    const double Jinv = 1.0 / J;
+    GPState<3> f;
+    f.E = 0; f.T = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        f.v[i] = 0; f.dT[i] = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) f.dV[i][j] = 0;
+    }
+#pragma unroll
+    for (int q = 0; q < NEQ; q++) {
+        f.U[q] = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) f.dU[q][j] = 0;
+    }
+    double SPhi = 0.0, SD[3] = {0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        double dphi[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * G.dpxi3[gp][n][j]);
+            dphi[i] = acc;
+            o.dphi[n][i] = acc;
+        }
+        const double ph = G.phi3[gp][n];
+        SPhi += ph;
+        f.E += ph * nd[n][ND_E];
+        const double Tnn = nd[n][ND_T];
+        f.T += ph * Tnn;
+#pragma unroll
+        for (int q = 0; q < NEQ; q++) {
+            const double u_ = nd[n][ND_U + q];
+            f.U[q] += ph * u_;
+#pragma unroll
+            for (int j = 0; j < 3; j++) f.dU[q][j] += dphi[j] * u_;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double v_ = nd[n][ND_V + i];
+            SD[i] += dphi[i];
+            f.v[i] += ph * v_;
+            f.dT[i] += dphi[i] * Tnn;
+#pragma unroll
+            for (int j = 0; j < 3; j++) f.dV[i][j] += dphi[j] * v_;
+        }
+    }
+    f.mu = calc_mu(th, f.T);
+    f.k = calc_k(th, f.mu);
+    calc_tau<3>(f);
+    o.wJ = G.w3[gp] * J;
+    o.SPhi = SPhi;
+    o.D[0] = 0.0;
+#pragma unroll
+    for (int s = 1; s < 4; s++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) acc += (-f.tau[j][s - 1] * reinv) * SD[j];
+        o.D[s] = acc;
+    }
+    {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double td = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) td += f.tau[j][k] * f.dV[k][j];
+            acc += (-td - f.k * f.dT[j] * SD[j]) * reinv;
+        }
+        o.D[4] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        o.P[i][0] = 0.0;
+#pragma unroll
+        for (int r = 1; r < 5; r++) {
+            double P;
+            if (r < 4) P = dt * (-f.tau[i][r - 1] * reinv) * SPhi;
+            else {
+                double tv = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) tv += f.tau[i][k] * f.v[k];
+                P = dt * ((-tv - f.k * f.dT[i] * SPhi) * reinv);
+            }
+            o.P[i][r] = P;
+        }
+    }
+    o.f = f;
+}
+
+// Phase 2 for lane (N, M): the domain contraction of every Gauss point.
+CFDV_HD void interior_phase2(const DevThermo& th, const GaussTables& G, const GPShared* gps, int lane,
+                             double dt, const double sp[4], double R[5][5], double rhs[5]) {
+    constexpr int NEQ = 5;
+    const int N = lane >> 3, M = lane & 7;
+    const double s1 = sp[0], s2 = sp[1], s3 = sp[2], s4 = sp[3];
+    const double dtsq_2 = (dt * dt) / 2.0;
+#pragma unroll
+    for (int r = 0; r < NEQ; r++) {
+        rhs[r] = 0.0;
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) R[r][s] = 0.0;
+    }
+#pragma unroll 1
+    for (int gp = 0; gp < 8; gp++) {
+        const GPShared& g = gps[gp];
+        const double PN = G.phi3[gp][N], PM = G.phi3[gp][M];
+        const double wJ = g.wJ;
+        double DN[3], DM[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) { DN[i] = g.dphi[N][i]; DM[i] = g.dphi[M][i]; }
+        const GPState<3>& f = g.f;   // read in place (LDS in the kernel)
+        double D[5];
+#pragma unroll
+        for (int s = 0; s < 5; s++) D[s] = g.D[s];
+        double wg[3], wr[3], W[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            wg[i] = -dt * wJ * PM * DN[i];
+            wr[i] = wJ * DN[i];
+#pragma unroll
+            for (int j = 0; j < 3; j++) W[i][j] = -dt * s3 * wJ * DN[i] * DM[j];
+        }
+        double AQ[5];
+        contract3(th, f, R, wJ * PN * PM, wg, DN, DM, dtsq_2 * wJ, wr, D, AQ, s1, s2, s3, s4);
+        accum_c3(th, f, W, R);
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) {
+            double acc = 0.0;
+            if (r > 0) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) acc += wr[i] * g.P[i][r];
+            }
+            rhs[r] += acc - dtsq_2 * AQ[r];
+        }
+    }
+}
+
+template <bool W64, bool W32>
+__global__ __launch_bounds__(256) void build3d_interior_kernel(BuildArgs A) {
+    constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600;
+    __shared__ double s_nd[4][NNOD][ND_STRIDE];
+    __shared__ GPShared s_gp[4][8];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const long e = (long)xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
+    const DevThermo th = A.th;
+    const bool in_range = e < A.n_elem_pad;
+    const long es = in_range ? e : 0;
+    const int* cn = A.conn + es * NNOD;
+    const bool real = in_range && cn[0] >= 0;
+    bool boundary = false;
+    if (real) {
+        const int* bcs = A.bc + es * 6;
+#pragma unroll
+        for (int fc = 0; fc < 6; fc++) boundary |= (bcs[fc] != 0);
+        boundary |= (A.cmask[es] != 0ULL);
+    }
+    const bool work = real && !boundary;
+    double (*nd)[ND_STRIDE] = s_nd[wv];
+    if (work && lane < NNOD) load_node3d(A, cn[lane], nd[lane]);
+    __syncthreads();
+    if (work && lane < 8) gp_phase1(th, c_gauss, nd, lane, A.dt, s_gp[wv][lane]);
+    __syncthreads();
+    if (!in_range || boundary) return;
+    if (!real) {   // padding slot of a color range: zero block
+        if (W64)
+            for (int q = lane; q < KK; q += 64) A.R64[blk_index<double>(e, q, KK)] = 0.0;
+        if (W32)
+            for (int q = lane; q < KK; q += 64) A.R32[blk_index<float>(e, q, KK)] = 0.0f;
+        if (lane < K) A.rhs[e * K + lane] = 0.0;
+        return;
+    }
+    const int N = lane >> 3, M = lane & 7;
+    double sp[4];
+    int tie = 0;
+    {
+        double vn[NNOD][3], Tn[NNOD], rhon[NNOD];
+#pragma unroll
+        for (int n = 0; n < NNOD; n++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) vn[n][i] = nd[n][ND_V + i];
+            Tn[n] = nd[n][ND_T];
+            rhon[n] = nd[n][ND_RHO];
+        }
+        fdvparam<3, 8>(th, vn, Tn, rhon, sp, &tie);
+    }
+    double R[NEQ][NEQ], rhs[NEQ];
+    interior_phase2(th, c_gauss, s_gp[wv], lane, A.dt, sp, R, rhs);
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) {
+            const int q = (NEQ * N + r) * K + NEQ * M + s;
+            if (W64) A.R64[blk_index<double>(e, q, KK)] = R[r][s];
+            if (W32) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
+        }
+    if (M == 0) {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
+    }
+    if (lane == 0) {
+        if (A.svals) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) A.svals[e * 4 + i] = sp[i];
+        }
+        if (tie) atomicAdd(A.ties, 1);
+    }
+}
+
 template <bool W64, bool W32>
 __global__ __launch_bounds__(256) void build3d_kernel(BuildArgs A) {
     constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600;
     __shared__ double s_nd[4][NNOD][ND_STRIDE];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const long e = (long)xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
+    // with a slot list (elements with faces or constrained dofs), entry i of
+    // the list; otherwise every slot
+    const long i_ = (long)blockIdx.x * 4 + wv;
+    const long e = A.bslots ? (i_ < A.n_bslots ? (long)A.bslots[i_] : A.n_elem_pad)
+                            : (long)xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
     const DevThermo th = A.th;
     const bool in_range = e < A.n_elem_pad;
     const int* cn = A.conn + (in_range ? e : 0) * NNOD;
@@ -1204,10 +1462,18 @@ __global__ __launch_bounds__(256) void build2d_kernel(BuildArgs A) {
 
 void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st) {
     if (a.ndim == 3) {
+        // interior elements (and padding) with the shared Gauss-point table
         const int nb = (int)((a.n_elem_pad + 3) / 4);
-        if (w64 && w32) hipLaunchKernelGGL((build3d_kernel<true, true>), dim3(nb), dim3(256), 0, st, a);
-        else if (w64) hipLaunchKernelGGL((build3d_kernel<true, false>), dim3(nb), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3(nb), dim3(256), 0, st, a);
+        if (w64 && w32) hipLaunchKernelGGL((build3d_interior_kernel<true, true>), dim3(nb), dim3(256), 0, st, a);
+        else if (w64) hipLaunchKernelGGL((build3d_interior_kernel<true, false>), dim3(nb), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((build3d_interior_kernel<false, true>), dim3(nb), dim3(256), 0, st, a);
+        // elements with faces or constrained dofs: the full per-lane kernel
+        if (a.n_bslots > 0) {
+            const int nbb = (int)((a.n_bslots + 3) / 4);
+            if (w64 && w32) hipLaunchKernelGGL((build3d_kernel<true, true>), dim3(nbb), dim3(256), 0, st, a);
+            else if (w64) hipLaunchKernelGGL((build3d_kernel<true, false>), dim3(nbb), dim3(256), 0, st, a);
+            else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3(nbb), dim3(256), 0, st, a);
+        }
     } else {
         const int nb = (int)((a.n_elem_pad * 16 + 255) / 256);
         if (w64 && w32) hipLaunchKernelGGL((build2d_kernel<true, true>), dim3(nb), dim3(256), 0, st, a);

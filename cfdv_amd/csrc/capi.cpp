@@ -130,6 +130,8 @@ struct cfdv_ctx {
 
     int *d_conn = nullptr, *d_bc = nullptr, *d_perm_inv = nullptr;
     unsigned long long* d_cmask = nullptr;
+    int* d_bslots = nullptr;        // 3D slots with faces / constrained dofs
+    long n_bslots = 0;
     double *d_x = nullptr, *d_U = nullptr, *d_U0 = nullptr, *d_area = nullptr, *d_cleng = nullptr;
     double *d_R64 = nullptr;
     float* d_R32 = nullptr;
@@ -397,6 +399,17 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
         }
         cmask[s] = mk;
     }
+    // 3D slots built by the full per-lane kernel: any face code or
+    // constrained dof (the interior kernel skips exactly these)
+    std::vector<int> bslots;
+    if (c->ndim == 3)
+        for (long s = 0; s < np; s++) {
+            if (c->perm[s] < 0) continue;
+            bool b = cmask[s] != 0ULL;
+            for (int f = 0; f < nf; f++) b = b || bc_s[s * nf + f] != 0;
+            if (b) bslots.push_back((int)s);
+        }
+    c->n_bslots = (long)bslots.size();
     // compose CSR entries point at slots; lists stay in original element order
     for (size_t i = 0; i < ent.size(); i++) {
         const int e = ent[i] / nn, l = ent[i] % nn;
@@ -416,6 +429,10 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
     HIP_CHECK_THROW(hipMemcpy(c->d_conn, conn_s.data(), sizeof(int) * np * nn, hipMemcpyHostToDevice));
     HIP_CHECK_THROW(hipMemcpy(c->d_bc, bc_s.data(), sizeof(int) * np * nf, hipMemcpyHostToDevice));
     HIP_CHECK_THROW(hipMemcpy(c->d_cmask, cmask.data(), sizeof(unsigned long long) * np, hipMemcpyHostToDevice));
+    if (c->n_bslots > 0) {
+        c->d_bslots = c->alloc<int>((size_t)c->n_bslots);
+        HIP_CHECK_THROW(hipMemcpy(c->d_bslots, bslots.data(), sizeof(int) * c->n_bslots, hipMemcpyHostToDevice));
+    }
     HIP_CHECK_THROW(hipMemcpy(c->d_x, m->x, sizeof(double) * nnode * c->ndim, hipMemcpyHostToDevice));
     HIP_CHECK_THROW(hipMemcpy(c->d_node_ptr, ptr.data(), sizeof(int) * (nnode + 1), hipMemcpyHostToDevice));
     HIP_CHECK_THROW(hipMemcpy(c->d_node_ent, ent.data(), sizeof(int) * ne * nn, hipMemcpyHostToDevice));
@@ -878,6 +895,8 @@ int cfdv_build(cfdv_ctx* ctx, double dt) {
     a.svals = ctx->d_s;
     a.ties = ctx->d_counters;
     a.status = ctx->d_counters + 1;
+    a.bslots = ctx->d_bslots;
+    a.n_bslots = ctx->n_bslots;
     launch_build(a, ctx->d_R64 != nullptr, ctx->d_R32 != nullptr, ctx->st);
     ctx->check_launch();
     int cnt[2];

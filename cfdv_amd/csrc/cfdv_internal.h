@@ -33,6 +33,8 @@ struct BuildArgs {
     double* svals;            // [n_elem_pad][4] or nullptr
     int* ties;                // count of FDVParam r == 0.01 ties
     int* status;              // bit 0: unknown bc code, bit 1: bad face detjac
+    const int* bslots;        // 3D: slots with faces or constrained dofs (built
+    long n_bslots;            //     by the full per-lane kernel), ascending
 };
 
 void upload_gauss_tables(const GaussTables& g);
