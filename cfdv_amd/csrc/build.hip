@@ -385,175 +385,17 @@ CFDV_HD void contract3(const DevThermo& th, const GPState<3>& f, double R[5][5],
         }
 }
 
-// One hex8 element seen from lane (N, M) = (lane/8, lane%8): FDVParam,
-// the 8-point domain integral, the face integrals and ApplyBC for the 5x5
-// sub-block R(5N+r, 5M+s) and (for M == 0) rhs(5N+r).  nd holds the element's
-// node data (x, U, v, E, T, rho per node).
-CFDV_HD void build3d_elem(const BuildArgs& A, const GaussTables& G, const double (*nd)[ND_STRIDE], long e,
-                          int lane, double R[5][5], double rhs[5], double sp[4], int* tie, int* status) {
-    constexpr int NNOD = 8, NEQ = 5;
+// Face integrals (FDVGalerkin.h:455-533, 643-756) of every face with
+// 0 < bc < 100, added for lane (N, M) to R and (for M == 0) rhs.
+CFDV_HD void build3d_faces(const BuildArgs& A, const GaussTables& G, const double (*nd)[ND_STRIDE], long e,
+                           int lane, const double sp[4], double R[5][5], double rhs[5], int* status) {
+    constexpr int NEQ = 5;
     const DevThermo th = A.th;
     const int N = lane >> 3, M = lane & 7;
-    {
-        double vn[NNOD][3], Tn[NNOD], rhon[NNOD];
-#pragma unroll
-        for (int n = 0; n < NNOD; n++) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) vn[n][i] = nd[n][ND_V + i];
-            Tn[n] = nd[n][ND_T];
-            rhon[n] = nd[n][ND_RHO];
-        }
-        fdvparam<3, 8>(th, vn, Tn, rhon, sp, tie);
-    }
     const double s1 = sp[0], s2 = sp[1], s3 = sp[2], s4 = sp[3];
     const double dt = A.dt, dtsq_2 = (dt * dt) / 2.0;
     const double reinv = 1.0 / th.creyn;
-
-#pragma unroll
-    for (int r = 0; r < NEQ; r++) {
-        rhs[r] = 0.0;
-#pragma unroll
-        for (int s = 0; s < NEQ; s++) R[r][s] = 0.0;
-    }
-
-    // ---------------- domain integral: 8 Gauss points ----------------
-    for (int gp = 0; gp < 8; gp++) {
-        double jac[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                double acc = 0.0;
-#pragma unroll
-                for (int n = 0; n < 8; n++) acc += G.dpxi3[gp][n][j] * nd[n][ND_X + i];
-                jac[i][j] = acc;
-            }
-        const double J = jac[0][0] * jac[1][1] * jac[2][2] + jac[0][1] * jac[1][2] * jac[2][0] +
-                         jac[0][2] * jac[1][0] * jac[2][1] - jac[0][2] * jac[1][1] * jac[2][0] -
-                         jac[0][1] * jac[1][0] * jac[2][2] - jac[0][0] * jac[1][2] * jac[2][1];
-        double ji[3][3];
-        ji[0][0] = jac[1][1] * jac[2][2] - jac[1][2] * jac[2][1];
-        ji[0][1] = -(jac[1][0] * jac[2][2] - jac[1][2] * jac[2][0]);
-        ji[0][2] = jac[1][0] * jac[2][1] - jac[1][1] * jac[2][0];
-        ji[1][0] = -(jac[0][1] * jac[2][2] - jac[0][2] * jac[2][1]);
-        ji[1][1] = jac[0][0] * jac[2][2] - jac[0][2] * jac[2][0];
-        ji[1][2] = -(jac[0][0] * jac[2][1] - jac[0][1] * jac[2][0]);
-        ji[2][0] = jac[0][1] * jac[1][2] - jac[0][2] * jac[1][1];
-        ji[2][1] = -(jac[0][0] * jac[1][2] - jac[0][2] * jac[1][0]);
-        ji[2][2] = jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0];
-        const double Jinv = 1.0 / J;
-        // Gauss-point state (FDVGalerkin::sum, :761-797) and my node pair
-        GPState<3> f;
-        f.E = 0; f.T = 0;
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            f.v[i] = 0; f.dT[i] = 0;
-#pragma unroll
-            for (int j = 0; j < 3; j++) f.dV[i][j] = 0;
-        }
-#pragma unroll
-        for (int q = 0; q < NEQ; q++) {
-            f.U[q] = 0;
-#pragma unroll
-            for (int j = 0; j < 3; j++) f.dU[q][j] = 0;
-        }
-        double SPhi = 0.0, SD[3] = {0, 0, 0};
-        double PN = 0, PM = 0, DN[3] = {0, 0, 0}, DM[3] = {0, 0, 0};
-#pragma unroll
-        for (int n = 0; n < 8; n++) {
-            double dphi[3];
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                double acc = 0.0;
-#pragma unroll
-                for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * G.dpxi3[gp][n][j]);
-                dphi[i] = acc;
-            }
-            const double ph = G.phi3[gp][n];
-            if (n == N) { PN = ph; DN[0] = dphi[0]; DN[1] = dphi[1]; DN[2] = dphi[2]; }
-            if (n == M) { PM = ph; DM[0] = dphi[0]; DM[1] = dphi[1]; DM[2] = dphi[2]; }
-            SPhi += ph;
-            f.E += ph * nd[n][ND_E];
-            const double Tnn = nd[n][ND_T];
-            f.T += ph * Tnn;
-#pragma unroll
-            for (int q = 0; q < NEQ; q++) {
-                const double u_ = nd[n][ND_U + q];
-                f.U[q] += ph * u_;
-#pragma unroll
-                for (int j = 0; j < 3; j++) f.dU[q][j] += dphi[j] * u_;
-            }
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const double v_ = nd[n][ND_V + i];
-                SD[i] += dphi[i];
-                f.v[i] += ph * v_;
-                f.dT[i] += dphi[i] * Tnn;
-#pragma unroll
-                for (int j = 0; j < 3; j++) f.dV[i][j] += dphi[j] * v_;
-            }
-        }
-        f.mu = calc_mu(th, f.T);
-        f.k = calc_k(th, f.mu);
-        calc_tau<3>(f);
-        const double wJ = G.w3[gp] * J;
-
-        // rhs pieces: P(i,r) = dt sum_b Phi_b G_b(i,r) and D(s) = sum_jb dPhi_bj
-        // G_b(j,s), G_b from the Gauss-point tau/dT/k with node velocities
-        // (sumFGB :801-811, calc_G NavierStokes3D.h:41-64; F = 0, quirk Q1)
-        double D[5];
-        D[0] = 0.0;
-#pragma unroll
-        for (int s = 1; s < 4; s++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int j = 0; j < 3; j++) acc += (-f.tau[j][s - 1] * reinv) * SD[j];
-            D[s] = acc;
-        }
-        {
-            double acc = 0.0;
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                double td = 0.0;
-#pragma unroll
-                for (int k = 0; k < 3; k++) td += f.tau[j][k] * f.dV[k][j];
-                acc += (-td - f.k * f.dT[j] * SD[j]) * reinv;
-            }
-            D[4] = acc;
-        }
-        double wg[3], wr[3], W[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            wg[i] = -dt * wJ * PM * DN[i];
-            wr[i] = wJ * DN[i];
-#pragma unroll
-            for (int j = 0; j < 3; j++) W[i][j] = -dt * s3 * wJ * DN[i] * DM[j];
-        }
-        double AQ[5];
-        contract3(th, f, R, wJ * PN * PM, wg, DN, DM, dtsq_2 * wJ, wr, D, AQ, s1, s2, s3, s4);
-        accum_c3(th, f, W, R);
-#pragma unroll
-        for (int r = 0; r < NEQ; r++) {
-            double acc = 0.0;
-            if (r > 0) {
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    double P;
-                    if (r < 4) P = dt * (-f.tau[i][r - 1] * reinv) * SPhi;
-                    else {
-                        double tv = 0.0;
-#pragma unroll
-                        for (int k = 0; k < 3; k++) tv += f.tau[i][k] * f.v[k];
-                        P = dt * ((-tv - f.k * f.dT[i] * SPhi) * reinv);
-                    }
-                    acc += wr[i] * P;
-                }
-            }
-            rhs[r] += acc - dtsq_2 * AQ[r];
-        }
-    }
-
-    // ---------------- boundary integral: faces with 0 < bc < 100 ----------------
+    (void)s1; (void)s2; (void)s4;
     const int* bcs = A.bc + e * 6;
     int anyface = 0;
 #pragma unroll
@@ -746,7 +588,13 @@ CFDV_HD void build3d_elem(const BuildArgs& A, const GaussTables& G, const double
         }
     }
 
-    // ---------------- ApplyBC: constrained rows ----------------
+}
+
+// ApplyBC (ApplyBC.h:14-133): constrained rows of lane (N, M) from the
+// element's dof mask; row -> 0.5 on the diagonal, rhs -> 0.
+CFDV_HD void apply_bc3(const BuildArgs& A, long e, int lane, double R[5][5], double rhs[5]) {
+    constexpr int NEQ = 5;
+    const int N = lane >> 3, M = lane & 7;
     const unsigned long long cm = A.cmask[e];
 #pragma unroll
     for (int r = 0; r < NEQ; r++) {
@@ -756,7 +604,178 @@ CFDV_HD void build3d_elem(const BuildArgs& A, const GaussTables& G, const double
             for (int s = 0; s < NEQ; s++) R[r][s] = (N == M && r == s) ? 0.5 : 0.0;
         }
     }
+}
 
+// One hex8 element seen from lane (N, M) = (lane/8, lane%8): FDVParam,
+// the 8-point domain integral, the face integrals and ApplyBC for the 5x5
+// sub-block R(5N+r, 5M+s) and (for M == 0) rhs(5N+r).  nd holds the element's
+// node data (x, U, v, E, T, rho per node).
+CFDV_HD void build3d_elem(const BuildArgs& A, const GaussTables& G, const double (*nd)[ND_STRIDE], long e,
+                          int lane, double R[5][5], double rhs[5], double sp[4], int* tie, int* status) {
+    constexpr int NNOD = 8, NEQ = 5;
+    const DevThermo th = A.th;
+    const int N = lane >> 3, M = lane & 7;
+    {
+        double vn[NNOD][3], Tn[NNOD], rhon[NNOD];
+#pragma unroll
+        for (int n = 0; n < NNOD; n++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) vn[n][i] = nd[n][ND_V + i];
+            Tn[n] = nd[n][ND_T];
+            rhon[n] = nd[n][ND_RHO];
+        }
+        fdvparam<3, 8>(th, vn, Tn, rhon, sp, tie);
+    }
+    const double s1 = sp[0], s2 = sp[1], s3 = sp[2], s4 = sp[3];
+    const double dt = A.dt, dtsq_2 = (dt * dt) / 2.0;
+    const double reinv = 1.0 / th.creyn;
+
+#pragma unroll
+    for (int r = 0; r < NEQ; r++) {
+        rhs[r] = 0.0;
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) R[r][s] = 0.0;
+    }
+
+    // ---------------- domain integral: 8 Gauss points ----------------
+    for (int gp = 0; gp < 8; gp++) {
+        double jac[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int n = 0; n < 8; n++) acc += G.dpxi3[gp][n][j] * nd[n][ND_X + i];
+                jac[i][j] = acc;
+            }
+        const double J = jac[0][0] * jac[1][1] * jac[2][2] + jac[0][1] * jac[1][2] * jac[2][0] +
+                         jac[0][2] * jac[1][0] * jac[2][1] - jac[0][2] * jac[1][1] * jac[2][0] -
+                         jac[0][1] * jac[1][0] * jac[2][2] - jac[0][0] * jac[1][2] * jac[2][1];
+        double ji[3][3];
+        ji[0][0] = jac[1][1] * jac[2][2] - jac[1][2] * jac[2][1];
+        ji[0][1] = -(jac[1][0] * jac[2][2] - jac[1][2] * jac[2][0]);
+        ji[0][2] = jac[1][0] * jac[2][1] - jac[1][1] * jac[2][0];
+        ji[1][0] = -(jac[0][1] * jac[2][2] - jac[0][2] * jac[2][1]);
+        ji[1][1] = jac[0][0] * jac[2][2] - jac[0][2] * jac[2][0];
+        ji[1][2] = -(jac[0][0] * jac[2][1] - jac[0][1] * jac[2][0]);
+        ji[2][0] = jac[0][1] * jac[1][2] - jac[0][2] * jac[1][1];
+        ji[2][1] = -(jac[0][0] * jac[1][2] - jac[0][2] * jac[1][0]);
+        ji[2][2] = jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0];
+        const double Jinv = 1.0 / J;
+        // Gauss-point state (FDVGalerkin::sum, :761-797) and my node pair
+        GPState<3> f;
+        f.E = 0; f.T = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            f.v[i] = 0; f.dT[i] = 0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) f.dV[i][j] = 0;
+        }
+#pragma unroll
+        for (int q = 0; q < NEQ; q++) {
+            f.U[q] = 0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) f.dU[q][j] = 0;
+        }
+        double SPhi = 0.0, SD[3] = {0, 0, 0};
+        double PN = 0, PM = 0, DN[3] = {0, 0, 0}, DM[3] = {0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            double dphi[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * G.dpxi3[gp][n][j]);
+                dphi[i] = acc;
+            }
+            const double ph = G.phi3[gp][n];
+            if (n == N) { PN = ph; DN[0] = dphi[0]; DN[1] = dphi[1]; DN[2] = dphi[2]; }
+            if (n == M) { PM = ph; DM[0] = dphi[0]; DM[1] = dphi[1]; DM[2] = dphi[2]; }
+            SPhi += ph;
+            f.E += ph * nd[n][ND_E];
+            const double Tnn = nd[n][ND_T];
+            f.T += ph * Tnn;
+#pragma unroll
+            for (int q = 0; q < NEQ; q++) {
+                const double u_ = nd[n][ND_U + q];
+                f.U[q] += ph * u_;
+#pragma unroll
+                for (int j = 0; j < 3; j++) f.dU[q][j] += dphi[j] * u_;
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const double v_ = nd[n][ND_V + i];
+                SD[i] += dphi[i];
+                f.v[i] += ph * v_;
+                f.dT[i] += dphi[i] * Tnn;
+#pragma unroll
+                for (int j = 0; j < 3; j++) f.dV[i][j] += dphi[j] * v_;
+            }
+        }
+        f.mu = calc_mu(th, f.T);
+        f.k = calc_k(th, f.mu);
+        calc_tau<3>(f);
+        const double wJ = G.w3[gp] * J;
+
+        // rhs pieces: P(i,r) = dt sum_b Phi_b G_b(i,r) and D(s) = sum_jb dPhi_bj
+        // G_b(j,s), G_b from the Gauss-point tau/dT/k with node velocities
+        // (sumFGB :801-811, calc_G NavierStokes3D.h:41-64; F = 0, quirk Q1)
+        double D[5];
+        D[0] = 0.0;
+#pragma unroll
+        for (int s = 1; s < 4; s++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) acc += (-f.tau[j][s - 1] * reinv) * SD[j];
+            D[s] = acc;
+        }
+        {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                double td = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) td += f.tau[j][k] * f.dV[k][j];
+                acc += (-td - f.k * f.dT[j] * SD[j]) * reinv;
+            }
+            D[4] = acc;
+        }
+        double wg[3], wr[3], W[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            wg[i] = -dt * wJ * PM * DN[i];
+            wr[i] = wJ * DN[i];
+#pragma unroll
+            for (int j = 0; j < 3; j++) W[i][j] = -dt * s3 * wJ * DN[i] * DM[j];
+        }
+        double AQ[5];
+        contract3(th, f, R, wJ * PN * PM, wg, DN, DM, dtsq_2 * wJ, wr, D, AQ, s1, s2, s3, s4);
+        accum_c3(th, f, W, R);
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) {
+            double acc = 0.0;
+            if (r > 0) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    double P;
+                    if (r < 4) P = dt * (-f.tau[i][r - 1] * reinv) * SPhi;
+                    else {
+                        double tv = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 3; k++) tv += f.tau[i][k] * f.v[k];
+                        P = dt * ((-tv - f.k * f.dT[i] * SPhi) * reinv);
+                    }
+                    acc += wr[i] * P;
+                }
+            }
+            rhs[r] += acc - dtsq_2 * AQ[r];
+        }
+    }
+
+    build3d_faces(A, G, nd, e, lane, sp, R, rhs, status);
+    apply_bc3(A, e, lane, R, rhs);
 }
 
 // ---------------------------------------------------------------------------
@@ -767,15 +786,15 @@ CFDV_HD void build3d_elem(const BuildArgs& A, const GaussTables& G, const double
 // as build3d_elem's domain loop, in the same order.
 // ---------------------------------------------------------------------------
 struct GPShared {
-    double dphi[8][3];   // dPhi_n/dx_j at the point
     double wJ, SPhi;     // w * det J, sum_n Phi_n
     double D[5];         // sum_jb dPhi_bj G_b(j, s)
     double P[3][5];      // P(i, r) for r = 1..4 (r = 0 unused)
     GPState<3> f;
 };
 
+// dPhi_n/dx_j go to dphi_out[n][j] when dphi_out is given.
 CFDV_HD void gp_phase1(const DevThermo& th, const GaussTables& G, const double (*nd)[ND_STRIDE], int gp,
-                       double dt, GPShared& o) {
+                       double dt, GPShared& o, double (*dphi_out)[3]) {
     constexpr int NEQ = 5;
     const double reinv = 1.0 / th.creyn;
     double jac[3][3];
@@ -817,7 +836,7 @@ CFDV_HD void gp_phase1(const DevThermo& th, const GaussTables& G, const double (
         for (int j = 0; j < 3; j++) f.dU[q][j] = 0;
     }
     double SPhi = 0.0, SD[3] = {0, 0, 0};
-#pragma unroll
+#pragma unroll 1
     for (int n = 0; n < 8; n++) {
         double dphi[3];
 #pragma unroll
@@ -826,7 +845,7 @@ CFDV_HD void gp_phase1(const DevThermo& th, const GaussTables& G, const double (
 #pragma unroll
             for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * G.dpxi3[gp][n][j]);
             dphi[i] = acc;
-            o.dphi[n][i] = acc;
+            if (dphi_out) dphi_out[n][i] = acc;
         }
         const double ph = G.phi3[gp][n];
         SPhi += ph;
@@ -893,12 +912,139 @@ CFDV_HD void gp_phase1(const DevThermo& th, const GaussTables& G, const double (
     o.f = f;
 }
 
-// Phase 2 for lane (N, M): the domain contraction of every Gauss point.
-CFDV_HD void interior_phase2(const DevThermo& th, const GaussTables& G, const GPShared* gps, int lane,
+// Structural nonzeros of the direction-I Jacobians a_I and b_I
+// (NavierStokes3D.h:74-343): row 0 {1+I}, row 1+I all, rows 1+j (j != I)
+// {0, 1+j, 1+I}, row 4 all -- 17 entries, row-major.
+constexpr int JPAT_R[3][17] = {{0, 1, 1, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 4, 4},
+                               {0, 1, 1, 1, 2, 2, 2, 2, 2, 3, 3, 3, 4, 4, 4, 4, 4},
+                               {0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 3, 3, 4, 4, 4, 4, 4}};
+constexpr int JPAT_S[3][17] = {{1, 0, 1, 2, 3, 4, 0, 1, 2, 0, 1, 3, 0, 1, 2, 3, 4},
+                               {2, 0, 1, 2, 0, 1, 2, 3, 4, 0, 2, 3, 0, 1, 2, 3, 4},
+                               {3, 0, 1, 3, 0, 2, 3, 0, 1, 2, 3, 4, 0, 1, 2, 3, 4}};
+// (r, s) is a structural nonzero of some direction
+CFDV_HD constexpr bool jnz(int r, int s) {
+    for (int I = 0; I < 3; I++)
+        for (int p = 0; p < 17; p++)
+            if (JPAT_R[I][p] == r && JPAT_S[I][p] == s) return true;
+    return false;
+}
+
+// Per Gauss point, what the node-pair contraction reads: the direction
+// combinations M1_I = s1 a_I + s3 b_I, A_I = a_I + b_I, S_I = s2 a_I + s4 b_I
+// on their 17 nonzeros, the rhs pieces and the scalars of c_ij.
+struct GPTab {
+    double dphi[8][3];
+    double wJ;
+    double M1[3][17], A[3][17], S[3][17];
+    double P[3][5];     // P(i, r), r = 1..4
+    double AD[3][5];    // (A_i D)(r)
+    double c_mu, c_lambda, c_muR, c_rinv, c_denom, c_l, c_m, c_n, c_TERM1, c_hkocv, c_en;
+};
+
+// Phase 1b, direction I of one Gauss point (one lane)
+template <int I>
+CFDV_HD void gp_tab_dir(const DevThermo& th, const GPShared& g, const double sp[4], GPTab& t) {
+    double a[5][5], b[5][5];
+    calc_a3_dir<I>(th, g.f, a);
+    calc_b3_dir<I>(th, g.f, b);
+    const double s1 = sp[0], s2 = sp[1], s3 = sp[2], s4 = sp[3];
+#pragma unroll
+    for (int p = 0; p < 17; p++) {
+        const int r = JPAT_R[I][p], c = JPAT_S[I][p];
+        t.M1[I][p] = s1 * a[r][c] + s3 * b[r][c];
+        t.A[I][p] = a[r][c] + b[r][c];
+        t.S[I][p] = s2 * a[r][c] + s4 * b[r][c];
+    }
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        double q = 0.0;
+#pragma unroll
+        for (int c = 0; c < 5; c++) q += (a[r][c] + b[r][c]) * g.D[c];
+        t.AD[I][r] = q;
+        t.P[I][r] = g.P[I][r];
+    }
+}
+
+// Phase 1b, w det J and the c_ij scalars of one Gauss point (one lane);
+// the scalars are accum_c3's preamble
+CFDV_HD void gp_tab_common(const DevThermo& th, const GPShared& g, GPTab& t) {
+    t.wJ = g.wJ;
+    const GPState<3>& f = g.f;
+    const double mu = f.mu;
+    const double lambda = -2. * mu / 3.;
+    const double rho = f.U[0];
+    const double rinv = 1.0 / rho;
+    const double denom = 1. / (rho * rho);
+    const double l = f.U[1], m = f.U[2], n = f.U[3], re = f.U[4];
+    const double TERM1 = f.k / th.Cv * denom;
+    t.c_mu = mu;
+    t.c_lambda = lambda;
+    t.c_muR = 2. * mu + lambda;
+    t.c_rinv = rinv;
+    t.c_denom = denom;
+    t.c_l = l;
+    t.c_m = m;
+    t.c_n = n;
+    t.c_TERM1 = TERM1;
+    t.c_hkocv = f.k / th.Cv;
+    t.c_en = TERM1 * (-re + (l * l + m * m + n * n) * rinv);
+}
+
+// accum_c3 (NavierStokes3D.h:347-477 contracted) from the tabulated scalars
+CFDV_HD void accum_c3_tab(const DevThermo& th, const GPTab& t, const double W[3][3], double out[5][5]) {
+    const double mu = t.c_mu, lambda = t.c_lambda, mu_R = t.c_muR;
+    const double rinv = t.c_rinv, denom = t.c_denom;
+    const double l = t.c_l, m = t.c_m, n = t.c_n;
+    const double hkocv = t.c_hkocv, TERM1 = t.c_TERM1, en = t.c_en;
+    const double reinv = 1.0 / th.creyn;
+    const double lr = l * rinv, mr = m * rinv, nr = n * rinv;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double MUX = (i == 0) ? mu_R : mu, MUY = (i == 1) ? mu_R : mu, MUZ = (i == 2) ? mu_R : mu;
+        double c10 = MUX * l * denom, c11 = (-MUX * rinv);
+        double c20 = MUY * m * denom, c22 = (-MUY * rinv);
+        double c30 = MUZ * n * denom, c33 = (-MUZ * rinv);
+        const double c40 = (l * c10 + m * c20 + n * c30) * rinv - en;
+        const double c41 = lr * c11 + TERM1 * l;
+        const double c42 = mr * c22 + TERM1 * m;
+        const double c43 = nr * c33 + TERM1 * n;
+        const double c44 = (-hkocv * rinv);
+        const double wgt = W[i][i] * reinv;
+        out[1][0] += wgt * c10; out[1][1] += wgt * c11;
+        out[2][0] += wgt * c20; out[2][2] += wgt * c22;
+        out[3][0] += wgt * c30; out[3][3] += wgt * c33;
+        out[4][0] += wgt * c40; out[4][1] += wgt * c41; out[4][2] += wgt * c42;
+        out[4][3] += wgt * c43; out[4][4] += wgt * c44;
+    }
+    const double Uc[3] = {l, m, n};
+    const double Ur[3] = {lr, mr, nr};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            if (i == j) continue;
+            const double wgt = W[i][j] * reinv;
+            const double c_i0 = lambda * Uc[j] * denom;
+            const double c_ij = (-lambda * rinv);
+            const double c_j0 = mu * Uc[i] * denom;
+            const double c_ji = (-mu * rinv);
+            out[1 + i][0] += wgt * c_i0;
+            out[1 + i][1 + j] += wgt * c_ij;
+            out[1 + j][0] += wgt * c_j0;
+            out[1 + j][1 + i] += wgt * c_ji;
+            out[4][0] += wgt * (Ur[i] * c_i0 + Ur[j] * c_j0);
+            out[4][1 + i] += wgt * (Ur[j] * c_ji);
+            out[4][1 + j] += wgt * (Ur[i] * c_ij);
+        }
+}
+
+// Phase 2 for lane (N, M): the domain contraction of every Gauss point,
+// in contract3's order (diagonal, direction terms, A S product, c_ij).
+CFDV_HD void interior_phase2(const DevThermo& th, const GaussTables& G, const GPTab* tabs, int lane,
                              double dt, const double sp[4], double R[5][5], double rhs[5]) {
     constexpr int NEQ = 5;
     const int N = lane >> 3, M = lane & 7;
-    const double s1 = sp[0], s2 = sp[1], s3 = sp[2], s4 = sp[3];
+    const double s3 = sp[2];
     const double dtsq_2 = (dt * dt) / 2.0;
 #pragma unroll
     for (int r = 0; r < NEQ; r++) {
@@ -908,16 +1054,12 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GaussTables& G, const GP
     }
 #pragma unroll 1
     for (int gp = 0; gp < 8; gp++) {
-        const GPShared& g = gps[gp];
+        const GPTab& g = tabs[gp];
         const double PN = G.phi3[gp][N], PM = G.phi3[gp][M];
         const double wJ = g.wJ;
         double DN[3], DM[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) { DN[i] = g.dphi[N][i]; DM[i] = g.dphi[M][i]; }
-        const GPState<3>& f = g.f;   // read in place (LDS in the kernel)
-        double D[5];
-#pragma unroll
-        for (int s = 0; s < 5; s++) D[s] = g.D[s];
         double wg[3], wr[3], W[3][3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -926,49 +1068,77 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GaussTables& G, const GP
 #pragma unroll
             for (int j = 0; j < 3; j++) W[i][j] = -dt * s3 * wJ * DN[i] * DM[j];
         }
-        double AQ[5];
-        contract3(th, f, R, wJ * PN * PM, wg, DN, DM, dtsq_2 * wJ, wr, D, AQ, s1, s2, s3, s4);
-        accum_c3(th, f, W, R);
+        const double cI = wJ * PN * PM, cA = dtsq_2 * wJ;
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) R[r][r] += cI;
+        double Sh[5][5];
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) Sh[r][s] = 0.0;
+#pragma unroll
+        for (int I = 0; I < 3; I++)
+#pragma unroll
+            for (int p = 0; p < 17; p++) {
+                R[JPAT_R[I][p]][JPAT_S[I][p]] += wg[I] * g.M1[I][p];
+                Sh[JPAT_R[I][p]][JPAT_S[I][p]] += DM[I] * g.S[I][p];
+            }
+        // rhs: P and the A D product (contract3's AQ)
 #pragma unroll
         for (int r = 0; r < NEQ; r++) {
-            double acc = 0.0;
-            if (r > 0) {
+            double acc = 0.0, aq = 0.0;
 #pragma unroll
-                for (int i = 0; i < 3; i++) acc += wr[i] * g.P[i][r];
+            for (int i = 0; i < 3; i++) {
+                if (r > 0) acc += wr[i] * g.P[i][r];
+                aq += wr[i] * g.AD[i][r];
             }
-            rhs[r] += acc - dtsq_2 * AQ[r];
+            rhs[r] += acc - dtsq_2 * aq;
         }
+        // R += cA (sum_i DN_i A_i)(sum_j DM_j S_j), one row of the left factor at a time
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) {
+            double Ah[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int I = 0; I < 3; I++)
+#pragma unroll
+                for (int p = 0; p < 17; p++)
+                    if (JPAT_R[I][p] == r) Ah[JPAT_S[I][p]] += DN[I] * g.A[I][p];
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) {
+                double pr = 0.0;
+#pragma unroll
+                for (int t = 0; t < NEQ; t++)
+                    if (jnz(r, t) && jnz(t, s)) pr += Ah[t] * Sh[t][s];
+                R[r][s] += cA * pr;
+            }
+        }
+        accum_c3_tab(th, g, W, R);
     }
 }
 
 template <bool W64, bool W32>
-__global__ __launch_bounds__(256) void build3d_interior_kernel(BuildArgs A) {
+__global__ __launch_bounds__(64) void build3d_interior_kernel(BuildArgs A) {
+    // one wavefront (workgroup) per element slot
     constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600;
-    __shared__ double s_nd[4][NNOD][ND_STRIDE];
-    __shared__ GPShared s_gp[4][8];
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    const long e = (long)xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
+    __shared__ double s_nd[NNOD][ND_STRIDE];
+    __shared__ GPTab s_tab[8];
+    const int lane = threadIdx.x;
+    const long e = (long)xcd_remap(blockIdx.x, gridDim.x);
     const DevThermo th = A.th;
-    const bool in_range = e < A.n_elem_pad;
-    const long es = in_range ? e : 0;
-    const int* cn = A.conn + es * NNOD;
-    const bool real = in_range && cn[0] >= 0;
+    if (e >= A.n_elem_pad) return;   // whole workgroup leaves together
+    const int* cn = A.conn + e * NNOD;
+    const bool real = cn[0] >= 0;
     bool boundary = false;
     if (real) {
-        const int* bcs = A.bc + es * 6;
+        const int* bcs = A.bc + e * 6;
+        boundary = A.cmask[e] != 0ULL;
 #pragma unroll
         for (int fc = 0; fc < 6; fc++) boundary |= (bcs[fc] != 0);
-        boundary |= (A.cmask[es] != 0ULL);
-    }
-    const bool work = real && !boundary;
-    double (*nd)[ND_STRIDE] = s_nd[wv];
-    if (work && lane < NNOD) load_node3d(A, cn[lane], nd[lane]);
-    __syncthreads();
-    if (work && lane < 8) gp_phase1(th, c_gauss, nd, lane, A.dt, s_gp[wv][lane]);
-    __syncthreads();
-    if (!in_range || boundary) return;
-    if (!real) {   // padding slot of a color range: zero block
+        // with FP64 blocks, a boundary element's domain part is written here
+        // (FP64 R and rhs) and build3d_face_kernel adds faces and ApplyBC;
+        // without, the per-lane kernel builds it whole
+        if (boundary && !W64) return;
+    } else {   // padding slot of a color range: zero block
         if (W64)
             for (int q = lane; q < KK; q += 64) A.R64[blk_index<double>(e, q, KK)] = 0.0;
         if (W32)
@@ -976,7 +1146,8 @@ __global__ __launch_bounds__(256) void build3d_interior_kernel(BuildArgs A) {
         if (lane < K) A.rhs[e * K + lane] = 0.0;
         return;
     }
-    const int N = lane >> 3, M = lane & 7;
+    if (lane < NNOD) load_node3d(A, cn[lane], s_nd[lane]);
+    __syncthreads();
     double sp[4];
     int tie = 0;
     {
@@ -984,21 +1155,36 @@ __global__ __launch_bounds__(256) void build3d_interior_kernel(BuildArgs A) {
 #pragma unroll
         for (int n = 0; n < NNOD; n++) {
 #pragma unroll
-            for (int i = 0; i < 3; i++) vn[n][i] = nd[n][ND_V + i];
-            Tn[n] = nd[n][ND_T];
-            rhon[n] = nd[n][ND_RHO];
+            for (int i = 0; i < 3; i++) vn[n][i] = s_nd[n][ND_V + i];
+            Tn[n] = s_nd[n][ND_T];
+            rhon[n] = s_nd[n][ND_RHO];
         }
         fdvparam<3, 8>(th, vn, Tn, rhon, sp, &tie);
     }
+    // phase 1: lanes (part, gp) = (lane / 8, lane % 8), part < 4: the
+    // Gauss-point state (evaluated by the four lanes of a point alike), then
+    // direction 0, 1, 2 of the Jacobian combinations or the common part
+    if (lane < 32) {
+        const int gp = lane & 7, part = lane >> 3;
+        GPShared g;
+        gp_phase1(th, c_gauss, s_nd, gp, A.dt, g, part == 3 ? s_tab[gp].dphi : nullptr);
+        if (part == 0) gp_tab_dir<0>(th, g, sp, s_tab[gp]);
+        else if (part == 1) gp_tab_dir<1>(th, g, sp, s_tab[gp]);
+        else if (part == 2) gp_tab_dir<2>(th, g, sp, s_tab[gp]);
+        else gp_tab_common(th, g, s_tab[gp]);
+    }
+    __syncthreads();
+    // phase 2: lane (N, M) contracts every point for its node pair
+    const int N = lane >> 3, M = lane & 7;
     double R[NEQ][NEQ], rhs[NEQ];
-    interior_phase2(th, c_gauss, s_gp[wv], lane, A.dt, sp, R, rhs);
+    interior_phase2(th, c_gauss, s_tab, lane, A.dt, sp, R, rhs);
 #pragma unroll
     for (int r = 0; r < NEQ; r++)
 #pragma unroll
         for (int s = 0; s < NEQ; s++) {
             const int q = (NEQ * N + r) * K + NEQ * M + s;
             if (W64) A.R64[blk_index<double>(e, q, KK)] = R[r][s];
-            if (W32) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
+            if (W32 && !boundary) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
         }
     if (M == 0) {
 #pragma unroll
@@ -1010,6 +1196,59 @@ __global__ __launch_bounds__(256) void build3d_interior_kernel(BuildArgs A) {
             for (int i = 0; i < 4; i++) A.svals[e * 4 + i] = sp[i];
         }
         if (tie) atomicAdd(A.ties, 1);
+    }
+}
+
+// Boundary elements whose domain part is already in R64/rhs: face integrals
+// and ApplyBC on top (the same accumulation order as build3d_elem).  One
+// wavefront per listed slot.
+template <bool W32>
+__global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
+    constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600;
+    __shared__ double s_nd[NNOD][ND_STRIDE];
+    const int lane = threadIdx.x;
+    if ((long)blockIdx.x >= A.n_bslots) return;
+    const long e = A.bslots[blockIdx.x];
+    const int* cn = A.conn + e * NNOD;
+    if (lane < NNOD) load_node3d(A, cn[lane], s_nd[lane]);
+    __syncthreads();
+    const DevThermo th = A.th;
+    double sp[4];
+    int tie = 0;
+    {
+        double vn[NNOD][3], Tn[NNOD], rhon[NNOD];
+#pragma unroll
+        for (int n = 0; n < NNOD; n++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) vn[n][i] = s_nd[n][ND_V + i];
+            Tn[n] = s_nd[n][ND_T];
+            rhon[n] = s_nd[n][ND_RHO];
+        }
+        fdvparam<3, 8>(th, vn, Tn, rhon, sp, &tie);
+    }
+    const int N = lane >> 3, M = lane & 7;
+    double R[NEQ][NEQ], rhs[NEQ];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++) {
+        rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) R[r][s] = A.R64[blk_index<double>(e, (NEQ * N + r) * K + NEQ * M + s, KK)];
+    }
+    int status = 0;
+    build3d_faces(A, c_gauss, s_nd, e, lane, sp, R, rhs, &status);
+    apply_bc3(A, e, lane, R, rhs);
+    if (status && lane == 0) atomicOr(A.status, status);
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) {
+            const int q = (NEQ * N + r) * K + NEQ * M + s;
+            A.R64[blk_index<double>(e, q, KK)] = R[r][s];
+            if (W32) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
+        }
+    if (M == 0) {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
     }
 }
 
@@ -1462,17 +1701,18 @@ __global__ __launch_bounds__(256) void build2d_kernel(BuildArgs A) {
 
 void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st) {
     if (a.ndim == 3) {
-        // interior elements (and padding) with the shared Gauss-point table
-        const int nb = (int)((a.n_elem_pad + 3) / 4);
-        if (w64 && w32) hipLaunchKernelGGL((build3d_interior_kernel<true, true>), dim3(nb), dim3(256), 0, st, a);
-        else if (w64) hipLaunchKernelGGL((build3d_interior_kernel<true, false>), dim3(nb), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((build3d_interior_kernel<false, true>), dim3(nb), dim3(256), 0, st, a);
-        // elements with faces or constrained dofs: the full per-lane kernel
+        // interior elements (and padding) with the shared Gauss-point table,
+        // one 64-lane workgroup per slot
+        const int nb = (int)a.n_elem_pad;
+        if (w64 && w32) hipLaunchKernelGGL((build3d_interior_kernel<true, true>), dim3(nb), dim3(64), 0, st, a);
+        else if (w64) hipLaunchKernelGGL((build3d_interior_kernel<true, false>), dim3(nb), dim3(64), 0, st, a);
+        else hipLaunchKernelGGL((build3d_interior_kernel<false, true>), dim3(nb), dim3(64), 0, st, a);
+        // elements with faces or constrained dofs: faces + ApplyBC on top of
+        // the FP64 domain part, or (FP32 blocks only) the full per-lane kernel
         if (a.n_bslots > 0) {
-            const int nbb = (int)((a.n_bslots + 3) / 4);
-            if (w64 && w32) hipLaunchKernelGGL((build3d_kernel<true, true>), dim3(nbb), dim3(256), 0, st, a);
-            else if (w64) hipLaunchKernelGGL((build3d_kernel<true, false>), dim3(nbb), dim3(256), 0, st, a);
-            else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3(nbb), dim3(256), 0, st, a);
+            if (w64 && w32) hipLaunchKernelGGL((build3d_face_kernel<true>), dim3((int)a.n_bslots), dim3(64), 0, st, a);
+            else if (w64) hipLaunchKernelGGL((build3d_face_kernel<false>), dim3((int)a.n_bslots), dim3(64), 0, st, a);
+            else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3((int)((a.n_bslots + 3) / 4)), dim3(256), 0, st, a);
         }
     } else {
         const int nb = (int)((a.n_elem_pad * 16 + 255) / 256);
