@@ -35,7 +35,8 @@ CFDV_PRECOND_NODE_BLOCK = 1
 EXPORTS = ["cfdv_create", "cfdv_destroy", "cfdv_last_error", "cfdv_mesh_info", "cfdv_set_state",
            "cfdv_get_state", "cfdv_cfl", "cfdv_build", "cfdv_get_blocks", "cfdv_get_blocks32",
            "cfdv_get_rhs_vector", "cfdv_solve", "cfdv_get_du", "cfdv_update", "cfdv_rms",
-           "cfdv_refine_flags", "cfdv_color", "cfdv_matvec", "cfdv_profile_matvec", "cfdv_step"]
+           "cfdv_refine_flags", "cfdv_color", "cfdv_matvec", "cfdv_profile_matvec", "cfdv_step",
+           "cfdv_set_thermo", "cfdv_adjoint_solve", "cfdv_build_vjp"]
 
 
 class Mesh(C.Structure):
@@ -109,5 +110,8 @@ def load() -> C.CDLL:
     L.cfdv_matvec.argtypes = [vp, dp, dp, C.c_int, C.c_int]
     L.cfdv_profile_matvec.argtypes = [vp, C.c_int, C.c_int, dp]
     L.cfdv_step.argtypes = [vp, C.c_double, C.POINTER(SolveOpts), C.POINTER(StepStats)]
+    L.cfdv_set_thermo.argtypes = [vp, C.POINTER(ThermoS)]
+    L.cfdv_adjoint_solve.argtypes = [vp, C.POINTER(SolveOpts), dp, dp, C.POINTER(SolveStats)]
+    L.cfdv_build_vjp.argtypes = [vp, dp, dp, dp, dp]
     _LIB = L
     return L

@@ -1,0 +1,96 @@
+"""Discrete adjoint of one FDV time step and its torch binding.
+
+One step at a fixed dt: U1 = U0 + du with K(U0, M) du = b(U0, M) (the
+element blocks of FDVGalerkin + ApplyBC, solved with the FP64 or mixed
+GMRES).  For a scalar J(U1) with g = dJ/dU1:
+
+    K^T lam = g,    dJ/dU0 = g + d/dU0 [lam^T (b - K du)],
+                    dJ/dM  =     d/dM  [lam^T (b - K du)],
+
+the bracket differentiated with lam and du held fixed (cfdv_build_vjp, the
+forward-mode tangents of the element build on the device).  M is the Mach
+number of the Thermo constants (Thermo::set, FDV/Thermo.h:22-33) with the
+conservative state held fixed.  The wall temperature enters the step only
+through the state of the wall nodes (ApplyBC.h's Twall branch is commented
+out in the reference), so a wall-temperature gradient is the chain rule
+through U: dU_E/dT = rho Cv at those nodes (``wall_temperature_gradient``).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from .fdv import FDVContext
+
+
+def step_fixed_dt(ctx: FDVContext, U0, dt, mode="fp64", rtol=1e-12, precond="nodeblock", **kw):
+    """U1 = U0 + du for one step at the given dt; returns (U1, du)."""
+    ctx.set_state(U0)
+    ctx.build(dt)
+    kw.setdefault("restarts", 200 if mode == "fp64" else 40)
+    ctx.solve(mode, m=kw.pop("m", 30), rtol=rtol, precond=precond, **kw)
+    du = ctx.du()
+    return np.asarray(U0).ravel() + du, du
+
+
+def step_gradient(ctx: FDVContext, U0, dt, g, mode="fp64", rtol=1e-12, precond="nodeblock"):
+    """(J-independent) gradient of J = g^T U1 with respect to U0 and the Mach
+    number; returns (dJ/dU0 [n_node][neqn], dJ/dM, U1)."""
+    U1, du = step_fixed_dt(ctx, U0, dt, mode=mode, rtol=rtol, precond=precond)
+    lam, _ = ctx.adjoint_solve(g, mode=mode, rtol=rtol, precond=precond)
+    gU, gM = ctx.build_vjp(lam, du)
+    n = ctx.mesh.n_node
+    return np.asarray(g).reshape(n, -1) + gU, gM, U1.reshape(n, -1)
+
+
+def wall_temperature_gradient(dJdU0, U0, Cv, wall_nodes):
+    """dJ/dT_n at the given nodes, T = (E/rho - |v|^2/2)/Cv with rho and the
+    velocity held: dU_E/dT = rho Cv (NodeUnpack.h:124-144 inverted)."""
+    U0 = np.asarray(U0)
+    rho = U0[wall_nodes, 0]
+    return dJdU0[wall_nodes, -1] * rho * Cv
+
+
+def _torch_function():
+    import torch
+
+    class FDVStep(torch.autograd.Function):
+        """U1 = step(U0, M) at a fixed dt on the GPU (float64 host tensors in
+        and out); backward through the discrete adjoint."""
+
+        @staticmethod
+        def forward(tctx, U0, mach, ctx: FDVContext, dt: float, mode: str = "fp64",
+                    rtol: float = 1e-12):
+            th = dataclasses.replace(ctx.thermo, mach=float(mach))
+            ctx.set_thermo(th)
+            U0n = U0.detach().cpu().numpy().astype(np.float64)
+            U1, du = step_fixed_dt(ctx, U0n, dt, mode=mode, rtol=rtol)
+            tctx.fdv = (ctx, dt, mode, rtol, U0n, du, float(mach))
+            return torch.from_numpy(U1.reshape(U0n.shape).copy())
+
+        @staticmethod
+        def backward(tctx, gU1):
+            ctx, dt, mode, rtol, U0n, du, mach = tctx.fdv
+            ctx.set_thermo(dataclasses.replace(ctx.thermo, mach=mach))
+            ctx.set_state(U0n)
+            ctx.build(dt)
+            g = gU1.detach().cpu().numpy().astype(np.float64)
+            lam, _ = ctx.adjoint_solve(g, mode=mode, rtol=rtol)
+            gU, gM = ctx.build_vjp(lam, du)
+            gU0 = torch.from_numpy(g.reshape(gU.shape) + gU)
+            return gU0, torch.tensor(gM, dtype=torch.float64), None, None, None, None
+
+    return FDVStep
+
+
+_FDVSTEP = None
+
+
+def fdv_step(U0, mach, ctx: FDVContext, dt: float, mode="fp64", rtol=1e-12):
+    """torch entry: U1 = FDV step of U0 at Mach ``mach`` and fixed ``dt``,
+    differentiable with respect to U0 and mach."""
+    global _FDVSTEP
+    if _FDVSTEP is None:
+        _FDVSTEP = _torch_function()
+    return _FDVSTEP.apply(U0, mach, ctx, dt, mode, rtol)

@@ -147,6 +147,9 @@ struct cfdv_ctx {
     double *d_P64 = nullptr, *d_z64 = nullptr, *d_t64 = nullptr;
     float *d_P32 = nullptr, *d_z32 = nullptr;
     bool pinv_valid = false;
+    // discrete adjoint: per-slot tangents [n_pad][41], host-given vectors
+    double *d_gelem = nullptr, *d_lam = nullptr, *d_duv = nullptr, *d_gU = nullptr, *d_g = nullptr;
+    double last_dt = 0.0;
     double *d_part = nullptr, *d_small = nullptr, *d_hcol = nullptr, *d_cflpart = nullptr;
     double* h_pin = nullptr;   // pinned staging for small host-to-device arrays
     int *d_counters = nullptr;     // [0] ties, [1] status, [2] singular node block
@@ -690,34 +693,39 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
     return iters;
 }
 
-int solve_fp64(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
+// K x = b (K^T x = b with o->transpose) for device b, x (both [ndof])
+int solve_fp64_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, const double* b, double* x) {
     const long n = c->ndof;
     if (!c->d_R64) return fail(c, CFDV_ERR_ARG, "FP64 solve needs CFDV_BLOCKS_FP64");
     c->ensure_krylov(o->m);
-    launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
-    vec_fill(c->d_du, 0.0, n, c->st);
-    const double bnorm = c->norm(c->d_b);
+    vec_fill(x, 0.0, n, c->st);
+    const double bnorm = c->norm(b);
     s->bnorm = bnorm;
     bool conv = false;
     const bool pc = o->precond == CFDV_PRECOND_NODE_BLOCK;
     if (pc) c->ensure_precond();
     if (bnorm > 0)
-        s->iters += gmres_core<double>(c, 64, o->transpose != 0, c->d_b, c->d_du, o->m, o->restarts, o->rtol * bnorm,
-                                       bnorm, s, &conv, c->d_V, c->d_w, pc ? c->d_P64 : nullptr,
-                                       pc ? c->d_z64 : nullptr);
-    c->matvec(c->d_du, c->d_y, 64, o->transpose != 0);
-    vec_sub(c->d_r, c->d_b, c->d_y, n, c->st);
+        s->iters += gmres_core<double>(c, 64, o->transpose != 0, b, x, o->m, o->restarts, o->rtol * bnorm, bnorm, s,
+                                       &conv, c->d_V, c->d_w, pc ? c->d_P64 : nullptr, pc ? c->d_z64 : nullptr);
+    c->matvec(x, c->d_y, 64, o->transpose != 0);
+    vec_sub(c->d_r, b, c->d_y, n, c->st);
     s->resid = c->norm(c->d_r);
     s->rel_resid = bnorm > 0 ? s->resid / bnorm : 0.0;
-    c->have_du = true;
     if (!std::isfinite(s->resid)) return fail(c, CFDV_ERR_NAN, "NaN in the FP64 solve");
     if (s->rel_resid > o->rtol * 10 && bnorm > 0) return fail(c, CFDV_ERR_NOCONV, "FP64 GMRES did not converge");
     return CFDV_OK;
 }
 
+int solve_fp64(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
+    launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
+    const int rc = solve_fp64_rhs(c, o, s, c->d_b, c->d_du);
+    c->have_du = rc == CFDV_OK || rc == CFDV_ERR_NOCONV;
+    return rc;
+}
+
 // FP64 iterative refinement around an FP32 GMRES: r = b - K64 du (FP64),
 // solve K32 d = r/||r|| in FP32, du += ||r|| d (FP64).
-int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
+int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, const double* b, double* x) {
     const long n = c->ndof;
     if (!c->d_R32) return fail(c, CFDV_ERR_ARG, "mixed solve needs CFDV_BLOCKS_FP32");
     const int outer_prec = c->d_R64 ? 64 : 3;   // FP64 residual with FP64 (or widened FP32) blocks
@@ -725,13 +733,12 @@ int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
     c->ensure_krylov32(o->inner_m);
     const bool pc = o->precond == CFDV_PRECOND_NODE_BLOCK;
     if (pc) c->ensure_precond();
-    launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
-    vec_fill(c->d_du, 0.0, n, c->st);
+    vec_fill(x, 0.0, n, c->st);
     const double one = 1.0;
     vec_set(c->d_small + 10, one, c->st);
-    const double bnorm = c->norm(c->d_b);
+    const double bnorm = c->norm(b);
     s->bnorm = bnorm;
-    vec_copy(c->d_r, c->d_b, n, c->st);
+    vec_copy(c->d_r, b, n, c->st);
     double rnorm = bnorm;
     int sweeps = 0;
     while (rnorm > o->rtol * bnorm && sweeps < o->restarts) {
@@ -750,11 +757,11 @@ int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
         s->matvecs64 += si.matvecs64;
         // du += ||r|| d  (FP64)
         vec_set(c->d_small + 11, rnorm, c->st);
-        vec_axpy(c->d_du, c->d_d64, c->d_small + 11, 1.0, n, c->st);
-        // r = b - K du in FP64
-        c->matvec(c->d_du, c->d_y, outer_prec, tr);
+        vec_axpy(x, c->d_d64, c->d_small + 11, 1.0, n, c->st);
+        // r = b - K x in FP64
+        c->matvec(x, c->d_y, outer_prec, tr);
         if (outer_prec == 64) s->matvecs64++; else s->matvecs32++;
-        vec_sub(c->d_r, c->d_b, c->d_y, n, c->st);
+        vec_sub(c->d_r, b, c->d_y, n, c->st);
         rnorm = c->norm(c->d_r);
         sweeps++;
         if (!std::isfinite(rnorm)) break;
@@ -762,10 +769,16 @@ int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
     s->restarts = sweeps;
     s->resid = rnorm;
     s->rel_resid = bnorm > 0 ? rnorm / bnorm : 0.0;
-    c->have_du = true;
     if (!std::isfinite(rnorm)) return fail(c, CFDV_ERR_NAN, "NaN in the mixed solve");
     if (s->rel_resid > o->rtol && bnorm > 0) return fail(c, CFDV_ERR_NOCONV, "mixed refinement did not converge");
     return CFDV_OK;
+}
+
+int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
+    launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
+    const int rc = solve_mixed_rhs(c, o, s, c->d_b, c->d_du);
+    c->have_du = rc == CFDV_OK || rc == CFDV_ERR_NOCONV;
+    return rc;
 }
 
 }  // namespace
@@ -889,6 +902,7 @@ int cfdv_build(cfdv_ctx* ctx, double dt) {
     a.x = ctx->d_x;
     a.U = ctx->d_U;
     a.dt = dt;
+    ctx->last_dt = dt;
     a.R64 = ctx->d_R64;
     a.R32 = ctx->d_R32;
     a.rhs = ctx->d_rhs;
@@ -1117,6 +1131,89 @@ int cfdv_profile_matvec(cfdv_ctx* ctx, int precision, int reps, double* ms) {
     HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
     *ms = t / reps;
     if (ytmp) { ctx->allocs.pop_back(); (void)hipFree(ytmp); }
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_set_thermo(cfdv_ctx* ctx, const cfdv_thermo* thermo) {
+    if (!ctx || !thermo) return CFDV_ERR_ARG;
+    if ((thermo->adiabatic != 0) != (ctx->th.adiabatic != 0))
+        return fail(ctx, CFDV_ERR_ARG, "the adiabatic flag is fixed at create (it shapes ApplyBC)");
+    if (!(thermo->mach > 0) || !(thermo->gamma > 1) || !(thermo->reynolds > 0) || !(thermo->prandtl > 0) ||
+        !(thermo->tinf > 0))
+        return fail(ctx, CFDV_ERR_ARG, "bad thermo constants");
+    ctx->th = make_thermo(thermo);
+    ctx->built = false;
+    ctx->pinv_valid = false;
+    ctx->have_du = false;
+    return CFDV_OK;
+}
+
+int cfdv_adjoint_solve(cfdv_ctx* ctx, const cfdv_solve_opts* o, const double* g, double* lambda,
+                       cfdv_solve_stats* s) {
+    if (!ctx || !o || !g || !lambda || !s) return CFDV_ERR_ARG;
+    if (!ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_adjoint_solve before cfdv_build");
+    if (o->mode != CFDV_SOLVE_FP64 && o->mode != CFDV_SOLVE_MIXED)
+        return fail(ctx, CFDV_ERR_ARG, "the adjoint needs the FP64 or mixed solver");
+    if (o->m < 1 || o->m > 500 || o->restarts < 1) return fail(ctx, CFDV_ERR_ARG, "bad Krylov dimension/restarts");
+    if (o->precond != CFDV_PRECOND_NONE && o->precond != CFDV_PRECOND_NODE_BLOCK)
+        return fail(ctx, CFDV_ERR_ARG, "unknown preconditioner");
+    if (o->mode == CFDV_SOLVE_MIXED && (o->inner_m < 2 || o->inner_m > 500 || o->inner_restarts < 1))
+        return fail(ctx, CFDV_ERR_ARG, "bad inner Krylov parameters");
+    std::memset(s, 0, sizeof(*s));
+    GUARD_BEGIN
+    if (!ctx->d_g) {
+        ctx->d_g = ctx->alloc<double>(ctx->ndof);
+        ctx->d_lam = ctx->alloc<double>(ctx->ndof);
+    }
+    HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_g, g, sizeof(double) * ctx->ndof, hipMemcpyHostToDevice, ctx->st));
+    cfdv_solve_opts ot = *o;
+    ot.transpose = 1;
+    const int rc = (o->mode == CFDV_SOLVE_FP64) ? solve_fp64_rhs(ctx, &ot, s, ctx->d_g, ctx->d_lam)
+                                                : solve_mixed_rhs(ctx, &ot, s, ctx->d_g, ctx->d_lam);
+    HIP_CHECK_THROW(hipMemcpyAsync(lambda, ctx->d_lam, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    return rc;
+    GUARD_END
+}
+
+int cfdv_build_vjp(cfdv_ctx* ctx, const double* lambda, const double* du, double* gU, double* gmach) {
+    if (!ctx || !lambda || !gU || !gmach) return CFDV_ERR_ARG;
+    if (ctx->ndim != 3) return fail(ctx, CFDV_ERR_ARG, "build tangents are implemented for hex8 meshes");
+    if (!ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_build_vjp before cfdv_build");
+    if (!du && !ctx->have_du) return fail(ctx, CFDV_ERR_ARG, "no du given and none solved");
+    GUARD_BEGIN
+    if (!ctx->d_gelem) {
+        ctx->d_gelem = ctx->alloc<double>((size_t)ctx->n_pad * 41);
+        ctx->d_duv = ctx->alloc<double>(ctx->ndof);
+        ctx->d_gU = ctx->alloc<double>(ctx->ndof);
+        if (!ctx->d_lam) {
+            ctx->d_g = ctx->alloc<double>(ctx->ndof);
+            ctx->d_lam = ctx->alloc<double>(ctx->ndof);
+        }
+    }
+    HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_lam, lambda, sizeof(double) * ctx->ndof, hipMemcpyHostToDevice, ctx->st));
+    if (du)
+        HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_duv, du, sizeof(double) * ctx->ndof, hipMemcpyHostToDevice, ctx->st));
+    else
+        vec_copy(ctx->d_duv, ctx->d_du, ctx->ndof, ctx->st);
+    BuildArgs a{};
+    a.th = ctx->th;
+    a.ndim = ctx->ndim;
+    a.n_elem_pad = ctx->n_pad;
+    a.conn = ctx->d_conn;
+    a.bc = ctx->d_bc;
+    a.cmask = ctx->d_cmask;
+    a.x = ctx->d_x;
+    a.U = ctx->d_U;
+    a.dt = ctx->last_dt;
+    launch_build_vjp(a, make_tables(), ctx->d_lam, ctx->d_duv, ctx->d_gelem, ctx->st);
+    launch_vjp_gather(ctx->d_node_ptr, ctx->d_node_ent, ctx->d_gelem, ctx->n_node, ctx->d_gU, ctx->st);
+    launch_column_sum(ctx->d_gelem, ctx->n_pad, 41, 40, ctx->d_part, NBLK_RED, ctx->d_small + 1000, ctx->st);
+    ctx->check_launch();
+    HIP_CHECK_THROW(hipMemcpyAsync(gU, ctx->d_gU, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    HIP_CHECK_THROW(hipMemcpyAsync(gmach, ctx->d_small + 1000, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
     return CFDV_OK;
     GUARD_END
 }

@@ -17,6 +17,8 @@
 
 namespace cfdv {
 
+struct GaussTables;
+
 struct BuildArgs {
     DevThermo th;
     int ndim;
@@ -38,6 +40,15 @@ struct BuildArgs {
 };
 
 void upload_gauss_tables(const GaussTables& g);
+// discrete adjoint of the hex8 build: gelem [n_elem_pad][41] = d phi_e /
+// d (U of the element's nodes, Mach), phi_e = lam_e^T (rhs_e - R_e du_e)
+void launch_build_vjp(const BuildArgs& a, const GaussTables& G, const double* lam, const double* du, double* gelem,
+                      hipStream_t st);
+void launch_vjp_gather(const int* node_ptr, const int* node_ent, const double* gelem, long n_node, double* gU,
+                       hipStream_t st);
+// out = sum_i a[i * stride + offset], deterministic (part: nblk doubles)
+void launch_column_sum(const double* a, long n, int stride, int offset, double* part, int nblk, double* out,
+                       hipStream_t st);
 void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st);
 void launch_flags(int ndim, const DevThermo& th, const int* conn, const double* U, const int* perm_inv,
                   long n_elem, uint8_t* flags, double* s1, int* ties, hipStream_t st);

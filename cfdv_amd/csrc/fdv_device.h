@@ -18,6 +18,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "dual.h"
+
 // Element-level physics is callable from host code too, so that the exact
 // device arithmetic can be exercised by a host-side debugging harness.
 #define CFDV_HD __host__ __device__ __forceinline__
@@ -62,33 +64,46 @@ template <int NDIM> struct Dims;
 template <> struct Dims<3> { static constexpr int NNOD = 8, NEQ = 5, NB = 4, NFACE = 6, NGP = 8, NFGP = 4; };
 template <> struct Dims<2> { static constexpr int NNOD = 4, NEQ = 4, NB = 2, NFACE = 4, NGP = 4, NFGP = 2; };
 
-CFDV_HD double calc_mu(const DevThermo& th, double T) {
+// Thermo with the Mach-dependent constants of scalar type S (double, or a
+// dual number for tangents with respect to the Mach number)
+template <typename S>
+struct ThermoT {
+    double gamma, gamm1, gm1d2, gamm3, gm3d2;
+    double Pr, csuth;
+    S Cv, cgas, cmach;
+    double creyn, Twall;
+    int adiabatic;
+};
+
+template <typename TH, typename S>
+CFDV_HD S calc_mu(const TH& th, const S& T) {
     // Thermo::calc_mu, Thermo.h:53-70
-    if (T < 1.0E-12) return 1.0;
+    if (T < 1.0E-12) return S(1.0);
     return ((1 + th.csuth) / (T + th.csuth)) * T * sqrt(T);
 }
-CFDV_HD double calc_k(const DevThermo& th, double mu) {
+template <typename TH, typename S>
+CFDV_HD S calc_k(const TH& th, const S& mu) {
     // Thermo::calc_k, Thermo.h:72-75
     return mu / ((th.gamma - 1.0) * (th.cmach * th.cmach) * th.Pr);
 }
 
 // Gauss-point state (Flow.h:9-62 with Tau)
-template <int NDIM> struct GPState {
+template <int NDIM, typename S = double> struct GPState {
     static constexpr int NEQ = Dims<NDIM>::NEQ;
-    double E, T;
-    double v[NDIM];
-    double U[NEQ];
-    double dU[NEQ][NDIM];
-    double dV[NDIM][NDIM];
-    double dT[NDIM];
-    double k, mu;
-    double tau[NDIM][NDIM];
+    S E, T;
+    S v[NDIM];
+    S U[NEQ];
+    S dU[NEQ][NDIM];
+    S dV[NDIM][NDIM];
+    S dT[NDIM];
+    S k, mu;
+    S tau[NDIM][NDIM];
 };
 
-template <int NDIM>
-CFDV_HD void calc_tau(GPState<NDIM>& f) {
+template <int NDIM, typename S>
+CFDV_HD void calc_tau(GPState<NDIM, S>& f) {
     // Tau::calculate, tau.h:14-42
-    double vkk = 0;
+    S vkk = 0;
 #pragma unroll
     for (int k = 0; k < NDIM; k++) vkk += f.dV[k][k];
 #pragma unroll
@@ -104,164 +119,22 @@ CFDV_HD void calc_tau(GPState<NDIM>& f) {
 // Flux Jacobians.  3D: NavierStokes3D.h:74-477, 2D: NavierStokes2D.h:59-222.
 // a[i][r][s] = dF_i/dU, b = dG_i/dU, c[i][j] = dG_i/dU_,j.
 // ---------------------------------------------------------------------------
-CFDV_HD void calc_a3(const DevThermo& th, const GPState<3>& f, double a[3][5][5]) {
-    const double gamma = th.gamma, gamma1 = th.gamm1, gamma3 = th.gamm3, gm3d2 = th.gm3d2, gm1d2 = th.gm1d2;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int r = 0; r < 5; r++)
-#pragma unroll
-            for (int s = 0; s < 5; s++) a[i][r][s] = 0.0;
-    const double u = f.v[0], v = f.v[1], w = f.v[2];
-    const double uu = u * u, uv = u * v, vv = v * v, ww = w * w, uw = u * w, vw = v * w;
-    const double E = f.E;
-    const double q2 = uu + vv + ww;
-    a[0][0][1] = 1.0;
-    a[0][1][0] = gm3d2 * uu + gm1d2 * (vv + ww);
-    a[0][1][1] = -gamma3 * u;
-    a[0][1][2] = -gamma1 * v;
-    a[0][1][3] = -gamma1 * w;
-    a[0][1][4] = gamma1;
-    a[0][2][0] = -uv; a[0][2][1] = v; a[0][2][2] = u;
-    a[0][3][0] = -uw; a[0][3][1] = w; a[0][3][3] = u;
-    a[0][4][0] = -gamma * E * u + gamma1 * u * q2;
-    a[0][4][1] = gamma * E - gm1d2 * (3. * uu + (vv + ww));
-    a[0][4][2] = -gamma1 * uv;
-    a[0][4][3] = -gamma1 * uw;
-    a[0][4][4] = gamma * u;
-
-    a[1][0][2] = 1.0;
-    a[1][1][0] = -uv; a[1][1][1] = v; a[1][1][2] = u;
-    a[1][2][0] = gm1d2 * (uu + ww) + gm3d2 * vv;
-    a[1][2][1] = -gamma1 * u;
-    a[1][2][2] = -gamma3 * v;
-    a[1][2][3] = -gamma1 * w;
-    a[1][2][4] = gamma1;
-    a[1][3][0] = -vw; a[1][3][2] = w; a[1][3][3] = v;
-    a[1][4][0] = -gamma * E * v + gamma1 * v * q2;
-    a[1][4][1] = -gamma1 * uv;
-    a[1][4][2] = gamma * E - gm1d2 * ((uu + ww) + 3. * vv);
-    a[1][4][3] = -gamma1 * vw;
-    a[1][4][4] = gamma * v;
-
-    a[2][0][3] = 1.0;
-    a[2][1][0] = -uw; a[2][1][1] = w; a[2][1][3] = u;
-    a[2][2][0] = -vw; a[2][2][2] = w; a[2][2][3] = v;
-    a[2][3][0] = gm1d2 * (uu + vv) + gm3d2 * ww;
-    a[2][3][1] = -gamma1 * u;
-    a[2][3][2] = -gamma1 * v;
-    a[2][3][3] = -gamma3 * w;
-    a[2][3][4] = gamma1;
-    a[2][4][0] = -gamma * E * w + gamma1 * w * q2;
-    a[2][4][1] = -gamma1 * uw;
-    a[2][4][2] = -gamma1 * vw;
-    a[2][4][3] = gamma * E - gm1d2 * ((uu + vv) + 3. * ww);
-    a[2][4][4] = gamma * w;
-}
-
-CFDV_HD void calc_b3(const DevThermo& th, const GPState<3>& f, double b[3][5][5]) {
-    // NavierStokes3D::calc_b, NavierStokes3D.h:193-343 (formulas evaluated
-    // with reinv = 1, then every entry scaled by 1/Re)
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int r = 0; r < 5; r++)
-#pragma unroll
-            for (int s = 0; s < 5; s++) b[i][r][s] = 0.0;
-    const double mu = f.mu, E = f.E;
-    const double lambda = -2. * mu / 3.;
-    const double mu_R = 2. * mu + lambda;
-    const double denom = 1. / (f.U[0] * f.U[0]);
-    const double TERM1 = f.k * denom / th.Cv;
-    const double rho = f.U[0];
-    const double u = f.v[0], v = f.v[1], w = f.v[2];
-    const double uu = u * u, vv = v * v, ww = w * w;
-    const double drdx = f.dU[0][0], dldx = f.dU[1][0], dmdx = f.dU[2][0], dndx = f.dU[3][0], dedx = f.dU[4][0];
-    const double drdy = f.dU[0][1], dldy = f.dU[1][1], dmdy = f.dU[2][1], dndy = f.dU[3][1], dedy = f.dU[4][1];
-    const double drdz = f.dU[0][2], dldz = f.dU[1][2], dmdz = f.dU[2][2], dndz = f.dU[3][2], dedz = f.dU[4][2];
-    const double l = f.U[1], m = f.U[2], n = f.U[3];
-    const double q2 = uu + vv + ww;
-    const double rinv = 1.0 / rho;
-
-    b[0][1][0] = denom * (mu_R * dldx + lambda * (dmdy + dndz) + mu_R * (-2. * u * drdx + v * drdy + w * drdz));
-    b[0][1][1] = mu_R * drdx * denom;
-    b[0][1][2] = lambda * drdy * denom;
-    b[0][1][3] = lambda * drdz * denom;
-    b[0][2][0] = mu * denom * (dldy + dmdx - 2. * (u * drdy + v * drdx));
-    b[0][2][1] = mu * drdy * denom;
-    b[0][2][2] = mu * drdx * denom;
-    b[0][3][0] = mu * denom * (dldz + dndx - 2. * (u * drdz + w * drdx));
-    b[0][3][1] = mu * drdz * denom;
-    b[0][3][3] = mu * drdx * denom;
-    b[0][4][0] = u * b[0][1][0] + v * b[0][2][0] + w * b[0][3][0] +
-                 (denom * (l * f.tau[0][0] + m * f.tau[0][1] + n * f.tau[0][2]) -
-                  TERM1 * (-dedx + (2. * E - 3. * q2) * drdx + 2. * (u * dldx + v * dmdx + w * dndx)));
-    b[0][4][1] = u * b[0][1][1] + v * b[0][2][1] + w * b[0][3][1] + (-f.tau[0][0] * rinv - TERM1 * (2. * u * drdx - dldx));
-    b[0][4][2] = u * b[0][1][2] + v * b[0][2][2] + w * b[0][3][2] + (-f.tau[0][1] * rinv - TERM1 * (-dmdx + 2. * v * drdx));
-    b[0][4][3] = u * b[0][1][3] + v * b[0][2][3] + w * b[0][3][3] + (-f.tau[0][2] * rinv - TERM1 * (2. * w * drdx - dndx));
-    b[0][4][4] = TERM1 * drdx;
-
-    b[1][1][0] = b[0][2][0];
-    b[1][1][1] = b[0][2][1];
-    b[1][1][2] = b[0][2][2];
-    b[1][2][0] = denom * (lambda * (dldx + dndz) + mu_R * (dmdy + u * drdx - 2. * v * drdy + w * drdz));
-    b[1][2][1] = lambda * drdx * denom;
-    b[1][2][2] = mu_R * drdy * denom;
-    b[1][2][3] = lambda * drdz * denom;
-    b[1][3][0] = mu * denom * (dmdz + dndy - 2. * (v * drdz + w * drdy));
-    b[1][3][2] = mu * denom * drdz;
-    b[1][3][3] = mu * denom * drdy;
-    b[1][4][0] = u * b[1][1][0] + v * b[1][2][0] + w * b[1][3][0] +
-                 (denom * (l * f.tau[1][0] + m * f.tau[1][1] + n * f.tau[1][2]) -
-                  TERM1 * (-dedy + (2. * E - 3. * q2) * drdy + 2. * (u * dldy + v * dmdy + w * dndy)));
-    b[1][4][1] = u * b[1][1][1] + v * b[1][2][1] + w * b[1][3][1] + (-f.tau[1][0] * rinv - TERM1 * (-dldy + 2. * u * drdy));
-    b[1][4][2] = u * b[1][1][2] + v * b[1][2][2] + w * b[1][3][2] + (-f.tau[1][1] * rinv - TERM1 * (2. * v * drdy - dmdy));
-    b[1][4][3] = u * b[1][1][3] + v * b[1][2][3] + w * b[1][3][3] + (-f.tau[1][2] * rinv - TERM1 * (2. * w * drdy - dndy));
-    b[1][4][4] = TERM1 * drdy;
-
-    b[2][1][0] = b[0][3][0];
-    b[2][1][1] = b[0][3][1];
-    b[2][1][3] = b[0][3][3];
-    b[2][2][0] = b[1][3][0];
-    b[2][2][2] = b[1][3][2];
-    b[2][2][3] = b[1][3][3];
-    b[2][3][0] = denom * (mu_R * (dndz + u * drdx + v * drdy - 2. * w * drdz) + lambda * (dldx + dmdy));
-    b[2][3][1] = lambda * drdx * denom;
-    b[2][3][2] = lambda * drdy * denom;
-    b[2][3][3] = mu_R * drdz * denom;
-    b[2][4][0] = u * b[2][1][0] + v * b[2][2][0] + w * b[2][3][0] +
-                 (denom * (l * f.tau[2][0] + m * f.tau[2][1] + n * f.tau[2][2]) -
-                  TERM1 * (-dedz + (2. * E - 3. * q2) * drdz + 2. * (u * dldz + v * dmdz + w * dndz)));
-    b[2][4][1] = u * b[2][1][1] + v * b[2][2][1] + w * b[2][3][1] + (-f.tau[2][0] * rinv - TERM1 * (2. * u * drdz - dldz));
-    b[2][4][2] = u * b[2][1][2] + v * b[2][2][2] + w * b[2][3][2] + (-f.tau[2][1] * rinv - TERM1 * (2. * v * drdz - dmdz));
-    b[2][4][3] = u * b[2][1][3] + v * b[2][2][3] + w * b[2][3][3] + (-f.tau[2][2] * rinv - TERM1 * (2. * w * drdz - dndz));
-    b[2][4][4] = TERM1 * drdz;
-
-    const double reinv = 1.0 / th.creyn;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int r = 0; r < 5; r++)
-#pragma unroll
-            for (int s = 0; s < 5; s++) b[i][r][s] *= reinv;
-}
-
 // Per-direction forms of calc_a/calc_b (3D) so that only one direction's
 // 2 x 25 coefficients are live at a time.  Entries shared between
 // directions in the reference (b(1,1,.) = b(0,2,.), b(2,1,.) = b(0,3,.),
 // b(2,2,.) = b(1,3,.), NavierStokes3D.h:275-277, 307-312) are recomputed
 // from the same expressions.
-template <int I>
-CFDV_HD void calc_a3_dir(const DevThermo& th, const GPState<3>& f, double a[5][5]) {
-    const double gamma = th.gamma, gamma1 = th.gamm1, gamma3 = th.gamm3, gm3d2 = th.gm3d2, gm1d2 = th.gm1d2;
+template <int I, typename TH, typename S>
+CFDV_HD void calc_a3_dir(const TH& th, const GPState<3, S>& f, S a[5][5]) {
+    const S gamma = th.gamma, gamma1 = th.gamm1, gamma3 = th.gamm3, gm3d2 = th.gm3d2, gm1d2 = th.gm1d2;
 #pragma unroll
     for (int r = 0; r < 5; r++)
 #pragma unroll
         for (int s = 0; s < 5; s++) a[r][s] = 0.0;
-    const double u = f.v[0], v = f.v[1], w = f.v[2];
-    const double uu = u * u, uv = u * v, vv = v * v, ww = w * w, uw = u * w, vw = v * w;
-    const double E = f.E;
-    const double q2 = uu + vv + ww;
+    const S u = f.v[0], v = f.v[1], w = f.v[2];
+    const S uu = u * u, uv = u * v, vv = v * v, ww = w * w, uw = u * w, vw = v * w;
+    const S E = f.E;
+    const S q2 = uu + vv + ww;
     if (I == 0) {
         a[0][1] = 1.0;
         a[1][0] = gm3d2 * uu + gm1d2 * (vv + ww);
@@ -294,23 +167,23 @@ CFDV_HD void calc_a3_dir(const DevThermo& th, const GPState<3>& f, double a[5][5
     }
 }
 
-template <int I>
-CFDV_HD void calc_b3_dir(const DevThermo& th, const GPState<3>& f, double b[5][5]) {
+template <int I, typename TH, typename S>
+CFDV_HD void calc_b3_dir(const TH& th, const GPState<3, S>& f, S b[5][5]) {
 #pragma unroll
     for (int r = 0; r < 5; r++)
 #pragma unroll
         for (int s = 0; s < 5; s++) b[r][s] = 0.0;
-    const double mu = f.mu, E = f.E;
-    const double lambda = -2. * mu / 3.;
-    const double mu_R = 2. * mu + lambda;
-    const double denom = 1. / (f.U[0] * f.U[0]);
-    const double TERM1 = f.k * denom / th.Cv;
-    const double rinv = 1.0 / f.U[0];
-    const double u = f.v[0], v = f.v[1], w = f.v[2];
-    const double q2 = u * u + v * v + w * w;
-    const double drdx = f.dU[0][0], dldx = f.dU[1][0], dmdx = f.dU[2][0], dndx = f.dU[3][0];
-    const double drdy = f.dU[0][1], dldy = f.dU[1][1], dmdy = f.dU[2][1], dndy = f.dU[3][1];
-    const double drdz = f.dU[0][2], dldz = f.dU[1][2], dmdz = f.dU[2][2], dndz = f.dU[3][2];
+    const S mu = f.mu, E = f.E;
+    const S lambda = -2. * mu / 3.;
+    const S mu_R = 2. * mu + lambda;
+    const S denom = 1. / (f.U[0] * f.U[0]);
+    const S TERM1 = f.k * denom / th.Cv;
+    const S rinv = 1.0 / f.U[0];
+    const S u = f.v[0], v = f.v[1], w = f.v[2];
+    const S q2 = u * u + v * v + w * w;
+    const S drdx = f.dU[0][0], dldx = f.dU[1][0], dmdx = f.dU[2][0], dndx = f.dU[3][0];
+    const S drdy = f.dU[0][1], dldy = f.dU[1][1], dmdy = f.dU[2][1], dndy = f.dU[3][1];
+    const S drdz = f.dU[0][2], dldz = f.dU[1][2], dmdz = f.dU[2][2], dndz = f.dU[3][2];
     // rows 1..3 of direction I (b(I, 1..3, .))
     if (I == 0) {
         b[1][0] = denom * (mu_R * dldx + lambda * (dmdy + dndz) + mu_R * (-2. * u * drdx + v * drdy + w * drdz));
@@ -335,7 +208,7 @@ CFDV_HD void calc_b3_dir(const DevThermo& th, const GPState<3>& f, double b[5][5
         b[3][1] = lambda * drdx * denom; b[3][2] = lambda * drdy * denom; b[3][3] = mu_R * drdz * denom;
     }
     // row 4 of direction I, derivative index X = I
-    const double drdX = f.dU[0][I], dldX = f.dU[1][I], dmdX = f.dU[2][I], dndX = f.dU[3][I], dedX = f.dU[4][I];
+    const S drdX = f.dU[0][I], dldX = f.dU[1][I], dmdX = f.dU[2][I], dndX = f.dU[3][I], dedX = f.dU[4][I];
     b[4][0] = u * b[1][0] + v * b[2][0] + w * b[3][0] +
               (denom * (f.U[1] * f.tau[I][0] + f.U[2] * f.tau[I][1] + f.U[3] * f.tau[I][2]) -
                TERM1 * (-dedX + (2. * E - 3. * q2) * drdX + 2. * (u * dldX + v * dmdX + w * dndX)));
@@ -343,7 +216,7 @@ CFDV_HD void calc_b3_dir(const DevThermo& th, const GPState<3>& f, double b[5][5
     b[4][2] = u * b[1][2] + v * b[2][2] + w * b[3][2] + (-f.tau[I][1] * rinv - TERM1 * (2. * v * drdX - dmdX));
     b[4][3] = u * b[1][3] + v * b[2][3] + w * b[3][3] + (-f.tau[I][2] * rinv - TERM1 * (2. * w * drdX - dndX));
     b[4][4] = TERM1 * drdX;
-    const double reinv = 1.0 / th.creyn;
+    const S reinv = 1.0 / th.creyn;
 #pragma unroll
     for (int r = 0; r < 5; r++)
 #pragma unroll
@@ -352,24 +225,24 @@ CFDV_HD void calc_b3_dir(const DevThermo& th, const GPState<3>& f, double b[5][5
 
 // c_ij (NavierStokes3D::calc_c, NavierStokes3D.h:347-477), contracted on
 // the fly:  out[r][s] += sum_ij W[i][j] c_ij[r][s].
-CFDV_HD void accum_c3(const DevThermo& th, const GPState<3>& f, const double W[3][3],
-                                         double out[5][5]) {
-    const double mu = f.mu;
-    const double lambda = -2. * mu / 3.;
-    const double mu_R = 2. * mu + lambda;
-    const double rho = f.U[0];
-    const double rinv = 1.0 / rho;
-    const double denom = 1. / (rho * rho);
-    const double l = f.U[1], m = f.U[2], n = f.U[3], re = f.U[4];
-    const double ll = l * l, mm = m * m, nn = n * n;
-    const double hkocv = f.k / th.Cv;
-    const double TERM1 = f.k / th.Cv * denom;
-    const double reinv = 1.0 / th.creyn;
-    const double lr = l * rinv, mr = m * rinv, nr = n * rinv;
-    const double en = TERM1 * (-re + (ll + mm + nn) * rinv);
+template <typename TH, typename S>
+CFDV_HD void accum_c3(const TH& th, const GPState<3, S>& f, const S W[3][3], S out[5][5]) {
+    const S mu = f.mu;
+    const S lambda = -2. * mu / 3.;
+    const S mu_R = 2. * mu + lambda;
+    const S rho = f.U[0];
+    const S rinv = 1.0 / rho;
+    const S denom = 1. / (rho * rho);
+    const S l = f.U[1], m = f.U[2], n = f.U[3], re = f.U[4];
+    const S ll = l * l, mm = m * m, nn = n * n;
+    const S hkocv = f.k / th.Cv;
+    const S TERM1 = f.k / th.Cv * denom;
+    const S reinv = 1.0 / th.creyn;
+    const S lr = l * rinv, mr = m * rinv, nr = n * rinv;
+    const S en = TERM1 * (-re + (ll + mm + nn) * rinv);
 
     // diagonal blocks c_ii
-    double c[5][5];
+    S c[5][5];
 #define CSET_DIAG(I, MUX, MUY, MUZ, LX, MX, NX)                                   \
     {                                                                             \
         for (int r = 0; r < 5; r++)                                               \
@@ -382,7 +255,7 @@ CFDV_HD void accum_c3(const DevThermo& th, const GPState<3>& f, const double W[3
         c[4][2] = mr * c[2][2] + TERM1 * m;                                       \
         c[4][3] = nr * c[3][3] + TERM1 * n;                                       \
         c[4][4] = (-hkocv * rinv);                                                \
-        const double wgt = W[I][I] * reinv;                                       \
+        const S wgt = W[I][I] * reinv;                                       \
         for (int r = 0; r < 5; r++)                                               \
             for (int s = 0; s < 5; s++) out[r][s] += wgt * c[r][s];               \
     }
@@ -396,18 +269,18 @@ CFDV_HD void accum_c3(const DevThermo& th, const GPState<3>& f, const double W[3
     // c(i,j,1+j,0) = mu * U(1+i) * denom,     c(i,j,1+j,1+i) = -mu/rho,
     // c(i,j,4,0)   = U(1+i)/rho c(i,j,1+i,0) + U(1+j)/rho c(i,j,1+j,0),
     // c(i,j,4,1+i) = U(1+j)/rho c(i,j,1+j,1+i), c(i,j,4,1+j) = U(1+i)/rho c(i,j,1+i,1+j).
-    const double Uc[3] = {l, m, n};
-    const double Ur[3] = {lr, mr, nr};
+    const S Uc[3] = {l, m, n};
+    const S Ur[3] = {lr, mr, nr};
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             if (i == j) continue;
-            const double wgt = W[i][j] * reinv;
-            const double c_i0 = lambda * Uc[j] * denom;
-            const double c_ij = (-lambda * rinv);
-            const double c_j0 = mu * Uc[i] * denom;
-            const double c_ji = (-mu * rinv);
+            const S wgt = W[i][j] * reinv;
+            const S c_i0 = lambda * Uc[j] * denom;
+            const S c_ij = (-lambda * rinv);
+            const S c_j0 = mu * Uc[i] * denom;
+            const S c_ji = (-mu * rinv);
             out[1 + i][0] += wgt * c_i0;
             out[1 + i][1 + j] += wgt * c_ij;
             out[1 + j][0] += wgt * c_j0;

@@ -192,6 +192,39 @@ class FDVContext:
         self._check(self.L.cfdv_profile_matvec(self.h, precision, reps, C.byref(ms)))
         return ms.value
 
+    # ---- discrete adjoint of one step ----
+    def set_thermo(self, thermo: Thermo):
+        """Replace the Thermo constants (e.g. the Mach number); rebuild after."""
+        t = _lib.ThermoS(thermo.gamma, thermo.mach, thermo.reynolds, thermo.prandtl,
+                         thermo.tinf, thermo.twall, int(thermo.adiabatic))
+        self._check(self.L.cfdv_set_thermo(self.h, C.byref(t)))
+        self.thermo = thermo
+
+    def adjoint_solve(self, g, mode="fp64", restarts=200, m=30, rtol=1e-12, inner_m=30,
+                      inner_rtol=1e-4, inner_restarts=4, precond="nodeblock"):
+        """lambda with K^T lambda = g for the blocks of the last build."""
+        md = {"fp64": _lib.CFDV_SOLVE_FP64, "mixed": _lib.CFDV_SOLVE_MIXED}[mode]
+        o = _lib.SolveOpts(md, restarts, m, rtol, inner_m, inner_rtol, inner_restarts, 1,
+                           _PRECOND[precond])
+        g = np.ascontiguousarray(g, dtype=np.float64).ravel()
+        lam = np.zeros(self.ndof)
+        st = _lib.SolveStats()
+        self._check(self.L.cfdv_adjoint_solve(self.h, C.byref(o), _dp(g), _dp(lam), C.byref(st)))
+        return lam, st
+
+    def build_vjp(self, lam, du=None):
+        """Gradient of Phi = lam^T (b - K du) with respect to the node state U
+        and the Mach number, at the state and dt of the last build."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64).ravel()
+        gU = np.zeros(self.ndof)
+        gM = C.c_double()
+        dup = None
+        if du is not None:
+            du = np.ascontiguousarray(du, dtype=np.float64).ravel()
+            dup = _dp(du)
+        self._check(self.L.cfdv_build_vjp(self.h, _dp(lam), dup, _dp(gU), C.byref(gM)))
+        return gU.reshape(self.mesh.n_node, self.mesh.neqn), gM.value
+
 
 class FDVGalerkin:
     """``FDVGalerkin<Element*>`` (src/FDV/FDVGalerkin.h:52-154) over the whole

@@ -35,6 +35,10 @@
  *   cfdv_color, cfdv_matvec, cfdv_profile_matvec: no reference counterpart
  *                        (element coloring for the atomic-free EBE matvec,
  *                        and test/measurement hooks).
+ *   cfdv_set_thermo      Thermo::set (FDV/Thermo.h:22-33) after create
+ *   cfdv_adjoint_solve, cfdv_build_vjp: no reference counterpart (discrete
+ *                        adjoint of one step: transpose solve, and the
+ *                        forward-mode tangents of the element build).
  */
 #ifndef CFDV_H
 #define CFDV_H
@@ -169,6 +173,23 @@ int cfdv_matvec(cfdv_ctx *ctx, const double *x, double *y, int precision, int tr
    on the context stream */
 int cfdv_profile_matvec(cfdv_ctx *ctx, int precision, int reps, double *ms);
 int cfdv_step(cfdv_ctx *ctx, double cfl, const cfdv_solve_opts *opts, cfdv_step_stats *st);
+
+/* ---- discrete adjoint of one step (no reference counterpart) ----------
+   For J = g^T U1 with U1 = U0 + du, K(U0, M) du = b(U0, M) at a fixed dt:
+     dJ/dU0 = g + gU,  dJ/dM = gM,  where K^T lambda = g and (gU, gM) is the
+   gradient of Phi = lambda^T (b - K du) from cfdv_build_vjp. */
+/* replace the Thermo constants (Thermo::set, FDV/Thermo.h:22-33); the
+   adiabatic flag must stay as at create; invalidates the build */
+int cfdv_set_thermo(cfdv_ctx *ctx, const cfdv_thermo *thermo);
+/* K^T lambda = g with the FP64 or mixed solver (opts->transpose ignored);
+   host arrays [n_node*neqn] */
+int cfdv_adjoint_solve(cfdv_ctx *ctx, const cfdv_solve_opts *opts, const double *g, double *lambda,
+                       cfdv_solve_stats *st);
+/* gradient of Phi = lambda^T (b - K du) with respect to the node state U
+   (gU [n_node][neqn]) and the Thermo Mach number (*gmach), at the state and
+   dt of the last cfdv_build, lambda and du held fixed (du NULL: the last
+   solved du).  hex8 meshes. */
+int cfdv_build_vjp(cfdv_ctx *ctx, const double *lambda, const double *du, double *gU, double *gmach);
 
 #ifdef __cplusplus
 }

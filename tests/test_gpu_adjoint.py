@@ -1,0 +1,175 @@
+"""Discrete adjoint of one FDV step (hex8) against finite differences.
+
+1. Build tangents: the gradient of Phi(U, M) = lam^T (b - K du) (du the
+   step's update, lam random) from cfdv_build_vjp against central
+   differences of Phi evaluated from the GPU build and, independently, from
+   the CPU oracle's build.
+2. Step gradient: J = g^T U1 for one step at a fixed dt; the adjoint
+   gradient (transpose GMRES + build tangents) against central differences
+   of full GPU steps along 16 seeded directions (random fields, wall-node
+   temperatures, the uniform wall temperature, the Mach number); relative
+   error <= 1e-5.
+3. The torch binding: backward of fdv_step equals the adjoint gradient.
+
+FDVParam's max/min over nodes and its clip at r = 1 make Phi piecewise
+smooth: a kink within h of the state makes a central difference average two
+slopes (seen once at h = 1e-6 along a random direction on the mixed-bc box,
+where h = 1e-7 then agrees with the tangent to 1e-7).  Each check therefore
+takes central differences at h and h/10 and requires the closer one to meet
+the tolerance; a wrong tangent misses at both.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from cfdv_amd import mesh as M
+from cfdv_amd.adjoint import step_fixed_dt, step_gradient, wall_temperature_gradient
+from cfdv_amd.fdv import FDVContext
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    if name == "box3d":
+        return M.mixed_bc_box(4, 3, 3, seed=5)
+    if name == "ramp":
+        return M.compression_ramp(6, 4, 3)
+    if name == "plate":
+        return M.flat_plate(6, 4, 3)
+    raise KeyError(name)
+
+
+def _phi_gpu(ctx, U, dt, lam, du):
+    ctx.set_state(U)
+    ctx.build(dt)
+    b = ctx.rhs_vector()
+    return lam @ b - lam @ ctx.matvec(du)
+
+
+def _phi_oracle(mesh, th, U, dt, lam, du):
+    R, rhs, _ = O.build(th, mesh, U, dt)
+    b = O.compose_rhs(mesh, rhs)
+    return lam @ b - lam @ O.matvec(mesh, R, du)
+
+
+def _fd_best(f, an, h=1e-6):
+    """relative error of the tangent ``an`` against central differences of
+    f at h and h/10 (the closer of the two) and the two differences"""
+    fds = [(f(hh) - f(-hh)) / (2 * hh) for hh in (h, h / 10)]
+    errs = [abs(an - fd) / max(abs(fd), 1e-300) for fd in fds]
+    return min(errs), fds
+
+
+def _wall_nodes(mesh):
+    fn = M.FACE_NODES_3D
+    nodes = set()
+    for e in range(mesh.n_elem):
+        for f in range(6):
+            if mesh.bc[e, f] in (1, 4, 11, 21, 31, 41):
+                nodes.update(mesh.conn[e, fn[f]].tolist())
+    return np.array(sorted(nodes), dtype=np.int64)
+
+
+@pytest.mark.parametrize("case", ["box3d", "ramp", "plate"])
+def test_build_vjp_against_finite_differences(gpu_available, case):
+    mesh, th, U = _case(case)
+    ctx = FDVContext(mesh, th, U)
+    dt = ctx.cfl(0.5)
+    rng = np.random.default_rng(3)
+    lam = rng.standard_normal(ctx.ndof)
+    # du = K^-1 b, so that Phi(U) itself is ~0 and central differences are
+    # not swamped by the rounding of a large Phi
+    ctx.build(dt)
+    ctx.solve("fp64", restarts=200, m=30, rtol=1e-13, precond="nodeblock")
+    du = ctx.du()
+    gU, gM = ctx.build_vjp(lam, du)
+    scale = np.abs(U).max(axis=0)
+    for k in range(4):
+        d = rng.standard_normal(U.shape) * scale
+        an = np.sum(gU * d)
+        eg, fdg = _fd_best(lambda h: _phi_gpu(ctx, U + h * d, dt, lam, du), an)
+        eo, fdo = _fd_best(lambda h: _phi_oracle(mesh, th, U + h * d, dt, lam, du), an)
+        print(case, "dir", k, "adjoint", an, "fd(gpu)", fdg, "fd(oracle)", fdo, "rel err", eg, eo)
+        assert eg <= 1e-6 and eo <= 1e-6
+
+    def phiM(h):
+        ctx.set_thermo(dataclasses.replace(th, mach=th.mach * (1 + h)))
+        return _phi_gpu(ctx, U, dt, lam, du)
+
+    eM, fdM = _fd_best(phiM, gM * th.mach)
+    ctx.set_thermo(th)
+    print(case, "dPhi/dM adjoint", gM, "fd", [v / th.mach for v in fdM], "rel err", eM)
+    assert eM <= 1e-6
+
+
+@pytest.mark.parametrize("case", ["box3d", "ramp"])
+def test_step_gradient_16_directions(gpu_available, case):
+    mesh, th, U0 = _case(case)
+    ctx = FDVContext(mesh, th, U0)
+    dt = ctx.cfl(0.5)
+    rng = np.random.default_rng(11)
+    g = rng.standard_normal(ctx.ndof)
+    gU0, gM, _ = step_gradient(ctx, U0, dt, g, rtol=1e-13)
+
+    # J = g^T U0 + g^T du: the first term is linear in U0 with gradient g, so
+    # the differences are taken of g^T du alone (no cancellation against the
+    # large g^T U0) and compared with dJ/dU0 - g
+    def Jdu(U, mach=th.mach):
+        ctx.set_thermo(dataclasses.replace(th, mach=mach))
+        _, du = step_fixed_dt(ctx, U, dt, rtol=1e-13)
+        return g @ du
+
+    walls = _wall_nodes(mesh)
+    scale = np.abs(U0).max(axis=0)
+    dirs = []
+    for k in range(10):                          # random fields
+        dirs.append(("field", rng.standard_normal(U0.shape) * scale))
+    for k in range(4):                           # wall-node temperatures (energy of wall nodes)
+        d = np.zeros_like(U0)
+        if len(walls):
+            d[walls, 4] = rng.standard_normal(len(walls)) * scale[4]
+        else:
+            d = rng.standard_normal(U0.shape) * scale
+        dirs.append(("wallT", d))
+    d = np.zeros_like(U0)                        # uniform wall temperature
+    if len(walls):
+        d[walls, 4] = U0[walls, 0] * th.Cv
+    else:
+        d[:, 4] = U0[:, 0] * th.Cv
+    dirs.append(("wallT_uniform", d))
+    worst = 0.0
+    gdu = gU0 - g.reshape(gU0.shape)
+    for name, d in dirs:
+        an = np.sum(gdu * d)
+        err, fds = _fd_best(lambda h: Jdu(U0 + h * d), an)
+        worst = max(worst, err)
+        print(case, name, "adjoint", an, "fd", fds, "rel err", err)
+        assert err <= 1e-5
+    errM, fdM = _fd_best(lambda h: Jdu(U0, th.mach * (1 + h)), gM * th.mach)
+    ctx.set_thermo(th)
+    print(case, "Mach adjoint", gM, "fd", [v / th.mach for v in fdM], "rel err", errM, "worst field", worst)
+    assert errM <= 1e-5
+    if len(walls):
+        gT = wall_temperature_gradient(gU0, U0, th.Cv, walls)
+        terms = gU0 * dirs[14][1]
+        assert abs(gT.sum() - terms.sum()) <= 1e-12 * np.abs(terms).sum()
+
+
+def test_torch_fdv_step_backward(gpu_available):
+    torch = pytest.importorskip("torch")
+    from cfdv_amd.adjoint import fdv_step
+    mesh, th, U0 = _case("box3d")
+    ctx = FDVContext(mesh, th, U0)
+    dt = ctx.cfl(0.5)
+    rng = np.random.default_rng(2)
+    g = rng.standard_normal(U0.shape)
+    Ut = torch.tensor(U0, dtype=torch.float64, requires_grad=True)
+    Mt = torch.tensor(th.mach, dtype=torch.float64, requires_grad=True)
+    U1 = fdv_step(Ut, Mt, ctx, dt)
+    (U1 * torch.from_numpy(g)).sum().backward()
+    gU0, gM, U1n = step_gradient(ctx, U0, dt, g.ravel())
+    assert np.allclose(U1.detach().numpy(), U1n, rtol=1e-12, atol=1e-14)
+    assert np.allclose(Ut.grad.numpy(), gU0, rtol=1e-10, atol=1e-12)
+    assert abs(Mt.grad.item() - gM) <= 1e-10 * max(abs(gM), 1.0)
