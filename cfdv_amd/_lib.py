@@ -43,7 +43,8 @@ class Mesh(C.Structure):
     _fields_ = [("ndim", C.c_int), ("nnod", C.c_int), ("neqn", C.c_int),
                 ("n_elem", C.c_int64), ("n_node", C.c_int64),
                 ("conn", C.POINTER(C.c_int32)), ("x", C.POINTER(C.c_double)),
-                ("bc", C.POINTER(C.c_int32)), ("dirichlet", C.POINTER(C.c_uint8))]
+                ("bc", C.POINTER(C.c_int32)), ("dirichlet", C.POINTER(C.c_uint8)),
+                ("adap", C.POINTER(C.c_uint8)), ("hnm", C.POINTER(C.c_double))]
 
 
 class ThermoS(C.Structure):

@@ -521,7 +521,9 @@ __global__ __launch_bounds__(64) void build3d_interior_kernel(BuildArgs A) {
         for (int fc = 0; fc < 6; fc++) boundary |= (bcs[fc] != 0);
         // with FP64 blocks, a boundary element's domain part is written here
         // (FP64 R and rhs) and build3d_face_kernel adds faces and ApplyBC;
-        // without, the per-lane kernel builds it whole
+        // without, the per-lane kernel builds it whole.  Hanging-node
+        // elements are built whole by the per-lane kernel.
+        if (A.adap && A.adap[e]) return;
         if (boundary && !W64) return;
     } else {   // padding slot of a color range: zero block
         if (W64)
@@ -638,16 +640,15 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
 }
 
 template <bool W64, bool W32>
-__global__ __launch_bounds__(256) void build3d_kernel(BuildArgs A) {
+__global__ __launch_bounds__(256) void build3d_kernel(BuildArgs A, const int* list, long n_list) {
     constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600;
     __shared__ double s_nd[4][NNOD][ND_STRIDE];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    // with a slot list (elements with faces or constrained dofs), entry i of
-    // the list; otherwise every slot
+    // with a slot list, entry i of the list; otherwise every slot
     const long i_ = (long)blockIdx.x * 4 + wv;
-    const long e = A.bslots ? (i_ < A.n_bslots ? (long)A.bslots[i_] : A.n_elem_pad)
-                            : (long)xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
+    const long e = list ? (i_ < n_list ? (long)list[i_] : A.n_elem_pad)
+                        : (long)xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
     const DevThermo th = A.th;
     const bool in_range = e < A.n_elem_pad;
     const int* cn = A.conn + (in_range ? e : 0) * NNOD;
@@ -1097,7 +1098,16 @@ void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st) {
         if (a.n_bslots > 0) {
             if (w64 && w32) hipLaunchKernelGGL((build3d_face_kernel<true>), dim3((int)a.n_bslots), dim3(64), 0, st, a);
             else if (w64) hipLaunchKernelGGL((build3d_face_kernel<false>), dim3((int)a.n_bslots), dim3(64), 0, st, a);
-            else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3((int)((a.n_bslots + 3) / 4)), dim3(256), 0, st, a);
+            else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3((int)((a.n_bslots + 3) / 4)), dim3(256), 0,
+                                    st, a, a.bslots, a.n_bslots);
+        }
+        // hanging-node elements (Hnm-transformed test functions): whole
+        // element on the per-lane kernel
+        if (a.n_aslots > 0) {
+            const int nba = (int)((a.n_aslots + 3) / 4);
+            if (w64 && w32) hipLaunchKernelGGL((build3d_kernel<true, true>), dim3(nba), dim3(256), 0, st, a, a.aslots, a.n_aslots);
+            else if (w64) hipLaunchKernelGGL((build3d_kernel<true, false>), dim3(nba), dim3(256), 0, st, a, a.aslots, a.n_aslots);
+            else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3(nba), dim3(256), 0, st, a, a.aslots, a.n_aslots);
         }
     } else {
         const int nb = (int)((a.n_elem_pad * 16 + 255) / 256);

@@ -132,6 +132,10 @@ struct cfdv_ctx {
     unsigned long long* d_cmask = nullptr;
     int* d_bslots = nullptr;        // 3D slots with faces / constrained dofs
     long n_bslots = 0;
+    unsigned char* d_adap = nullptr;  // hanging-node flags per slot (or none)
+    double* d_hnm = nullptr;          // [n_pad][64] Hnm(N, M) at N + 8 M
+    int* d_aslots = nullptr;
+    long n_aslots = 0;
     double *d_x = nullptr, *d_U = nullptr, *d_U0 = nullptr, *d_area = nullptr, *d_cleng = nullptr;
     double *d_R64 = nullptr;
     float* d_R32 = nullptr;
@@ -402,17 +406,34 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
         }
         cmask[s] = mk;
     }
-    // 3D slots built by the full per-lane kernel: any face code or
-    // constrained dof (the interior kernel skips exactly these)
-    std::vector<int> bslots;
+    // 3D slot lists: faces or constrained dofs (face kernel on top of the
+    // domain part) and hanging-node elements (whole element per lane)
+    std::vector<int> bslots, aslots;
+    std::vector<unsigned char> adap_s;
+    std::vector<double> hnm_s;
+    if (m->adap) {
+        adap_s.assign((size_t)np, 0);
+        hnm_s.assign((size_t)np * 64, 0.0);
+        for (long s = 0; s < np; s++) {
+            const int e = c->perm[s];
+            if (e < 0 || !m->adap[e]) continue;
+            adap_s[s] = 1;
+            std::memcpy(hnm_s.data() + (size_t)s * 64, m->hnm + (size_t)e * 64, sizeof(double) * 64);
+        }
+    }
     if (c->ndim == 3)
         for (long s = 0; s < np; s++) {
             if (c->perm[s] < 0) continue;
+            if (!adap_s.empty() && adap_s[s]) {
+                aslots.push_back((int)s);
+                continue;
+            }
             bool b = cmask[s] != 0ULL;
             for (int f = 0; f < nf; f++) b = b || bc_s[s * nf + f] != 0;
             if (b) bslots.push_back((int)s);
         }
     c->n_bslots = (long)bslots.size();
+    c->n_aslots = (long)aslots.size();
     // compose CSR entries point at slots; lists stay in original element order
     for (size_t i = 0; i < ent.size(); i++) {
         const int e = ent[i] / nn, l = ent[i] % nn;
@@ -435,6 +456,16 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
     if (c->n_bslots > 0) {
         c->d_bslots = c->alloc<int>((size_t)c->n_bslots);
         HIP_CHECK_THROW(hipMemcpy(c->d_bslots, bslots.data(), sizeof(int) * c->n_bslots, hipMemcpyHostToDevice));
+    }
+    if (!adap_s.empty()) {
+        c->d_adap = c->alloc<unsigned char>((size_t)np);
+        c->d_hnm = c->alloc<double>((size_t)np * 64);
+        HIP_CHECK_THROW(hipMemcpy(c->d_adap, adap_s.data(), (size_t)np, hipMemcpyHostToDevice));
+        HIP_CHECK_THROW(hipMemcpy(c->d_hnm, hnm_s.data(), sizeof(double) * np * 64, hipMemcpyHostToDevice));
+    }
+    if (c->n_aslots > 0) {
+        c->d_aslots = c->alloc<int>((size_t)c->n_aslots);
+        HIP_CHECK_THROW(hipMemcpy(c->d_aslots, aslots.data(), sizeof(int) * c->n_aslots, hipMemcpyHostToDevice));
     }
     HIP_CHECK_THROW(hipMemcpy(c->d_x, m->x, sizeof(double) * nnode * c->ndim, hipMemcpyHostToDevice));
     HIP_CHECK_THROW(hipMemcpy(c->d_node_ptr, ptr.data(), sizeof(int) * (nnode + 1), hipMemcpyHostToDevice));
@@ -803,6 +834,7 @@ int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flag
         return CFDV_ERR_ARG;
     if (mesh->n_elem <= 0 || mesh->n_node <= 0 || !mesh->conn || !mesh->x || !mesh->bc || !mesh->dirichlet)
         return CFDV_ERR_ARG;
+    if (mesh->adap && (!mesh->hnm || mesh->ndim != 3)) return CFDV_ERR_ARG;   // Hnm elements: hex8 only
     if (!(block_flags & (CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32))) return CFDV_ERR_ARG;
     cfdv_ctx* ctx = new (std::nothrow) cfdv_ctx();
     if (!ctx) return CFDV_ERR_HIP;
@@ -911,6 +943,10 @@ int cfdv_build(cfdv_ctx* ctx, double dt) {
     a.status = ctx->d_counters + 1;
     a.bslots = ctx->d_bslots;
     a.n_bslots = ctx->n_bslots;
+    a.adap = ctx->d_adap;
+    a.hnm = ctx->d_hnm;
+    a.aslots = ctx->d_aslots;
+    a.n_aslots = ctx->n_aslots;
     launch_build(a, ctx->d_R64 != nullptr, ctx->d_R32 != nullptr, ctx->st);
     ctx->check_launch();
     int cnt[2];
@@ -1207,6 +1243,8 @@ int cfdv_build_vjp(cfdv_ctx* ctx, const double* lambda, const double* du, double
     a.x = ctx->d_x;
     a.U = ctx->d_U;
     a.dt = ctx->last_dt;
+    a.adap = ctx->d_adap;
+    a.hnm = ctx->d_hnm;
     launch_build_vjp(a, make_tables(), ctx->d_lam, ctx->d_duv, ctx->d_gelem, ctx->st);
     launch_vjp_gather(ctx->d_node_ptr, ctx->d_node_ent, ctx->d_gelem, ctx->n_node, ctx->d_gU, ctx->st);
     launch_column_sum(ctx->d_gelem, ctx->n_pad, 41, 40, ctx->d_part, NBLK_RED, ctx->d_small + 1000, ctx->st);

@@ -35,8 +35,12 @@ struct BuildArgs {
     double* svals;            // [n_elem_pad][4] or nullptr
     int* ties;                // count of FDVParam r == 0.01 ties
     int* status;              // bit 0: unknown bc code, bit 1: bad face detjac
-    const int* bslots;        // 3D: slots with faces or constrained dofs (built
-    long n_bslots;            //     by the full per-lane kernel), ascending
+    const int* bslots;        // 3D: slots with faces or constrained dofs and no
+    long n_bslots;            //     Hnm (face kernel), ascending
+    const unsigned char* adap;  // [n_elem_pad] hanging-node element flags or nullptr
+    const double* hnm;        // [n_elem_pad][64] Hnm(N, M) at N + 8 M (adapted slots)
+    const int* aslots;        // 3D adapted slots (full per-lane kernel), ascending
+    long n_aslots;
 };
 
 void upload_gauss_tables(const GaussTables& g);

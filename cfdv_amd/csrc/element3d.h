@@ -246,6 +246,7 @@ CFDV_HD void build3d_faces(const BuildArgs& A, const TH& th, const GaussTables& 
     const double reinv = 1.0 / th.creyn;
     (void)s1; (void)s2; (void)s4;
     const int* bcs = A.bc + e * 6;
+    const double* Hn = (A.adap && A.adap[e]) ? A.hnm + e * 64 : nullptr;
     int anyface = 0;
 #pragma unroll
     for (int fc = 0; fc < 6; fc++) anyface |= (bcs[fc] != 0 && bcs[fc] < 100);
@@ -265,6 +266,17 @@ CFDV_HD void build3d_faces(const BuildArgs& A, const TH& th, const GaussTables& 
 #pragma unroll
                     for (int i = 0; i < 3; i++) nl[t][i] = value_of(nd[fl[t]][ND_X + i]);
                 }
+                if (Hn) {
+                    // face corners from the Hnm-mapped element nodes
+                    // (calc_TF_norm3D.h:169-200)
+                    for (int t = 0; t < 4; t++)
+#pragma unroll
+                        for (int i = 0; i < 3; i++) {
+                            double acc = 0.0;
+                            for (int m = 0; m < 8; m++) acc += Hn[fl[t] + 8 * m] * value_of(nd[m][ND_X + i]);
+                            nl[t][i] = acc;
+                        }
+                }
                 // TFNorm3D (calc_TF_norm3D.h:143-301), including the Lambda/Mu
                 // magnitude recomputed after each component (:208-212)
                 double phib[4];
@@ -272,6 +284,15 @@ CFDV_HD void build3d_faces(const BuildArgs& A, const TH& th, const GaussTables& 
                 phib[1] = 0.25 * (1 + z0) * (1 - z1);
                 phib[2] = 0.25 * (1 + z0) * (1 + z1);
                 phib[3] = 0.25 * (1 - z0) * (1 + z1);
+                if (Hn) {
+                    // adap_face (MeshRefine2D.h:903-952): only phibou is
+                    // carried over, dpidx stays untransformed
+                    double pt[4] = {0.0, 0.0, 0.0, 0.0};
+                    for (int n = 0; n < 4; n++)
+                        for (int m = 0; m < 4; m++) pt[m] += phib[n] * Hn[fl[n] + 8 * fl[m]];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) phib[t] = pt[t];
+                }
                 double Lm[3], Mu[3];
 #pragma unroll
                 for (int i = 0; i < 3; i++) { Lm[i] = nl[1][i] - nl[0][i]; Mu[i] = nl[3][i] - nl[0][i]; }
@@ -488,8 +509,34 @@ CFDV_HD void build3d_elem(const BuildArgs& A, const TH& th, const GaussTables& G
         for (int s = 0; s < NEQ; s++) R[r][s] = 0.0;
     }
 
+    // hanging-node element: test functions Phi Hnm (adap(), MeshRefine2D.h:877-900)
+    const double* Hn = (A.adap && A.adap[e]) ? A.hnm + e * 64 : nullptr;
+
     // ---------------- domain integral: 8 Gauss points ----------------
     for (int gp = 0; gp < 8; gp++) {
+        double phg[8], dpg[8][3];
+        if (Hn) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                phg[m] = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; j++) dpg[m][j] = 0.0;
+            }
+            for (int n = 0; n < 8; n++)
+                for (int m = 0; m < 8; m++) {
+                    const double h = Hn[n + 8 * m];
+                    phg[m] += G.phi3[gp][n] * h;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) dpg[m][j] += G.dpxi3[gp][n][j] * h;
+                }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                phg[m] = G.phi3[gp][m];
+#pragma unroll
+                for (int j = 0; j < 3; j++) dpg[m][j] = G.dpxi3[gp][m][j];
+            }
+        }
         double jac[3][3];
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -497,7 +544,7 @@ CFDV_HD void build3d_elem(const BuildArgs& A, const TH& th, const GaussTables& G
             for (int j = 0; j < 3; j++) {
                 double acc = 0.0;
 #pragma unroll
-                for (int n = 0; n < 8; n++) acc += G.dpxi3[gp][n][j] * value_of(nd[n][ND_X + i]);
+                for (int n = 0; n < 8; n++) acc += dpg[n][j] * value_of(nd[n][ND_X + i]);
                 jac[i][j] = acc;
             }
         const double J = jac[0][0] * jac[1][1] * jac[2][2] + jac[0][1] * jac[1][2] * jac[2][0] +
@@ -538,10 +585,10 @@ CFDV_HD void build3d_elem(const BuildArgs& A, const TH& th, const GaussTables& G
             for (int i = 0; i < 3; i++) {
                 double acc = 0.0;
 #pragma unroll
-                for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * G.dpxi3[gp][n][j]);
+                for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * dpg[n][j]);
                 dphi[i] = acc;
             }
-            const double ph = G.phi3[gp][n];
+            const double ph = phg[n];
             if (n == N) { PN = ph; DN[0] = dphi[0]; DN[1] = dphi[1]; DN[2] = dphi[2]; }
             if (n == M) { PM = ph; DM[0] = dphi[0]; DM[1] = dphi[1]; DM[2] = dphi[2]; }
             SPhi += ph;

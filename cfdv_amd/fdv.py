@@ -50,10 +50,18 @@ class FDVContext:
         self._x = np.ascontiguousarray(mesh.x, dtype=np.float64)
         self._bc = np.ascontiguousarray(mesh.bc, dtype=np.int32)
         self._d = np.ascontiguousarray(mesh.dirichlet, dtype=np.uint8)
+        self._adap = None if mesh.adap is None else np.ascontiguousarray(mesh.adap, dtype=np.uint8)
+        self._hnm = None
+        if mesh.hnm is not None:
+            # Hnm(N, M) at N + nnod*M (the reference Tensor's column-major order)
+            self._hnm = np.ascontiguousarray(np.transpose(mesh.hnm, (0, 2, 1)), dtype=np.float64)
         m = _lib.Mesh(mesh.ndim, mesh.nnod, mesh.neqn, mesh.n_elem, mesh.n_node,
                       self._conn.ctypes.data_as(C.POINTER(C.c_int32)), _dp(self._x),
                       self._bc.ctypes.data_as(C.POINTER(C.c_int32)),
-                      self._d.ctypes.data_as(C.POINTER(C.c_uint8)))
+                      self._d.ctypes.data_as(C.POINTER(C.c_uint8)),
+                      self._adap.ctypes.data_as(C.POINTER(C.c_uint8)) if self._adap is not None
+                      else C.cast(None, C.POINTER(C.c_uint8)),
+                      _dp(self._hnm) if self._hnm is not None else C.cast(None, C.POINTER(C.c_double)))
         t = _lib.ThermoS(thermo.gamma, thermo.mach, thermo.reynolds, thermo.prandtl,
                          thermo.tinf, thermo.twall, int(thermo.adiabatic))
         flags = (_lib.CFDV_BLOCKS_FP64 if "fp64" in blocks else 0) | \

@@ -49,6 +49,8 @@ struct MeshArrays {
     std::vector<int32_t> conn, bc;
     std::vector<double> x;
     std::vector<uint8_t> dirichlet;
+    std::vector<uint8_t> adap;   // empty when no element is adapted
+    std::vector<double> hnm;
 
     cfdv_mesh view() const {
         cfdv_mesh m;
@@ -61,6 +63,8 @@ struct MeshArrays {
         m.x = x.data();
         m.bc = bc.data();
         m.dirichlet = dirichlet.data();
+        m.adap = adap.empty() ? nullptr : adap.data();
+        m.hnm = adap.empty() ? nullptr : hnm.data();
         return m;
     }
 };

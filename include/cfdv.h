@@ -74,6 +74,11 @@ typedef struct {
                               get_face.h:80-165)                                  */
     const uint8_t *dirichlet; /* [n_node][neqn] Node::dirichlet (LoadBinary.h:
                               277-316)                                            */
+    const uint8_t *adap;   /* [n_elem] Element::adap (hanging-node elements of a
+                              refined mesh, FDV/Element.h:39-41) or NULL         */
+    const double *hnm;     /* [n_elem][nnod*nnod] Element::Hnm, Hnm(N, M) at
+                              N + nnod*M (Tensor column-major), read where adap
+                              is set; NULL when adap is NULL.  hex8 only        */
 } cfdv_mesh;
 
 typedef struct {          /* the aero deck keys (MPIBinaryConfiguration.h:54-63) */

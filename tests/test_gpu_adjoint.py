@@ -173,3 +173,26 @@ def test_torch_fdv_step_backward(gpu_available):
     assert np.allclose(U1.detach().numpy(), U1n, rtol=1e-12, atol=1e-14)
     assert np.allclose(Ut.grad.numpy(), gU0, rtol=1e-10, atol=1e-12)
     assert abs(Mt.grad.item() - gM) <= 1e-10 * max(abs(gM), 1.0)
+
+
+def test_build_vjp_on_hanging_node_elements(gpu_available):
+    rng = np.random.default_rng(8)
+    m, th, U = M.mixed_bc_box(4, 3, 3, seed=6)
+    adap = (rng.random(m.n_elem) < 0.5).astype(np.uint8)
+    A = rng.standard_normal((m.n_elem, 8, 8))
+    H = np.tile(np.eye(8), (m.n_elem, 1, 1)) + 0.02 * (A - A.mean(axis=2, keepdims=True))
+    mesh = M.Mesh(3, 8, 5, m.conn, m.x, m.bc, m.dirichlet, adap=adap, hnm=H)
+    ctx = FDVContext(mesh, th, U)
+    dt = ctx.cfl(0.5)
+    ctx.build(dt)
+    ctx.solve("fp64", restarts=200, m=30, rtol=1e-13, precond="nodeblock")
+    du = ctx.du()
+    lam = rng.standard_normal(ctx.ndof)
+    gU, gM = ctx.build_vjp(lam, du)
+    scale = np.abs(U).max(axis=0)
+    for k in range(3):
+        d = rng.standard_normal(U.shape) * scale
+        an = np.sum(gU * d)
+        eo, fdo = _fd_best(lambda h: _phi_oracle(mesh, th, U + h * d, dt, lam, du), an)
+        print("adapted dir", k, "adjoint", an, "fd(oracle)", fdo, "rel err", eo)
+        assert eo <= 1e-6

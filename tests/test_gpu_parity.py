@@ -201,3 +201,35 @@ def test_preconditioner_rejected_in_reference_mode(gpu_available):
     ctx.build(ctx.cfl(0.5))
     _, rc = ctx.solve("reference", restarts=1, m=10, precond="nodeblock", check=False)
     assert rc == -1
+
+
+def _adapted_box(seed=4):
+    m, th, U = M.mixed_bc_box(6, 5, 4, seed=seed)
+    rng = np.random.default_rng(seed)
+    adap = (rng.random(m.n_elem) < 0.35).astype(np.uint8)
+    # rows summing to one (partition of unity kept), a small perturbation of
+    # the identity so that the mapped faces stay valid
+    A = rng.standard_normal((m.n_elem, 8, 8))
+    H = np.tile(np.eye(8), (m.n_elem, 1, 1)) + 0.02 * (A - A.mean(axis=2, keepdims=True))
+    ma = M.Mesh(3, 8, 5, m.conn, m.x, m.bc, m.dirichlet, adap=adap, hnm=H)
+    return ma, th, U
+
+
+def test_hanging_node_elements_build_parity(gpu_available):
+    """Elements with Hnm (adap(), adap_face(); MeshRefine2D.h:877-952) on
+    interior and boundary slots: GPU blocks and rhs against the oracle."""
+    m, th, U = _adapted_box()
+    print("adapted elements", int(m.adap.sum()), "of", m.n_elem)
+    ctx = FDVContext(m, th, U, blocks=("fp64", "fp32"))
+    dt = ctx.cfl(0.5)
+    ctx.build(dt)
+    R, rhs, s = ctx.blocks()
+    Ro, rhso, so = O.build(th, m, U, dt)
+    err = _rel_fro(R, Ro)
+    print("R rel err max", err.max(), "adapted", err[m.adap == 1].max())
+    assert err.max() < 1e-11
+    assert np.linalg.norm(rhs - rhso) <= 1e-10 * np.linalg.norm(rhso)
+    assert np.array_equal(ctx.blocks32(), R.astype(np.float32))
+    # the step runs end to end on the adapted mesh
+    st, _ = ctx.solve("fp64", restarts=60, m=30, rtol=1e-10, precond="nodeblock")
+    assert st.rel_resid <= 1e-10

@@ -329,3 +329,40 @@ def test_eval2dboundary_edits(adiabatic):
     # unknown code is an error
     rc, _, _, _ = run(99)
     assert rc == -1
+
+
+def test_hnm_permutation_is_a_node_relabelling():
+    """adap() (MeshRefine2D.h:877-900) maps Phi -> Phi Hnm.  With Hnm the
+    permutation matrix of a cube rotation pi (Hnm(N, M) = 1 for M = pi(N)),
+    an interior element behaves as the unadapted element whose reference
+    corner k carries global node conn[pi(k)]: R_adapted(M, M') =
+    R_relabelled(pi^-1(M), pi^-1(M')) block by block."""
+    m, th, U = M.mixed_bc_box(4, 4, 4, seed=21)
+    pi = np.array([1, 2, 3, 0, 5, 6, 7, 4])          # quarter turn about z
+    inv = np.argsort(pi)
+    interior = np.nonzero((m.bc == 0).all(axis=1))[0]
+    adap = np.zeros(m.n_elem, np.uint8)
+    adap[interior] = 1
+    H = np.zeros((m.n_elem, 8, 8))
+    H[:] = np.eye(8)
+    for e in interior:
+        H[e] = 0.0
+        H[e, np.arange(8), pi] = 1.0                 # H[e, N, pi(N)] = 1
+    ma = M.Mesh(3, 8, 5, m.conn, m.x, m.bc, m.dirichlet, adap=adap, hnm=H)
+    conn_r = m.conn.copy()
+    conn_r[interior] = m.conn[interior][:, pi]       # conn'[k] = conn[pi(k)]
+    mr = M.Mesh(3, 8, 5, conn_r, m.x, m.bc, m.dirichlet)
+    dt = 1e-3
+    Ra, rhsa, _ = O.build(th, ma, U, dt, apply_bc=False)
+    Rr, rhsr, _ = O.build(th, mr, U, dt, apply_bc=False)
+    idx = (inv[:, None] * 5 + np.arange(5)).ravel()  # row M of Ra <- row pi^-1(M) of Rr
+    for e in interior:
+        expect = Rr[e][np.ix_(idx, idx)]
+        assert np.allclose(Ra[e], expect, rtol=1e-12, atol=1e-14 * np.abs(expect).max())
+        assert np.allclose(rhsa[e], rhsr[e][idx], rtol=1e-10, atol=1e-12 * np.abs(rhsr[e]).max())
+    # identity Hnm changes nothing anywhere
+    mi = M.Mesh(3, 8, 5, m.conn, m.x, m.bc, m.dirichlet, adap=np.ones(m.n_elem, np.uint8),
+                hnm=np.tile(np.eye(8), (m.n_elem, 1, 1)))
+    Ri, rhsi, _ = O.build(th, mi, U, dt)
+    R0, rhs0, _ = O.build(th, m, U, dt)
+    assert np.array_equal(Ri, R0) and np.array_equal(rhsi, rhs0)
