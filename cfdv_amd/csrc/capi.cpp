@@ -1256,6 +1256,60 @@ int cfdv_build_vjp(cfdv_ctx* ctx, const double* lambda, const double* du, double
     GUARD_END
 }
 
+// Y = K X (or K^T X) for 8 vectors at once with the FP32 blocks
+static void matvec_multi_dev(cfdv_ctx* ctx, const float* X, float* Y, bool tr) {
+    vec_fill_f(Y, 0.0f, ctx->ndof * 8, ctx->st);
+    for (int c = 0; c < ctx->ncolor; c++) {
+        MatvecArgs a{ctx->ndim, ctx->nnod, ctx->neqn, ctx->cstart[c], ctx->cstart[c + 1], ctx->d_conn};
+        launch_matvec_multi_color(a, ctx->d_R32, X, Y, tr, ctx->st);
+    }
+    ctx->check_launch();
+}
+
+int cfdv_matvec_multi(cfdv_ctx* ctx, const float* X, float* Y, int nvec, int transpose) {
+    if (!ctx || !X || !Y || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_matvec_multi before cfdv_build");
+    if (nvec != 8) return fail(ctx, CFDV_ERR_ARG, "cfdv_matvec_multi takes 8 vectors");
+    if (ctx->ndim != 3 || !ctx->d_R32) return fail(ctx, CFDV_ERR_ARG, "needs hex8 FP32 blocks");
+    GUARD_BEGIN
+    const size_t n = (size_t)ctx->ndof * 8;
+    float* dX = ctx->alloc<float>(n);
+    float* dY = ctx->alloc<float>(n);
+    HIP_CHECK_THROW(hipMemcpyAsync(dX, X, sizeof(float) * n, hipMemcpyHostToDevice, ctx->st));
+    matvec_multi_dev(ctx, dX, dY, transpose != 0);
+    HIP_CHECK_THROW(hipMemcpyAsync(Y, dY, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    ctx->allocs.pop_back();
+    ctx->allocs.pop_back();
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_profile_matvec_multi(cfdv_ctx* ctx, int reps, double* ms) {
+    if (!ctx || !ms || reps < 1 || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "profile before build");
+    if (ctx->ndim != 3 || !ctx->d_R32) return fail(ctx, CFDV_ERR_ARG, "needs hex8 FP32 blocks");
+    GUARD_BEGIN
+    const size_t n = (size_t)ctx->ndof * 8;
+    float* dX = ctx->alloc<float>(n);
+    float* dY = ctx->alloc<float>(n);
+    vec_fill_f(dX, 1.0f, (long)n, ctx->st);
+    matvec_multi_dev(ctx, dX, dY, false);   // warm
+    HIP_CHECK_THROW(hipEventRecord(ctx->ev[0], ctx->st));
+    for (int i = 0; i < reps; i++) matvec_multi_dev(ctx, dX, dY, false);
+    HIP_CHECK_THROW(hipEventRecord(ctx->ev[1], ctx->st));
+    HIP_CHECK_THROW(hipEventSynchronize(ctx->ev[1]));
+    float t = 0;
+    HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
+    *ms = t / reps;
+    ctx->allocs.pop_back();
+    ctx->allocs.pop_back();
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    return CFDV_OK;
+    GUARD_END
+}
+
 int cfdv_step(cfdv_ctx* ctx, double cfl, const cfdv_solve_opts* o, cfdv_step_stats* st) {
     if (!ctx || !o || !st) return CFDV_ERR_ARG;
     std::memset(st, 0, sizeof(*st));

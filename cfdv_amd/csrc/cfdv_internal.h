@@ -70,6 +70,10 @@ struct MatvecArgs {
 // y += sum over elements of one color of R_e x_e (or R_e^T x_e)
 void launch_matvec_color(const MatvecArgs& a, const double* R64, const float* R32, const void* x, void* y,
                          int prec, bool transpose, hipStream_t st);
+// hex8 EBE product with 8 right-hand sides, FP32 blocks/vectors, vectors
+// [n_node*neqn][8]; one color range per launch, Y accumulated
+void launch_matvec_multi_color(const MatvecArgs& a, const float* R32, const float* X, float* Y, bool tr,
+                               hipStream_t st);
 void launch_compose(const int* node_ptr, const int* node_ent, const double* rhs, int k, int neqn, long n_node,
                     double* b, hipStream_t st);
 

@@ -121,6 +121,93 @@ void launch_matvec_color(const MatvecArgs& a, const double* R64, const float* R3
 }
 
 // ---------------------------------------------------------------------------
+// EBE product with NV right-hand sides at once (FP32 blocks and vectors,
+// FP32 accumulation): Y[n][NV] = K X[n][NV] (or K^T X).  NV lanes per element,
+// one vector each: the NV lanes of an element issue the same 16-byte block
+// loads (one transaction for all of them) and the NV/... elements of a
+// wavefront read consecutive pieces of the chunked layout, so one pass over
+// the blocks serves NV products.  The vectors are node-major with the NV
+// components contiguous, so an element's lanes gather and scatter 4*NV
+// contiguous bytes per dof.  This is HBM-bound like the single product
+// (25.6 kflop per 6.4 KB block at NV = 8): plain FP32 FMAs keep up with the
+// block stream, so no matrix-core tiling is needed.
+// ---------------------------------------------------------------------------
+template <int NV, bool TR>
+__global__ __launch_bounds__(256) void matvec_multi_kernel(long e0, long e1, const int* __restrict__ conn,
+                                                           const float* __restrict__ R,
+                                                           const float* __restrict__ X, float* __restrict__ Y) {
+    constexpr int NNOD = 8, NEQ = 5, K = NNOD * NEQ, KK = K * K, V = 4, GPR = K / V;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long e = e0 + t / NV;
+    const int v = (int)(t % NV);
+    if (e >= e1) return;
+    const int* ce = conn + e * NNOD;
+    if (ce[0] < 0) return;
+    const v4f_t* Rb = reinterpret_cast<const v4f_t*>(R + (size_t)(e >> 6) * 64 * KK) + (e & 63);
+    if (!TR) {
+        float xv[K];
+#pragma unroll
+        for (int m = 0; m < NNOD; m++) {
+            const int gm = ce[m];
+#pragma unroll
+            for (int q = 0; q < NEQ; q++) xv[m * NEQ + q] = X[((size_t)gm * NEQ + q) * NV + v];
+        }
+        for (int n = 0; n < NNOD; n++) {
+            const int gn = ce[n];
+#pragma unroll
+            for (int r = 0; r < NEQ; r++) {
+                v4f_t rv[GPR];
+#pragma unroll
+                for (int c = 0; c < GPR; c++) rv[c] = __builtin_nontemporal_load(Rb + (size_t)((n * NEQ + r) * GPR + c) * 64);
+                float acc = 0.0f;
+#pragma unroll
+                for (int c = 0; c < GPR; c++)
+#pragma unroll
+                    for (int w = 0; w < V; w++) acc += rv[c][w] * xv[c * V + w];
+                float* yp = Y + ((size_t)gn * NEQ + r) * NV + v;
+                *yp += acc;
+            }
+        }
+    } else {
+        float yv[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) yv[j] = 0.0f;
+        for (int n = 0; n < NNOD; n++) {
+            const int gn = ce[n];
+#pragma unroll
+            for (int r = 0; r < NEQ; r++) {
+                const float xi = X[((size_t)gn * NEQ + r) * NV + v];
+#pragma unroll
+                for (int c = 0; c < GPR; c++) {
+                    const v4f_t rv = __builtin_nontemporal_load(Rb + (size_t)((n * NEQ + r) * GPR + c) * 64);
+#pragma unroll
+                    for (int w = 0; w < V; w++) yv[c * V + w] += rv[w] * xi;
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NNOD; m++) {
+            const int gm = ce[m];
+#pragma unroll
+            for (int q = 0; q < NEQ; q++) Y[((size_t)gm * NEQ + q) * NV + v] += yv[m * NEQ + q];
+        }
+    }
+}
+
+void launch_matvec_multi_color(const MatvecArgs& a, const float* R32, const float* X, float* Y, bool tr,
+                               hipStream_t st) {
+    constexpr int NV = 8;
+    const long n = (a.e_end - a.e_begin) * NV;
+    if (n <= 0) return;
+    const int bs = 256;
+    const int nb = (int)((n + bs - 1) / bs);
+    if (tr)
+        hipLaunchKernelGGL((matvec_multi_kernel<NV, true>), dim3(nb), dim3(bs), 0, st, a.e_begin, a.e_end, a.conn, R32, X, Y);
+    else
+        hipLaunchKernelGGL((matvec_multi_kernel<NV, false>), dim3(nb), dim3(bs), 0, st, a.e_begin, a.e_end, a.conn, R32, X, Y);
+}
+
+// ---------------------------------------------------------------------------
 // compose_rhs (MPI_GMRES.h:369-389) as a node-centric gather: for each node
 // the contributing (element, local node) pairs are listed in element order,
 // so b is summed in the same order as the reference's element loop.

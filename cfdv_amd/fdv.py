@@ -200,6 +200,20 @@ class FDVContext:
         self._check(self.L.cfdv_profile_matvec(self.h, precision, reps, C.byref(ms)))
         return ms.value
 
+    def matvec_multi(self, X, transpose=False):
+        """Y = K X for X [ndof][8] (float32) in one pass over the FP32 blocks."""
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(self.ndof, 8)
+        Y = np.zeros_like(X)
+        self._check(self.L.cfdv_matvec_multi(self.h, X.ctypes.data_as(C.POINTER(C.c_float)),
+                                             Y.ctypes.data_as(C.POINTER(C.c_float)), 8,
+                                             int(transpose)))
+        return Y
+
+    def profile_matvec_multi(self, reps=10):
+        ms = C.c_double()
+        self._check(self.L.cfdv_profile_matvec_multi(self.h, reps, C.byref(ms)))
+        return ms.value
+
     # ---- discrete adjoint of one step ----
     def set_thermo(self, thermo: Thermo):
         """Replace the Thermo constants (e.g. the Mach number); rebuild after."""

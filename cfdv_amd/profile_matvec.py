@@ -3,8 +3,9 @@
     python -m cfdv_amd.profile_matvec --size 256x128x64 --reps 20
 
 Builds the compression-ramp mesh of the given size, runs one element build,
-then ``--reps`` full EBE matvecs in FP32 and in FP64 blocks, and prints one
-JSON line with the event-timed milliseconds and algorithmic bytes of each.
+then ``--reps`` full EBE matvecs in FP32 and in FP64 blocks and ``--reps``
+8-vector FP32 products, and prints one JSON line with the event-timed
+milliseconds and algorithmic bytes of each.
 With ``--copy-mib N`` it first runs one plain device copy of N MiB (an
 elementwise ``mul(src, 1)`` through torch) whose HBM bytes are known, so that
 hardware byte counters can be scaled against it.
@@ -56,6 +57,12 @@ def main(argv=None):
         out[f"matvec{prec}_ms"] = ms
         out[f"matvec{prec}_bytes"] = nbytes
         out[f"matvec{prec}_GBps"] = nbytes / (ms * 1e-3) / 1e9
+    ms8 = ctx.profile_matvec_multi(a.reps)
+    b8 = ne * k * k * 4 + nn * mesh.neqn * 8 * 2 * 4
+    out["matvec32x8_ms"] = ms8
+    out["matvec32x8_bytes"] = b8
+    out["matvec32x8_GBps"] = b8 / (ms8 * 1e-3) / 1e9
+    out["matvec32x8_speedup_vs_8_single"] = 8 * out.get("matvec32_ms", ms8) / ms8
     out["elements"] = ne
     out["nodes"] = nn
     out["colors"] = ctx.info()["ncolors"]

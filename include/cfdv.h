@@ -32,7 +32,8 @@
  *                        (adap/RefineListAdaptive.h:28-38)
  *   cfdv_step            one iteration of the time loop (main.cpp:162-206)
  *                        without Save/Output/Sensor
- *   cfdv_color, cfdv_matvec, cfdv_profile_matvec: no reference counterpart
+ *   cfdv_color, cfdv_matvec, cfdv_profile_matvec, cfdv_matvec_multi,
+ *   cfdv_profile_matvec_multi: no reference counterpart
  *                        (element coloring for the atomic-free EBE matvec,
  *                        and test/measurement hooks).
  *   cfdv_set_thermo      Thermo::set (FDV/Thermo.h:22-33) after create
@@ -178,6 +179,12 @@ int cfdv_matvec(cfdv_ctx *ctx, const double *x, double *y, int precision, int tr
    on the context stream */
 int cfdv_profile_matvec(cfdv_ctx *ctx, int precision, int reps, double *ms);
 int cfdv_step(cfdv_ctx *ctx, double cfl, const cfdv_solve_opts *opts, cfdv_step_stats *st);
+/* Y = K X (or K^T X) for 8 vectors in one pass over the FP32 blocks (no
+   reference counterpart): X, Y host arrays [n_node*neqn][8] (the 8 values
+   of a dof contiguous); nvec must be 8; hex8 meshes with FP32 blocks */
+int cfdv_matvec_multi(cfdv_ctx *ctx, const float *X, float *Y, int nvec, int transpose);
+/* average time of one 8-vector product (all colors), HIP events */
+int cfdv_profile_matvec_multi(cfdv_ctx *ctx, int reps, double *ms);
 
 /* ---- discrete adjoint of one step (no reference counterpart) ----------
    For J = g^T U1 with U1 = U0 + du, K(U0, M) du = b(U0, M) at a fixed dt:

@@ -233,3 +233,20 @@ def test_hanging_node_elements_build_parity(gpu_available):
     # the step runs end to end on the adapted mesh
     st, _ = ctx.solve("fp64", restarts=60, m=30, rtol=1e-10, precond="nodeblock")
     assert st.rel_resid <= 1e-10
+
+
+@pytest.mark.parametrize("transpose", [False, True])
+def test_matvec_multi_equals_single_products(gpu_available, transpose):
+    """The 8-vector FP32 product equals 8 single FP32 products (1e-6)."""
+    m, th, U = _case("box3d")
+    ctx = FDVContext(m, th, U, blocks=("fp64", "fp32"))
+    ctx.build(ctx.cfl(0.5))
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((ctx.ndof, 8)).astype(np.float32)
+    Y = ctx.matvec_multi(X, transpose=transpose)
+    R32 = ctx.blocks32().astype(np.float64)
+    for v in range(8):
+        y1 = ctx.matvec(X[:, v].astype(np.float64), precision=32, transpose=transpose)
+        yo = O.matvec(m, R32, X[:, v].astype(np.float64), transpose=transpose)
+        assert np.linalg.norm(Y[:, v] - y1) <= 1e-6 * np.linalg.norm(y1)
+        assert np.linalg.norm(Y[:, v] - yo) <= 1e-6 * np.linalg.norm(yo)
