@@ -19,7 +19,7 @@ using namespace cfdv;
 
 namespace {
 
-constexpr int NBLK_RED = 512;   // blocks of the reduction kernels
+constexpr int NBLK_RED = 1024;  // blocks of the reduction kernels (partials: NBLK_RED x 32 doubles)
 
 // 2-point Gauss-Legendre by Newton iteration, the rule of
 // GaussQuad::get_gauss (src/FiniteElement/GaussQuad.h:258-301).
@@ -236,6 +236,9 @@ struct cfdv_ctx {
         sync();
         return std::sqrt(h);
     }
+    // Krylov basis stride: ndof rounded up to 64 elements, so that every
+    // basis vector starts 256-byte aligned for 16-byte vector loads
+    long ldv() const { return (ndof + 63) & ~63L; }
     void ensure_krylov(int m) {
         if (vcap >= m + 1) return;
         if (d_V) {
@@ -243,7 +246,7 @@ struct cfdv_ctx {
             if (it != allocs.end()) allocs.erase(it);
             (void)hipFree(d_V);
         }
-        d_V = alloc<double>((size_t)(m + 1) * ndof);
+        d_V = alloc<double>((size_t)(m + 1) * ldv());
         vcap = m + 1;
     }
     void ensure_krylov32(int m) {
@@ -253,14 +256,16 @@ struct cfdv_ctx {
             if (it != allocs.end()) allocs.erase(it);
             (void)hipFree(d_V32);
         }
-        d_V32 = alloc<float>((size_t)(m + 1) * ndof);
+        d_V32 = alloc<float>((size_t)(m + 1) * ldv());
         vcap32 = m + 1;
     }
     // node-block inverses from the current blocks (FP64 if held, else FP32)
     void ensure_precond() {
         if (!d_P64) {
-            d_P64 = alloc<double>((size_t)n_node * neqn * neqn);
-            d_P32 = alloc<float>((size_t)n_node * neqn * neqn);
+            // chunks of 64 nodes (the last one padded)
+            const size_t np = (size_t)((n_node + 63) / 64) * 64;
+            d_P64 = alloc<double>(np * neqn * neqn);
+            d_P32 = alloc<float>(np * neqn * neqn);
             d_z64 = alloc<double>((size_t)ndof);
             d_z32 = alloc<float>((size_t)ndof);
             d_t64 = alloc<double>((size_t)ndof);
@@ -485,7 +490,7 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
     c->d_y = c->alloc<double>(c->ndof);
     c->d_d64 = c->alloc<double>(c->ndof);
     c->d_w32 = c->alloc<float>(c->ndof);
-    c->d_part = c->alloc<double>((size_t)NBLK_RED * 16 + 64);
+    c->d_part = c->alloc<double>((size_t)NBLK_RED * 32 + 64);
     c->d_small = c->alloc<double>(1024);
     c->d_hcol = c->alloc<double>(2048);
     c->d_cflpart = c->alloc<double>((np + 255) / 256);
@@ -506,6 +511,7 @@ int solve_reference(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) 
     const long n = c->ndof;
     c->ensure_krylov(m_);
     double* V = c->d_V;
+    const long ldv = c->ldv();
     std::vector<double> h((size_t)(m_ + 1) * m_), g(m_ + 1), y(m_);
     auto H = [&](int i, int j) -> double& { return h[i + (size_t)(m_ + 1) * j]; };
     vec_fill(c->d_du, 0.0, n, c->st);
@@ -529,13 +535,13 @@ int solve_reference(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) 
         vec_scale_inv<double>(V, c->d_r, c->d_small + 8, n, c->st);
         for (int j = 0; j < mm; j++) {
             double* w = c->d_w;
-            c->matvec(V + (size_t)j * n, w, 64, false);
+            c->matvec(V + (size_t)j * ldv, w, 64, false);
             s->matvecs64++;
             s->iters++;
             // modified Gram-Schmidt, one inner product at a time (:166-174)
             for (int i = 0; i <= j; i++) {
-                c->multidot<double>(V + (size_t)i * n, 1, w, c->d_hcol + i);
-                vec_axpy(w, V + (size_t)i * n, c->d_hcol + i, -1.0, n, c->st);
+                c->multidot<double>(V + (size_t)i * ldv, 1, w, c->d_hcol + i);
+                vec_axpy(w, V + (size_t)i * ldv, c->d_hcol + i, -1.0, n, c->st);
             }
             c->multidot<double>(w, 1, w, c->d_hcol + j + 1);
             std::vector<double> col(j + 2);
@@ -545,7 +551,7 @@ int solve_reference(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) 
             H(j + 1, j) = std::sqrt(col[j + 1]);
             if (!(H(j + 1, j) > 0)) s->breakdown = 1;
             vec_set(c->d_small + 9, H(j + 1, j), c->st);
-            vec_scale_inv<double>(V + (size_t)(j + 1) * n, w, c->d_small + 9, n, c->st);
+            vec_scale_inv<double>(V + (size_t)(j + 1) * ldv, w, c->d_small + 9, n, c->st);
             // rotate_and_check (:696-730): c, s from the unrotated h (Q3)
             const int nn = j + 1;
             std::vector<double> hh((size_t)(nn + 1) * nn), gg(nn + 1, 0.0);
@@ -590,7 +596,7 @@ int solve_reference(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) 
         }
         std::memcpy(c->h_pin, y.data(), sizeof(double) * mm);
         HIP_CHECK_THROW(hipMemcpyAsync(c->d_small + 16, c->h_pin, sizeof(double) * mm, hipMemcpyHostToDevice, c->st));
-        vec_combine<double>(c->d_du, V, n, mm, c->d_small + 16, n, c->st);
+        vec_combine<double>(c->d_du, V, ldv, mm, c->d_small + 16, n, c->st);
         s->restarts++;   // Q4: no early exit from the restart loop
     }
     // final true residual for reporting
@@ -616,6 +622,7 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
                double rnorm0, cfdv_solve_stats* s, bool* conv, T* V, T* w, const T* Pinv = nullptr,
                T* z = nullptr) {
     const long n = c->ndof;
+    const long ldv = c->ldv();
     std::vector<double> h((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m);
     auto H = [&](int i, int j) -> double& { return h[i + (size_t)(m + 1) * j]; };
     int iters = 0;
@@ -656,19 +663,27 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
         bool brk = false;
         for (int j = 0; j < m; j++) {
             if (Pinv) {
-                vec_nodeblock_apply<T, T>(Pinv, V + (size_t)j * n, z, c->neqn, c->n_node, false, tr, c->st);
+                vec_nodeblock_apply<T, T>(Pinv, V + (size_t)j * ldv, z, c->neqn, c->n_node, false, tr, c->st);
                 c->matvec(z, w, prec, tr);
             } else {
-                c->matvec(V + (size_t)j * n, w, prec, tr);
+                c->matvec(V + (size_t)j * ldv, w, prec, tr);
             }
             if (prec == 64) s->matvecs64++; else s->matvecs32++;
             iters++;
-            // CGS2: two classical Gram-Schmidt passes
-            c->multidot<T>(V, j + 1, w, c->d_hcol);
-            vec_multi_axpy<T>(w, V, n, j + 1, c->d_hcol, 1.0, n, c->st);
-            c->multidot<T>(V, j + 1, w, c->d_hcol + 1024);
-            vec_multi_axpy<T>(w, V, n, j + 1, c->d_hcol + 1024, 1.0, n, c->st);
-            c->multidot<T>(w, 1, w, c->d_hcol + 2040);
+            // CGS2: two classical Gram-Schmidt passes, h1 = V^T w,
+            // w -= V h1, h2 = V^T w, w -= V h2, |w|^2; up to 32 basis vectors
+            // the projections fuse into three passes over the basis
+            if (j + 1 <= 32) {
+                vec_cgs<T>(0, w, V, ldv, j + 1, nullptr, n, c->d_part, NBLK_RED, c->d_hcol, c->st);
+                vec_cgs<T>(1, w, V, ldv, j + 1, c->d_hcol, n, c->d_part, NBLK_RED, c->d_hcol + 1024, c->st);
+                vec_cgs<T>(2, w, V, ldv, j + 1, c->d_hcol + 1024, n, c->d_part, NBLK_RED, c->d_hcol + 2040, c->st);
+            } else {
+                vec_multidot<T>(V, ldv, j + 1, w, n, c->d_part, c->d_hcol, NBLK_RED, c->st);
+                vec_multi_axpy<T>(w, V, ldv, j + 1, c->d_hcol, 1.0, n, c->st);
+                vec_multidot<T>(V, ldv, j + 1, w, n, c->d_part, c->d_hcol + 1024, NBLK_RED, c->st);
+                vec_multi_axpy<T>(w, V, ldv, j + 1, c->d_hcol + 1024, 1.0, n, c->st);
+                c->multidot<T>(w, 1, w, c->d_hcol + 2040);
+            }
             std::vector<double> col(j + 1), col2(j + 1);
             double nrm2;
             HIP_CHECK_THROW(hipMemcpyAsync(col.data(), c->d_hcol, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->st));
@@ -681,7 +696,7 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
             if (!std::isfinite(hn)) { s->breakdown = 1; brk = true; }
             if (hn > 0 && !brk) {
                 vec_set(c->d_small + 9, H(j + 1, j), c->st);
-                vec_scale_inv<T>(V + (size_t)(j + 1) * n, w, c->d_small + 9, n, c->st);
+                vec_scale_inv<T>(V + (size_t)(j + 1) * ldv, w, c->d_small + 9, n, c->st);
             } else {
                 brk = true;   // happy breakdown or failure: the space is invariant
             }
@@ -712,10 +727,10 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
         if (Pinv) {
             // x += Pinv (V y), the combination formed in FP64
             vec_fill(c->d_t64, 0.0, n, c->st);
-            vec_combine<T>(c->d_t64, V, n, jdone, c->d_small + 16, n, c->st);
+            vec_combine<T>(c->d_t64, V, ldv, jdone, c->d_small + 16, n, c->st);
             vec_nodeblock_apply<double, T>(Pinv, c->d_t64, x, c->neqn, c->n_node, true, tr, c->st);
         } else {
-            vec_combine<T>(x, V, n, jdone, c->d_small + 16, n, c->st);
+            vec_combine<T>(x, V, ldv, jdone, c->d_small + 16, n, c->st);
         }
         s->restarts++;
         if (s->est_resid <= tol_abs) { *conv = true; break; }
@@ -780,7 +795,7 @@ int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, 
         cfdv_solve_stats si{};
         // the FP32 right-hand side stays in d_w32; the last basis slot is the
         // Arnoldi work vector
-        float* wbuf = c->d_V32 + (size_t)o->inner_m * n;   // last slot as the Arnoldi work vector
+        float* wbuf = c->d_V32 + (size_t)o->inner_m * c->ldv();   // last slot as the Arnoldi work vector
         s->iters += gmres_core<float>(c, 32, tr, c->d_w32, c->d_d64, o->inner_m - 1, o->inner_restarts,
                                       o->inner_rtol, 1.0, &si, &conv, c->d_V32, wbuf, pc ? c->d_P32 : nullptr,
                                       pc ? c->d_z32 : nullptr);

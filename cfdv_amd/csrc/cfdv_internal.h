@@ -93,14 +93,23 @@ void vec_update_state(double* U, double* U0, const double* du, long n, hipStream
 template <typename T>
 void vec_multidot(const T* V, long ldv, int nv, const T* w, long n, double* part, double* out, int nblk,
                   hipStream_t st);
+// Classical Gram-Schmidt passes on 16-byte loads (V_j = V + j ldv, ldv a
+// multiple of 64, nv <= 32): mode 0 out[j] = <V_j, w>; mode 1 w -= sum h_j
+// V_j then out[j] = <V_j, w>; mode 2 w -= sum h_j V_j then out[0] = <w, w>.
+// part holds nblk * nv doubles.
+template <typename T>
+void vec_cgs(int mode, T* w, const T* V, long ldv, int nv, const double* h, long n, double* part, int nblk,
+             double* out, hipStream_t st);
 template <typename T>
 void vec_multi_axpy(T* w, const T* V, long ldv, int nv, const double* h, double sign, long n, hipStream_t st);
 template <typename T>
 void vec_scale_inv(T* y, const T* x, const double* denom_dev, long n, hipStream_t st);
 template <typename T>
 void vec_combine(double* x, const T* V, long ldv, int nv, const double* y, long n, hipStream_t st);
-// node-block Jacobi: Pinv per node, row-major neqn x neqn, FP64 and FP32
-// copies; *status set to 1 when a node block is singular
+// node-block Jacobi: Pinv per node (entry r neqn + c), FP64 and FP32 copies,
+// in chunks of 64 nodes: node n entry q at (n / 64) 64 neqn^2 + q 64 + n % 64
+// (buffers hold round_up(n_node, 64) nodes); *status set to 1 when a node
+// block is singular
 void launch_nodeblock(const int* node_ptr, const int* node_ent, const double* R64, const float* R32, int nnod,
                       int neqn, long n_node, double* P64, float* P32, int* status, hipStream_t st);
 // z = Pinv v (accumulate: z += Pinv v; trans: Pinv^T)

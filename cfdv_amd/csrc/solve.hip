@@ -378,6 +378,165 @@ void vec_multidot(const T* V, long ldv, int nv, const T* w, long n, double* part
 template void vec_multidot<double>(const double*, long, int, const double*, long, double*, double*, int, hipStream_t);
 template void vec_multidot<float>(const float*, long, int, const float*, long, double*, double*, int, hipStream_t);
 
+// ---------------------------------------------------------------------------
+// Classical Gram-Schmidt kernels on 16-byte vector loads (V_j = V + j ldv,
+// ldv a multiple of 64 elements, w 256-byte aligned; the n % VL tail is done
+// by scalar loads).  Each block writes its partial sums (fixed grid, fixed
+// order) for finish_dot_kernel, so results do not depend on timing.
+//   cgs_dot:       part[b][j] = sum_i V_j[i] w[i]
+//   cgs_axpy_dot:  w -= sum_j h[j] V_j,  part[b][j] = sum_i V_j[i] w_new[i]
+//   cgs_axpy_norm: w -= sum_j h[j] V_j,  part[b][0] = sum_i w_new[i]^2
+// ---------------------------------------------------------------------------
+// entries per load: 16 bytes, except in the FP32 fused projection (mode 1)
+// over 5..16 vectors, which holds every basis entry of the thread in
+// registers and loads 8 bytes to stay at 3 waves per SIMD (145 VGPRs; 16-byte
+// loads there need 247)
+template <typename T, int L> struct VecT { typedef T type __attribute__((ext_vector_type(L))); };
+template <typename T, int NV, int MODE> constexpr int cgs_lanes() {
+    return sizeof(T) == 8 ? 2 : ((MODE == 1 && NV > 4 && NV <= 16) ? 2 : 4);
+}
+
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&acc)[NV], int nv, double* __restrict__ part) {
+    __shared__ double sm[NV][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        double v = acc[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) sm[j][wv] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nv && threadIdx.x < NV) {
+        const int j = threadIdx.x;
+        part[(size_t)blockIdx.x * nv + j] = ((sm[j][0] + sm[j][1]) + sm[j][2]) + sm[j][3];
+    }
+}
+
+template <typename T, int NV, int MODE>   // MODE 0: dot, 1: axpy + dot, 2: axpy + norm
+__global__ __launch_bounds__(256) void cgs_kernel(T* __restrict__ w, const T* __restrict__ V, long ldv, int nv,
+                                                  const double* __restrict__ h, long n, double* __restrict__ part) {
+    constexpr int L = cgs_lanes<T, NV, MODE>();
+    typedef typename VecT<T, L>::type vt;
+    double hh[NV];
+    if (MODE > 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) hh[j] = (j < nv) ? h[j] : 0.0;
+    }
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) acc[j] = 0.0;
+    const long nvec = n / L;
+    const long stride = (long)gridDim.x * blockDim.x;
+#pragma unroll 1
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride) {
+        vt wv = reinterpret_cast<const vt*>(w)[i];
+        if constexpr (MODE == 1) {
+            // the basis entries are loaded once and held in registers for
+            // both the projection and the dot products
+            vt vv[NV];
+#pragma unroll
+            for (int j = 0; j < NV; j++)
+                if (j < nv) vv[j] = reinterpret_cast<const vt*>(V + (size_t)j * ldv)[i];
+            double a[L];
+#pragma unroll
+            for (int l = 0; l < L; l++) a[l] = 0.0;
+#pragma unroll
+            for (int j = 0; j < NV; j++)
+                if (j < nv) {
+#pragma unroll
+                    for (int l = 0; l < L; l++) a[l] += hh[j] * (double)vv[j][l];
+                }
+#pragma unroll
+            for (int l = 0; l < L; l++) wv[l] = (T)((double)wv[l] - a[l]);
+            reinterpret_cast<vt*>(w)[i] = wv;
+#pragma unroll
+            for (int j = 0; j < NV; j++)
+                if (j < nv) {
+#pragma unroll
+                    for (int l = 0; l < L; l++) acc[j] += (double)vv[j][l] * (double)wv[l];
+                }
+        } else if constexpr (MODE == 2) {
+            double a[L];
+#pragma unroll
+            for (int l = 0; l < L; l++) a[l] = 0.0;
+#pragma unroll
+            for (int j = 0; j < NV; j++)
+                if (j < nv) {
+                    const vt vj = reinterpret_cast<const vt*>(V + (size_t)j * ldv)[i];
+#pragma unroll
+                    for (int l = 0; l < L; l++) a[l] += hh[j] * (double)vj[l];
+                }
+#pragma unroll
+            for (int l = 0; l < L; l++) wv[l] = (T)((double)wv[l] - a[l]);
+            reinterpret_cast<vt*>(w)[i] = wv;
+#pragma unroll
+            for (int l = 0; l < L; l++) acc[0] += (double)wv[l] * (double)wv[l];
+        } else {
+#pragma unroll
+            for (int j = 0; j < NV; j++)
+                if (j < nv) {
+                    const vt vj = reinterpret_cast<const vt*>(V + (size_t)j * ldv)[i];
+#pragma unroll
+                    for (int l = 0; l < L; l++) acc[j] += (double)vj[l] * (double)wv[l];
+                }
+        }
+    }
+    // tail elements (n % L) by the first threads of block 0
+    if (blockIdx.x == 0) {
+        const long i = nvec * L + threadIdx.x;
+        if (i < n) {
+            double wi = (double)w[i];
+            if (MODE > 0) {
+                double a = 0.0;
+#pragma unroll
+                for (int j = 0; j < NV; j++)
+                    if (j < nv) a += hh[j] * (double)V[(size_t)j * ldv + i];
+                const T wn = (T)(wi - a);
+                w[i] = wn;
+                wi = (double)wn;
+            }
+            if (MODE == 2) acc[0] += wi * wi;
+            else {
+#pragma unroll
+                for (int j = 0; j < NV; j++)
+                    if (j < nv) acc[j] += (double)V[(size_t)j * ldv + i] * wi;
+            }
+        }
+    }
+    block_partials<NV>(acc, MODE == 2 ? 1 : nv, part);
+}
+
+template <typename T>
+static void launch_cgs(int mode, T* w, const T* V, long ldv, int nv, const double* h, long n, double* part,
+                       int nblk, double* out, hipStream_t st) {
+#define CGS(NVV, MODE) hipLaunchKernelGGL((cgs_kernel<T, NVV, MODE>), dim3(nblk), dim3(256), 0, st, w, V, ldv, nv, h, n, part)
+#define CGS_NV(MODE)                                   \
+    if (nv <= 1) CGS(1, MODE);                         \
+    else if (nv <= 4) CGS(4, MODE);                    \
+    else if (nv <= 8) CGS(8, MODE);                    \
+    else if (nv <= 16) CGS(16, MODE);                  \
+    else CGS(32, MODE);
+    if (mode == 0) { CGS_NV(0) }
+    else if (mode == 1) { CGS_NV(1) }
+    else { CGS_NV(2) }
+#undef CGS_NV
+#undef CGS
+    hipLaunchKernelGGL(finish_dot_kernel, dim3(1), dim3(256), 0, st, part, nblk, mode == 2 ? 1 : nv, out);
+}
+
+template <typename T>
+void vec_cgs(int mode, T* w, const T* V, long ldv, int nv, const double* h, long n, double* part, int nblk,
+             double* out, hipStream_t st) {
+    if (nv > 32) throw std::runtime_error("vec_cgs: at most 32 basis vectors per call");
+    launch_cgs<T>(mode, w, V, ldv, nv, h, n, part, nblk, out, st);
+}
+template void vec_cgs<double>(int, double*, const double*, long, int, const double*, long, double*, int, double*,
+                              hipStream_t);
+template void vec_cgs<float>(int, float*, const float*, long, int, const double*, long, double*, int, double*,
+                             hipStream_t);
+
 // w -= sign * sum_j h[j] V_j  (h on device)
 template <typename T>
 __global__ __launch_bounds__(256) void multi_axpy_kernel(T* __restrict__ w, const T* __restrict__ V, long ldv, int nv,
@@ -431,6 +590,15 @@ template void vec_combine<float>(double*, const float*, long, int, const double*
 // summed in element order; Pinv_n by Gauss-Jordan with partial pivoting in
 // FP64.  One thread per node.  The same node CSR as compose_kernel.
 // ---------------------------------------------------------------------------
+// Inverse blocks are stored in chunks of 64 nodes, entry-major inside a
+// chunk: entry q = r NEQ + c of node n at (n / 64) 64 NEQ^2 + q 64 + n % 64,
+// so that a wavefront's load of one entry for 64 consecutive nodes is one
+// contiguous 512-byte (FP64) or 256-byte (FP32) segment.
+template <int NEQ>
+__device__ __forceinline__ size_t pblk_offset(long n, int q) {
+    return (size_t)(n >> 6) * 64 * NEQ * NEQ + (size_t)q * 64 + (size_t)(n & 63);
+}
+
 template <typename TB, int NEQ>
 __global__ __launch_bounds__(256) void nodeblock_kernel(const int* __restrict__ node_ptr,
                                                         const int* __restrict__ node_ent,
@@ -496,8 +664,8 @@ __global__ __launch_bounds__(256) void nodeblock_kernel(const int* __restrict__ 
 #pragma unroll
         for (int c = 0; c < NEQ; c++) {
             const double v = singular ? ((r == c) ? 1.0 : 0.0) : B[r][c];
-            P64[(size_t)n * NEQ * NEQ + r * NEQ + c] = v;
-            P32[(size_t)n * NEQ * NEQ + r * NEQ + c] = (float)v;
+            P64[pblk_offset<NEQ>(n, r * NEQ + c)] = v;
+            P32[pblk_offset<NEQ>(n, r * NEQ + c)] = (float)v;
         }
 }
 
@@ -528,7 +696,7 @@ __global__ __launch_bounds__(256) void nodeblock_apply_kernel(const TP* __restri
             TV acc = accumulate ? z[n * NEQ + r] : (TV)0;
 #pragma unroll
             for (int c = 0; c < NEQ; c++)
-                acc += (TV)P[(size_t)n * NEQ * NEQ + (trans ? c * NEQ + r : r * NEQ + c)] * x[c];
+                acc += (TV)P[pblk_offset<NEQ>(n, trans ? c * NEQ + r : r * NEQ + c)] * x[c];
             z[n * NEQ + r] = acc;
         }
     }
