@@ -20,6 +20,10 @@ CFDV_ERR_NAN = -5
 CFDV_ERR_BREAKDOWN = -6
 CFDV_ERR_CFL = -7
 CFDV_ERR_NOCONV = -8
+CFDV_ERR_REFINE = -9
+
+CFDV_REFINE_PARITY = 0
+CFDV_REFINE_3D = 1
 
 CFDV_BLOCKS_FP64 = 1
 CFDV_BLOCKS_FP32 = 2
@@ -37,7 +41,9 @@ EXPORTS = ["cfdv_create", "cfdv_destroy", "cfdv_last_error", "cfdv_mesh_info", "
            "cfdv_get_rhs_vector", "cfdv_solve", "cfdv_get_du", "cfdv_update", "cfdv_rms",
            "cfdv_refine_flags", "cfdv_color", "cfdv_matvec", "cfdv_profile_matvec", "cfdv_step",
            "cfdv_set_thermo", "cfdv_adjoint_solve", "cfdv_build_vjp", "cfdv_matvec_multi",
-           "cfdv_profile_matvec_multi"]
+           "cfdv_profile_matvec_multi", "cfdv_refine_create", "cfdv_refine_destroy",
+           "cfdv_refine_last_error", "cfdv_refine_list", "cfdv_refine_apply", "cfdv_refine_size",
+           "cfdv_refine_get"]
 
 
 class Mesh(C.Structure):
@@ -118,5 +124,16 @@ def load() -> C.CDLL:
     fp = C.POINTER(C.c_float)
     L.cfdv_matvec_multi.argtypes = [vp, fp, fp, C.c_int, C.c_int]
     L.cfdv_profile_matvec_multi.argtypes = [vp, C.c_int, dp]
+    ip = C.POINTER(C.c_int32)
+    up = C.POINTER(C.c_uint8)
+    L.cfdv_refine_create.argtypes = [C.POINTER(Mesh), dp, dp, ip, C.POINTER(vp)]
+    L.cfdv_refine_destroy.argtypes = [vp]
+    L.cfdv_refine_destroy.restype = None
+    L.cfdv_refine_last_error.argtypes = [vp]
+    L.cfdv_refine_last_error.restype = C.c_char_p
+    L.cfdv_refine_list.argtypes = [vp, up, ip, ip, ip, ip]
+    L.cfdv_refine_apply.argtypes = [vp, ip, C.c_int32, ip, C.c_int32, C.c_int, C.c_double]
+    L.cfdv_refine_size.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), ip]
+    L.cfdv_refine_get.argtypes = [vp, ip, ip, dp, dp, dp, up, up, dp, ip]
     _LIB = L
     return L

@@ -13,7 +13,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcfdv_amd.so")
-SOURCES = ["build.hip", "solve.hip", "adjoint.hip", "capi.cpp"]
+SOURCES = ["build.hip", "solve.hip", "adjoint.hip", "capi.cpp", "../host/refine.cpp"]
 ROOT = os.path.dirname(HERE)
 RUNNER = os.path.join(HERE, "cfdv_run")
 ARCH = os.environ.get("CFDV_OFFLOAD_ARCH", "gfx950")
@@ -33,7 +33,7 @@ def flags() -> list:
 
 
 def _compile(src: str, objdir: str) -> str:
-    obj = os.path.join(objdir, src + ".o")
+    obj = os.path.join(objdir, os.path.basename(src) + ".o")
     cmd = [hipcc()] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

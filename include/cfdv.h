@@ -36,6 +36,13 @@
  *   cfdv_profile_matvec_multi: no reference counterpart
  *                        (element coloring for the atomic-free EBE matvec,
  *                        and test/measurement hooks).
+ *   cfdv_refine_*        host-side refinement, no GPU: cfdv_refine_list is
+ *                        AdaptiveRefineList (adap/RefineListAdaptive.h:8-126)
+ *                        after its flag pass, with ElementAssociation
+ *                        (adap/ElementAssociation.h:42-249); cfdv_refine_apply
+ *                        is MeshRefine2D (adap/MeshRefine2D.h:39-872)
+ *                        (main.cpp:143-151); cfdv_refine_create/_get stand for
+ *                        the element and node vectors those mutate
  *   cfdv_set_thermo      Thermo::set (FDV/Thermo.h:22-33) after create
  *   cfdv_adjoint_solve, cfdv_build_vjp: no reference counterpart (discrete
  *                        adjoint of one step: transpose solve, and the
@@ -59,6 +66,9 @@ extern "C" {
 #define CFDV_ERR_BREAKDOWN (-6)  /* Krylov breakdown (zero Arnoldi vector)   */
 #define CFDV_ERR_CFL (-7)        /* no element set dt                        */
 #define CFDV_ERR_NOCONV (-8)     /* tolerance not met within the limits      */
+#define CFDV_ERR_REFINE (-9)     /* refinement reached a state where the
+                                    reference blocks on stdin or recurses
+                                    without end                             */
 
 typedef struct cfdv_ctx cfdv_ctx;
 
@@ -202,6 +212,42 @@ int cfdv_adjoint_solve(cfdv_ctx *ctx, const cfdv_solve_opts *opts, const double 
    dt of the last cfdv_build, lambda and du held fixed (du NULL: the last
    solved du).  hex8 meshes. */
 int cfdv_build_vjp(cfdv_ctx *ctx, const double *lambda, const double *du, double *gU, double *gmach);
+
+/* ---- host-side adaptive refinement (no GPU needed) ---------------------
+   A cfdv_refine object holds a copy of the mesh and node states that the
+   reference's AdaptiveRefineList and MeshRefine2D read and append to:
+   element and node numbers are append order, a refined element keeps its
+   number and appends nnod-1 children, a hanging local node keeps the node of
+   the element before refinement and gets an Hnm row (Element::Hnm, adap).
+   Elements without adap carry the identity as Hnm. */
+typedef struct cfdv_refine cfdv_refine;
+#define CFDV_REFINE_PARITY 0   /* repeated-node test on x and y only, |d| < tol
+                                  (MeshRefine2D.h:337-361, quirk Q14: in 3D,
+                                  nodes differing only in z merge)            */
+#define CFDV_REFINE_3D 1       /* repeated-node test on every coordinate      */
+/* U0 may be NULL (U0 = U); refine_level [n_elem] may be NULL (all 0); adap
+   and hnm of the mesh are read when adap is set (hex8 and quad4) */
+int cfdv_refine_create(const cfdv_mesh *mesh, const double *U, const double *U0,
+                       const int32_t *refine_level, cfdv_refine **out);
+void cfdv_refine_destroy(cfdv_refine *r);
+const char *cfdv_refine_last_error(const cfdv_refine *r);
+/* AdaptiveRefineList from the flags [n_elem] (s1 > 0.8, cfdv_refine_flags):
+   dependency closure over face and edge neighbours, first-occurrence dedupe,
+   buckets by refine level.  elelist capacity n_elem, level_ptr capacity
+   (max refine level + 2); *n_level_ptr = max refine level + 2 */
+int cfdv_refine_list(cfdv_refine *r, const uint8_t *flags, int32_t *elelist, int32_t *n_list,
+                     int32_t *level_ptr, int32_t *n_level_ptr);
+/* MeshRefine2D: refine elelist level bucket by level bucket (level_ptr as
+   returned by cfdv_refine_list); mode CFDV_REFINE_*; tol <= 0 selects the
+   reference's 1e-3 */
+int cfdv_refine_apply(cfdv_refine *r, const int32_t *elelist, int32_t n_list, const int32_t *level_ptr,
+                      int32_t n_level_ptr, int mode, double tol);
+int cfdv_refine_size(const cfdv_refine *r, int64_t *n_elem, int64_t *n_node, int32_t *max_level);
+/* copy the current mesh out (any pointer may be NULL): conn [n_elem][nnod],
+   bc [n_elem][nface] (SaveBinary.h:59-69), x, U, U0, dirichlet, adap,
+   hnm [n_elem][nnod*nnod] with Hnm(N, M) at N + nnod*M, refine_level */
+int cfdv_refine_get(const cfdv_refine *r, int32_t *conn, int32_t *bc, double *x, double *U, double *U0,
+                    uint8_t *dirichlet, uint8_t *adap, double *hnm, int32_t *refine_level);
 
 #ifdef __cplusplus
 }

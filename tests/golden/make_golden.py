@@ -20,7 +20,17 @@
    that spread would test summation order, not the algorithm.  Finally the textbook-GMRES solution
    of the first step's system to a 1e-12 relative residual.
 
-Run from the repository root:  python tests/golden/make_golden.py
+3. Refinement (``refine_<deck>.npz``, ``--refine`` writes only these): two
+   --adap passes of the refinement oracle (oracle/refine.py:
+   AdaptiveRefineList + MeshRefine2D, node match on x and y as the reference,
+   quirk Q14) on lid50 and 3dtestlagrange from the state after the two golden
+   steps.  The flags of each pass are the FDVParam s1 > 0.8 pass of the
+   oracle at that pass's state, restricted to a window of the domain so that
+   the fixtures stay small; they are stored with the outputs (elelist, level
+   breakpoints, connectivity, bc, coordinates, U, U0, Dirichlet flags,
+   adap, Hnm, refine levels).
+
+Run from the repository root:  python tests/golden/make_golden.py [--refine]
 """
 from __future__ import annotations
 
@@ -36,6 +46,7 @@ sys.path.insert(0, ROOT)
 
 from cfdv_amd import mesh as M  # noqa: E402
 from oracle import oracle as O  # noqa: E402
+from oracle import refine as OR  # noqa: E402
 
 SRC = "/root/reference/example"
 DECKS = {
@@ -113,11 +124,50 @@ def make(name):
           f"true GMRES iters {gst.iters}, -> {os.path.getsize(out)} bytes")
 
 
+# refinement windows per pass: (lower corner, upper corner) of element centroids
+REFINE_WINDOWS = {
+    "lid50": (((0.75, 0.75), (1.0, 1.0)), ((0.88, 0.88), (1.0, 1.0))),
+    "3dtestlagrange": (((0.0, 0.0, 0.0), (0.5, 0.25, 0.25)), ((0.0, 0.0, 0.0), (0.5, 0.25, 0.25))),
+}
+
+
+def make_refine(name):
+    mesh, th, _, _ = M.read_deck(os.path.join(HERE, "decks", name))
+    g = np.load(os.path.join(HERE, f"{name}.npz"))
+    U, U0 = g["U_steps"][-1], g["U_steps"][-2]
+    rm = OR.RMesh.from_arrays(mesh.ndim, mesh.nnod, mesh.neqn, mesh.conn, mesh.bc, mesh.x, U, U0,
+                              mesh.dirichlet)
+    out = {}
+    cur = mesh
+    for p, (lo, hi) in enumerate(REFINE_WINDOWS[name]):
+        Uc = np.array(rm.U)
+        s, _ = O.fdvparam_all(th, cur, Uc)
+        cent = np.array(rm.x)[np.array(rm.conn)].mean(axis=1)
+        inside = np.all((cent >= np.array(lo)) & (cent <= np.array(hi)), axis=1)
+        flags = (s[:, 0] > 0.8) & inside
+        el, rl = OR.adaptive_refine_list(rm, flags)
+        OR.MeshRefine(rm, el, rl, mode="parity")
+        conn, bc, x, Un, U0n, d, adap, hnm, lv = rm.arrays()
+        cur = M.Mesh(mesh.ndim, mesh.nnod, mesh.neqn, conn, x, bc, d, adap=adap, hnm=hnm)
+        out.update({f"p{p}_flags": flags.astype(np.uint8), f"p{p}_elelist": np.array(el, np.int32),
+                    f"p{p}_breaks": np.array(rl, np.int32), f"p{p}_conn": conn, f"p{p}_bc": bc,
+                    f"p{p}_x": x, f"p{p}_U": Un, f"p{p}_U0": U0n, f"p{p}_dirichlet": d,
+                    f"p{p}_adap": adap, f"p{p}_hnm": hnm, f"p{p}_level": lv})
+        print(f"refine {name} pass {p}: {int(flags.sum())} flagged, elelist {len(el)}, levels {rl}, "
+              f"-> {rm.n_elem} elements, {rm.n_node} nodes, {int(adap.sum())} adapted")
+    path = os.path.join(HERE, f"refine_{name}.npz")
+    np.savez_compressed(path, U=U, U0=U0, **out)
+    print(f"  -> {os.path.getsize(path)} bytes")
+
+
 def main():
-    if os.path.isdir(SRC):
-        copy_decks()
-    for name in DECKS:
-        make(name)
+    if "--refine" not in sys.argv:
+        if os.path.isdir(SRC):
+            copy_decks()
+        for name in DECKS:
+            make(name)
+    for name in REFINE_WINDOWS:
+        make_refine(name)
 
 
 if __name__ == "__main__":
