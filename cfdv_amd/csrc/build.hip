@@ -733,9 +733,19 @@ CFDV_HD void build2d_elem(const BuildArgs& A, const GaussTables& G, long e, int 
         Fn[n][0][0] = l; Fn[n][0][1] = p + (l * l) / rho; Fn[n][0][2] = (l * m) / rho; Fn[n][0][3] = (p + en) * (l / rho);
         Fn[n][1][0] = m; Fn[n][1][1] = (l * m) / rho; Fn[n][1][2] = p + (m * m) / rho; Fn[n][1][3] = (p + en) * (m / rho);
     }
-    // element sizes for the numerical diffusion (FDVGalerkin.h:284-297, Q9)
-    const double h_xi = (0.5 * xn[1][0] + 0.5 * xn[2][0]) - (0.5 * xn[0][0] + 0.5 * xn[3][0]);
-    const double h_eta = (0.5 * xn[0][1] + 0.5 * xn[1][1]) - (0.5 * xn[2][1] + 0.5 * xn[3][1]);
+    // element sizes for the numerical diffusion (FDVGalerkin.h:284-297, Q9),
+    // summed node by node as the reference does: alpha = coth(R/2) - 2/R
+    // cancels for small R, so the last bit of h matters
+    double h_xi = 0.0, h_eta = 0.0;
+    {
+        constexpr double XR[4] = {0.0, 0.5, 0.5, 0.0}, XL[4] = {0.5, 0.0, 0.0, 0.5};
+        constexpr double XT[4] = {0.5, 0.5, 0.0, 0.0}, XB[4] = {0.0, 0.0, 0.5, 0.5};
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            h_xi += XR[n] * xn[n][0] - XL[n] * xn[n][0];
+            h_eta += XT[n] * xn[n][1] - XB[n] * xn[n][1];
+        }
+    }
 
 #pragma unroll
     for (int r = 0; r < NEQ; r++) {
@@ -743,8 +753,20 @@ CFDV_HD void build2d_elem(const BuildArgs& A, const GaussTables& G, long e, int 
 #pragma unroll
         for (int s = 0; s < NEQ; s++) R[r][s] = 0.0;
     }
+    // hanging-node element: Phi <- Phi Hnm, dPhi/dxi <- dPhi/dxi Hnm
+    // (adap(), MeshRefine2D.h:877-900; FDVGalerkin.h:183-186)
+    const double* H = (A.adap && A.adap[e]) ? A.hnm + e * 16 : nullptr;
     for (int gp = 0; gp < 4; gp++) {
-        double jac[2][2];
+        double phl[4], dpl[4][2];
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            phl[n] = G.phi2[gp][n];
+            dpl[n][0] = G.dpxi2[gp][n][0];
+            dpl[n][1] = G.dpxi2[gp][n][1];
+        }
+        // the numerical diffusion's Jacobian is always the plain one
+        // (a fresh dtestfunction, FDVGalerkin.h:237-241)
+        double jn[2][2];
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -752,14 +774,42 @@ CFDV_HD void build2d_elem(const BuildArgs& A, const GaussTables& G, long e, int 
                 double acc = 0.0;
 #pragma unroll
                 for (int n = 0; n < 4; n++) acc += G.dpxi2[gp][n][j] * xn[n][i];
-                jac[i][j] = acc;
+                jn[i][j] = acc;
             }
+        double jac[2][2] = {{jn[0][0], jn[0][1]}, {jn[1][0], jn[1][1]}};
+        if (H) {
+            double pa[4] = {0, 0, 0, 0}, da[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+            for (int Nn = 0; Nn < 4; Nn++)
+#pragma unroll
+                for (int Mm = 0; Mm < 4; Mm++) {
+                    const double h = H[Nn + 4 * Mm];
+                    pa[Mm] += phl[Nn] * h;
+                    da[Mm][0] += dpl[Nn][0] * h;
+                    da[Mm][1] += dpl[Nn][1] * h;
+                }
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                phl[n] = pa[n];
+                dpl[n][0] = da[n][0];
+                dpl[n][1] = da[n][1];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int n = 0; n < 4; n++) acc += dpl[n][j] * xn[n][i];
+                    jac[i][j] = acc;
+                }
+        }
         const double J = jac[0][0] * jac[1][1] - jac[1][0] * jac[0][1];
         const double Jinv = 1.0 / J;
         double dphi[4][2];
 #pragma unroll
         for (int n = 0; n < 4; n++) {
-            const double d0 = G.dpxi2[gp][n][0], d1 = G.dpxi2[gp][n][1];
+            const double d0 = dpl[n][0], d1 = dpl[n][1];
             dphi[n][0] = Jinv * (d0 * jac[1][1] - jac[1][0] * d1);
             dphi[n][1] = Jinv * (jac[0][0] * d1 - d0 * jac[0][1]);
         }
@@ -771,7 +821,7 @@ CFDV_HD void build2d_elem(const BuildArgs& A, const GaussTables& G, long e, int 
         for (int q = 0; q < 4; q++) { f.U[q] = 0; f.dU[q][0] = f.dU[q][1] = 0; }
 #pragma unroll
         for (int n = 0; n < 4; n++) {
-            const double ph = G.phi2[gp][n];
+            const double ph = phl[n];
             f.E += ph * En[n];
             f.T += ph * Tn[n];
 #pragma unroll
@@ -794,11 +844,11 @@ CFDV_HD void build2d_elem(const BuildArgs& A, const GaussTables& G, long e, int 
         // numerical diffusion and discontinuity capturing (:225-387)
         double Psi[4];
         {
-            const double J_xi = jac[0][0] * jac[0][0] + jac[1][0] * jac[1][0];
-            const double J_eta = jac[0][1] * jac[0][1] + jac[1][1] * jac[1][1];
+            const double J_xi = jn[0][0] * jn[0][0] + jn[1][0] * jn[1][0];
+            const double J_eta = jn[0][1] * jn[0][1] + jn[1][1] * jn[1][1];
             const double ixi = 1. / pow(J_xi, 0.5), ieta = 1. / pow(J_eta, 0.5);
-            const double v_xi = f.v[0] * (ixi * jac[0][0]) + f.v[1] * (ixi * jac[1][0]);
-            const double v_eta = f.v[0] * (ieta * jac[0][1]) + f.v[1] * (ieta * jac[1][1]);
+            const double v_xi = f.v[0] * (ixi * jn[0][0]) + f.v[1] * (ixi * jn[1][0]);
+            const double v_eta = f.v[0] * (ieta * jn[0][1]) + f.v[1] * (ieta * jn[1][1]);
             const double S = f.v[0] * f.v[0] + f.v[1] * f.v[1];
             const double R_xi = f.U[0] * v_xi * h_xi / f.mu;
             const double R_eta = f.U[0] * v_eta * h_eta / f.mu;
@@ -832,7 +882,7 @@ CFDV_HD void build2d_elem(const BuildArgs& A, const GaussTables& G, long e, int 
         double PN = 0, PM = 0, DN[2] = {0, 0}, DM[2] = {0, 0}, PsiN = 0;
 #pragma unroll
         for (int n = 0; n < 4; n++) {
-            const double ph = G.phi2[gp][n];
+            const double ph = phl[n];
             if (n == N) { PN = ph; DN[0] = dphi[n][0]; DN[1] = dphi[n][1]; PsiN = Psi[n]; }
             if (n == M) { PM = ph; DM[0] = dphi[n][0]; DM[1] = dphi[n][1]; }
         }
@@ -856,7 +906,7 @@ CFDV_HD void build2d_elem(const BuildArgs& A, const GaussTables& G, long e, int 
                     Gv[i][2] = -f.tau[i][1] * reinv;
                     Gv[i][3] = (-f.tau[i][0] * vn[n][0] - f.tau[i][1] * vn[n][1] - f.k * f.dT[i]) * reinv;
                 }
-                const double ph = G.phi2[gp][n];
+                const double ph = phl[n];
 #pragma unroll
                 for (int i = 0; i < 2; i++)
 #pragma unroll

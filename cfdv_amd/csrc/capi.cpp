@@ -416,14 +416,15 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
     std::vector<int> bslots, aslots;
     std::vector<unsigned char> adap_s;
     std::vector<double> hnm_s;
+    const int nn2 = c->nnod * c->nnod;   // Hnm entries per element
     if (m->adap) {
         adap_s.assign((size_t)np, 0);
-        hnm_s.assign((size_t)np * 64, 0.0);
+        hnm_s.assign((size_t)np * nn2, 0.0);
         for (long s = 0; s < np; s++) {
             const int e = c->perm[s];
             if (e < 0 || !m->adap[e]) continue;
             adap_s[s] = 1;
-            std::memcpy(hnm_s.data() + (size_t)s * 64, m->hnm + (size_t)e * 64, sizeof(double) * 64);
+            std::memcpy(hnm_s.data() + (size_t)s * nn2, m->hnm + (size_t)e * nn2, sizeof(double) * nn2);
         }
     }
     if (c->ndim == 3)
@@ -464,9 +465,9 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
     }
     if (!adap_s.empty()) {
         c->d_adap = c->alloc<unsigned char>((size_t)np);
-        c->d_hnm = c->alloc<double>((size_t)np * 64);
+        c->d_hnm = c->alloc<double>((size_t)np * nn2);
         HIP_CHECK_THROW(hipMemcpy(c->d_adap, adap_s.data(), (size_t)np, hipMemcpyHostToDevice));
-        HIP_CHECK_THROW(hipMemcpy(c->d_hnm, hnm_s.data(), sizeof(double) * np * 64, hipMemcpyHostToDevice));
+        HIP_CHECK_THROW(hipMemcpy(c->d_hnm, hnm_s.data(), sizeof(double) * np * nn2, hipMemcpyHostToDevice));
     }
     if (c->n_aslots > 0) {
         c->d_aslots = c->alloc<int>((size_t)c->n_aslots);
@@ -849,7 +850,7 @@ int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flag
         return CFDV_ERR_ARG;
     if (mesh->n_elem <= 0 || mesh->n_node <= 0 || !mesh->conn || !mesh->x || !mesh->bc || !mesh->dirichlet)
         return CFDV_ERR_ARG;
-    if (mesh->adap && (!mesh->hnm || mesh->ndim != 3)) return CFDV_ERR_ARG;   // Hnm elements: hex8 only
+    if (mesh->adap && !mesh->hnm) return CFDV_ERR_ARG;
     if (!(block_flags & (CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32))) return CFDV_ERR_ARG;
     cfdv_ctx* ctx = new (std::nothrow) cfdv_ctx();
     if (!ctx) return CFDV_ERR_HIP;

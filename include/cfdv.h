@@ -89,7 +89,8 @@ typedef struct {
                               refined mesh, FDV/Element.h:39-41) or NULL         */
     const double *hnm;     /* [n_elem][nnod*nnod] Element::Hnm, Hnm(N, M) at
                               N + nnod*M (Tensor column-major), read where adap
-                              is set; NULL when adap is NULL.  hex8 only        */
+                              is set; NULL when adap is NULL.  The adjoint
+                              entries (cfdv_build_vjp) take hex8 only          */
 } cfdv_mesh;
 
 typedef struct {          /* the aero deck keys (MPIBinaryConfiguration.h:54-63) */
