@@ -3,7 +3,7 @@
 //
 //   cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed]
 //            [--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE]
-//            [--flatten FILE]
+//            [--flatten FILE] [--adap parity|3d]
 //
 // Reads a reference binary deck (control, aero, binarynodes, binaryelements;
 // LoadBinary.h:156-245) into element/node objects shaped like the
@@ -16,11 +16,24 @@
 // --flatten writes the arrays handed to cfdv_create (conn int32, bc int32,
 // x double, dirichlet uint8, U double, in that order) and exits without
 // touching the device.
+// --adap runs the refinement branch instead of the time loop (main.cpp:
+// 143-157): the s1 > 0.8 flag pass on the device, AdaptiveRefineList's
+// closure and MeshRefine2D on the host ("parity": the reference's x, y node
+// match, quirk Q14; "3d": every coordinate), NodeUnpack, then saves the
+// refined deck into <out> (binarynodes, binaryelements, control with adap = 1,
+// aero, and the adap file of SaveBinary::write_adap, SaveBinary.h:153-168).
+// Records are written at the deck's own dimension: the reference's 3D
+// SaveBinary writes 2D-sized records (quirk Q16).
+// A deck whose control has adap = 1 is read with its adap file
+// (LoadBinary.h:247-261: "elno adap refine_level" then the serialised Hnm
+// tensor, TensorStream.h:95-175, column-major).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <deque>
 #include <fstream>
+#include <functional>
 #include <iostream>
 #include <map>
 #include <memory>
@@ -40,11 +53,12 @@ struct Vec {
     T& operator()(int i) { return d[(size_t)i]; }
     T operator()(int i) const { return d[(size_t)i]; }
 };
-struct BVec {
-    std::vector<unsigned char> d;
-    explicit BVec(size_t n = 0) : d(n, 0) {}
-    void set(int i) { d[(size_t)i] = 1; }
-    bool operator()(int i) const { return d[(size_t)i] != 0; }
+struct BVec {   // Tensor<bool,1>: element access by reference
+    std::deque<bool> d;
+    explicit BVec(size_t n = 0) : d(n, false) {}
+    void set(int i) { d[(size_t)i] = true; }
+    bool& operator()(int i) { return d[(size_t)i]; }
+    bool operator()(int i) const { return d[(size_t)i]; }
 };
 
 struct Node {
@@ -57,9 +71,24 @@ struct Face {
     int face = 0;
     int bc = 0;
 };
+// Tensor<double,2>-like storage with the reference's call syntax,
+// column-major as the reference's Tensor (Tensor.h:53-117)
+struct Mat {
+    int n = 0;
+    std::vector<double> d;
+    explicit Mat(int n_ = 0) : n(n_), d((size_t)n_ * n_, 0.0) {}
+    double& operator()(int i, int j) { return d[(size_t)i + (size_t)n * j]; }
+    double operator()(int i, int j) const { return d[(size_t)i + (size_t)n * j]; }
+};
 struct Element {
+    Element() = default;
+    Element(int nnod, int /*neqn*/, int /*ndim*/) : node((size_t)nnod), Hnm(nnod) {}
+    int number = 0;
     std::vector<Node*> node;
     std::vector<Face*> face;
+    Mat Hnm;
+    bool adap = false;
+    int refine_level = 0;
 };
 
 // face -> local node ids (get_face.h:80-165)
@@ -143,10 +172,11 @@ Deck load(const std::string& dir) {
     for (long i = 0; i < ne; i++) {
         ein.read((char*)erec.data(), (std::streamsize)(erec.size() * sizeof(int32_t)));
         if (!ein) throw std::runtime_error("short binaryelements");
-        auto e = std::make_unique<Element>();
+        auto e = std::make_unique<Element>(nnod, neqn, ndim);
+        e->number = (int)i;
         for (int j = 0; j < nnod; j++) {
             if (erec[j] < 0 || erec[j] >= nn) throw std::runtime_error("node id out of range");
-            e->node.push_back(d.nodes[(size_t)erec[j]]);
+            e->node[(size_t)j] = d.nodes[(size_t)erec[j]];
         }
         for (int f = 0; f < nface; f++) {
             const int bc = erec[nnod + f];
@@ -175,7 +205,85 @@ Deck load(const std::string& dir) {
         d.elements.push_back(e.get());
         d.elem_store.push_back(std::move(e));
     }
+    if (d.control.count("adap") && d.control.at("adap") != 0) {
+        // adap file (LoadBinary.h:247-261)
+        std::ifstream af(dir + "/adap");
+        if (!af) throw std::runtime_error("control has adap = 1 but " + dir + "/adap is missing");
+        long elno;
+        while (af >> elno) {
+            int adap = 0, level = 0, tdim = 0, imax = 0, jmax = 0;
+            std::string type;
+            if (!(af >> adap >> level >> type >> tdim >> imax >> jmax) || tdim != 2 || imax != nnod || jmax != nnod ||
+                elno < 0 || elno >= ne)
+                throw std::runtime_error("bad record in " + dir + "/adap");
+            Element* e = d.elements[(size_t)elno];
+            e->adap = adap != 0;
+            e->refine_level = level;
+            for (size_t q = 0; q < e->Hnm.d.size(); q++)
+                if (!(af >> e->Hnm.d[q])) throw std::runtime_error("short Hnm in " + dir + "/adap");
+        }
+    }
     return d;
+}
+
+// SaveBinaryData (SaveBinary.h:34-168) for the refined deck, after NodeUnpack
+// (NodeUnpack.h:124-144): node x, rho, v, T; element node ids then bc codes
+void save_deck(const Deck& d, const std::string& dir, const std::string& src) {
+    const int ndim = d.ndim, nnod = d.nnod, nface = ndim == 3 ? 6 : 4;
+    const double g = d.th.gamma, M = d.th.mach;
+    const double Cv = 1.0 / (g * (g - 1.0) * M * M);
+    std::ofstream nf(dir + "/binarynodes", std::ios::binary);
+    for (const Node* n : d.nodes) {
+        std::vector<double> rec(2 * ndim + 2);
+        const double rho = n->U(0);
+        double vs = 0;
+        for (int j = 0; j < ndim; j++) {
+            rec[j] = n->x(j);
+            const double v = n->U(1 + j) / rho;
+            rec[ndim + 1 + j] = v;
+            vs += v * v;
+        }
+        rec[ndim] = rho;
+        const double E = n->U(ndim + 1) / rho;
+        rec[2 * ndim + 1] = (E - 0.5 * vs) / Cv;
+        nf.write((const char*)rec.data(), (std::streamsize)(rec.size() * sizeof(double)));
+    }
+    std::ofstream ef(dir + "/binaryelements", std::ios::binary);
+    for (const Element* e : d.elements) {
+        std::vector<int32_t> rec(nnod + nface, 0);
+        for (int j = 0; j < nnod; j++) rec[j] = (int32_t)e->node[(size_t)j]->number;
+        for (const Face* f : e->face) rec[nnod + f->face] = f->bc;
+        ef.write((const char*)rec.data(), (std::streamsize)(rec.size() * sizeof(int32_t)));
+    }
+    // control: the source deck's keys with the new sizes and adap = 1
+    std::ifstream cin_(src + "/control");
+    std::ofstream cf(dir + "/control");
+    std::string line;
+    bool have_adap = false;
+    while (std::getline(cin_, line)) {
+        const size_t eq = line.find('=');
+        std::string key = eq == std::string::npos ? "" : line.substr(0, eq);
+        key.erase(key.find_last_not_of(" \t") + 1);
+        key.erase(0, key.find_first_not_of(" \t"));
+        if (key == "nodes") line = "nodes = " + std::to_string(d.nodes.size());
+        else if (key == "elements") line = "elements = " + std::to_string(d.elements.size());
+        else if (key == "adap") { line = "adap = 1"; have_adap = true; }
+        cf << line << "\n";
+    }
+    if (!have_adap) cf << "adap = 1\n";
+    std::ifstream ain(src + "/aero");
+    std::ofstream aout(dir + "/aero");
+    aout << ain.rdbuf();
+    // write_adap: adapted elements only, Hnm serialised column-major
+    std::ofstream adf(dir + "/adap");
+    adf.precision(17);
+    for (const Element* e : d.elements) {
+        if (!e->adap) continue;
+        adf << e->number << " " << 1 << " " << e->refine_level << " double 2 " << nnod << " " << nnod;
+        for (double v : e->Hnm.d) adf << " " << v;
+        adf << "\n";
+    }
+    if (!nf || !ef || !cf || !adf) throw std::runtime_error("cannot write the refined deck into " + dir);
 }
 
 }  // namespace
@@ -183,10 +291,11 @@ Deck load(const std::string& dir) {
 int main(int argc, char** argv) {
     if (argc < 2) {
         std::fprintf(stderr, "usage: cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed] "
-                             "[--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE] [--flatten FILE]\n");
+                             "[--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE] [--flatten FILE] "
+                             "[--adap parity|3d]\n");
         return 2;
     }
-    std::string dir = argv[1], out = ".", dump, mode = "reference", flat, precond = "none";
+    std::string dir = argv[1], out = ".", dump, mode = "reference", flat, precond = "none", adap;
     int iters = -1;
     double rtol = 1e-8;
     for (int i = 2; i + 1 < argc; i += 2) {
@@ -198,6 +307,7 @@ int main(int argc, char** argv) {
         else if (a == "--dump") dump = v;
         else if (a == "--flatten") flat = v;
         else if (a == "--precond") precond = v;
+        else if (a == "--adap") adap = v;
         else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
@@ -224,6 +334,38 @@ int main(int argc, char** argv) {
         const int blocks = md == CFDV_SOLVE_MIXED ? (CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32) : CFDV_BLOCKS_FP64;
         std::printf("nodes/eles: %zu %zu\n", d.nodes.size(), d.elements.size());
         cfdv::Context gpu(d.elements, d.nodes, d.ndim, d.nnod, d.neqn, d.th, blocks);
+        if (!adap.empty()) {
+            if (adap != "parity" && adap != "3d") throw std::runtime_error("--adap takes parity or 3d");
+            cfdv::MeshArrays a = cfdv::flatten(d.elements, d.nodes, d.ndim, d.nnod, d.neqn);
+            std::vector<double> U, U0;
+            std::vector<int32_t> level;
+            for (const Node* n : d.nodes)
+                for (int q = 0; q < d.neqn; q++) {
+                    U.push_back(n->U(q));
+                    U0.push_back(n->U0(q));
+                }
+            for (const Element* e : d.elements) level.push_back(e->refine_level);
+            cfdv::RefineResult res =
+                cfdv::refine_pass(gpu, a, U, U0, level, adap == "3d" ? CFDV_REFINE_3D : CFDV_REFINE_PARITY);
+            std::function<Face*(int, int)> make_face = [&d](int f, int bc) {
+                auto fc = std::make_unique<Face>();
+                fc->face = f;
+                fc->bc = bc;
+                d.face_store.push_back(std::move(fc));
+                return d.face_store.back().get();
+            };
+            std::vector<Element*> els = d.elements;
+            std::vector<Node*> nds = d.nodes;
+            cfdv::apply_refinement<Element, Node, Face>(res, els, nds, make_face);
+            for (size_t i = d.elements.size(); i < els.size(); i++) d.elem_store.emplace_back(els[i]);
+            for (size_t i = d.nodes.size(); i < nds.size(); i++) d.node_store.emplace_back(nds[i]);
+            d.elements = els;
+            d.nodes = nds;
+            std::printf("refined %zu elements in %zu levels -> %zu elements, %zu nodes\n", res.elelist.size(),
+                        res.level_ptr.size() - 1, d.elements.size(), d.nodes.size());
+            save_deck(d, out, dir);
+            return 0;
+        }
         cfdv::HipGMRES<Element, Node> gmres(gpu, restart, m, md, rtol);
         if (md != CFDV_SOLVE_REFERENCE) gmres.options().restarts = std::max(restart, 200);
         if (precond == "nodeblock") gmres.options().precond = CFDV_PRECOND_NODE_BLOCK;

@@ -56,6 +56,7 @@ class Mesh:
     adap: Optional[np.ndarray] = None   # uint8 [n_elem]
     hnm: Optional[np.ndarray] = None    # f64 [n_elem][nnod][nnod], Hnm[e, N, M]
     shape: Optional[tuple] = None       # structured (nx, ny[, nz]) if generated
+    refine_level: Optional[np.ndarray] = None   # int32 [n_elem] (Element::refine_level)
 
     @property
     def nbnod(self) -> int:
@@ -377,7 +378,36 @@ def read_deck(path: str, node_range=None) -> tuple:
     U = conservative(rho, v, T, th.Cv)
     d = dirichlet_flags(ndim, neqn, nn, conn, bc, th.adiabatic)
     mesh = Mesh(ndim, nnod, neqn, conn, x, bc, d)
+    if int(float(control.get("adap", 0))):
+        mesh.adap, mesh.hnm, mesh.refine_level = read_adap(os.path.join(path, "adap"), ne, nnod)
     return mesh, th, U, control
+
+
+def read_adap(path: str, n_elem: int, nnod: int):
+    """The adap file (LoadBinary.h:247-261): per record ``elno adap
+    refine_level`` then the serialised Hnm tensor (TensorStream.h:95-175:
+    type, rank, sizes, data in the Tensor's column-major order).  Elements
+    without a record have adap = 0, level 0, Hnm = identity.  Parsing stops
+    at the end of the last complete record (the reference's ``while
+    (!file.eof())`` loop would read one record past it)."""
+    adap = np.zeros(n_elem, dtype=np.uint8)
+    hnm = np.tile(np.eye(nnod), (n_elem, 1, 1))
+    level = np.zeros(n_elem, dtype=np.int32)
+    tok = open(path).read().split()
+    i = 0
+    while i + 7 <= len(tok):
+        e, a, lv = int(tok[i]), int(tok[i + 1]), int(tok[i + 2])
+        rank, ni, nj = int(tok[i + 4]), int(tok[i + 5]), int(tok[i + 6])
+        if rank != 2 or ni != nnod or nj != nnod or not 0 <= e < n_elem:
+            raise ValueError(f"bad adap record for element {e} in {path}")
+        vals = np.array([float(t) for t in tok[i + 7:i + 7 + ni * nj]])
+        if vals.size != ni * nj:
+            raise ValueError(f"short Hnm for element {e} in {path}")
+        adap[e] = 1 if a else 0
+        level[e] = lv
+        hnm[e] = vals.reshape(nnod, nnod, order="F")
+        i += 7 + ni * nj
+    return adap, hnm, level
 
 
 def write_deck(path: str, mesh: Mesh, th: Thermo, rho, v, T, control_extra=None) -> None:
@@ -396,7 +426,7 @@ def write_deck(path: str, mesh: Mesh, th: Thermo, rho, v, T, control_extra=None)
     ctl = {"itermax": 100, "cfl": 0.5, "gmresrestart": 1, "gmresiter": 20,
            "ndim": nd, "nnod": mesh.nnod, "neqn": mesh.neqn, "nbnod": mesh.nbnod,
            "nface": mesh.nface, "nodes": mesh.n_node, "elements": mesh.n_elem,
-           "iter": 0, "adap": 0, "axi": 0}
+           "iter": 0, "adap": int(mesh.adap is not None and bool(mesh.adap.any())), "axi": 0}
     if control_extra:
         ctl.update(control_extra)
     with open(os.path.join(path, "control"), "w") as fh:
@@ -406,6 +436,13 @@ def write_deck(path: str, mesh: Mesh, th: Thermo, rho, v, T, control_extra=None)
         fh.write(f"M = {th.mach!r}\nPr = {th.prandtl!r}\nRe = {th.reynolds!r}\n"
                  f"Tinf = {th.tinf!r}\nTwall = {th.twall!r}\n"
                  f"Adiabatic = {int(th.adiabatic)}\ngamma = {th.gamma!r}\n")
+    if mesh.adap is not None and mesh.adap.any():
+        # SaveBinary::write_adap (SaveBinary.h:153-168): adapted elements only
+        lv = mesh.refine_level if mesh.refine_level is not None else np.zeros(mesh.n_elem, np.int32)
+        with open(os.path.join(path, "adap"), "w") as fh:
+            for e in np.nonzero(mesh.adap)[0]:
+                data = " ".join(repr(float(v)) for v in np.asarray(mesh.hnm[e]).ravel(order="F"))
+                fh.write(f"{e} 1 {int(lv[e])} double 2 {mesh.nnod} {mesh.nnod} {data}\n")
     with open(os.path.join(path, "MPIBreakdown"), "w") as fh:
         fh.write("1\nelemin, elemax, nodemin, nodemax, nedgenode_l, nedgenode_r, "
                  "NodeIteratorMin, NodeIteratorMax\n")

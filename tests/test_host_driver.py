@@ -83,3 +83,72 @@ def test_driver_two_reference_iterations(gpu_available, runner, tmp_path):
         assert int(it) == k
         v = np.array([float(t) for t in vals.split()])
         assert np.allclose(v, g["rms_steps"][k], rtol=1e-5, atol=1e-12)
+
+
+def _refined_deck(tmp_path):
+    """a flat-plate deck refined twice on the host (3D node matching)"""
+    from cfdv_amd import refine as PR
+    mesh, th, U = M.flat_plate(6, 3, 3)
+    r = PR.RefinableMesh(mesh, U)
+    for every, off in ((5, 2), (7, 1)):
+        ne = r.sizes()[0]
+        r.apply(*r.refine_list((np.arange(ne) % every == off).astype(np.uint8)), PR.MODE_3D)
+    m2, U2, _, lv = r.get()
+    m2.refine_level = lv
+    p = M.primitives(U2, 3, th.Cv, th.gamma)
+    deck = tmp_path / "refined"
+    M.write_deck(str(deck), m2, th, p["rho"], p["v"], p["T"])
+    return deck, m2, U2
+
+
+def test_adap_deck_round_trip(runner, tmp_path):
+    deck, m2, U2 = _refined_deck(tmp_path)
+    back, th, U, ctl = M.read_deck(str(deck))
+    assert int(ctl["adap"]) == 1
+    assert np.array_equal(back.conn, m2.conn) and np.array_equal(back.bc, m2.bc)
+    assert np.array_equal(back.adap, m2.adap) and np.array_equal(back.refine_level * back.adap,
+                                                                  m2.refine_level * m2.adap)
+    sel = m2.adap.astype(bool)
+    assert np.array_equal(back.hnm[sel], m2.hnm[sel])
+    assert np.allclose(U, U2, rtol=1e-14, atol=0)
+    # the C++ driver reads the same adap file into Element::adap / Hnm
+    out = tmp_path / "flat.bin"
+    subprocess.run([runner, str(deck), "--flatten", str(out)], check=True, capture_output=True)
+    raw = out.read_bytes()
+    conn = np.frombuffer(raw, np.int32, back.n_elem * 8, 0).reshape(-1, 8)
+    assert np.array_equal(conn, back.conn)
+
+
+@pytest.mark.gpu
+def test_driver_adap_pass_matches_host_refinement(gpu_available, runner, tmp_path):
+    """cfdv_run --adap 3d (device flags, host closure and split, SaveBinary)
+    writes the deck that the Python mirror of the same pass produces"""
+    from cfdv_amd import refine as PR
+    from cfdv_amd.fdv import FDVContext
+    mesh, th, U = M.flat_plate(8, 4, 4)
+    p = M.primitives(U, 3, th.Cv, th.gamma)
+    src = tmp_path / "plate"
+    M.write_deck(str(src), mesh, th, p["rho"], p["v"], p["T"])
+    mesh, th, U, _ = M.read_deck(str(src))
+    out = tmp_path / "plate_refined"
+    os.makedirs(out)
+    r = subprocess.run([runner, str(src), "--adap", "3d", "--out", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    ctx = FDVContext(mesh, th, U)
+    flags, _, _ = ctx.refine_flags()
+    assert flags.any()
+    m2, U2, _, lv, el, rl = PR.refine_pass(mesh, U, flags, mode=PR.MODE_3D)
+    got, _, Ug, _ = M.read_deck(str(out))
+    assert np.array_equal(got.conn, m2.conn) and np.array_equal(got.bc, m2.bc)
+    assert np.array_equal(got.x, m2.x)
+    assert np.array_equal(got.adap, m2.adap)
+    sel = m2.adap.astype(bool)
+    assert np.array_equal(got.hnm[sel], m2.hnm[sel])
+    assert np.array_equal(got.refine_level[sel], lv[sel])
+    assert np.allclose(Ug, U2, rtol=1e-13, atol=0)
+    # the refined deck runs: one reference-mode iteration of the driver
+    r = subprocess.run([runner, str(out), "--iters", "1", "--out", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
