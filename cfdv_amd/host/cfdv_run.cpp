@@ -3,7 +3,7 @@
 //
 //   cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed]
 //            [--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE]
-//            [--flatten FILE] [--adap parity|3d]
+//            [--flatten FILE] [--adap parity|3d] [--tecplot N] [--tecplot-only]
 //
 // Reads a reference binary deck (control, aero, binarynodes, binaryelements;
 // LoadBinary.h:156-245) into element/node objects shaped like the
@@ -24,6 +24,11 @@
 // aero, and the adap file of SaveBinary::write_adap, SaveBinary.h:153-168).
 // Records are written at the deck's own dimension: the reference's 3D
 // SaveBinary writes 2D-sized records (quirk Q16).
+// --tecplot N writes <out>/tecplot_<iter>.plt every N iterations (the
+// reference's Output every 100, main.cpp:183) with the state the build of
+// that iteration reads and the element s1..s4 of that build (ele_data, as
+// main.cpp passes it); --tecplot-only writes <out>/tecplot_<iter>.plt of the
+// loaded deck without s1..s4 and without touching the device.
 // A deck whose control has adap = 1 is read with its adap file
 // (LoadBinary.h:247-261: "elno adap refine_level" then the serialised Hnm
 // tensor, TensorStream.h:95-175, column-major).
@@ -34,6 +39,7 @@
 #include <deque>
 #include <fstream>
 #include <functional>
+#include <iomanip>
 #include <iostream>
 #include <map>
 #include <memory>
@@ -112,6 +118,9 @@ std::map<std::string, double> read_keys(const std::string& path) {
     return k;
 }
 
+// Thermo::set (Thermo.h:22-33), in its operation order
+double thermo_cv(const cfdv_thermo& th) { return 1. / th.gamma / (th.gamma - 1.0) / std::pow(th.mach, 2); }
+
 struct Deck {
     int ndim = 0, nnod = 0, neqn = 0;
     std::map<std::string, double> control, aero;
@@ -138,9 +147,7 @@ Deck load(const std::string& dir) {
     d.th.twall = d.aero.at("Twall");
     d.th.adiabatic = d.aero.at("Adiabatic") != 0;
     const int ndim = d.ndim, nnod = d.nnod, neqn = d.neqn, nface = ndim == 3 ? 6 : 4;
-    // Thermo::set (Thermo.h:22-33): Cv = 1 / (gamma (gamma-1) M^2)
-    const double g = d.th.gamma, M = d.th.mach;
-    const double Cv = 1.0 / (g * (g - 1.0) * M * M);
+    const double Cv = thermo_cv(d.th);
 
     std::ifstream nin(dir + "/binarynodes", std::ios::binary);
     std::ifstream ein(dir + "/binaryelements", std::ios::binary);
@@ -226,12 +233,88 @@ Deck load(const std::string& dir) {
     return d;
 }
 
+// the file-name number: to_string<double>(iter) (Utility/tostring.h:5-11,
+// MPI_TecplotOut.h:189), i.e. the default stream format of a double
+std::string iter_name(int iter) {
+    std::ostringstream ss;
+    ss << (double)iter;
+    return ss.str();
+}
+
+// MPI_TecplotOut (Utility/IO/MPI_TecplotOut.h:43-372) on one rank: node
+// primitives from NodeUnpack (FDV/Function/NodeUnpack.h:10-30), BLOCK data
+// of 7-digit scientific values, a line break after value 100, 200, ...,
+// p tab-separated, optional per-element s1..s4 (tab-separated, no line
+// break between the four blocks), then the 1-based connectivity.
+void write_tecplot(const Deck& d, const std::string& path, double time, const double* s /* [n_elem][4] or null */) {
+    const int ndim = d.ndim, nnod = d.nnod;
+    const double Cv = thermo_cv(d.th), gamma = d.th.gamma;
+    const size_t nn = d.nodes.size(), ne = d.elements.size();
+    std::vector<double> rho(nn), T(nn), p(nn), v(nn * ndim);
+    for (size_t n = 0; n < nn; n++) {
+        const Node* nd = d.nodes[n];
+        rho[n] = nd->U(0);
+        double vsqr = 0;
+        for (int i = 0; i < ndim; i++) {
+            v[n * ndim + i] = nd->U(1 + i) / nd->U(0);
+            vsqr += (v[n * ndim + i] * v[n * ndim + i]);
+        }
+        const double E = nd->U(ndim + 1) / nd->U(0);
+        const double e = E - 0.5 * vsqr;
+        T[n] = e / Cv;
+        p[n] = rho[n] * (gamma - 1.0) * e;
+    }
+    std::ofstream tout(path.c_str(), std::ios::out);
+    const bool ele_data = s != nullptr;
+    tout << "title = \"Output at time = " << time << " \"" << std::endl;
+    tout << "variables = \"x\", \"y\",";
+    if (ndim == 3) tout << " \"z\",";
+    tout << "\"rho\", \"u\", \"v\",";
+    if (ndim == 3) tout << " \"w\",";
+    tout << "\"T\", \"p\",";
+    if (ele_data) tout << "\"s1\", \"s2\", \"s3\", \"s4\", ";
+    tout << std::endl;
+    tout << "zone E=" << ne << ", N=" << nn;
+    tout << ", DATAPACKING=BLOCK, ZONETYPE=FEQUADRILATERAL";
+    if (ndim == 3) tout << ", DATAPACKING=BLOCK, ZONETYPE=FEBRICK";
+    tout << std::endl;
+    if (ele_data) {
+        if (ndim == 2) tout << "VARLOCATION=([8,9,10,11]=CellCentered)" << std::endl;
+        else tout << "VARLOCATION=([10,11,12,13]=CellCentered)" << std::endl;
+    }
+    tout << std::endl;
+    tout << std::scientific << std::endl;
+    tout << std::setprecision(7) << std::endl;
+    auto block = [&](auto value, const char* sep, bool end_line) {
+        for (size_t n = 0; n < nn; n++) {
+            tout << value(n) << sep;
+            if (n % 100 == 0 && n != 0) tout << std::endl;
+        }
+        if (end_line) tout << std::endl;
+    };
+    for (int i = 0; i < ndim; i++) block([&](size_t n) { return d.nodes[n]->x(i); }, " ", true);
+    block([&](size_t n) { return rho[n]; }, " ", true);
+    for (int i = 0; i < ndim; i++) block([&](size_t n) { return v[n * ndim + i]; }, " ", true);
+    block([&](size_t n) { return T[n]; }, " ", true);
+    block([&](size_t n) { return p[n]; }, "\t", true);
+    if (ele_data)
+        for (int q = 0; q < 4; q++)
+            for (size_t f = 0; f < ne; f++) {
+                tout << s[f * 4 + q] << "\t";
+                if (f % 100 == 0 && f != 0) tout << std::endl;
+            }
+    for (size_t e = 0; e < ne; e++) {
+        for (int n = 0; n < nnod; n++) tout << d.elements[e]->node[(size_t)n]->number + 1 << " ";
+        tout << std::endl;
+    }
+    if (!tout) throw std::runtime_error("cannot write " + path);
+}
+
 // SaveBinaryData (SaveBinary.h:34-168) for the refined deck, after NodeUnpack
 // (NodeUnpack.h:124-144): node x, rho, v, T; element node ids then bc codes
 void save_deck(const Deck& d, const std::string& dir, const std::string& src) {
     const int ndim = d.ndim, nnod = d.nnod, nface = ndim == 3 ? 6 : 4;
-    const double g = d.th.gamma, M = d.th.mach;
-    const double Cv = 1.0 / (g * (g - 1.0) * M * M);
+    const double Cv = thermo_cv(d.th);
     std::ofstream nf(dir + "/binarynodes", std::ios::binary);
     for (const Node* n : d.nodes) {
         std::vector<double> rec(2 * ndim + 2);
@@ -296,10 +379,22 @@ int main(int argc, char** argv) {
         return 2;
     }
     std::string dir = argv[1], out = ".", dump, mode = "reference", flat, precond = "none", adap;
+    int tecplot = 0;
+    bool tecplot_only = false;
     int iters = -1;
     double rtol = 1e-8;
-    for (int i = 2; i + 1 < argc; i += 2) {
-        std::string a = argv[i], v = argv[i + 1];
+    for (int i = 2; i < argc; i += 2) {
+        std::string a = argv[i];
+        if (a == "--tecplot-only") {
+            tecplot_only = true;
+            i--;
+            continue;
+        }
+        if (i + 1 >= argc) {
+            std::fprintf(stderr, "option %s needs a value\n", a.c_str());
+            return 2;
+        }
+        std::string v = argv[i + 1];
         if (a == "--iters") iters = std::atoi(v.c_str());
         else if (a == "--mode") mode = v;
         else if (a == "--rtol") rtol = std::atof(v.c_str());
@@ -308,6 +403,7 @@ int main(int argc, char** argv) {
         else if (a == "--flatten") flat = v;
         else if (a == "--precond") precond = v;
         else if (a == "--adap") adap = v;
+        else if (a == "--tecplot") tecplot = std::atoi(v.c_str());
         else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
@@ -324,6 +420,11 @@ int main(int argc, char** argv) {
             f.write((const char*)a.dirichlet.data(), (std::streamsize)a.dirichlet.size());
             for (const Node* n : d.nodes) f.write((const char*)n->U.d.data(), (std::streamsize)(d.neqn * sizeof(double)));
             return f ? 0 : 1;
+        }
+        if (tecplot_only) {
+            const int it0 = d.control.count("iter") ? (int)d.control.at("iter") : 0;
+            write_tecplot(d, out + "/tecplot_" + iter_name(it0) + ".plt", 0.0, nullptr);
+            return 0;
         }
         const double cfl = d.control.at("cfl");
         const int restart = (int)d.control.at("gmresrestart");
@@ -376,6 +477,12 @@ int main(int argc, char** argv) {
             const double dt = gpu.cfl(cfl);
             cfdv::FDVGalerkin fdv(gpu, dt);
             fdv();
+            if (tecplot > 0 && iter % tecplot == 0) {   // Output (main.cpp:183)
+                std::vector<double> sv(d.elements.size() * 4);
+                cfdv::check(cfdv_get_blocks(gpu.handle(), nullptr, nullptr, sv.data()), gpu.handle(),
+                            "cfdv_get_blocks");
+                write_tecplot(d, out + "/tecplot_" + iter_name(iter) + ".plt", t, sv.data());
+            }
             gmres.iterate(d.elements);
             gmres.update(d.nodes);
             // MPI_RMSError (MPI_RMSError.h:19-74), from the updated nodes

@@ -30,7 +30,11 @@
    breakpoints, connectivity, bc, coordinates, U, U0, Dirichlet flags,
    adap, Hnm, refine levels).
 
-Run from the repository root:  python tests/golden/make_golden.py [--refine]
+4. The Tecplot writer's fixture (``--tecplot`` writes only this): the deck
+   backstep_refined_new (control, aero, gzipped binary records) and the
+   reference's own output on it at t = 0, tecplot_0.plt (gzipped).
+
+Run from the repository root:  python tests/golden/make_golden.py [--refine | --tecplot]
 """
 from __future__ import annotations
 
@@ -56,6 +60,26 @@ DECKS = {
 }
 BIN_FILES = ("control", "aero", "binarynodes", "binaryelements")
 ASCII_FILES = ("nodes", "elements", "faces")
+
+
+TECPLOT_DECK = "backstep_refined_new"   # ships tecplot_0.plt, the writer's own output at t = 0
+
+
+def copy_tecplot_fixture():
+    """backstep_refined_new (18400 quad4): its control, aero and binary
+    records (gzipped) and the reference's own tecplot_0.plt (gzipped)"""
+    import gzip
+    src = os.path.join(SRC, TECPLOT_DECK)
+    dst = os.path.join(HERE, "decks", TECPLOT_DECK)
+    os.makedirs(dst, exist_ok=True)
+    for f in ("control", "aero"):
+        shutil.copyfile(os.path.join(src, f), os.path.join(dst, f))
+        os.chmod(os.path.join(dst, f), 0o644)
+    for f, out in (("binarynodes", os.path.join(dst, "binarynodes.gz")),
+                   ("binaryelements", os.path.join(dst, "binaryelements.gz")),
+                   ("tecplot_0.plt", os.path.join(HERE, "tecplot_backstep_0.plt.gz"))):
+        with open(os.path.join(src, f), "rb") as fi, gzip.open(out, "wb", compresslevel=9) as fo:
+            fo.write(fi.read())
 
 
 def copy_decks():
@@ -161,9 +185,13 @@ def make_refine(name):
 
 
 def main():
+    if "--tecplot" in sys.argv:
+        copy_tecplot_fixture()
+        return
     if "--refine" not in sys.argv:
         if os.path.isdir(SRC):
             copy_decks()
+            copy_tecplot_fixture()
         for name in DECKS:
             make(name)
     for name in REFINE_WINDOWS:

@@ -152,3 +152,66 @@ def test_driver_adap_pass_matches_host_refinement(gpu_available, runner, tmp_pat
     r = subprocess.run([runner, str(out), "--iters", "1", "--out", str(out)], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr
+
+
+def _unpack_deck(tmp_path, name):
+    """a golden deck stored with gzipped binary records, unpacked to tmp"""
+    import gzip
+    import shutil
+    src = os.path.join(GOLD, "decks", name)
+    dst = tmp_path / name
+    os.makedirs(dst)
+    for f in os.listdir(src):
+        if f.endswith(".gz"):
+            with gzip.open(os.path.join(src, f), "rb") as fi, open(dst / f[:-3], "wb") as fo:
+                fo.write(fi.read())
+        else:
+            shutil.copyfile(os.path.join(src, f), dst / f)
+    return dst
+
+
+def test_tecplot_writer_reproduces_reference_output(runner, tmp_path):
+    """MPI_TecplotOut's file for backstep_refined_new at t = 0, byte for
+    byte against the tecplot_0.plt the reference ships with that deck"""
+    import gzip
+    deck = _unpack_deck(tmp_path, "backstep_refined_new")
+    out = tmp_path / "out"
+    os.makedirs(out)
+    subprocess.run([runner, str(deck), "--tecplot-only", "--out", str(out)], check=True, capture_output=True)
+    got = (out / "tecplot_0.plt").read_bytes()
+    with gzip.open(os.path.join(GOLD, "tecplot_backstep_0.plt.gz"), "rb") as f:
+        want = f.read()
+    assert len(got) == len(want)
+    assert got == want
+
+
+@pytest.mark.gpu
+def test_tecplot_during_time_loop_has_element_s(gpu_available, runner, tmp_path):
+    """with the time loop, the Output file also carries s1..s4 per element
+    (ele_data, main.cpp:183): the node blocks equal the t = 0 file and the s
+    blocks the oracle's FDVParam to the printed 7 digits"""
+    from oracle import oracle as O
+    deck = os.path.join(GOLD, "decks", "lid50")
+    out = tmp_path / "out"
+    os.makedirs(out)
+    r = subprocess.run([runner, deck, "--iters", "1", "--tecplot", "1", "--out", str(out)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    only = tmp_path / "only"
+    os.makedirs(only)
+    subprocess.run([runner, deck, "--tecplot-only", "--out", str(only)], check=True, capture_output=True)
+    a = (out / "tecplot_0.plt").read_text().split("\n")
+    b = (only / "tecplot_0.plt").read_text().split("\n")
+    assert a[1].startswith(b[1]) and a[1].endswith('"s1", "s2", "s3", "s4", ')
+    assert a[3] == "VARLOCATION=([8,9,10,11]=CellCentered)"
+    mesh, th, U, _ = M.read_deck(deck)
+    # node blocks (x y rho u v T p): identical text
+    body_a = "\n".join(a[4:]).split()
+    body_b = "\n".join(b[3:]).split()
+    nvals = 7 * mesh.n_node
+    assert body_a[:nvals] == body_b[:nvals]
+    s_file = np.array([float(t) for t in body_a[nvals:nvals + 4 * mesh.n_elem]]).reshape(4, -1).T
+    so, _ = O.fdvparam_all(th, mesh, U)
+    assert np.allclose(s_file, so, rtol=1e-6, atol=1e-300)
+    conn = np.array([int(t) for t in body_a[nvals + 4 * mesh.n_elem:]]).reshape(-1, 4) - 1
+    assert np.array_equal(conn, mesh.conn)
