@@ -12,6 +12,10 @@ Two paths are timed from the same initial state:
   fp64  : FP64 blocks, FP64 restarted GMRES(m)
   mixed : FP32 blocks with FP32 accumulation inside FP64 iterative
           refinement (outer residual with FP64 blocks)
+The solver's products K x run on the assembled node rows (the element
+blocks summed per node pair after each build: 27 blocks per node instead of
+64 per element); ``--storage ebe`` multiplies element by element as the
+reference does.
 ``value`` is the mixed path's steps/s; the FP64 path and the ratio are
 reported beside it.  The CPU baseline times the CPU oracle (a port of the
 reference algorithm) on a bounded random sample of the same mesh and
@@ -32,12 +36,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per full EBE matvec (8 color launches) measured with rocprofv3
-# FETCH_SIZE / WRITE_SIZE on MI355X, each scaled by the counter's reading for
-# a plain 1 GiB copy kernel (fetch x2.0, write x1.0); summary and commands in
-# profiles/ebe_matvec_ramp_256x128x64_hbm_counters.txt.  Only the default
-# workload size has a measurement.
-MEASURED_TRAFFIC = {"256x128x64": {"fp32": 14.607e9, "fp64": 29.121e9}}
+# HBM bytes per full product measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# on MI355X, each scaled by the counter's reading for a plain 1 GiB copy
+# kernel (fetch x2.0, write x1.0); summaries and commands in
+# profiles/ebe_matvec_ramp_256x128x64_hbm_counters.txt (EBE, 8 color
+# launches).  Only the default workload size has a measurement.
+MEASURED_TRAFFIC = {("ebe", "256x128x64"): {"fp32": 14.607e9, "fp64": 29.121e9}}
 
 
 def log(*a):
@@ -59,6 +63,7 @@ def parse():
     # sweeps, 1e-5: more inner iterations)
     p.add_argument("--inner-rtol", type=float, default=1e-4)
     p.add_argument("--precond", choices=["nodeblock", "none"], default="nodeblock")
+    p.add_argument("--storage", choices=["rows", "ebe"], default="rows")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=32768)
     return p.parse_args()
@@ -81,7 +86,8 @@ def run_steps(ctx, U0, mode, n, args):
                       rtol=args.rtol, inner_m=args.inner_m, inner_rtol=args.inner_rtol,
                       inner_restarts=3, precond=args.precond)
         out.append(st)
-        log(f"  {mode} step {i}: build {st.ms_build:.1f} ms solve {st.ms_solve:.1f} ms "
+        log(f"  {mode} step {i}: build {st.ms_build:.1f} ms assemble {st.ms_assemble:.1f} ms "
+            f"solve {st.ms_solve:.1f} ms "
             f"mv64 {st.solve.matvecs64} mv32 {st.solve.matvecs32} rel {st.solve.rel_resid:.2e}")
     return out
 
@@ -127,7 +133,7 @@ def main():
     mesh, th, U0 = M.compression_ramp(nx, ny, nz)
     log(f"mesh {nx}x{ny}x{nz}: {mesh.n_elem} elements, {mesh.n_node} nodes ({time.time() - t0:.1f}s)")
     t0 = time.time()
-    ctx = FDVContext(mesh, th, U0, blocks=("fp64", "fp32"))
+    ctx = FDVContext(mesh, th, U0, blocks=("fp64", "rows") if args.storage == "rows" else ("fp64", "fp32"))
     info = ctx.info()
     log(f"context: {info} ({time.time() - t0:.1f}s)")
 
@@ -148,8 +154,20 @@ def main():
     fp64_sps = args.steps / el_64
 
     ne, nn = mesh.n_elem, mesh.n_node
-    b32 = ne * 1600 * 4 + nn * 5 * 2 * 4
-    b64 = ne * 1600 * 8 + nn * 5 * 2 * 8
+    if args.storage == "rows":
+        # one block per node pair sharing an element: on the structured
+        # (nx+1)(ny+1)(nz+1) node grid, (3nx+1)(3ny+1)(3nz+1) blocks; each
+        # block's 25 values plus its column index, x read and y written once
+        nnzb = (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1)
+        b32 = nnzb * (25 * 4 + 4) + nn * 5 * 2 * 4
+        b64 = nnzb * (25 * 8 + 4) + nn * 5 * 2 * 8
+        kname = "rows_matvec_kernel<float,float,float,5> (assembled node rows, FP32)"
+        per = f"{nnzb} node blocks"
+    else:
+        b32 = ne * 1600 * 4 + nn * 5 * 2 * 4
+        b64 = ne * 1600 * 8 + nn * 5 * 2 * 8
+        kname = "matvec_kernel<float,float,float,8,5> (EBE matvec, FP32 blocks)"
+        per = f"{info['ncolors']} color launches"
     ms32 = sum(s.ms_matvec32 for s in st_mx)
     n32 = sum(s.n_matvec32 for s in st_mx)
     ms64 = sum(s.ms_matvec64 for s in st_64)
@@ -169,7 +187,7 @@ def main():
                           f"({det['t_matvec_sample_s'] * 1e3:.1f} ms), scaled to {ne} elements "
                           f"with {mv_per_step} matvecs per step (the FP64 GPU path's count)")}
 
-    traffic = MEASURED_TRAFFIC.get(args.size, {})
+    traffic = MEASURED_TRAFFIC.get((args.storage, args.size), {})
     line = {
         "metric": "FDV steps/s at 2.1M hex8: FP32+FP64-refine vs FP64 GPU, equal 1e-8 residual",
         "value": value,
@@ -188,12 +206,14 @@ def main():
                                f"CFL {args.cfl}, GMRES({args.m}) to rel. residual {args.rtol}, "
                                f"{args.precond} right preconditioner on both paths",
                    "elements": ne, "nodes": nn, "colors": info["ncolors"],
-                   "precond": args.precond, "inner_m": args.inner_m, "inner_rtol": args.inner_rtol},
+                   "precond": args.precond, "inner_m": args.inner_m, "inner_rtol": args.inner_rtol,
+                   "storage": args.storage},
         "fp64_steps_per_s": fp64_sps,
         "mixed_over_fp64": value / fp64_sps,
         "rel_resid_mixed": max(s.solve.rel_resid for s in st_mx),
         "rel_resid_fp64": max(s.solve.rel_resid for s in st_64),
         "ms_build": float(np.mean([s.ms_build for s in st_mx])),
+        "ms_assemble": float(np.mean([s.ms_assemble for s in st_mx])),
         "ms_solve_mixed": float(np.mean([s.ms_solve for s in st_mx])),
         "ms_solve_fp64": float(np.mean([s.ms_solve for s in st_64])),
         "matvecs_per_step_mixed": {"fp32": n32 / args.steps,
@@ -202,9 +222,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (ach32 / HBM_PEAK_GBS) if ach32 else None,
                      "traffic": traffic.get("fp32"),
-                     "kernel": "matvec_kernel<float,float,float,8,5> (EBE matvec, FP32 blocks), "
-                               f"{b32} algorithmic bytes per full matvec "
-                               f"({info['ncolors']} color launches)"},
+                     "kernel": f"{kname}, {b32} algorithmic bytes per full product ({per})"},
         "roofline_fp64_matvec": {"achieved": ach64, "frac": (ach64 / HBM_PEAK_GBS) if ach64 else None,
                                  "bytes_per_matvec": b64, "traffic": traffic.get("fp64")},
         "cpu_baseline": cpu,

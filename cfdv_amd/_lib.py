@@ -27,6 +27,7 @@ CFDV_REFINE_3D = 1
 
 CFDV_BLOCKS_FP64 = 1
 CFDV_BLOCKS_FP32 = 2
+CFDV_BLOCKS_ROWS = 4
 
 CFDV_SOLVE_REFERENCE = 0
 CFDV_SOLVE_FP64 = 1
@@ -77,7 +78,7 @@ class StepStats(C.Structure):
                 ("ms_build", C.c_double), ("ms_solve", C.c_double), ("ms_update", C.c_double),
                 ("ms_matvec64", C.c_double), ("ms_matvec32", C.c_double),
                 ("n_matvec64", C.c_int), ("n_matvec32", C.c_int),
-                ("solve", SolveStats)]
+                ("solve", SolveStats), ("ms_assemble", C.c_double)]
 
 
 _LIB = None

@@ -13,7 +13,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcfdv_amd.so")
-SOURCES = ["build.hip", "solve.hip", "adjoint.hip", "capi.cpp", "../host/refine.cpp"]
+SOURCES = ["build.hip", "solve.hip", "rows.hip", "adjoint.hip", "capi.cpp", "../host/refine.cpp"]
 ROOT = os.path.dirname(HERE)
 RUNNER = os.path.join(HERE, "cfdv_run")
 ARCH = os.environ.get("CFDV_OFFLOAD_ARCH", "gfx950")

@@ -151,6 +151,14 @@ struct cfdv_ctx {
     double *d_P64 = nullptr, *d_z64 = nullptr, *d_t64 = nullptr;
     float *d_P32 = nullptr, *d_z32 = nullptr;
     bool pinv_valid = false;
+    // assembled node rows (CFDV_BLOCKS_ROWS), formed after each build
+    RowsArgs ra{};
+    long n_groups = 0;
+    long *d_goff = nullptr, *d_coff = nullptr;
+    int *d_cols = nullptr, *d_contrib = nullptr;
+    double* d_B64 = nullptr;
+    float* d_B32 = nullptr;
+    bool rows_off = false;    // set while the reference solve runs (EBE only)
     // discrete adjoint: per-slot tangents [n_pad][41], host-given vectors
     double *d_gelem = nullptr, *d_lam = nullptr, *d_duv = nullptr, *d_gU = nullptr, *d_g = nullptr;
     double last_dt = 0.0;
@@ -184,7 +192,18 @@ struct cfdv_ctx {
     void sync() { HIP_CHECK_THROW(hipStreamSynchronize(st)); }
     void check_launch() { HIP_CHECK_THROW(hipGetLastError()); }
 
-    // ---- EBE matvec over all colors: y = K x (or K^T x) ----
+    // products K x use the assembled rows when the context holds them (not
+    // for K^T x, nor inside the reference solve)
+    bool use_rows(int prec, bool tr) const {
+        if (tr || rows_off) return false;
+        return prec == 64 ? d_B64 != nullptr : d_B32 != nullptr;
+    }
+    // blocks available for a product of this precision
+    bool have_blocks(int prec, bool tr) const {
+        if (use_rows(prec, tr)) return true;
+        return prec == 64 ? d_R64 != nullptr : d_R32 != nullptr;
+    }
+    // ---- matvec y = K x (or K^T x): assembled rows, or EBE over all colors ----
     hipEvent_t next_event() {
         if (ev_used == (int)evpool.size()) {
             hipEvent_t e;
@@ -199,11 +218,15 @@ struct cfdv_ctx {
             mark = ev_used;
             HIP_CHECK_THROW(hipEventRecord(next_event(), st));
         }
-        if (prec == 32) vec_fill_f((float*)y, 0.0f, ndof, st);
-        else vec_fill((double*)y, 0.0, ndof, st);
-        for (int c = 0; c < ncolor; c++) {
-            MatvecArgs a{ndim, nnod, neqn, cstart[c], cstart[c + 1], d_conn};
-            launch_matvec_color(a, d_R64, d_R32, x, y, prec, tr, st);
+        if (use_rows(prec, tr)) {
+            launch_rows_matvec(ra, d_B64, d_B32, x, y, prec, st);
+        } else {
+            if (prec == 32) vec_fill_f((float*)y, 0.0f, ndof, st);
+            else vec_fill((double*)y, 0.0, ndof, st);
+            for (int c = 0; c < ncolor; c++) {
+                MatvecArgs a{ndim, nnod, neqn, cstart[c], cstart[c + 1], d_conn};
+                launch_matvec_color(a, d_R64, d_R32, x, y, prec, tr, st);
+            }
         }
         check_launch();
         if (timing) {
@@ -306,6 +329,125 @@ DevThermo make_thermo(const cfdv_thermo* t) {
     th.Twall = t->twall;
     th.adiabatic = t->adiabatic;
     return th;
+}
+
+// Structure of the assembled node rows (see rows.hip): per node the sorted
+// neighbour nodes (nodes sharing an element, itself included), chunks of 64
+// rows with groups of 4 slots, and per slot the element blocks that sum
+// into it, listed in element order.  ptr/elem: node -> element CSR.
+struct RowsHost {
+    std::vector<long> goff, coff;
+    std::vector<int> cols, contrib;
+};
+
+RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* conn, const std::vector<int>& perm_inv,
+                        const std::vector<int>& ptr, const std::vector<int>& elem) {
+    const int nn2 = nn * nn;
+    if ((double)n_pad * nn2 >= 2147483647.0) throw std::invalid_argument("mesh too large for assembled rows");
+    // neighbour lists
+    std::vector<long> aptr(nnode + 1, 0);
+    std::vector<int> adj;
+    adj.reserve((size_t)nnode * (nn == 8 ? 27 : 9));
+    std::vector<int> tmp;
+    for (long n = 0; n < nnode; n++) {
+        tmp.clear();
+        for (int p = ptr[n]; p < ptr[n + 1]; p++) {
+            const long e = elem[p];
+            for (int l = 0; l < nn; l++) tmp.push_back(conn[e * nn + l]);
+        }
+        if (tmp.empty()) tmp.push_back((int)n);   // a node of no element: a zero row
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        adj.insert(adj.end(), tmp.begin(), tmp.end());
+        aptr[n + 1] = (long)adj.size();
+    }
+    // contributions per adjacency entry (CSR over adj), element order
+    std::vector<long> eptr(adj.size() + 1, 0);
+    auto slot_of = [&](long row, int col) -> long {
+        const int* b = adj.data() + aptr[row];
+        const int* e = adj.data() + aptr[row + 1];
+        const int* p = std::lower_bound(b, e, col);
+        return aptr[row] + (p - b);
+    };
+    for (long e = 0; e < ne; e++)
+        for (int N = 0; N < nn; N++)
+            for (int M = 0; M < nn; M++)
+                eptr[slot_of(conn[e * nn + N], conn[e * nn + M]) + 1]++;
+    for (size_t i = 0; i < adj.size(); i++) eptr[i + 1] += eptr[i];
+    std::vector<int> codes((size_t)eptr[adj.size()]);
+    {
+        std::vector<long> pos(eptr.begin(), eptr.end() - 1);
+        for (long e = 0; e < ne; e++) {
+            const long sl = perm_inv[e];
+            for (int N = 0; N < nn; N++)
+                for (int M = 0; M < nn; M++) {
+                    const long a = slot_of(conn[e * nn + N], conn[e * nn + M]);
+                    codes[pos[a]++] = (int)(sl * nn2 + N * nn + M);
+                }
+        }
+    }
+    // chunks and groups
+    const long nch = (nnode + 63) / 64;
+    RowsHost h;
+    std::vector<long>& goff = h.goff;
+    goff.assign(nch + 1, 0);
+    for (long ch = 0; ch < nch; ch++) {
+        long kmax = 0;
+        for (long r = ch * 64; r < std::min(nnode, ch * 64 + 64); r++) kmax = std::max(kmax, aptr[r + 1] - aptr[r]);
+        goff[ch + 1] = goff[ch] + (kmax + 3) / 4;
+    }
+    const long ng = goff[nch];
+    std::vector<int>& cols = h.cols;
+    std::vector<long>& coff = h.coff;
+    cols.assign((size_t)ng * 4 * 64, 0);
+    coff.assign((size_t)ng * 4 + 1, 0);
+    for (long ch = 0; ch < nch; ch++)
+        for (long k = goff[ch] * 4; k < goff[ch + 1] * 4; k++) {
+            const long kl = k - goff[ch] * 4;
+            long cmax = 0;
+            for (int lane = 0; lane < 64; lane++) {
+                const long r = ch * 64 + lane;
+                if (r >= nnode) continue;
+                const long deg = aptr[r + 1] - aptr[r];
+                cols[(size_t)k * 64 + lane] = kl < deg ? adj[aptr[r] + kl] : (int)r;
+                if (kl < deg) cmax = std::max(cmax, eptr[aptr[r] + kl + 1] - eptr[aptr[r] + kl]);
+            }
+            coff[k + 1] = coff[k] + cmax;
+        }
+    std::vector<int>& contrib = h.contrib;
+    contrib.assign((size_t)coff[ng * 4] * 64, -1);
+    for (long ch = 0; ch < nch; ch++)
+        for (long k = goff[ch] * 4; k < goff[ch + 1] * 4; k++) {
+            const long kl = k - goff[ch] * 4;
+            for (int lane = 0; lane < 64; lane++) {
+                const long r = ch * 64 + lane;
+                if (r >= nnode || kl >= aptr[r + 1] - aptr[r]) continue;
+                const long a = aptr[r] + kl;
+                for (long i = eptr[a]; i < eptr[a + 1]; i++)
+                    contrib[(size_t)(coff[k] + (i - eptr[a])) * 64 + lane] = codes[i];
+            }
+        }
+    return h;
+}
+
+void setup_rows(cfdv_ctx* c, const cfdv_mesh* m, const std::vector<int>& ptr, const std::vector<int>& elem) {
+    RowsHost h = rows_structure(c->nnod, c->n_elem, c->n_node, c->n_pad, m->conn, c->perm_inv, ptr, elem);
+    const long nch = (long)h.goff.size() - 1, ng = h.goff[nch], nnode = c->n_node;
+    const auto &goff = h.goff, &coff = h.coff;
+    const auto &cols = h.cols, &contrib = h.contrib;
+    c->n_groups = ng;
+    c->d_goff = c->alloc<long>(nch + 1);
+    c->d_cols = c->alloc<int>(cols.size());
+    c->d_coff = c->alloc<long>(coff.size());
+    c->d_contrib = c->alloc<int>(contrib.size());
+    HIP_CHECK_THROW(hipMemcpy(c->d_goff, goff.data(), sizeof(long) * goff.size(), hipMemcpyHostToDevice));
+    HIP_CHECK_THROW(hipMemcpy(c->d_cols, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice));
+    HIP_CHECK_THROW(hipMemcpy(c->d_coff, coff.data(), sizeof(long) * coff.size(), hipMemcpyHostToDevice));
+    HIP_CHECK_THROW(hipMemcpy(c->d_contrib, contrib.data(), sizeof(int) * contrib.size(), hipMemcpyHostToDevice));
+    const size_t nval = (size_t)ng * 4 * c->neqn * c->neqn * 64;
+    c->d_B64 = c->alloc<double>(nval);
+    c->d_B32 = c->alloc<float>(nval);
+    c->ra = RowsArgs{nnode, nch, c->nnod, c->neqn, c->d_goff, c->d_cols, c->d_coff, c->d_contrib};
 }
 
 void setup(cfdv_ctx* c, const cfdv_mesh* m) {
@@ -482,6 +624,7 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
     const size_t kk = (size_t)c->k * c->k;
     if (c->block_flags & CFDV_BLOCKS_FP64) c->d_R64 = c->alloc<double>((size_t)np * kk);
     if (c->block_flags & CFDV_BLOCKS_FP32) c->d_R32 = c->alloc<float>((size_t)np * kk);
+    if (c->block_flags & CFDV_BLOCKS_ROWS) setup_rows(c, m, ptr, elem);
     c->d_rhs = c->alloc<double>((size_t)np * c->k);
     c->d_s = c->alloc<double>((size_t)np * 4);
     c->d_b = c->alloc<double>(c->ndof);
@@ -743,7 +886,7 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
 // K x = b (K^T x = b with o->transpose) for device b, x (both [ndof])
 int solve_fp64_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, const double* b, double* x) {
     const long n = c->ndof;
-    if (!c->d_R64) return fail(c, CFDV_ERR_ARG, "FP64 solve needs CFDV_BLOCKS_FP64");
+    if (!c->have_blocks(64, o->transpose != 0)) return fail(c, CFDV_ERR_ARG, "FP64 solve needs CFDV_BLOCKS_FP64");
     c->ensure_krylov(o->m);
     vec_fill(x, 0.0, n, c->st);
     const double bnorm = c->norm(b);
@@ -774,9 +917,11 @@ int solve_fp64(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
 // solve K32 d = r/||r|| in FP32, du += ||r|| d (FP64).
 int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, const double* b, double* x) {
     const long n = c->ndof;
-    if (!c->d_R32) return fail(c, CFDV_ERR_ARG, "mixed solve needs CFDV_BLOCKS_FP32");
-    const int outer_prec = c->d_R64 ? 64 : 3;   // FP64 residual with FP64 (or widened FP32) blocks
     const bool tr = o->transpose != 0;
+    if (!c->have_blocks(32, tr))
+        return fail(c, CFDV_ERR_ARG, tr ? "transposed mixed solve needs CFDV_BLOCKS_FP32"
+                                        : "mixed solve needs CFDV_BLOCKS_FP32 or CFDV_BLOCKS_ROWS");
+    const int outer_prec = c->have_blocks(64, tr) ? 64 : 3;   // FP64 residual with FP64 (or widened FP32) blocks
     c->ensure_krylov32(o->inner_m);
     const bool pc = o->precond == CFDV_PRECOND_NODE_BLOCK;
     if (pc) c->ensure_precond();
@@ -852,6 +997,8 @@ int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flag
         return CFDV_ERR_ARG;
     if (mesh->adap && !mesh->hnm) return CFDV_ERR_ARG;
     if (!(block_flags & (CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32))) return CFDV_ERR_ARG;
+    if (block_flags & ~(CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32 | CFDV_BLOCKS_ROWS)) return CFDV_ERR_ARG;
+    if ((block_flags & CFDV_BLOCKS_ROWS) && !(block_flags & CFDV_BLOCKS_FP64)) return CFDV_ERR_ARG;
     cfdv_ctx* ctx = new (std::nothrow) cfdv_ctx();
     if (!ctx) return CFDV_ERR_HIP;
     try {
@@ -964,6 +1111,9 @@ int cfdv_build(cfdv_ctx* ctx, double dt) {
     a.aslots = ctx->d_aslots;
     a.n_aslots = ctx->n_aslots;
     launch_build(a, ctx->d_R64 != nullptr, ctx->d_R32 != nullptr, ctx->st);
+    if (ctx->timing) HIP_CHECK_THROW(hipEventRecord(ctx->ev[5], ctx->st));
+    if (ctx->d_B64) launch_rows_assemble(ctx->ra, ctx->d_R64, ctx->d_B64, ctx->d_B32, ctx->st);
+    if (ctx->timing) HIP_CHECK_THROW(hipEventRecord(ctx->ev[6], ctx->st));
     ctx->check_launch();
     int cnt[2];
     HIP_CHECK_THROW(hipMemcpyAsync(cnt, ctx->d_counters, sizeof(int) * 2, hipMemcpyDeviceToHost, ctx->st));
@@ -1050,7 +1200,15 @@ int cfdv_solve(cfdv_ctx* ctx, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
     case CFDV_SOLVE_REFERENCE:
         if (!ctx->d_R64) return fail(ctx, CFDV_ERR_ARG, "reference solve needs FP64 blocks");
         if (o->transpose) return fail(ctx, CFDV_ERR_ARG, "reference solve has no transpose");
-        return solve_reference(ctx, o, s);
+        {
+            // the reference multiplies element by element
+            struct RowsOff {
+                cfdv_ctx* c;
+                ~RowsOff() { c->rows_off = false; }
+            } guard{ctx};
+            ctx->rows_off = true;
+            return solve_reference(ctx, o, s);
+        }
     case CFDV_SOLVE_FP64:
         return solve_fp64(ctx, o, s);
     case CFDV_SOLVE_MIXED:
@@ -1136,9 +1294,9 @@ int cfdv_color(cfdv_ctx* ctx, int32_t* color, int32_t* ncolors) {
 
 int cfdv_matvec(cfdv_ctx* ctx, const double* x, double* y, int precision, int transpose) {
     if (!ctx || !x || !y || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_matvec before cfdv_build");
-    if (precision == 64 && !ctx->d_R64) return fail(ctx, CFDV_ERR_ARG, "no FP64 blocks");
-    if ((precision == 32 || precision == 3) && !ctx->d_R32) return fail(ctx, CFDV_ERR_ARG, "no FP32 blocks");
     if (precision != 64 && precision != 32 && precision != 3) return fail(ctx, CFDV_ERR_ARG, "bad precision");
+    if (!ctx->have_blocks(precision, transpose != 0))
+        return fail(ctx, CFDV_ERR_ARG, precision == 64 ? "no FP64 blocks" : "no FP32 blocks");
     GUARD_BEGIN
     const long n = ctx->ndof;
     HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_r, x, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
@@ -1162,8 +1320,9 @@ int cfdv_matvec(cfdv_ctx* ctx, const double* x, double* y, int precision, int tr
 
 int cfdv_profile_matvec(cfdv_ctx* ctx, int precision, int reps, double* ms) {
     if (!ctx || !ms || reps < 1 || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "profile before build");
-    if (precision == 64 && !ctx->d_R64) return fail(ctx, CFDV_ERR_ARG, "no FP64 blocks");
-    if (precision == 32 && !ctx->d_R32) return fail(ctx, CFDV_ERR_ARG, "no FP32 blocks");
+    if (precision != 64 && precision != 32) return fail(ctx, CFDV_ERR_ARG, "bad precision");
+    if (!ctx->have_blocks(precision, false))
+        return fail(ctx, CFDV_ERR_ARG, precision == 64 ? "no FP64 blocks" : "no FP32 blocks");
     GUARD_BEGIN
     void* x = precision == 32 ? (void*)ctx->d_w32 : (void*)ctx->d_r;
     void* y = precision == 32 ? (void*)(ctx->d_V32 ? ctx->d_V32 : nullptr) : (void*)ctx->d_y;
@@ -1355,6 +1514,11 @@ int cfdv_step(cfdv_ctx* ctx, double cfl, const cfdv_solve_opts* o, cfdv_step_sta
     st->ms_cfl = t;
     HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[1], ctx->ev[2]));
     st->ms_build = t;
+    if (ctx->d_B64) {
+        HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[5], ctx->ev[6]));
+        st->ms_assemble = t;
+        st->ms_build -= t;
+    }
     HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
     st->ms_solve = t;
     HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[3], ctx->ev[4]));

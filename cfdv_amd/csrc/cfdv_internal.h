@@ -119,7 +119,25 @@ void vec_nodeblock_apply(const TP* P, const TV* v, TV* z, int neqn, long n_node,
 void vec_sumsq_eqn(const double* U, int neqn, long n_node, double* part, double* out, int nblk, hipStream_t st);
 void min_reduce(const double* part, int n, double* out, hipStream_t st);
 
-// ---- coloring (color.hip) ----
+// ---- assembled node rows (rows.hip) ----
+// Sliced block rows: chunks of 64 nodes, groups of 4 slots (neighbour
+// nodes) per chunk; block values 16 bytes per lane at a time.
+struct RowsArgs {
+    long n_node, n_chunks;
+    int nnod, neqn;
+    const long* goff;      // [n_chunks + 1] first group of each chunk
+    const int* cols;       // [n_groups * 4][64] column node of each slot
+    const long* coff;      // [n_groups * 4 + 1] contributor rows per slot
+    const int* contrib;    // [coff[end]][64] element block codes, -1 = none
+};
+// y = K x over the rows; prec 64 (FP64 rows and vectors), 32 (FP32), 3 (FP32
+// rows, FP64 vectors).  y is written, not accumulated.
+void launch_rows_matvec(const RowsArgs& a, const double* B64, const float* B32, const void* x, void* y, int prec,
+                        hipStream_t st);
+// rows (FP64 and/or FP32, either may be null) from the FP64 element blocks
+void launch_rows_assemble(const RowsArgs& a, const double* R64, double* B64, float* B32, hipStream_t st);
+
+// ---- coloring (solve.hip) ----
 void launch_color_round(const int* conn, int nnod, const int* node_ptr, const int* node_elem, long n_elem,
                         const unsigned long long* prio, const int* color_in, int* color_out, int* changed,
                         hipStream_t st);

@@ -1,0 +1,176 @@
+// Assembled node rows of K: the element blocks summed per node pair, stored
+// as sliced block rows for the solver's products.
+//
+// The reference multiplies element by element (MPI_GMRES.h:442-566 sums
+// R_e x_e over the elements).  A hex8 element holds 64 blocks of 5x5; summed
+// per node pair an interior node has 27 blocks, so one product over the
+// assembled rows moves 27 * 25 values per node instead of 64 * 25 per
+// element: 2.4x fewer bytes for the same operator.  The rows are formed
+// after every build from the FP64 element blocks (each element block is read
+// once) and stored in FP64 and FP32.
+//
+// Layout.  Rows come in chunks of 64 nodes (lane = node % 64).  Chunk c has
+// G_c groups of 4 slots (slot = one neighbour node of the row, ascending
+// node number, padded with the row itself and a zero block); goff[c] is the
+// first group of chunk c.  A group's 4 blocks are 4 * NEQ^2 values per lane,
+// row-major per block, stored 16 bytes per lane at a time: value f of group
+// g for a lane sits at g * GV * 64 + (f - f % V) * 64 + lane * V + f % V
+// (V = 16 / sizeof(T), GV = 4 * NEQ^2), so a wavefront loads 1 KB of
+// contiguous memory per 16-byte vector.  cols[(4 g + t) * 64 + lane] is the
+// column node of slot t of the group.
+//
+// Assembly.  For slot k of a chunk, the contributing element blocks of each
+// lane are listed in contrib[(coff[k] + i) * 64 + lane] (i < coff[k + 1] -
+// coff[k], -1 where a lane has fewer): code = s nnod^2 + N nnod + M for the
+// block (N, M) of the element in storage slot s.  The sum runs in list
+// order, so the rows are deterministic.
+#include "cfdv_internal.h"
+
+namespace cfdv {
+
+namespace {
+
+template <typename T> struct V16;
+template <> struct V16<double> { typedef double __attribute__((ext_vector_type(2))) type; };
+template <> struct V16<float> { typedef float __attribute__((ext_vector_type(4))) type; };
+
+template <typename T>
+__device__ __forceinline__ size_t row_offset(long g, int lane, int f, int GV) {
+    constexpr int V = 16 / (int)sizeof(T);
+    const int r = f % V;
+    return (size_t)g * GV * 64 + (size_t)(f - r) * 64 + (size_t)lane * V + r;
+}
+
+// Chunk handled by wave w of block b.  Blocks are dealt to the 8 XCDs
+// round-robin; this mapping gives each XCD a contiguous range of chunks so
+// that the x entries neighbouring chunks share stay in one L2.
+__device__ __forceinline__ long chunk_of_wave(long n_chunks) {
+    const long nb = gridDim.x;
+    const long b = blockIdx.x;
+    const long per = (nb + 7) / 8;
+    const long xcd = b % 8, idx = b / 8;
+    const long bb = xcd * per + idx;   // < 8 * per
+    return bb * 4 + threadIdx.x / 64;
+}
+
+}  // namespace
+
+// y = K x over the assembled rows; TB blocks, TV vectors, TA accumulation
+template <typename TB, typename TV, typename TA, int NEQ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void rows_matvec_kernel(
+    long n_node, long n_chunks, const long* __restrict__ goff, const int* __restrict__ cols,
+    const TB* __restrict__ B, const TV* __restrict__ x, TV* __restrict__ y) {
+    constexpr int BB = NEQ * NEQ, GV = 4 * BB, V = 16 / (int)sizeof(TB), NVEC = GV / V;
+    typedef typename V16<TB>::type vec_t;
+    const long c = chunk_of_wave(n_chunks);
+    if (c >= n_chunks) return;
+    const int lane = threadIdx.x & 63;
+    const long g0 = goff[c], g1 = goff[c + 1];
+    TA acc[NEQ];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++) acc[r] = 0;
+    const vec_t* Bv = reinterpret_cast<const vec_t*>(B) + lane;
+    for (long g = g0; g < g1; g++) {
+        int cj[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) cj[t] = cols[(g * 4 + t) * 64 + lane];
+        vec_t bv[NVEC];
+#pragma unroll
+        for (int i = 0; i < NVEC; i++) bv[i] = __builtin_nontemporal_load(Bv + (size_t)(g * NVEC + i) * 64);
+        TA xv[4][NEQ];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int q = 0; q < NEQ; q++) xv[t][q] = (TA)x[(size_t)cj[t] * NEQ + q];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int r = 0; r < NEQ; r++)
+#pragma unroll
+                for (int q = 0; q < NEQ; q++) {
+                    const int f = t * BB + r * NEQ + q;
+                    acc[r] += (TA)bv[f / V][f % V] * xv[t][q];
+                }
+    }
+    const long row = c * 64 + lane;
+    if (row < n_node) {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) y[(size_t)row * NEQ + r] = (TV)acc[r];
+    }
+}
+
+// rows from the FP64 element blocks (chunked layout of blk_offset)
+template <int NNOD, int NEQ>
+__global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const long* __restrict__ goff,
+                                                            const long* __restrict__ coff,
+                                                            const int* __restrict__ contrib,
+                                                            const double* __restrict__ R, double* __restrict__ B64,
+                                                            float* __restrict__ B32) {
+    constexpr int BB = NEQ * NEQ, GV = 4 * BB, K = NNOD * NEQ, KK = K * K, NN2 = NNOD * NNOD;
+    const long c = chunk_of_wave(n_chunks);
+    if (c >= n_chunks) return;
+    const int lane = threadIdx.x & 63;
+    const long g0 = goff[c], g1 = goff[c + 1];
+    for (long g = g0; g < g1; g++) {
+        for (int t = 0; t < 4; t++) {
+            const long k = g * 4 + t;
+            double acc[BB];
+#pragma unroll
+            for (int i = 0; i < BB; i++) acc[i] = 0.0;
+            for (long p = coff[k]; p < coff[k + 1]; p++) {
+                const int code = contrib[p * 64 + lane];
+                if (code < 0) continue;
+                const long s = code / NN2;
+                const int nm = code % NN2, N = nm / NNOD, M = nm % NNOD;
+#pragma unroll
+                for (int r = 0; r < NEQ; r++)
+#pragma unroll
+                    for (int q = 0; q < NEQ; q++)
+                        acc[r * NEQ + q] += R[blk_offset<double>(s, (N * NEQ + r) * K + M * NEQ + q, KK)];
+            }
+#pragma unroll
+            for (int i = 0; i < BB; i++) {
+                const int f = t * BB + i;
+                if (B64) B64[row_offset<double>(g, lane, f, GV)] = acc[i];
+                if (B32) B32[row_offset<float>(g, lane, f, GV)] = (float)acc[i];
+            }
+        }
+    }
+}
+
+static int row_grid(long n_chunks) {
+    const long nb = (n_chunks + 3) / 4;
+    return (int)(((nb + 7) / 8) * 8);   // whole rounds of 8 XCDs
+}
+
+void launch_rows_matvec(const RowsArgs& a, const double* B64, const float* B32, const void* x, void* y, int prec,
+                        hipStream_t st) {
+    if (a.n_chunks <= 0) return;
+    const dim3 grid(row_grid(a.n_chunks)), blk(256);
+#define L(TB, TV, TA, NQ, BP)                                                                              \
+    hipLaunchKernelGGL((rows_matvec_kernel<TB, TV, TA, NQ>), grid, blk, 0, st, a.n_node, a.n_chunks, a.goff, \
+                       a.cols, BP, (const TV*)x, (TV*)y)
+    if (a.neqn == 5) {
+        if (prec == 64) L(double, double, double, 5, B64);
+        else if (prec == 32) L(float, float, float, 5, B32);
+        else L(float, double, double, 5, B32);
+    } else {
+        if (prec == 64) L(double, double, double, 4, B64);
+        else if (prec == 32) L(float, float, float, 4, B32);
+        else L(float, double, double, 4, B32);
+    }
+#undef L
+}
+
+void launch_rows_assemble(const RowsArgs& a, const double* R64, double* B64, float* B32, hipStream_t st) {
+    if (a.n_chunks <= 0) return;
+    const dim3 grid(row_grid(a.n_chunks)), blk(256);
+    if (a.nnod == 8)
+        hipLaunchKernelGGL((rows_assemble_kernel<8, 5>), grid, blk, 0, st, a.n_chunks, a.goff, a.coff, a.contrib, R64,
+                           B64, B32);
+    else
+        hipLaunchKernelGGL((rows_assemble_kernel<4, 4>), grid, blk, 0, st, a.n_chunks, a.goff, a.coff, a.contrib, R64,
+                           B64, B32);
+}
+
+}  // namespace cfdv

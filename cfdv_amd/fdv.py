@@ -65,7 +65,8 @@ class FDVContext:
         t = _lib.ThermoS(thermo.gamma, thermo.mach, thermo.reynolds, thermo.prandtl,
                          thermo.tinf, thermo.twall, int(thermo.adiabatic))
         flags = (_lib.CFDV_BLOCKS_FP64 if "fp64" in blocks else 0) | \
-                (_lib.CFDV_BLOCKS_FP32 if "fp32" in blocks else 0)
+                (_lib.CFDV_BLOCKS_FP32 if "fp32" in blocks else 0) | \
+                (_lib.CFDV_BLOCKS_ROWS if "rows" in blocks else 0)
         h = C.c_void_p()
         rc = self.L.cfdv_create(C.byref(m), C.byref(t), flags, C.byref(h))
         if rc != 0:

@@ -101,6 +101,14 @@ typedef struct {          /* the aero deck keys (MPIBinaryConfiguration.h:54-63)
 /* block storage requested at create time */
 #define CFDV_BLOCKS_FP64 1
 #define CFDV_BLOCKS_FP32 2
+/* also sum the FP64 element blocks into assembled node rows (FP64 and FP32
+   copies) after every build (no reference counterpart: the reference
+   multiplies element by element, MPI_GMRES.h:442-566).  Needs
+   CFDV_BLOCKS_FP64.  Products K x of the FP64 and mixed solves and of
+   cfdv_matvec then use the rows (27 blocks per hex8 node instead of 64 per
+   element); K^T x and the reference solve stay element by element, so a
+   transposed mixed solve still needs CFDV_BLOCKS_FP32. */
+#define CFDV_BLOCKS_ROWS 4
 
 /* solver modes */
 #define CFDV_SOLVE_REFERENCE 0   /* MPI_GMRES::iterate as written: MGS Arnoldi,
@@ -139,8 +147,8 @@ typedef struct {
     double resid;        /* final ||b - K du|| recomputed in FP64                  */
     double rel_resid;    /* resid / bnorm                                          */
     double est_resid;    /* last Givens residual estimate                          */
-    int matvecs64;       /* EBE matvecs with FP64 blocks                           */
-    int matvecs32;       /* EBE matvecs with FP32 blocks                           */
+    int matvecs64;       /* products with FP64 blocks (rows or EBE)                */
+    int matvecs32;       /* products with FP32 blocks (rows or EBE)                */
     int iters;           /* Arnoldi iterations (all restarts / sweeps)             */
     int restarts;        /* restarts (FP64, reference) or refinement sweeps        */
     int breakdown;
@@ -150,11 +158,14 @@ typedef struct {
     double dt;
     double urms[5];
     double ms_cfl, ms_build, ms_solve, ms_update;   /* HIP-event timings */
-    double ms_matvec64, ms_matvec32;   /* summed EBE matvec time (events around
-                                          each full matvec: zero-fill + one
-                                          launch per color)                  */
+    double ms_matvec64, ms_matvec32;   /* summed matvec time (events around
+                                          each full product: the row kernel,
+                                          or zero-fill + one EBE launch per
+                                          color)                             */
     int n_matvec64, n_matvec32;
     cfdv_solve_stats solve;
+    double ms_assemble;   /* node rows from the element blocks (CFDV_BLOCKS_ROWS),
+                             not part of ms_build                           */
 } cfdv_step_stats;
 
 int cfdv_create(const cfdv_mesh *mesh, const cfdv_thermo *thermo, int block_flags,
@@ -184,10 +195,11 @@ int cfdv_refine_flags(cfdv_ctx *ctx, uint8_t *flags, double *s1, int32_t *nflagg
                       int32_t *nties);
 int cfdv_color(cfdv_ctx *ctx, int32_t *color, int32_t *ncolors);
 /* y = K x (or K^T x) with the built blocks; precision 64, 32 (FP32 blocks and
-   vectors) or 3 (FP32 blocks, FP64 vectors); host arrays [n_node*neqn] */
+   vectors) or 3 (FP32 blocks, FP64 vectors); host arrays [n_node*neqn].
+   K x uses the assembled rows when the context holds them */
 int cfdv_matvec(cfdv_ctx *ctx, const double *x, double *y, int precision, int transpose);
-/* average time of one full EBE matvec (all colors), measured with HIP events
-   on the context stream */
+/* average time of one full K x (the rows, or EBE over all colors),
+   measured with HIP events on the context stream; precision 64 or 32 */
 int cfdv_profile_matvec(cfdv_ctx *ctx, int precision, int reps, double *ms);
 int cfdv_step(cfdv_ctx *ctx, double cfl, const cfdv_solve_opts *opts, cfdv_step_stats *st);
 /* Y = K X (or K^T X) for 8 vectors in one pass over the FP32 blocks (no

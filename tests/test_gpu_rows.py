@@ -1,0 +1,174 @@
+"""Assembled node rows (CFDV_BLOCKS_ROWS) against the oracle's EBE product.
+
+The rows are the element blocks summed per node pair.  The reference has no
+such storage: it multiplies element by element (MPI_GMRES.h:442-566), so
+K x from the rows equals the oracle's EBE product up to summation order, to
+1e-13 relative in FP64.  The FP32 rows are the FP64 sums rounded once; K32 x
+agrees to 2e-6 relative (FP32 inputs and accumulation).  Meshes: hex8 box
+with mixed bc, flat plate, quad4 lid, a box with shuffled node numbers
+(ragged chunks, neighbour slots that differ lane by lane) and a refined box
+with hanging-node elements.  Solves on the rows (FP64, node-block
+preconditioned FP64 and mixed) reach the dense solution to 1e-8, the
+reference solve still multiplies element by element (bit-identical to a
+context without rows), and K^T x stays on the element blocks.
+"""
+import numpy as np
+import pytest
+
+from cfdv_amd import _lib
+from cfdv_amd import mesh as M
+from cfdv_amd import refine as PR
+from cfdv_amd.fdv import CFDVError, FDVContext
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _shuffled_box(seed=9):
+    m, th, U = M.mixed_bc_box(5, 4, 3, seed=2)
+    rng = np.random.default_rng(seed)
+    p = rng.permutation(m.n_node)            # old node -> new node
+    inv = np.argsort(p)
+    conn = p[m.conn]
+    x = m.x[inv]
+    d = m.dirichlet[inv]
+    return M.Mesh(3, 8, 5, conn, x, m.bc, d), th, U[inv]
+
+
+def _refined_box():
+    m, th, U = M.mixed_bc_box(4, 3, 3, seed=5)
+    r = PR.RefinableMesh(m, U)
+    for every, off in ((7, 3), (5, 1)):
+        ne = r.sizes()[0]
+        el, rl = r.refine_list((np.arange(ne) % every == off).astype(np.uint8))
+        r.apply(el, rl, PR.MODE_3D)
+    m2, U2, _, _ = r.get()
+    return m2, th, U2
+
+
+def _case(name):
+    if name == "box3d":
+        return M.mixed_bc_box()
+    if name == "plate":
+        return M.flat_plate(12, 6, 5)
+    if name == "lid":
+        return M.lid_cavity(8)
+    if name == "shuffled":
+        return _shuffled_box()
+    if name == "refined":
+        return _refined_box()
+    raise KeyError(name)
+
+
+def _dense(m, R):
+    n = m.n_node * m.neqn
+    A = np.zeros((n, n))
+    for e in range(m.n_elem):
+        dofs = (m.conn[e][:, None] * m.neqn + np.arange(m.neqn)).ravel()
+        A[np.ix_(dofs, dofs)] += R[e]
+    return A
+
+
+@pytest.mark.parametrize("case", ["box3d", "plate", "lid", "shuffled", "refined"])
+def test_rows_product_parity(gpu_available, case):
+    m, th, U = _case(case)
+    ctx = FDVContext(m, th, U, blocks=("fp64", "rows"))
+    ebe = FDVContext(m, th, U, blocks=("fp64", "fp32"))
+    dt = ctx.cfl(0.5)
+    ctx.build(dt)
+    ebe.build(dt)
+    R, _, _ = ctx.blocks()
+    assert np.array_equal(R, ebe.blocks()[0])
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(m.n_node * m.neqn)
+    yo = O.matvec(m, R, x)
+    nrm = np.linalg.norm(yo)
+    y = ctx.matvec(x)
+    err64 = np.linalg.norm(y - yo) / nrm
+    assert np.linalg.norm(y - ebe.matvec(x)) <= 1e-13 * nrm
+    y32 = ctx.matvec(x, precision=32)
+    y3 = ctx.matvec(x, precision=3)
+    err32 = np.linalg.norm(y32 - yo) / nrm
+    err3 = np.linalg.norm(y3 - yo) / nrm
+    print(case, "nodes", m.n_node, "rows FP64 rel err %.2e FP32 %.2e FP32 rows/FP64 vectors %.2e" % (err64, err32, err3))
+    assert err64 <= 1e-13
+    assert err32 <= 2e-6 and err3 <= 2e-6
+    # FP32 rows are rounded FP64 sums: the product with FP64 vectors is that
+    # of the rounded dense K
+    A = _dense(m, R)
+    y3d = A.astype(np.float32).astype(np.float64) @ x
+    assert np.linalg.norm(y3 - y3d) <= 1e-13 * nrm
+    # K^T x stays on the element blocks
+    yt = ctx.matvec(x, transpose=True)
+    assert np.linalg.norm(yt - O.matvec(m, R, x, transpose=True)) <= 1e-13 * nrm
+
+
+@pytest.mark.parametrize("case", ["box3d", "lid", "shuffled", "refined"])
+def test_rows_solves(gpu_available, case):
+    m, th, U = _case(case)
+    ctx = FDVContext(m, th, U, blocks=("fp64", "rows"))
+    dt = ctx.cfl(0.5)
+    ctx.build(dt)
+    R, rhs, _ = ctx.blocks()
+    b = O.compose_rhs(m, rhs)
+    A = _dense(m, R)
+    xs = np.linalg.solve(A, b)
+    nx, nb = np.linalg.norm(xs), np.linalg.norm(b)
+    # a relative residual of 1e-10 bounds the solution error by cond(A) 1e-10
+    tol_x = 2e-10 * np.linalg.cond(A)
+
+    def check(st, AA, xref):
+        du = ctx.du()
+        assert st.rel_resid <= 1e-10
+        assert np.linalg.norm(AA @ du - b) <= 1.5e-10 * nb      # the true residual, on the CPU
+        assert np.linalg.norm(du - xref) <= tol_x * np.linalg.norm(xref)
+
+    st, _ = ctx.solve("fp64", restarts=60, m=30, rtol=1e-10)
+    check(st, A, xs)
+    st, _ = ctx.solve("fp64", restarts=60, m=30, rtol=1e-10, precond="nodeblock")
+    check(st, A, xs)
+    st, _ = ctx.solve("mixed", restarts=20, m=30, rtol=1e-10, inner_m=20, inner_rtol=1e-4,
+                      inner_restarts=3, precond="nodeblock")
+    print(case, "cond %.1e" % np.linalg.cond(A), "mixed on rows: sweeps", st.restarts,
+          "FP32 products", st.matvecs32, "FP64", st.matvecs64)
+    assert st.matvecs32 > 0
+    check(st, A, xs)
+    # the transposed FP64 solve runs on the element blocks
+    xt = np.linalg.solve(A.T, b)
+    st, _ = ctx.solve("fp64", restarts=60, m=30, rtol=1e-10, transpose=True, precond="nodeblock")
+    check(st, A.T, xt)
+    assert nx > 0
+    # a transposed mixed solve needs the FP32 element blocks
+    st, rc = ctx.solve("mixed", restarts=5, m=30, rtol=1e-10, transpose=True, check=False)
+    assert rc == _lib.CFDV_ERR_ARG
+
+
+def test_reference_solve_stays_element_by_element(gpu_available):
+    m, th, U = _case("box3d")
+    a = FDVContext(m, th, U, blocks=("fp64", "rows"))
+    b = FDVContext(m, th, U)
+    sa = a.step(0.5, mode="reference", restarts=1, m=20)
+    sb = b.step(0.5, mode="reference", restarts=1, m=20)
+    assert sa.solve.iters == sb.solve.iters
+    Ua, _ = a.get_state()
+    Ub, _ = b.get_state()
+    assert np.array_equal(Ua, Ub)
+
+
+def test_mixed_step_on_rows(gpu_available):
+    m, th, U = M.compression_ramp(12, 5, 6)
+    ctx = FDVContext(m, th, U, blocks=("fp64", "rows"))
+    ref = FDVContext(m, th, U, blocks=("fp64", "fp32"))
+    kw = dict(restarts=10, m=30, rtol=1e-10, inner_m=30, inner_rtol=1e-4, inner_restarts=3, precond="nodeblock")
+    st = ctx.step(0.5, mode="mixed", **kw)
+    sr = ref.step(0.5, mode="mixed", **kw)
+    print("rows step: build %.3f ms assemble %.3f ms solve %.3f ms, FP32 products %d" %
+          (st.ms_build, st.ms_assemble, st.ms_solve, st.n_matvec32))
+    assert st.ms_assemble > 0 and sr.ms_assemble == 0
+    assert st.n_matvec32 > 0 and st.solve.rel_resid <= 1e-10
+    U1, U0 = ctx.get_state()
+    R1, _ = ref.get_state()
+    du, dr = U1 - U0, R1 - U0
+    assert np.linalg.norm(du - dr) <= 1e-8 * np.linalg.norm(dr)
+    with pytest.raises(CFDVError):
+        FDVContext(m, th, U, blocks=("rows",))
