@@ -763,8 +763,7 @@ int solve_reference(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) 
 // Arnoldi iterations; *conv set when the Givens estimate met tol_abs.
 template <typename T>
 int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, int maxrestart, double tol_abs,
-               double rnorm0, cfdv_solve_stats* s, bool* conv, T* V, T* w, const T* Pinv = nullptr,
-               T* z = nullptr) {
+               double rnorm0, cfdv_solve_stats* s, bool* conv, T* V, T* w, const T* Pinv, T* z, bool cgs2) {
     const long n = c->ndof;
     const long ldv = c->ldv();
     std::vector<double> h((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m);
@@ -814,24 +813,33 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
             }
             if (prec == 64) s->matvecs64++; else s->matvecs32++;
             iters++;
-            // CGS2: two classical Gram-Schmidt passes, h1 = V^T w,
-            // w -= V h1, h2 = V^T w, w -= V h2, |w|^2; up to 32 basis vectors
-            // the projections fuse into three passes over the basis
-            if (j + 1 <= 32) {
-                vec_cgs<T>(0, w, V, ldv, j + 1, nullptr, n, c->d_part, NBLK_RED, c->d_hcol, c->st);
-                vec_cgs<T>(1, w, V, ldv, j + 1, c->d_hcol, n, c->d_part, NBLK_RED, c->d_hcol + 1024, c->st);
-                vec_cgs<T>(2, w, V, ldv, j + 1, c->d_hcol + 1024, n, c->d_part, NBLK_RED, c->d_hcol + 2040, c->st);
+            // Classical Gram-Schmidt.  CGS2 (two passes: h1 = V^T w,
+            // w -= V h1, h2 = V^T w, w -= V h2, |w|^2) keeps the basis
+            // orthogonal to working precision; CGS1 (h1 = V^T w,
+            // w -= V h1, |w|^2) saves a pass over the basis and serves the
+            // FP32 inner solve, whose result the FP64 refinement checks and
+            // corrects.  Up to 32 basis vectors each pass is one fused kernel.
+            const int nv = j + 1;
+            std::vector<double> col(nv), col2(nv, 0.0);
+            double nrm2;
+            if (nv <= 32) {
+                vec_cgs<T>(0, w, V, ldv, nv, nullptr, n, c->d_part, NBLK_RED, c->d_hcol, c->st);
+                if (cgs2)
+                    vec_cgs<T>(1, w, V, ldv, nv, c->d_hcol, n, c->d_part, NBLK_RED, c->d_hcol + 1024, c->st);
+                vec_cgs<T>(2, w, V, ldv, nv, cgs2 ? c->d_hcol + 1024 : c->d_hcol, n, c->d_part, NBLK_RED,
+                           c->d_hcol + 2040, c->st);
             } else {
-                vec_multidot<T>(V, ldv, j + 1, w, n, c->d_part, c->d_hcol, NBLK_RED, c->st);
-                vec_multi_axpy<T>(w, V, ldv, j + 1, c->d_hcol, 1.0, n, c->st);
-                vec_multidot<T>(V, ldv, j + 1, w, n, c->d_part, c->d_hcol + 1024, NBLK_RED, c->st);
-                vec_multi_axpy<T>(w, V, ldv, j + 1, c->d_hcol + 1024, 1.0, n, c->st);
+                vec_multidot<T>(V, ldv, nv, w, n, c->d_part, c->d_hcol, NBLK_RED, c->st);
+                vec_multi_axpy<T>(w, V, ldv, nv, c->d_hcol, 1.0, n, c->st);
+                if (cgs2) {
+                    vec_multidot<T>(V, ldv, nv, w, n, c->d_part, c->d_hcol + 1024, NBLK_RED, c->st);
+                    vec_multi_axpy<T>(w, V, ldv, nv, c->d_hcol + 1024, 1.0, n, c->st);
+                }
                 c->multidot<T>(w, 1, w, c->d_hcol + 2040);
             }
-            std::vector<double> col(j + 1), col2(j + 1);
-            double nrm2;
-            HIP_CHECK_THROW(hipMemcpyAsync(col.data(), c->d_hcol, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->st));
-            HIP_CHECK_THROW(hipMemcpyAsync(col2.data(), c->d_hcol + 1024, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->st));
+            HIP_CHECK_THROW(hipMemcpyAsync(col.data(), c->d_hcol, sizeof(double) * nv, hipMemcpyDeviceToHost, c->st));
+            if (cgs2)
+                HIP_CHECK_THROW(hipMemcpyAsync(col2.data(), c->d_hcol + 1024, sizeof(double) * nv, hipMemcpyDeviceToHost, c->st));
             HIP_CHECK_THROW(hipMemcpyAsync(&nrm2, c->d_hcol + 2040, sizeof(double), hipMemcpyDeviceToHost, c->st));
             c->sync();
             for (int i = 0; i <= j; i++) H(i, j) = col[i] + col2[i];
@@ -896,7 +904,8 @@ int solve_fp64_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, c
     if (pc) c->ensure_precond();
     if (bnorm > 0)
         s->iters += gmres_core<double>(c, 64, o->transpose != 0, b, x, o->m, o->restarts, o->rtol * bnorm, bnorm, s,
-                                       &conv, c->d_V, c->d_w, pc ? c->d_P64 : nullptr, pc ? c->d_z64 : nullptr);
+                                       &conv, c->d_V, c->d_w, pc ? c->d_P64 : nullptr, pc ? c->d_z64 : nullptr,
+                                       true);
     c->matvec(x, c->d_y, 64, o->transpose != 0);
     vec_sub(c->d_r, b, c->d_y, n, c->st);
     s->resid = c->norm(c->d_r);
@@ -944,7 +953,7 @@ int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, 
         float* wbuf = c->d_V32 + (size_t)o->inner_m * c->ldv();   // last slot as the Arnoldi work vector
         s->iters += gmres_core<float>(c, 32, tr, c->d_w32, c->d_d64, o->inner_m - 1, o->inner_restarts,
                                       o->inner_rtol, 1.0, &si, &conv, c->d_V32, wbuf, pc ? c->d_P32 : nullptr,
-                                      pc ? c->d_z32 : nullptr);
+                                      pc ? c->d_z32 : nullptr, false);
         s->matvecs32 += si.matvecs32;
         s->matvecs64 += si.matvecs64;
         // du += ||r|| d  (FP64)

@@ -99,7 +99,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
 }
 
-// rows from the FP64 element blocks (chunked layout of blk_offset)
+// rows from the FP64 element blocks (chunked layout of blk_offset: the
+// entries of an element row come in 16-byte pairs).  A block row of NEQ
+// entries starting at column j0 = M NEQ is read as whole pairs: NEQ / 2 of
+// them for even NEQ, (NEQ + 1) / 2 from the pair holding j0 for odd NEQ
+// (one entry of a neighbouring block read and dropped), so each lane's
+// loads are full 16-byte vectors.
 template <int NNOD, int NEQ>
 __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const long* __restrict__ goff,
                                                             const long* __restrict__ coff,
@@ -107,6 +112,9 @@ __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const
                                                             const double* __restrict__ R, double* __restrict__ B64,
                                                             float* __restrict__ B32) {
     constexpr int BB = NEQ * NEQ, GV = 4 * BB, K = NNOD * NEQ, KK = K * K, NN2 = NNOD * NNOD;
+    constexpr int NP = (NEQ + 1) / 2;   // pairs read per block row: 3 for NEQ 5, 2 for NEQ 4
+    typedef typename V16<double>::type d2_t;
+    const d2_t* Rv = reinterpret_cast<const d2_t*>(R);
     const long c = chunk_of_wave(n_chunks);
     if (c >= n_chunks) return;
     const int lane = threadIdx.x & 63;
@@ -122,11 +130,27 @@ __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const
                 if (code < 0) continue;
                 const long s = code / NN2;
                 const int nm = code % NN2, N = nm / NNOD, M = nm % NNOD;
+                const int j0 = M * NEQ, odd = j0 & 1;
+                // pair p of element s sits at (s / 64) 64 KK / 2 + p 64 + s % 64
+                const d2_t* Rs = Rv + (size_t)(s >> 6) * 64 * (KK / 2) + (s & 63);
 #pragma unroll
-                for (int r = 0; r < NEQ; r++)
+                for (int r = 0; r < NEQ; r++) {
+                    const int p0 = ((N * NEQ + r) * K + j0 - odd) >> 1;
+                    double v[2 * NP];
 #pragma unroll
-                    for (int q = 0; q < NEQ; q++)
-                        acc[r * NEQ + q] += R[blk_offset<double>(s, (N * NEQ + r) * K + M * NEQ + q, KK)];
+                    for (int i = 0; i < NP; i++) {
+                        const d2_t w = Rs[(size_t)(p0 + i) * 64];
+                        v[2 * i] = w[0];
+                        v[2 * i + 1] = w[1];
+                    }
+                    if constexpr (NEQ % 2 == 1) {
+#pragma unroll
+                        for (int q = 0; q < NEQ; q++) acc[r * NEQ + q] += odd ? v[q + 1] : v[q];
+                    } else {   // j0 even: the pairs hold exactly the row
+#pragma unroll
+                        for (int q = 0; q < NEQ; q++) acc[r * NEQ + q] += v[q];
+                    }
+                }
             }
 #pragma unroll
             for (int i = 0; i < BB; i++) {

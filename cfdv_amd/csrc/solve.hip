@@ -345,20 +345,20 @@ __global__ __launch_bounds__(256) void multidot_kernel(const T* __restrict__ V, 
 }
 
 // out[j] = sum over blocks of part[blk*nv + j], fixed order
+// out[j] = sum_b part[b nv + j]: one block per j, each thread a strided
+// partial sum, then a fixed tree
 __global__ void finish_dot_kernel(const double* __restrict__ part, int nblk, int nv, double* __restrict__ out) {
     __shared__ double sm[256];
-    for (int j = 0; j < nv; j++) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += 256) acc += part[(size_t)b * nv + j];
-        sm[threadIdx.x] = acc;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[j] = sm[0];
+    const int j = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) acc += part[(size_t)b * nv + j];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
         __syncthreads();
     }
+    if (threadIdx.x == 0) out[j] = sm[0];
 }
 
 template <typename T>
@@ -372,7 +372,7 @@ void vec_multidot(const T* V, long ldv, int nv, const T* w, long n, double* part
             hipLaunchKernelGGL((multidot_kernel<T, 4>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part);
         else
             hipLaunchKernelGGL((multidot_kernel<T, 16>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part);
-        hipLaunchKernelGGL(finish_dot_kernel, dim3(1), dim3(256), 0, st, part, nblk, c, out + j0);
+        hipLaunchKernelGGL(finish_dot_kernel, dim3(c), dim3(256), 0, st, part, nblk, c, out + j0);
     }
 }
 template void vec_multidot<double>(const double*, long, int, const double*, long, double*, double*, int, hipStream_t);
@@ -383,9 +383,9 @@ template void vec_multidot<float>(const float*, long, int, const float*, long, d
 // ldv a multiple of 64 elements, w 256-byte aligned; the n % VL tail is done
 // by scalar loads).  Each block writes its partial sums (fixed grid, fixed
 // order) for finish_dot_kernel, so results do not depend on timing.
-//   cgs_dot:       part[b][j] = sum_i V_j[i] w[i]
-//   cgs_axpy_dot:  w -= sum_j h[j] V_j,  part[b][j] = sum_i V_j[i] w_new[i]
-//   cgs_axpy_norm: w -= sum_j h[j] V_j,  part[b][0] = sum_i w_new[i]^2
+//   mode 0: part[b][j] = sum_i V_j[i] w[i]
+//   mode 1: w -= sum_j h[j] V_j,  part[b][j] = sum_i V_j[i] w_new[i]
+//   mode 2: w -= sum_j h[j] V_j,  part[b][0] = sum_i w_new[i]^2
 // ---------------------------------------------------------------------------
 // entries per load: 16 bytes, except in the FP32 fused projection (mode 1)
 // over 5..16 vectors, which holds every basis entry of the thread in
@@ -396,8 +396,9 @@ template <typename T, int NV, int MODE> constexpr int cgs_lanes() {
     return sizeof(T) == 8 ? 2 : ((MODE == 1 && NV > 4 && NV <= 16) ? 2 : 4);
 }
 
+// part[b stride + j] = block b's sum of acc[j], j < nv
 template <int NV>
-__device__ __forceinline__ void block_partials(double (&acc)[NV], int nv, double* __restrict__ part) {
+__device__ __forceinline__ void block_partials(double (&acc)[NV], int nv, int stride, double* __restrict__ part) {
     __shared__ double sm[NV][4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -410,7 +411,7 @@ __device__ __forceinline__ void block_partials(double (&acc)[NV], int nv, double
     __syncthreads();
     if ((int)threadIdx.x < nv && threadIdx.x < NV) {
         const int j = threadIdx.x;
-        part[(size_t)blockIdx.x * nv + j] = ((sm[j][0] + sm[j][1]) + sm[j][2]) + sm[j][3];
+        part[(size_t)blockIdx.x * stride + j] = ((sm[j][0] + sm[j][1]) + sm[j][2]) + sm[j][3];
     }
 }
 
@@ -505,7 +506,7 @@ __global__ __launch_bounds__(256) void cgs_kernel(T* __restrict__ w, const T* __
             }
         }
     }
-    block_partials<NV>(acc, MODE == 2 ? 1 : nv, part);
+    block_partials<NV>(acc, MODE == 2 ? 1 : nv, MODE == 2 ? 1 : nv, part);
 }
 
 template <typename T>
@@ -523,7 +524,8 @@ static void launch_cgs(int mode, T* w, const T* V, long ldv, int nv, const doubl
     else { CGS_NV(2) }
 #undef CGS_NV
 #undef CGS
-    hipLaunchKernelGGL(finish_dot_kernel, dim3(1), dim3(256), 0, st, part, nblk, mode == 2 ? 1 : nv, out);
+    const int cnt = mode == 2 ? 1 : nv;
+    hipLaunchKernelGGL(finish_dot_kernel, dim3(cnt), dim3(256), 0, st, part, nblk, cnt, out);
 }
 
 template <typename T>
@@ -740,7 +742,7 @@ __global__ void sumsq_eqn_kernel(const double* __restrict__ U, int neqn, long n_
 }
 void vec_sumsq_eqn(const double* U, int neqn, long n_node, double* part, double* out, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(sumsq_eqn_kernel, dim3(nblk), dim3(256), 0, st, U, neqn, n_node, part);
-    hipLaunchKernelGGL(finish_dot_kernel, dim3(1), dim3(256), 0, st, part, nblk, neqn, out);
+    hipLaunchKernelGGL(finish_dot_kernel, dim3(neqn), dim3(256), 0, st, part, nblk, neqn, out);
 }
 
 __global__ void min_reduce_kernel(const double* part, int n, double* out) {

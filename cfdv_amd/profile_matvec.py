@@ -1,11 +1,13 @@
-"""Stand-alone run of the element build and the EBE matvec, for profilers.
+"""Stand-alone run of the element build and the products K x, for profilers.
 
-    python -m cfdv_amd.profile_matvec --size 256x128x64 --reps 20
+    python -m cfdv_amd.profile_matvec --size 256x128x64 --reps 20 [--storage rows]
 
-Builds the compression-ramp mesh of the given size, runs one element build,
-then ``--reps`` full EBE matvecs in FP32 and in FP64 blocks and ``--reps``
-8-vector FP32 products, and prints one JSON line with the event-timed
-milliseconds and algorithmic bytes of each.
+Builds the compression-ramp mesh of the given size, runs one element build
+(with ``--storage rows`` followed by the assembly of the node rows), then
+``--reps`` full products in FP32 and in FP64 (element by element, or over
+the assembled rows) and ``--reps`` 8-vector FP32 EBE products, and prints
+one JSON line with the event-timed milliseconds and algorithmic bytes of
+each.
 With ``--copy-mib N`` it first runs one plain device copy of N MiB (an
 elementwise ``mul(src, 1)`` through torch) whose HBM bytes are known, so that
 hardware byte counters can be scaled against it.
@@ -28,6 +30,7 @@ def main(argv=None):
     p.add_argument("--cfl", type=float, default=0.5)
     p.add_argument("--copy-mib", type=int, default=0)
     p.add_argument("--precision", choices=["32", "64", "both"], default="both")
+    p.add_argument("--storage", choices=["ebe", "rows"], default="ebe")
     a = p.parse_args(argv)
     nx, ny, nz = (int(v) for v in a.size.split("x"))
     out = {"size": a.size}
@@ -42,7 +45,8 @@ def main(argv=None):
         out["copy_bytes_written"] = n * 4
         del src, dst
     mesh, th, U = M.compression_ramp(nx, ny, nz)
-    ctx = FDVContext(mesh, th, U, blocks=("fp64", "fp32"))
+    rows = a.storage == "rows"
+    ctx = FDVContext(mesh, th, U, blocks=("fp64", "fp32", "rows") if rows else ("fp64", "fp32"))
     dt = ctx.cfl(a.cfl)
     t0 = time.perf_counter()
     ctx.build(dt)
@@ -53,7 +57,13 @@ def main(argv=None):
     for prec in ((32, 64) if a.precision == "both" else (int(a.precision),)):
         ms = ctx.profile_matvec(prec, a.reps)
         w = prec // 8
-        nbytes = ne * k * k * w + nn * mesh.neqn * 2 * w
+        if rows:
+            # node blocks of the structured node grid: (3nx+1)(3ny+1)(3nz+1),
+            # 25 values and a column index each; x read and y written once
+            nnzb = (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1)
+            nbytes = nnzb * (mesh.neqn ** 2 * w + 4) + nn * mesh.neqn * 2 * w
+        else:
+            nbytes = ne * k * k * w + nn * mesh.neqn * 2 * w
         out[f"matvec{prec}_ms"] = ms
         out[f"matvec{prec}_bytes"] = nbytes
         out[f"matvec{prec}_GBps"] = nbytes / (ms * 1e-3) / 1e9
@@ -66,6 +76,7 @@ def main(argv=None):
     out["elements"] = ne
     out["nodes"] = nn
     out["colors"] = ctx.info()["ncolors"]
+    out["storage"] = a.storage
     print(json.dumps(out), flush=True)
     return 0
 
