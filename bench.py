@@ -40,8 +40,10 @@ HBM_PEAK_GBS = 8000.0
 # on MI355X, each scaled by the counter's reading for a plain 1 GiB copy
 # kernel (fetch x2.0, write x1.0); summaries and commands in
 # profiles/ebe_matvec_ramp_256x128x64_hbm_counters.txt (EBE, 8 color
-# launches).  Only the default workload size has a measurement.
-MEASURED_TRAFFIC = {("ebe", "256x128x64"): {"fp32": 14.607e9, "fp64": 29.121e9}}
+# launches) and profiles/rows_matvec_ramp_256x128x64_hbm_counters.txt (node
+# rows, one launch).  Only the default workload size has a measurement.
+MEASURED_TRAFFIC = {("ebe", "256x128x64"): {"fp32": 14.607e9, "fp64": 29.121e9},
+                    ("rows", "256x128x64"): {"fp32": 6.342e9, "fp64": 12.502e9}}
 
 
 def log(*a):

@@ -177,8 +177,19 @@ struct GPShared {
     GPState<3> f;
 };
 
-// dPhi_n/dx_j go to dphi_out[n][j] when dphi_out is given.
-CFDV_HD void gp_phase1(const DevThermo& th, const GaussTables& G, const double (*nd)[ND_STRIDE], int gp,
+// The hex8 volume tables the interior kernel reads per point and node pair,
+// staged in LDS once per element (lane-dependent indices would otherwise be
+// vector loads from the constant buffer in every loop iteration)
+struct GaussHex {
+    double phi3[8][8];
+    double dpxi3[8][8][3];
+    double w3[8];
+};
+
+// dPhi_n/dx_j go to dphi_out[n][j] when dphi_out is given.  GT: GaussTables
+// or GaussHex.
+template <typename GT>
+CFDV_HD void gp_phase1(const DevThermo& th, const GT& G, const double (*nd)[ND_STRIDE], int gp,
                        double dt, GPShared& o, double (*dphi_out)[3]) {
     constexpr int NEQ = 5;
     const double reinv = 1.0 / th.creyn;
@@ -425,7 +436,8 @@ CFDV_HD void accum_c3_tab(const DevThermo& th, const GPTab& t, const double W[3]
 
 // Phase 2 for lane (N, M): the domain contraction of every Gauss point,
 // in contract3's order (diagonal, direction terms, A S product, c_ij).
-CFDV_HD void interior_phase2(const DevThermo& th, const GaussTables& G, const GPTab* tabs, int lane,
+template <typename GT>
+CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs, int lane,
                              double dt, const double sp[4], double R[5][5], double rhs[5]) {
     constexpr int NEQ = 5;
     const int N = lane >> 3, M = lane & 7;
@@ -507,6 +519,7 @@ __global__ __launch_bounds__(64) void build3d_interior_kernel(BuildArgs A) {
     constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600;
     __shared__ double s_nd[NNOD][ND_STRIDE];
     __shared__ GPTab s_tab[8];
+    __shared__ GaussHex s_g;
     const int lane = threadIdx.x;
     const long e = (long)xcd_remap(blockIdx.x, gridDim.x);
     const DevThermo th = A.th;
@@ -534,6 +547,10 @@ __global__ __launch_bounds__(64) void build3d_interior_kernel(BuildArgs A) {
         return;
     }
     if (lane < NNOD) load_node3d(A, cn[lane], s_nd[lane]);
+    s_g.phi3[lane >> 3][lane & 7] = c_gauss.phi3[lane >> 3][lane & 7];
+#pragma unroll
+    for (int j = 0; j < 3; j++) s_g.dpxi3[lane >> 3][lane & 7][j] = c_gauss.dpxi3[lane >> 3][lane & 7][j];
+    if (lane < 8) s_g.w3[lane] = c_gauss.w3[lane];
     __syncthreads();
     double sp[4];
     int tie = 0;
@@ -554,7 +571,7 @@ __global__ __launch_bounds__(64) void build3d_interior_kernel(BuildArgs A) {
     if (lane < 32) {
         const int gp = lane & 7, part = lane >> 3;
         GPShared g;
-        gp_phase1(th, c_gauss, s_nd, gp, A.dt, g, part == 3 ? s_tab[gp].dphi : nullptr);
+        gp_phase1(th, s_g, s_nd, gp, A.dt, g, part == 3 ? s_tab[gp].dphi : nullptr);
         if (part == 0) gp_tab_dir<0>(th, g, sp, s_tab[gp]);
         else if (part == 1) gp_tab_dir<1>(th, g, sp, s_tab[gp]);
         else if (part == 2) gp_tab_dir<2>(th, g, sp, s_tab[gp]);
@@ -564,7 +581,7 @@ __global__ __launch_bounds__(64) void build3d_interior_kernel(BuildArgs A) {
     // phase 2: lane (N, M) contracts every point for its node pair
     const int N = lane >> 3, M = lane & 7;
     double R[NEQ][NEQ], rhs[NEQ];
-    interior_phase2(th, c_gauss, s_tab, lane, A.dt, sp, R, rhs);
+    interior_phase2(th, s_g, s_tab, lane, A.dt, sp, R, rhs);
 #pragma unroll
     for (int r = 0; r < NEQ; r++)
 #pragma unroll

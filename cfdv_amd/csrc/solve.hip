@@ -217,10 +217,31 @@ __global__ void compose_kernel(const int* __restrict__ node_ptr, const int* __re
                                double* __restrict__ b) {
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= n_node) return;
+    const int nnod = k / neqn;
     double acc[5] = {0, 0, 0, 0, 0};
-    for (int p = node_ptr[n]; p < node_ptr[n + 1]; p++) {
-        const long ent = node_ent[p];           // e * nnod + local
-        const int nnod = k / neqn;
+    const int p0 = node_ptr[n], p1 = node_ptr[n + 1];
+    // the first 8 entries (every node of a hex8 mesh without hanging
+    // nodes) are fetched before any is summed, so their loads overlap
+    constexpr int PF = 8;
+    long off[PF];
+#pragma unroll
+    for (int i = 0; i < PF; i++) {
+        const long ent = (p0 + i < p1) ? node_ent[p0 + i] : 0;   // e * nnod + local
+        off[i] = (ent / nnod) * k + (ent % nnod) * neqn;
+    }
+    double v[PF][5];
+#pragma unroll
+    for (int i = 0; i < PF; i++)
+#pragma unroll
+        for (int q = 0; q < 5; q++) v[i][q] = (p0 + i < p1 && q < neqn) ? rhs[off[i] + q] : 0.0;
+#pragma unroll
+    for (int i = 0; i < PF; i++)
+        if (p0 + i < p1) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) acc[q] += v[i][q];
+        }
+    for (int p = p0 + PF; p < p1; p++) {
+        const long ent = node_ent[p];
         const long e = ent / nnod;
         const int l = (int)(ent % nnod);
         for (int q = 0; q < neqn; q++) acc[q] += rhs[e * k + l * neqn + q];
