@@ -326,12 +326,14 @@ CFDV_HD constexpr bool jnz(int r, int s) {
 }
 
 // Per Gauss point, what the node-pair contraction reads: the direction
-// combinations M1_I = s1 a_I + s3 b_I, A_I = a_I + b_I, S_I = s2 a_I + s4 b_I
-// on their 17 nonzeros, the rhs pieces and the scalars of c_ij.
+// combinations M1_I = s1 a_I + s3 b_I on their 17 nonzeros, A_I = a_I + b_I
+// and S_I = s2 a_I + s4 b_I as full row-major 5x5 (operands of the matrix-core
+// product), the rhs pieces and the scalars of c_ij.
 struct GPTab {
     double dphi[8][3];
     double wJ;
-    double M1[3][17], A[3][17], S[3][17];
+    double M1[3][17];
+    double Af[3][25], Sf[3][25];
     double P[3][5];     // P(i, r), r = 1..4
     double AD[3][5];    // (A_i D)(r)
     double c_mu, c_lambda, c_muR, c_rinv, c_denom, c_l, c_m, c_n, c_TERM1, c_hkocv, c_en;
@@ -348,9 +350,14 @@ CFDV_HD void gp_tab_dir(const DevThermo& th, const GPShared& g, const double sp[
     for (int p = 0; p < 17; p++) {
         const int r = JPAT_R[I][p], c = JPAT_S[I][p];
         t.M1[I][p] = s1 * a[r][c] + s3 * b[r][c];
-        t.A[I][p] = a[r][c] + b[r][c];
-        t.S[I][p] = s2 * a[r][c] + s4 * b[r][c];
     }
+#pragma unroll
+    for (int r = 0; r < 5; r++)
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            t.Af[I][r * 5 + c] = a[r][c] + b[r][c];
+            t.Sf[I][r * 5 + c] = s2 * a[r][c] + s4 * b[r][c];
+        }
 #pragma unroll
     for (int r = 0; r < 5; r++) {
         double q = 0.0;
@@ -434,8 +441,9 @@ CFDV_HD void accum_c3_tab(const DevThermo& th, const GPTab& t, const double W[3]
         }
 }
 
-// Phase 2 for lane (N, M): the domain contraction of every Gauss point,
-// in contract3's order (diagonal, direction terms, A S product, c_ij).
+// Phase 2 for lane (N, M): the domain contraction of every Gauss point
+// without the c_A A S product (mfma_as_product below): diagonal, direction
+// terms, c_ij, in contract3's order, and the rhs.
 template <typename GT>
 CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs, int lane,
                              double dt, const double sp[4], double R[5][5], double rhs[5]) {
@@ -465,21 +473,13 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs
 #pragma unroll
             for (int j = 0; j < 3; j++) W[i][j] = -dt * s3 * wJ * DN[i] * DM[j];
         }
-        const double cI = wJ * PN * PM, cA = dtsq_2 * wJ;
+        const double cI = wJ * PN * PM;
 #pragma unroll
         for (int r = 0; r < NEQ; r++) R[r][r] += cI;
-        double Sh[5][5];
-#pragma unroll
-        for (int r = 0; r < NEQ; r++)
-#pragma unroll
-            for (int s = 0; s < NEQ; s++) Sh[r][s] = 0.0;
 #pragma unroll
         for (int I = 0; I < 3; I++)
 #pragma unroll
-            for (int p = 0; p < 17; p++) {
-                R[JPAT_R[I][p]][JPAT_S[I][p]] += wg[I] * g.M1[I][p];
-                Sh[JPAT_R[I][p]][JPAT_S[I][p]] += DM[I] * g.S[I][p];
-            }
+            for (int p = 0; p < 17; p++) R[JPAT_R[I][p]][JPAT_S[I][p]] += wg[I] * g.M1[I][p];
         // rhs: P and the A D product (contract3's AQ)
 #pragma unroll
         for (int r = 0; r < NEQ; r++) {
@@ -491,26 +491,77 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs
             }
             rhs[r] += acc - dtsq_2 * aq;
         }
-        // R += cA (sum_i DN_i A_i)(sum_j DM_j S_j), one row of the left factor at a time
-#pragma unroll
-        for (int r = 0; r < NEQ; r++) {
-            double Ah[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int I = 0; I < 3; I++)
-#pragma unroll
-                for (int p = 0; p < 17; p++)
-                    if (JPAT_R[I][p] == r) Ah[JPAT_S[I][p]] += DN[I] * g.A[I][p];
-#pragma unroll
-            for (int s = 0; s < NEQ; s++) {
-                double pr = 0.0;
-#pragma unroll
-                for (int t = 0; t < NEQ; t++)
-                    if (jnz(r, t) && jnz(t, s)) pr += Ah[t] * Sh[t][s];
-                R[r][s] += cA * pr;
-            }
-        }
         accum_c3_tab(th, g, W, R);
     }
+}
+
+// The c_A (sum_i dPhi_N,i A_i)(sum_j dPhi_M,j S_j) term of every node pair
+// and Gauss point at once, on the FP64 matrix cores.  Over the element it is
+// one 40 x 40 x 40 product: rows (N, r), columns (M, s), inner index
+// (point g, u):
+//   R_AS[5N + r][5M + s] = sum_g sum_u AH_g[5N + r][u] SH_g[u][5M + s],
+//   AH_g[5N + r][u] = c_A,g sum_i dPhi_N,i(g) A_i(g)[r][u],
+//   SH_g[u][5M + s] = sum_j dPhi_M,j(g) S_j(g)[u][s],
+// c_A,g = dt^2 / 2 w_g det J_g.  Tiles of v_mfma_f64_16x16x4f64: 3 x 3 output
+// tiles (rows and columns padded to 48), 10 steps of 4 along the inner
+// index.  Operand layout of that instruction on gfx950: A[i][k] in lane
+// i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in register i / 4 of lane
+// j + 16 (i % 4).  Each operand entry is formed once, by the lane that feeds
+// it to the matrix core.  The 40 x 40 result goes to out[i * 40 + j].
+typedef double dvec4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void mfma_as_product(const GPTab* tabs, double dt, int lane, double* out) {
+    const double dtsq_2 = (dt * dt) / 2.0;
+    const int li = lane & 15, kk = lane >> 4;
+    dvec4_t acc[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) acc[a][b] = dvec4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int ks = 0; ks < 10; ks++) {
+        const int k = 4 * ks + kk, g = k / 5, u = k - 5 * g;
+        const GPTab& t = tabs[g];
+        const double cA = dtsq_2 * t.wJ;
+        double av[3], bv[3];
+#pragma unroll
+        for (int mt = 0; mt < 3; mt++) {
+            const int i = 16 * mt + li;       // row (N, r) of the element block
+            double v = 0.0;
+            if (i < 40) {
+                const int N = i / 5, r = i - 5 * (i / 5);
+                v = cA * (t.dphi[N][0] * t.Af[0][r * 5 + u] + t.dphi[N][1] * t.Af[1][r * 5 + u] +
+                          t.dphi[N][2] * t.Af[2][r * 5 + u]);
+            }
+            av[mt] = v;
+        }
+#pragma unroll
+        for (int nt = 0; nt < 3; nt++) {
+            const int j = 16 * nt + li;       // column (M, s)
+            double v = 0.0;
+            if (j < 40) {
+                const int M = j / 5, sc = j - 5 * (j / 5);
+                v = t.dphi[M][0] * t.Sf[0][u * 5 + sc] + t.dphi[M][1] * t.Sf[1][u * 5 + sc] +
+                    t.dphi[M][2] * t.Sf[2][u * 5 + sc];
+            }
+            bv[nt] = v;
+        }
+#pragma unroll
+        for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 3; nt++)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv[nt], acc[mt][nt], 0, 0, 0);
+    }
+    __syncthreads();   // every lane is done reading tabs; out may alias them
+#pragma unroll
+    for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 3; nt++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = 16 * mt + 4 * q + kk, j = 16 * nt + li;
+                if (i < 40 && j < 40) out[i * 40 + j] = acc[mt][nt][q];
+            }
+    __syncthreads();
 }
 
 template <bool W64, bool W32>
@@ -578,10 +629,19 @@ __global__ __launch_bounds__(64) void build3d_interior_kernel(BuildArgs A) {
         else gp_tab_common(th, g, s_tab[gp]);
     }
     __syncthreads();
-    // phase 2: lane (N, M) contracts every point for its node pair
+    // phase 2: lane (N, M) contracts every point for its node pair; the
+    // c_A A S term of all pairs runs on the matrix cores, its 40 x 40 result
+    // lands in LDS over the (then consumed) point tables
     const int N = lane >> 3, M = lane & 7;
     double R[NEQ][NEQ], rhs[NEQ];
     interior_phase2(th, s_g, s_tab, lane, A.dt, sp, R, rhs);
+    double* s_as = reinterpret_cast<double*>(s_tab);
+    static_assert(sizeof(s_tab) >= 1600 * sizeof(double), "point tables hold the 40 x 40 product");
+    mfma_as_product(s_tab, A.dt, lane, s_as);
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) R[r][s] += s_as[(NEQ * N + r) * K + NEQ * M + s];
 #pragma unroll
     for (int r = 0; r < NEQ; r++)
 #pragma unroll
