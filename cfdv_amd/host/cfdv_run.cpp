@@ -4,6 +4,7 @@
 //   cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed]
 //            [--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE]
 //            [--flatten FILE] [--adap parity|3d] [--tecplot N] [--tecplot-only]
+//            [--storage rows|ebe]
 //
 // Reads a reference binary deck (control, aero, binarynodes, binaryelements;
 // LoadBinary.h:156-245) into element/node objects shaped like the
@@ -29,6 +30,9 @@
 // that iteration reads and the element s1..s4 of that build (ele_data, as
 // main.cpp passes it); --tecplot-only writes <out>/tecplot_<iter>.plt of the
 // loaded deck without s1..s4 and without touching the device.
+// --storage: the FP64 and mixed solvers (--mode fp64|mixed) multiply over
+// the element blocks summed into node rows (rows, the default) or element by
+// element (ebe); the reference solve always runs element by element.
 // A deck whose control has adap = 1 is read with its adap file
 // (LoadBinary.h:247-261: "elno adap refine_level" then the serialised Hnm
 // tensor, TensorStream.h:95-175, column-major).
@@ -375,10 +379,10 @@ int main(int argc, char** argv) {
     if (argc < 2) {
         std::fprintf(stderr, "usage: cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed] "
                              "[--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE] [--flatten FILE] "
-                             "[--adap parity|3d]\n");
+                             "[--adap parity|3d] [--tecplot N] [--tecplot-only] [--storage rows|ebe]\n");
         return 2;
     }
-    std::string dir = argv[1], out = ".", dump, mode = "reference", flat, precond = "none", adap;
+    std::string dir = argv[1], out = ".", dump, mode = "reference", flat, precond = "none", adap, storage = "rows";
     int tecplot = 0;
     bool tecplot_only = false;
     int iters = -1;
@@ -404,6 +408,7 @@ int main(int argc, char** argv) {
         else if (a == "--precond") precond = v;
         else if (a == "--adap") adap = v;
         else if (a == "--tecplot") tecplot = std::atoi(v.c_str());
+        else if (a == "--storage") storage = v;
         else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
@@ -432,7 +437,13 @@ int main(int argc, char** argv) {
         int iter = (int)d.control.count("iter") ? (int)d.control.at("iter") : 0;
         const int itermax = iters >= 0 ? iter + iters : (int)d.control.at("itermax");
         const int md = mode == "fp64" ? CFDV_SOLVE_FP64 : mode == "mixed" ? CFDV_SOLVE_MIXED : CFDV_SOLVE_REFERENCE;
-        const int blocks = md == CFDV_SOLVE_MIXED ? (CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32) : CFDV_BLOCKS_FP64;
+        if (storage != "rows" && storage != "ebe") throw std::runtime_error("--storage takes rows or ebe");
+        // the FP64 and mixed solvers multiply over assembled node rows unless
+        // --storage ebe; the reference solve always multiplies element by
+        // element
+        int blocks = CFDV_BLOCKS_FP64;
+        if (md != CFDV_SOLVE_REFERENCE && storage == "rows") blocks |= CFDV_BLOCKS_ROWS;
+        else if (md == CFDV_SOLVE_MIXED) blocks |= CFDV_BLOCKS_FP32;
         std::printf("nodes/eles: %zu %zu\n", d.nodes.size(), d.elements.size());
         cfdv::Context gpu(d.elements, d.nodes, d.ndim, d.nnod, d.neqn, d.th, blocks);
         if (!adap.empty()) {

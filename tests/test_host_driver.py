@@ -85,6 +85,34 @@ def test_driver_two_reference_iterations(gpu_available, runner, tmp_path):
         assert np.allclose(v, g["rms_steps"][k], rtol=1e-5, atol=1e-12)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_driver_solvers_on_rows_and_elements(gpu_available, runner, tmp_path, mode):
+    """One iteration of the FP64 and mixed solvers on the ramp deck, products
+    over node rows (the default) and element by element: both reach the
+    golden textbook solution of the first step's system (du_true, a dense
+    solve of the oracle's K) and agree with each other."""
+    name = "3dtestramp_new"
+    g = np.load(os.path.join(GOLD, f"{name}.npz"))
+    out = {}
+    for storage in ("rows", "ebe"):
+        d = tmp_path / storage
+        d.mkdir()
+        r = subprocess.run([runner, os.path.join(GOLD, "decks", name), "--iters", "1", "--mode", mode,
+                            "--rtol", "1e-10", "--precond", "nodeblock", "--storage", storage,
+                            "--out", str(d), "--dump", str(d / "U.bin")],
+                           capture_output=True, text=True, timeout=300)
+        print(storage, r.stdout[-400:], r.stderr[-400:])
+        assert r.returncode == 0
+        out[storage] = np.fromfile(d / "U.bin", dtype=np.float64).reshape(g["U0"].shape) - g["U0"]
+    ref = g["du_true"].reshape(g["U0"].shape)
+    for storage, du in out.items():
+        err = np.linalg.norm(du - ref) / np.linalg.norm(ref)
+        print(mode, storage, "du vs textbook solution", err)
+        assert err < 1e-7
+    assert np.linalg.norm(out["rows"] - out["ebe"]) <= 1e-7 * np.linalg.norm(ref)
+
+
 def _refined_deck(tmp_path):
     """a flat-plate deck refined twice on the host (3D node matching)"""
     from cfdv_amd import refine as PR
