@@ -6,7 +6,9 @@ Re=5e5, adiabatic wall, CFL 0.5, synthetic initial state (uniform
 freestream, wall at rest).  A step is one full FDV time step on the GPU:
 CFL, element build (FDVParam + domain + boundary + ApplyBC, FP64 arithmetic),
 GMRES to a 1e-8 relative true residual (both paths right-preconditioned by
-node-block Jacobi unless --precond none), update and RMS monitor.
+node-block Jacobi unless --precond none; each solve after the first starts
+from the previous step's du scaled by dt / dt_prev unless --warm-start 0),
+update and RMS monitor.
 
 Two paths are timed from the same initial state:
   fp64  : FP64 blocks, FP64 restarted GMRES(m)
@@ -66,6 +68,10 @@ def parse():
     p.add_argument("--inner-rtol", type=float, default=1e-4)
     p.add_argument("--precond", choices=["nodeblock", "none"], default="nodeblock")
     p.add_argument("--storage", choices=["rows", "ebe"], default="rows")
+    # each solve starts from the previous step's du scaled by dt / dt_prev
+    # (the first step after the initial state starts from 0); --warm-start 0
+    # starts every solve from 0 as the reference does
+    p.add_argument("--warm-start", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=32768)
     return p.parse_args()
@@ -86,7 +92,7 @@ def run_steps(ctx, U0, mode, n, args):
     for i in range(n):
         st = ctx.step(args.cfl, mode=mode, restarts=60 if mode == "fp64" else 20, m=args.m,
                       rtol=args.rtol, inner_m=args.inner_m, inner_rtol=args.inner_rtol,
-                      inner_restarts=3, precond=args.precond)
+                      inner_restarts=3, precond=args.precond, warm_start=bool(args.warm_start))
         out.append(st)
         log(f"  {mode} step {i}: build {st.ms_build:.1f} ms assemble {st.ms_assemble:.1f} ms "
             f"solve {st.ms_solve:.1f} ms "
@@ -150,6 +156,18 @@ def main():
         el = time.perf_counter() - t0
         results[mode] = (el, sts)
 
+    # the mixed path again with every solve started from 0, for comparison
+    cold = None
+    if args.warm_start:
+        cargs = argparse.Namespace(**vars(args))
+        cargs.warm_start = 0
+        run_steps(ctx, U0, "mixed", args.warmup, cargs)
+        sync()
+        t0 = time.perf_counter()
+        sts_c = run_steps(ctx, U0, "mixed", args.steps, cargs)
+        sync()
+        cold = {"steps_per_s": args.steps / (time.perf_counter() - t0),
+                "fp32_products_per_step": sum(s.n_matvec32 for s in sts_c) / args.steps}
     el_mx, st_mx = results["mixed"]
     el_64, st_64 = results["fp64"]
     value = args.steps / el_mx
@@ -209,8 +227,9 @@ def main():
                                f"{args.precond} right preconditioner on both paths",
                    "elements": ne, "nodes": nn, "colors": info["ncolors"],
                    "precond": args.precond, "inner_m": args.inner_m, "inner_rtol": args.inner_rtol,
-                   "storage": args.storage},
+                   "storage": args.storage, "warm_start": bool(args.warm_start)},
         "fp64_steps_per_s": fp64_sps,
+        "mixed_cold_start": cold,
         "mixed_over_fp64": value / fp64_sps,
         "rel_resid_mixed": max(s.solve.rel_resid for s in st_mx),
         "rel_resid_fp64": max(s.solve.rel_resid for s in st_64),

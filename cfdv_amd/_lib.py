@@ -64,7 +64,7 @@ class ThermoS(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("restarts", C.c_int), ("m", C.c_int), ("rtol", C.c_double),
                 ("inner_m", C.c_int), ("inner_rtol", C.c_double), ("inner_restarts", C.c_int),
-                ("transpose", C.c_int), ("precond", C.c_int)]
+                ("transpose", C.c_int), ("precond", C.c_int), ("warm_start", C.c_int)]
 
 
 class SolveStats(C.Structure):

@@ -162,6 +162,10 @@ struct cfdv_ctx {
     // discrete adjoint: per-slot tangents [n_pad][41], host-given vectors
     double *d_gelem = nullptr, *d_lam = nullptr, *d_duv = nullptr, *d_gU = nullptr, *d_g = nullptr;
     double last_dt = 0.0;
+    // warm start: d_du keeps the last update's increment, built with prev_dt
+    bool have_prev = false;
+    double prev_dt = 0.0;
+    double* d_r0 = nullptr;
     double *d_part = nullptr, *d_small = nullptr, *d_hcol = nullptr, *d_cflpart = nullptr;
     double* h_pin = nullptr;   // pinned staging for small host-to-device arrays
     int *d_counters = nullptr;     // [0] ties, [1] status, [2] singular node block
@@ -891,21 +895,40 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
     return iters;
 }
 
-// K x = b (K^T x = b with o->transpose) for device b, x (both [ndof])
-int solve_fp64_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, const double* b, double* x) {
+// K x = b (K^T x = b with o->transpose) for device b, x (both [ndof]).
+// warm: x holds a starting guess x0 on entry; K d = b - K x0 is solved to
+// the same absolute tolerance rtol ||b|| and x = x0 + d.
+int solve_fp64_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, const double* b, double* x,
+                   bool warm = false) {
     const long n = c->ndof;
     if (!c->have_blocks(64, o->transpose != 0)) return fail(c, CFDV_ERR_ARG, "FP64 solve needs CFDV_BLOCKS_FP64");
     c->ensure_krylov(o->m);
-    vec_fill(x, 0.0, n, c->st);
     const double bnorm = c->norm(b);
     s->bnorm = bnorm;
     bool conv = false;
     const bool pc = o->precond == CFDV_PRECOND_NODE_BLOCK;
     if (pc) c->ensure_precond();
-    if (bnorm > 0)
-        s->iters += gmres_core<double>(c, 64, o->transpose != 0, b, x, o->m, o->restarts, o->rtol * bnorm, bnorm, s,
-                                       &conv, c->d_V, c->d_w, pc ? c->d_P64 : nullptr, pc ? c->d_z64 : nullptr,
-                                       true);
+    const bool tr = o->transpose != 0;
+    if (!warm) {
+        vec_fill(x, 0.0, n, c->st);
+        if (bnorm > 0)
+            s->iters += gmres_core<double>(c, 64, tr, b, x, o->m, o->restarts, o->rtol * bnorm, bnorm, s, &conv,
+                                           c->d_V, c->d_w, pc ? c->d_P64 : nullptr, pc ? c->d_z64 : nullptr, true);
+    } else {
+        if (!c->d_r0) c->d_r0 = c->alloc<double>((size_t)n);
+        c->matvec(x, c->d_y, 64, tr);
+        s->matvecs64++;
+        vec_sub(c->d_r0, b, c->d_y, n, c->st);
+        const double r0 = c->norm(c->d_r0);
+        if (r0 > o->rtol * bnorm) {
+            vec_fill(c->d_d64, 0.0, n, c->st);
+            s->iters += gmres_core<double>(c, 64, tr, c->d_r0, c->d_d64, o->m, o->restarts, o->rtol * bnorm, r0, s,
+                                           &conv, c->d_V, c->d_w, pc ? c->d_P64 : nullptr, pc ? c->d_z64 : nullptr,
+                                           true);
+            vec_set(c->d_small + 10, 1.0, c->st);
+            vec_axpy(x, c->d_d64, c->d_small + 10, 1.0, n, c->st);
+        }
+    }
     c->matvec(x, c->d_y, 64, o->transpose != 0);
     vec_sub(c->d_r, b, c->d_y, n, c->st);
     s->resid = c->norm(c->d_r);
@@ -915,16 +938,27 @@ int solve_fp64_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, c
     return CFDV_OK;
 }
 
+// x0 = du of the last update scaled by dt / dt_prev, in place in d_du
+bool warm_guess(cfdv_ctx* c, const cfdv_solve_opts* o) {
+    if (!o->warm_start || o->transpose || !c->have_prev || !(c->prev_dt > 0) || !(c->last_dt > 0)) return false;
+    vec_set(c->d_small + 12, c->prev_dt / c->last_dt, c->st);
+    vec_scale_to(c->d_du, c->d_du, c->d_small + 12, c->ndof, c->st);
+    return true;
+}
+
 int solve_fp64(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
     launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
-    const int rc = solve_fp64_rhs(c, o, s, c->d_b, c->d_du);
+    const bool warm = warm_guess(c, o);
+    const int rc = solve_fp64_rhs(c, o, s, c->d_b, c->d_du, warm);
     c->have_du = rc == CFDV_OK || rc == CFDV_ERR_NOCONV;
     return rc;
 }
 
 // FP64 iterative refinement around an FP32 GMRES: r = b - K64 du (FP64),
-// solve K32 d = r/||r|| in FP32, du += ||r|| d (FP64).
-int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, const double* b, double* x) {
+// solve K32 d = r/||r|| in FP32, du += ||r|| d (FP64).  warm: du starts
+// from the guess held in x.
+int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, const double* b, double* x,
+                    bool warm = false) {
     const long n = c->ndof;
     const bool tr = o->transpose != 0;
     if (!c->have_blocks(32, tr))
@@ -934,13 +968,20 @@ int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, 
     c->ensure_krylov32(o->inner_m);
     const bool pc = o->precond == CFDV_PRECOND_NODE_BLOCK;
     if (pc) c->ensure_precond();
-    vec_fill(x, 0.0, n, c->st);
     const double one = 1.0;
     vec_set(c->d_small + 10, one, c->st);
     const double bnorm = c->norm(b);
     s->bnorm = bnorm;
-    vec_copy(c->d_r, b, n, c->st);
     double rnorm = bnorm;
+    if (!warm) {
+        vec_fill(x, 0.0, n, c->st);
+        vec_copy(c->d_r, b, n, c->st);
+    } else {
+        c->matvec(x, c->d_y, outer_prec, tr);
+        if (outer_prec == 64) s->matvecs64++; else s->matvecs32++;
+        vec_sub(c->d_r, b, c->d_y, n, c->st);
+        rnorm = c->norm(c->d_r);
+    }
     int sweeps = 0;
     while (rnorm > o->rtol * bnorm && sweeps < o->restarts) {
         // inner right-hand side: FP32 copy of r / ||r||
@@ -951,9 +992,13 @@ int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, 
         // the FP32 right-hand side stays in d_w32; the last basis slot is the
         // Arnoldi work vector
         float* wbuf = c->d_V32 + (size_t)o->inner_m * c->ldv();   // last slot as the Arnoldi work vector
-        s->iters += gmres_core<float>(c, 32, tr, c->d_w32, c->d_d64, o->inner_m - 1, o->inner_restarts,
-                                      o->inner_rtol, 1.0, &si, &conv, c->d_V32, wbuf, pc ? c->d_P32 : nullptr,
-                                      pc ? c->d_z32 : nullptr, false);
+        // the inner solve reduces r by inner_rtol, or only by what is left to
+        // reach the target (half of rtol ||b|| / ||r||) when that is less
+        const double need = 0.5 * o->rtol * bnorm / rnorm;
+        const double itol = std::min(0.5, std::max(o->inner_rtol, need));
+        s->iters += gmres_core<float>(c, 32, tr, c->d_w32, c->d_d64, o->inner_m - 1, o->inner_restarts, itol, 1.0,
+                                      &si, &conv, c->d_V32, wbuf, pc ? c->d_P32 : nullptr, pc ? c->d_z32 : nullptr,
+                                      false);
         s->matvecs32 += si.matvecs32;
         s->matvecs64 += si.matvecs64;
         // du += ||r|| d  (FP64)
@@ -977,7 +1022,8 @@ int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, 
 
 int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
     launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
-    const int rc = solve_mixed_rhs(c, o, s, c->d_b, c->d_du);
+    const bool warm = warm_guess(c, o);
+    const int rc = solve_mixed_rhs(c, o, s, c->d_b, c->d_du, warm);
     c->have_du = rc == CFDV_OK || rc == CFDV_ERR_NOCONV;
     return rc;
 }
@@ -1062,6 +1108,7 @@ int cfdv_set_state(cfdv_ctx* ctx, const double* U) {
     ctx->sync();
     ctx->built = false;
     ctx->pinv_valid = false;
+    ctx->have_prev = false;
     return CFDV_OK;
     GUARD_END
 }
@@ -1248,6 +1295,8 @@ int cfdv_update(cfdv_ctx* ctx) {
     ctx->built = false;
     ctx->pinv_valid = false;
     ctx->have_du = false;
+    ctx->have_prev = true;   // d_du keeps the increment for a warm start
+    ctx->prev_dt = ctx->last_dt;
     return CFDV_OK;
     GUARD_END
 }
@@ -1366,6 +1415,7 @@ int cfdv_set_thermo(cfdv_ctx* ctx, const cfdv_thermo* thermo) {
     ctx->built = false;
     ctx->pinv_valid = false;
     ctx->have_du = false;
+    ctx->have_prev = false;
     return CFDV_OK;
 }
 

@@ -140,13 +140,15 @@ class FDVContext:
         return b
 
     def solve(self, mode="reference", restarts=1, m=20, rtol=1e-8, inner_m=30,
-              inner_rtol=1e-3, inner_restarts=4, transpose=False, check=True, precond="none"):
+              inner_rtol=1e-3, inner_restarts=4, transpose=False, check=True, precond="none",
+              warm_start=False):
         """``precond``: "none" or "nodeblock" (node-block Jacobi, right-applied;
-        FP64 and mixed modes)."""
+        FP64 and mixed modes).  ``warm_start``: start from the last update's du
+        scaled by dt / dt_prev (FP64 and mixed modes)."""
         md = {"reference": _lib.CFDV_SOLVE_REFERENCE, "fp64": _lib.CFDV_SOLVE_FP64,
               "mixed": _lib.CFDV_SOLVE_MIXED}[mode]
         o = _lib.SolveOpts(md, restarts, m, rtol, inner_m, inner_rtol, inner_restarts,
-                           int(transpose), _PRECOND[precond])
+                           int(transpose), _PRECOND[precond], int(warm_start))
         st = _lib.SolveStats()
         rc = self.L.cfdv_solve(self.h, C.byref(o), C.byref(st))
         if check:
@@ -171,7 +173,8 @@ class FDVContext:
               "mixed": _lib.CFDV_SOLVE_MIXED}[mode]
         o = _lib.SolveOpts(md, kw.get("restarts", 1), kw.get("m", 20), kw.get("rtol", 1e-8),
                            kw.get("inner_m", 30), kw.get("inner_rtol", 1e-3),
-                           kw.get("inner_restarts", 4), 0, _PRECOND[kw.get("precond", "none")])
+                           kw.get("inner_restarts", 4), 0, _PRECOND[kw.get("precond", "none")],
+                           int(kw.get("warm_start", False)))
         st = _lib.StepStats()
         self._check(self.L.cfdv_step(self.h, cfl, C.byref(o), C.byref(st)))
         return st

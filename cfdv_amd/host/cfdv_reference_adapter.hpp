@@ -226,6 +226,7 @@ public:
         opts_.inner_restarts = 4;
         opts_.transpose = 0;
         opts_.precond = CFDV_PRECOND_NONE;
+        opts_.warm_start = 0;
     }
     virtual ~HipGMRES() = default;
     // blocks and rhs are already on the device (FDVGalerkin); the element
@@ -240,7 +241,7 @@ public:
 
 private:
     const Context& ctx_;
-    cfdv_solve_opts opts_;
+    cfdv_solve_opts opts_{};
     cfdv_solve_stats last_{};
 };
 

@@ -129,12 +129,19 @@ typedef struct {
     int m;             /* Krylov dimension (gmresiter)                            */
     double rtol;       /* FP64/mixed: relative residual target                   */
     int inner_m;       /* mixed: inner Krylov dimension                           */
-    double inner_rtol; /* mixed: inner relative residual target                  */
+    double inner_rtol; /* mixed: inner relative residual target (a sweep that
+                          needs less to reach rtol ||b|| stops at what it
+                          needs, at most 0.5)                                  */
     int inner_restarts;/* mixed: max inner restarts per sweep                    */
     int transpose;     /* solve K^T du = b (adjoint); FP64 and mixed modes only   */
     int precond;       /* FP64 and mixed modes: CFDV_PRECOND_NONE or
                           CFDV_PRECOND_NODE_BLOCK (right preconditioner; the
                           stopping test stays on the true residual)            */
+    int warm_start;    /* FP64 and mixed modes, K du = b: start from the last
+                          update's du scaled by dt / dt_prev instead of 0 (the
+                          stopping test is unchanged, ||b - K du|| <= rtol
+                          ||b||); ignored for the first solve after create,
+                          cfdv_set_state or cfdv_set_thermo                  */
 } cfdv_solve_opts;
 
 #define CFDV_PRECOND_NONE 0
