@@ -154,7 +154,7 @@ struct cfdv_ctx {
     // assembled node rows (CFDV_BLOCKS_ROWS), formed after each build
     RowsArgs ra{};
     long n_groups = 0;
-    long *d_goff = nullptr, *d_coff = nullptr;
+    long *d_goff = nullptr, *d_coff = nullptr, *d_diag = nullptr;
     int *d_cols = nullptr, *d_contrib = nullptr;
     double* d_B64 = nullptr;
     float* d_B32 = nullptr;
@@ -299,7 +299,8 @@ struct cfdv_ctx {
         }
         if (pinv_valid) return;
         HIP_CHECK_THROW(hipMemsetAsync(d_counters + 2, 0, sizeof(int), st));
-        launch_nodeblock(d_node_ptr, d_node_ent, d_R64, d_R32, nnod, neqn, n_node, d_P64, d_P32, d_counters + 2, st);
+        if (d_B64) launch_nodeblock_rows(ra, d_diag, d_B64, d_P64, d_P32, d_counters + 2, st);
+        else launch_nodeblock(d_node_ptr, d_node_ent, d_R64, d_R32, nnod, neqn, n_node, d_P64, d_P32, d_counters + 2, st);
         check_launch();
         int singular = 0;
         HIP_CHECK_THROW(hipMemcpyAsync(&singular, d_counters + 2, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -340,7 +341,7 @@ DevThermo make_thermo(const cfdv_thermo* t) {
 // rows with groups of 4 slots, and per slot the element blocks that sum
 // into it, listed in element order.  ptr/elem: node -> element CSR.
 struct RowsHost {
-    std::vector<long> goff, coff;
+    std::vector<long> goff, coff, diag;
     std::vector<int> cols, contrib;
 };
 
@@ -420,6 +421,13 @@ RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* 
         }
     std::vector<int>& contrib = h.contrib;
     contrib.assign((size_t)coff[ng * 4] * 64, -1);
+    // global slot of each row's own column (its diagonal block)
+    h.diag.assign(nnode, 0);
+    for (long r = 0; r < nnode; r++) {
+        const int* b = adj.data() + aptr[r];
+        const long kl = std::lower_bound(b, adj.data() + aptr[r + 1], (int)r) - b;
+        h.diag[r] = goff[r / 64] * 4 + kl;
+    }
     for (long ch = 0; ch < nch; ch++)
         for (long k = goff[ch] * 4; k < goff[ch + 1] * 4; k++) {
             const long kl = k - goff[ch] * 4;
@@ -444,6 +452,8 @@ void setup_rows(cfdv_ctx* c, const cfdv_mesh* m, const std::vector<int>& ptr, co
     c->d_cols = c->alloc<int>(cols.size());
     c->d_coff = c->alloc<long>(coff.size());
     c->d_contrib = c->alloc<int>(contrib.size());
+    c->d_diag = c->alloc<long>(h.diag.size());
+    HIP_CHECK_THROW(hipMemcpy(c->d_diag, h.diag.data(), sizeof(long) * h.diag.size(), hipMemcpyHostToDevice));
     HIP_CHECK_THROW(hipMemcpy(c->d_goff, goff.data(), sizeof(long) * goff.size(), hipMemcpyHostToDevice));
     HIP_CHECK_THROW(hipMemcpy(c->d_cols, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice));
     HIP_CHECK_THROW(hipMemcpy(c->d_coff, coff.data(), sizeof(long) * coff.size(), hipMemcpyHostToDevice));

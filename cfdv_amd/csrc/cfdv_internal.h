@@ -136,10 +136,74 @@ void launch_rows_matvec(const RowsArgs& a, const double* B64, const float* B32, 
                         hipStream_t st);
 // rows (FP64 and/or FP32, either may be null) from the FP64 element blocks
 void launch_rows_assemble(const RowsArgs& a, const double* R64, double* B64, float* B32, hipStream_t st);
+// node-block inverses (as launch_nodeblock) from the diagonal slots of the
+// FP64 rows (diag[n]: global slot index of node n's own column); the slot
+// sums are the element blocks' in element order, so P equals
+// launch_nodeblock's bit for bit
+void launch_nodeblock_rows(const RowsArgs& a, const long* diag, const double* B64, double* P64, float* P32,
+                           int* status, hipStream_t st);
 
 // ---- coloring (solve.hip) ----
 void launch_color_round(const int* conn, int nnod, const int* node_ptr, const int* node_elem, long n_elem,
                         const unsigned long long* prio, const int* color_in, int* color_out, int* changed,
                         hipStream_t st);
+
+// node-block Jacobi storage: node n entry q in chunks of 64 nodes
+template <int NEQ>
+__host__ __device__ __forceinline__ size_t pblk_offset(long n, int q) {
+    return (size_t)(n >> 6) * 64 * NEQ * NEQ + (size_t)q * 64 + (size_t)(n & 63);
+}
+
+// Gauss-Jordan inverse with partial pivoting of one node block A (FP64),
+// stored in FP64 and FP32 at node n; a singular block sets *status = 1 and
+// stores the identity
+template <int NEQ>
+__device__ __forceinline__ void nodeblock_invert_store(double (&A)[NEQ][NEQ], long n, double* __restrict__ P64,
+                                                       float* __restrict__ P32, int* __restrict__ status) {
+    double B[NEQ][NEQ];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int c = 0; c < NEQ; c++) B[r][c] = (r == c) ? 1.0 : 0.0;
+    bool singular = false;
+#pragma unroll
+    for (int c = 0; c < NEQ; c++) {
+        // pivot: largest |A[r][c]| for r >= c (rows swapped by value)
+        int piv = c;
+        double best = fabs(A[c][c]);
+#pragma unroll
+        for (int r = c + 1; r < NEQ; r++)
+            if (fabs(A[r][c]) > best) { best = fabs(A[r][c]); piv = r; }
+        if (!(best > 0.0)) singular = true;
+#pragma unroll
+        for (int r = c + 1; r < NEQ; r++)
+            if (r == piv) {
+#pragma unroll
+                for (int t = 0; t < NEQ; t++) {
+                    double a = A[c][t]; A[c][t] = A[r][t]; A[r][t] = a;
+                    double b = B[c][t]; B[c][t] = B[r][t]; B[r][t] = b;
+                }
+            }
+        const double inv = 1.0 / A[c][c];
+#pragma unroll
+        for (int t = 0; t < NEQ; t++) { A[c][t] *= inv; B[c][t] *= inv; }
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+            if (r != c) {
+                const double f = A[r][c];
+#pragma unroll
+                for (int t = 0; t < NEQ; t++) { A[r][t] -= f * A[c][t]; B[r][t] -= f * B[c][t]; }
+            }
+    }
+    if (singular) *status = 1;
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int c = 0; c < NEQ; c++) {
+            const double v = singular ? ((r == c) ? 1.0 : 0.0) : B[r][c];
+            P64[pblk_offset<NEQ>(n, r * NEQ + c)] = v;
+            P32[pblk_offset<NEQ>(n, r * NEQ + c)] = (float)v;
+        }
+}
 
 }  // namespace cfdv

@@ -162,6 +162,37 @@ __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const
     }
 }
 
+// node-block inverses from the diagonal slot of each FP64 row
+template <int NEQ>
+__global__ __launch_bounds__(256) void nodeblock_rows_kernel(long n_node, const long* __restrict__ diag,
+                                                             const double* __restrict__ B64,
+                                                             double* __restrict__ P64, float* __restrict__ P32,
+                                                             int* __restrict__ status) {
+    constexpr int BB = NEQ * NEQ, GV = 4 * BB;
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_node) return;
+    const long k = diag[n];
+    const long g = k >> 2;
+    const int t = (int)(k & 3), lane = (int)(n & 63);
+    double A[NEQ][NEQ];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int c = 0; c < NEQ; c++) A[r][c] = B64[row_offset<double>(g, lane, t * BB + r * NEQ + c, GV)];
+    nodeblock_invert_store<NEQ>(A, n, P64, P32, status);
+}
+
+void launch_nodeblock_rows(const RowsArgs& a, const long* diag, const double* B64, double* P64, float* P32,
+                           int* status, hipStream_t st) {
+    const int nb = (int)((a.n_node + 255) / 256);
+    if (a.neqn == 5)
+        hipLaunchKernelGGL((nodeblock_rows_kernel<5>), dim3(nb), dim3(256), 0, st, a.n_node, diag, B64, P64, P32,
+                           status);
+    else
+        hipLaunchKernelGGL((nodeblock_rows_kernel<4>), dim3(nb), dim3(256), 0, st, a.n_node, diag, B64, P64, P32,
+                           status);
+}
+
 static int row_grid(long n_chunks) {
     const long nb = (n_chunks + 3) / 4;
     return (int)(((nb + 7) / 8) * 8);   // whole rounds of 8 XCDs

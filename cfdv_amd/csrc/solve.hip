@@ -617,10 +617,6 @@ template void vec_combine<float>(double*, const float*, long, int, const double*
 // chunk: entry q = r NEQ + c of node n at (n / 64) 64 NEQ^2 + q 64 + n % 64,
 // so that a wavefront's load of one entry for 64 consecutive nodes is one
 // contiguous 512-byte (FP64) or 256-byte (FP32) segment.
-template <int NEQ>
-__device__ __forceinline__ size_t pblk_offset(long n, int q) {
-    return (size_t)(n >> 6) * 64 * NEQ * NEQ + (size_t)q * 64 + (size_t)(n & 63);
-}
 
 template <typename TB, int NEQ>
 __global__ __launch_bounds__(256) void nodeblock_kernel(const int* __restrict__ node_ptr,
@@ -631,14 +627,11 @@ __global__ __launch_bounds__(256) void nodeblock_kernel(const int* __restrict__ 
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= n_node) return;
     const int k = nnod * NEQ, KK = k * k;
-    double A[NEQ][NEQ], B[NEQ][NEQ];
+    double A[NEQ][NEQ];
 #pragma unroll
     for (int r = 0; r < NEQ; r++)
 #pragma unroll
-        for (int c = 0; c < NEQ; c++) {
-            A[r][c] = 0.0;
-            B[r][c] = (r == c) ? 1.0 : 0.0;
-        }
+        for (int c = 0; c < NEQ; c++) A[r][c] = 0.0;
     for (int p = node_ptr[n]; p < node_ptr[n + 1]; p++) {
         const long ent = node_ent[p];   // slot * nnod + local node
         const long e = ent / nnod;
@@ -651,45 +644,7 @@ __global__ __launch_bounds__(256) void nodeblock_kernel(const int* __restrict__ 
                 A[r][c] += (double)R[blk_offset<TB>(e, q, KK)];
             }
     }
-    bool singular = false;
-#pragma unroll
-    for (int c = 0; c < NEQ; c++) {
-        // pivot: largest |A[r][c]| for r >= c (rows swapped by value)
-        int piv = c;
-        double best = fabs(A[c][c]);
-#pragma unroll
-        for (int r = c + 1; r < NEQ; r++)
-            if (fabs(A[r][c]) > best) { best = fabs(A[r][c]); piv = r; }
-        if (!(best > 0.0)) singular = true;
-#pragma unroll
-        for (int r = c + 1; r < NEQ; r++)
-            if (r == piv) {
-#pragma unroll
-                for (int t = 0; t < NEQ; t++) {
-                    double a = A[c][t]; A[c][t] = A[r][t]; A[r][t] = a;
-                    double b = B[c][t]; B[c][t] = B[r][t]; B[r][t] = b;
-                }
-            }
-        const double inv = 1.0 / A[c][c];
-#pragma unroll
-        for (int t = 0; t < NEQ; t++) { A[c][t] *= inv; B[c][t] *= inv; }
-#pragma unroll
-        for (int r = 0; r < NEQ; r++)
-            if (r != c) {
-                const double f = A[r][c];
-#pragma unroll
-                for (int t = 0; t < NEQ; t++) { A[r][t] -= f * A[c][t]; B[r][t] -= f * B[c][t]; }
-            }
-    }
-    if (singular) *status = 1;
-#pragma unroll
-    for (int r = 0; r < NEQ; r++)
-#pragma unroll
-        for (int c = 0; c < NEQ; c++) {
-            const double v = singular ? ((r == c) ? 1.0 : 0.0) : B[r][c];
-            P64[pblk_offset<NEQ>(n, r * NEQ + c)] = v;
-            P32[pblk_offset<NEQ>(n, r * NEQ + c)] = (float)v;
-        }
+    nodeblock_invert_store<NEQ>(A, n, P64, P32, status);
 }
 
 void launch_nodeblock(const int* node_ptr, const int* node_ent, const double* R64, const float* R32, int nnod,
