@@ -184,6 +184,7 @@ struct cfdv_ctx {
         if (h_pin) (void)hipHostFree(h_pin);
         if (st) {
             for (auto& e : ev) (void)hipEventDestroy(e);
+            if (ev_sync) (void)hipEventDestroy(ev_sync);
             (void)hipStreamDestroy(st);
         }
     }
@@ -193,7 +194,17 @@ struct cfdv_ctx {
         allocs.push_back(p);
         return p;
     }
-    void sync() { HIP_CHECK_THROW(hipStreamSynchronize(st)); }
+    // wait for the stream by polling an event: the solvers wait once per
+    // Arnoldi step for a few scalars, and a blocking wait adds its wake-up
+    // latency to every step
+    hipEvent_t ev_sync = nullptr;
+    void sync() {
+        HIP_CHECK_THROW(hipEventRecord(ev_sync, st));
+        hipError_t e;
+        while ((e = hipEventQuery(ev_sync)) == hipErrorNotReady) {
+        }
+        HIP_CHECK_THROW(e);
+    }
     void check_launch() { HIP_CHECK_THROW(hipGetLastError()); }
 
     // products K x use the assembled rows when the context holds them (not
@@ -509,7 +520,7 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
             launch_color_round(d_conn_o, nn, d_ptr, d_elem, ne, d_prio, d_c0, d_c1, d_chg, c->st);
             HIP_CHECK_THROW(hipGetLastError());
             HIP_CHECK_THROW(hipMemcpyAsync(&chg, d_chg, sizeof(int), hipMemcpyDeviceToHost, c->st));
-            HIP_CHECK_THROW(hipStreamSynchronize(c->st));
+            c->sync();
             std::swap(d_c0, d_c1);
             if (!chg) break;
             if (++rounds > 100000) throw std::runtime_error("coloring did not converge");
@@ -1080,6 +1091,7 @@ int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flag
         ctx->th = make_thermo(thermo);
         HIP_CHECK_THROW(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
         for (auto& e : ctx->ev) HIP_CHECK_THROW(hipEventCreate(&e));
+        HIP_CHECK_THROW(hipEventCreateWithFlags(&ctx->ev_sync, hipEventDisableTiming));
         upload_gauss_tables(make_tables());
         setup(ctx, mesh);
     } catch (const std::invalid_argument& ex) {
