@@ -436,7 +436,8 @@ RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* 
     h.diag.assign(nnode, 0);
     for (long r = 0; r < nnode; r++) {
         const int* b = adj.data() + aptr[r];
-        const long kl = std::lower_bound(b, adj.data() + aptr[r + 1], (int)r) - b;
+        const int* e = adj.data() + aptr[r + 1];
+        const long kl = std::lower_bound(b, e, (int)r) - b;
         h.diag[r] = goff[r / 64] * 4 + kl;
     }
     for (long ch = 0; ch < nch; ch++)
