@@ -457,7 +457,7 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs
 #pragma unroll
         for (int s = 0; s < NEQ; s++) R[r][s] = 0.0;
     }
-#pragma unroll 1
+#pragma unroll 2
     for (int gp = 0; gp < 8; gp++) {
         const GPTab& g = tabs[gp];
         const double PN = G.phi3[gp][N], PM = G.phi3[gp][M];
