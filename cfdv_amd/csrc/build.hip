@@ -232,7 +232,7 @@ CFDV_HD void gp_phase1(const DevThermo& th, const GT& G, const double (*nd)[ND_S
         for (int j = 0; j < 3; j++) f.dU[q][j] = 0;
     }
     double SPhi = 0.0, SD[3] = {0, 0, 0};
-#pragma unroll 1
+#pragma unroll 2
     for (int n = 0; n < 8; n++) {
         double dphi[3];
 #pragma unroll
