@@ -663,18 +663,285 @@ __global__ __launch_bounds__(64) void build3d_interior_kernel(BuildArgs A) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Boundary faces of non-hanging hex8 elements, in two phases like the
+// interior kernel: the state and Jacobian combinations of each face Gauss
+// point are formed once, by a lane group of their own, into LDS; then lane
+// (N, M) contracts them for its node pair.  Same terms and point order as
+// build3d_faces (element3d.h), which the per-lane kernel still uses for
+// hanging-node elements.
+// ---------------------------------------------------------------------------
+struct FaceTab {
+    GPTab t;             // M1, Af, Sf, AD, P and the c_ij scalars (dphi, wJ unused)
+    double en[3];        // w0 w1 det J times the unit face normal (TFNorm3D)
+    double phib[4];      // face shape functions at the point
+    double dpx[4][3];    // their x derivatives
+    int fl[4];           // element-local nodes of the face
+    int ok;              // 0: unknown bc code, the point adds nothing
+};
+
+// State of face point fgp on face fc (TFNorm3D, calc_TF_norm3D.h:143-301, the
+// face interpolation and eval2dboundary's edits, FDVGalerkin.h:455-533),
+// as build3d_faces computes it, into g (f, D, P, SPhi) and the geometry.
+// Returns false (and sets *status bit 0) for an unknown bc code.
+__device__ bool face_point(const DevThermo& th, const GaussTables& G, const double (*nd)[ND_STRIDE], int fc, int fgp,
+                           int bc, double dt, GPShared& g, double en[3], double phib[4], double dpx[4][3], int fl[4],
+                           int* status) {
+    constexpr int NEQ = 5;
+    const double reinv = 1.0 / th.creyn;
+    const int FN[6][4] = {{3, 2, 1, 0}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {6, 7, 4, 5}};
+    const double w0 = G.wf[fgp][0], w1 = G.wf[fgp][1];
+    const double z0 = G.zf[fgp][0], z1 = G.zf[fgp][1];
+    double nl[4][3];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        fl[t] = FN[fc][t];
+#pragma unroll
+        for (int i = 0; i < 3; i++) nl[t][i] = nd[fl[t]][ND_X + i];
+    }
+    phib[0] = 0.25 * (1 - z0) * (1 - z1);
+    phib[1] = 0.25 * (1 + z0) * (1 - z1);
+    phib[2] = 0.25 * (1 + z0) * (1 + z1);
+    phib[3] = 0.25 * (1 - z0) * (1 + z1);
+    // the Lambda/Mu magnitude is recomputed after each component
+    // (calc_TF_norm3D.h:208-212)
+    double Lm[3], Mu[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) { Lm[i] = nl[1][i] - nl[0][i]; Mu[i] = nl[3][i] - nl[0][i]; }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        Lm[j] /= sqrt(Lm[0] * Lm[0] + Lm[1] * Lm[1] + Lm[2] * Lm[2]);
+        Mu[j] /= sqrt(Mu[0] * Mu[0] + Mu[1] * Mu[1] + Mu[2] * Mu[2]);
+    }
+    double en_[3], ga[3];
+    en_[0] = Lm[1] * Mu[2] - Lm[2] * Mu[1];
+    en_[1] = Lm[2] * Mu[0] - Lm[0] * Mu[2];
+    en_[2] = Lm[0] * Mu[1] - Lm[1] * Mu[0];
+    ga[0] = en_[1] * Lm[2] - en_[2] * Lm[1];
+    ga[1] = en_[2] * Lm[0] - en_[0] * Lm[2];
+    ga[2] = en_[0] * Lm[1] - en_[1] * Lm[0];
+    double dp[4][2];
+    dp[0][0] = -0.25 * (1 - z1); dp[1][0] = 0.25 * (1 - z1);
+    dp[2][0] = 0.25 * (1 + z1);  dp[3][0] = -0.25 * (1 + z1);
+    dp[0][1] = -0.25 * (1 - z0); dp[1][1] = -0.25 * (1 + z0);
+    dp[2][1] = 0.25 * (1 + z0);  dp[3][1] = 0.25 * (1 - z0);
+    double dxd[3][2];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc += dp[t][j] * nl[t][i];
+            dxd[i][j] = acc;
+        }
+    double dtr[2][2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        dtr[0][j] = Lm[0] * dxd[0][j] + Lm[1] * dxd[1][j] + Lm[2] * dxd[2][j];
+        dtr[1][j] = ga[0] * dxd[0][j] + ga[1] * dxd[1][j] + ga[2] * dxd[2][j];
+    }
+    const double detjac = dtr[0][0] * dtr[1][1] - dtr[1][0] * dtr[0][1];
+    const double djinv = 1. / detjac;
+    if (detjac < 1.0E-12) *status |= 2;
+#pragma unroll
+    for (int i = 0; i < 3; i++) en[i] = en_[i] * w0 * w1 * detjac;
+    GPState<3>& f = g.f;
+    f.E = 0; f.T = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        f.v[i] = 0; f.dT[i] = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) { f.dV[i][j] = 0; f.tau[i][j] = 0; }
+    }
+#pragma unroll
+    for (int q = 0; q < NEQ; q++) {
+        f.U[q] = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) f.dU[q][j] = 0;
+    }
+    double Sph = 0.0, Sdp[3] = {0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int n = fl[t];
+        const double d0 = (dtr[1][1] * dp[t][0] - dtr[1][0] * dp[t][1]) * djinv;
+        const double d1 = (-dtr[0][1] * dp[t][0] + dtr[0][0] * dp[t][1]) * djinv;
+#pragma unroll
+        for (int i = 0; i < 3; i++) dpx[t][i] = Lm[i] * d0 + ga[i] * d1;
+        const double ph = phib[t];
+        Sph += ph;
+        f.E += ph * nd[n][ND_E];
+        const double Tt = nd[n][ND_T];
+        f.T += ph * Tt;
+#pragma unroll
+        for (int q = 0; q < NEQ; q++) {
+            const double u_ = nd[n][ND_U + q];
+            f.U[q] += ph * u_;
+#pragma unroll
+            for (int j = 0; j < 3; j++) f.dU[q][j] += dpx[t][j] * u_;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double v_ = nd[n][ND_V + i];
+            Sdp[i] += dpx[t][i];
+            f.v[i] += ph * v_;
+            f.dT[i] += dpx[t][i] * Tt;
+#pragma unroll
+            for (int j = 0; j < 3; j++) f.dV[i][j] += dpx[t][j] * v_;
+        }
+    }
+    // the unedited sums feed D and P
+    double vph[3], dVp[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        vph[i] = f.v[i];
+#pragma unroll
+        for (int j = 0; j < 3; j++) dVp[i][j] = f.dV[i][j];
+    }
+    f.mu = calc_mu(th, f.T);
+    f.k = calc_k(th, f.mu);
+    if (!eval2dboundary(th, bc, f, en)) {
+        *status |= 1;
+        return false;
+    }
+    g.wJ = 0.0;
+    g.SPhi = Sph;
+    g.D[0] = 0.0;
+#pragma unroll
+    for (int s = 1; s < 4; s++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) acc += (-f.tau[j][s - 1] * reinv) * Sdp[j];
+        g.D[s] = acc;
+    }
+    {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double td = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) td += f.tau[j][k] * dVp[k][j];
+            acc += (-td - f.k * f.dT[j] * Sdp[j]) * reinv;
+        }
+        g.D[4] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        g.P[i][0] = 0.0;
+#pragma unroll
+        for (int r = 1; r < NEQ; r++) {
+            double P;
+            if (r < 4) P = dt * (-f.tau[i][r - 1] * reinv) * Sph;
+            else {
+                double tv = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) tv += f.tau[i][k] * vph[k];
+                P = dt * ((-tv - f.k * f.dT[i] * Sph) * reinv);
+            }
+            g.P[i][r] = P;
+        }
+    }
+    return true;
+}
+
+// Face point p's contribution for lane (N, M): contract3 with cI = 0,
+// weights from the face shape functions, then the c_ij terms and the rhs
+// (build3d_faces' order).
+__device__ __forceinline__ void face_contract(const DevThermo& th, const FaceTab& F, int N, int M, double dt,
+                                              double s3, double R[5][5], double rhs[5]) {
+    constexpr int NEQ = 5;
+    const double dtsq_2 = (dt * dt) / 2.0;
+    int fN = -1, fM = -1;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        if (F.fl[t] == N) fN = t;
+        if (F.fl[t] == M) fM = t;
+    }
+    if (fN < 0) return;   // this lane's row is not on the face
+    const double phN = F.phib[fN];
+    const double onM = (fM >= 0) ? 1.0 : 0.0;
+    const double phM = (fM >= 0) ? F.phib[fM] : 0.0;
+    double dpM[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) dpM[i] = (fM >= 0) ? F.dpx[fM][i] : 0.0;
+    double wg[3], wr[3], wa[3], W[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        wg[i] = dt * phN * phM * F.en[i] * onM;
+        wr[i] = phN * F.en[i];
+        wa[i] = F.en[i] * onM;
+#pragma unroll
+        for (int j = 0; j < 3; j++) W[i][j] = dt * s3 * phN * F.en[i] * dpM[j] * onM;
+    }
+    const GPTab& t = F.t;
+#pragma unroll
+    for (int I = 0; I < 3; I++)
+#pragma unroll
+        for (int p = 0; p < 17; p++) R[JPAT_R[I][p]][JPAT_S[I][p]] += wg[I] * t.M1[I][p];
+    // R += cA (sum_i wa_i A_i)(sum_j dpM_j S_j), cA = -dt^2/2 phi_N, one row
+    // of the left factor at a time
+    const double cA = -dtsq_2 * phN;
+    double Sh[5][5];
+#pragma unroll
+    for (int u = 0; u < NEQ; u++)
+#pragma unroll
+        for (int s = 0; s < NEQ; s++)
+            Sh[u][s] = dpM[0] * t.Sf[0][u * 5 + s] + dpM[1] * t.Sf[1][u * 5 + s] + dpM[2] * t.Sf[2][u * 5 + s];
+#pragma unroll 1
+    for (int r = 0; r < NEQ; r++) {
+        double Ah[5];
+#pragma unroll
+        for (int u = 0; u < NEQ; u++)
+            Ah[u] = wa[0] * t.Af[0][r * 5 + u] + wa[1] * t.Af[1][r * 5 + u] + wa[2] * t.Af[2][r * 5 + u];
+        double pr[5];
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) {
+            pr[s] = 0.0;
+#pragma unroll
+            for (int u = 0; u < NEQ; u++) pr[s] += Ah[u] * Sh[u][s];
+        }
+        // R[r][.] with a runtime r: select the row by unrolled comparisons
+#pragma unroll
+        for (int rr = 0; rr < NEQ; rr++)
+            if (rr == r) {
+#pragma unroll
+                for (int s = 0; s < NEQ; s++) R[rr][s] += cA * pr[s];
+            }
+    }
+    accum_c3_tab(th, t, W, R);
+    if (M == 0) {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) {
+            double acc = 0.0, aq = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                if (r > 0) acc += wr[i] * t.P[i][r];
+                aq += wr[i] * t.AD[i][r];
+            }
+            rhs[r] += -acc + dtsq_2 * aq;
+        }
+    }
+}
+
 // Boundary elements whose domain part is already in R64/rhs: face integrals
-// and ApplyBC on top (the same accumulation order as build3d_elem).  One
-// wavefront per listed slot.
+// and ApplyBC on top.  One wavefront per listed slot (no hanging-node
+// elements: those are built whole by build3d_kernel).  Face points come in
+// build3d_faces' order (point-major over the faces with 0 < bc < 100), eight
+// at a time: lanes (point, part) for part < 4 form the tables, part 0..2 one
+// direction's Jacobian combinations each, part 3 the c_ij scalars and the
+// geometry.
 template <bool W32>
 __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
     constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600;
     __shared__ double s_nd[NNOD][ND_STRIDE];
+    __shared__ FaceTab s_ft[8];
+    __shared__ int s_status;
     const int lane = threadIdx.x;
     if ((long)blockIdx.x >= A.n_bslots) return;
     const long e = A.bslots[blockIdx.x];
     const int* cn = A.conn + e * NNOD;
     if (lane < NNOD) load_node3d(A, cn[lane], s_nd[lane]);
+    if (lane == 0) s_status = 0;
     __syncthreads();
     const DevThermo th = A.th;
     double sp[4];
@@ -691,29 +958,79 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
         fdvparam<3, 8>(th, vn, Tn, rhon, sp, &tie);
     }
     const int N = lane >> 3, M = lane & 7;
-    double R[NEQ][NEQ], rhs[NEQ];
+    const int* bcs = A.bc + e * 6;
+    int nfa = 0;
 #pragma unroll
-    for (int r = 0; r < NEQ; r++) {
-        rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
+    for (int fc = 0; fc < 6; fc++) nfa += (bcs[fc] != 0 && bcs[fc] < 100);
+    const int npts = 4 * nfa;
+    // R and rhs are read and written per batch of points, so that the point
+    // tables are formed with no node-pair block live (every element with at
+    // most two boundary faces takes one batch)
+    for (int b0 = 0; b0 < npts || b0 == 0; b0 += 8) {
+        const int nb = (npts - b0 < 8) ? npts - b0 : 8;
+        if (lane < 32 && (lane & 7) < nb) {
+            const int p = lane & 7, part = lane >> 3;
+            const int i = b0 + p, fgp = i / nfa, k = i - fgp * nfa;
+            int fc = 0, seen = 0;
+            for (int c = 0; c < 6; c++)
+                if (bcs[c] != 0 && bcs[c] < 100) {
+                    if (seen == k) fc = c;
+                    seen++;
+                }
+            GPShared g;
+            double en[3], phib[4], dpx[4][3];
+            int fl[4], st = 0;
+            const bool ok = face_point(th, c_gauss, s_nd, fc, fgp, bcs[fc], A.dt, g, en, phib, dpx, fl, &st);
+            FaceTab& F = s_ft[p];
+            if (part == 3) {
+                F.ok = ok ? 1 : 0;
 #pragma unroll
-        for (int s = 0; s < NEQ; s++) R[r][s] = A.R64[blk_index<double>(e, (NEQ * N + r) * K + NEQ * M + s, KK)];
-    }
-    int status = 0;
-    build3d_faces(A, A.th, c_gauss, s_nd, e, lane, sp, R, rhs, &status);
-    apply_bc3(A, e, lane, R, rhs);
-    if (status && lane == 0) atomicOr(A.status, status);
+                for (int t = 0; t < 4; t++) {
+                    F.fl[t] = fl[t];
+                    F.phib[t] = phib[t];
 #pragma unroll
-    for (int r = 0; r < NEQ; r++)
+                    for (int j = 0; j < 3; j++) F.dpx[t][j] = dpx[t][j];
+                }
 #pragma unroll
-        for (int s = 0; s < NEQ; s++) {
-            const int q = (NEQ * N + r) * K + NEQ * M + s;
-            A.R64[blk_index<double>(e, q, KK)] = R[r][s];
-            if (W32) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
+                for (int j = 0; j < 3; j++) F.en[j] = en[j];
+                if (st) atomicOr(&s_status, st);
+            }
+            if (ok) {
+                if (part == 0) gp_tab_dir<0>(th, g, sp, F.t);
+                else if (part == 1) gp_tab_dir<1>(th, g, sp, F.t);
+                else if (part == 2) gp_tab_dir<2>(th, g, sp, F.t);
+                else gp_tab_common(th, g, F.t);
+            }
         }
-    if (M == 0) {
+        __syncthreads();
+        double R[NEQ][NEQ], rhs[NEQ];
 #pragma unroll
-        for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
+        for (int r = 0; r < NEQ; r++) {
+            rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) R[r][s] = A.R64[blk_index<double>(e, (NEQ * N + r) * K + NEQ * M + s, KK)];
+        }
+#pragma unroll 1
+        for (int p = 0; p < nb; p++)
+            if (s_ft[p].ok) face_contract(th, s_ft[p], N, M, A.dt, sp[2], R, rhs);
+        const bool last = b0 + 8 >= npts;
+        if (last) apply_bc3(A, e, lane, R, rhs);
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) {
+                const int q = (NEQ * N + r) * K + NEQ * M + s;
+                A.R64[blk_index<double>(e, q, KK)] = R[r][s];
+                if (W32 && last) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
+            }
+        if (M == 0) {
+#pragma unroll
+            for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
+        }
+        if (last) break;
+        __syncthreads();   // the tables are rewritten by the next batch
     }
+    if (lane == 0 && s_status) atomicOr(A.status, s_status);
 }
 
 template <bool W64, bool W32>
