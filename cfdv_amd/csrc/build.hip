@@ -680,6 +680,21 @@ struct FaceTab {
     int ok;              // 0: unknown bc code, the point adds nothing
 };
 
+// What lane (N, M) adds for one face point, with t = the face position of
+// node M: R_NM += dt phi_N phi_M G + c_A,N PT[t] + dt s3 phi_N CT[t] and
+// rhs_N += phi_N Q (c_A,N = -dt^2/2 phi_N), where
+//   G     = sum_i en_i M1_i,
+//   PT[t] = (sum_i en_i A_i)(sum_j dphi_t,j S_j),
+//   CT[t] = sum_ij en_i dphi_t,j c_ij / Re,
+//   Q     = -sum_i en_i P_i + dt^2/2 sum_i en_i (A_i D);
+// none of them depends on N, so each is formed once per point.
+struct FaceMat {
+    double G[25];
+    double PT[4][25];
+    double CT[4][25];
+    double Q[5];
+};
+
 // State of face point fgp on face fc (TFNorm3D, calc_TF_norm3D.h:143-301, the
 // face interpolation and eval2dboundary's edits, FDVGalerkin.h:455-533),
 // as build3d_faces computes it, into g (f, D, P, SPhi) and the geometry.
@@ -844,13 +859,78 @@ __device__ bool face_point(const DevThermo& th, const GaussTables& G, const doub
     return true;
 }
 
-// Face point p's contribution for lane (N, M): contract3 with cI = 0,
-// weights from the face shape functions, then the c_ij terms and the rhs
-// (build3d_faces' order).
-__device__ __forceinline__ void face_contract(const DevThermo& th, const FaceTab& F, int N, int M, double dt,
+// Part `part` of face point F's node-pair matrices: part t < 4 forms PT[t]
+// and CT[t], part 4 forms G and Q.
+__device__ void face_mats(const DevThermo& th, const FaceTab& F, int part, double dt, FaceMat& out) {
+    constexpr int NEQ = 5;
+    const GPTab& t = F.t;
+    if (part < 4) {
+        double Ah[5][5];
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int u = 0; u < NEQ; u++)
+                Ah[r][u] = F.en[0] * t.Af[0][r * 5 + u] + F.en[1] * t.Af[1][r * 5 + u] + F.en[2] * t.Af[2][r * 5 + u];
+        const double d0 = F.dpx[part][0], d1 = F.dpx[part][1], d2 = F.dpx[part][2];
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) {
+            double Sh[5];
+#pragma unroll
+            for (int u = 0; u < NEQ; u++) Sh[u] = d0 * t.Sf[0][u * 5 + s] + d1 * t.Sf[1][u * 5 + s] + d2 * t.Sf[2][u * 5 + s];
+#pragma unroll
+            for (int r = 0; r < NEQ; r++) {
+                double pr = 0.0;
+#pragma unroll
+                for (int u = 0; u < NEQ; u++) pr += Ah[r][u] * Sh[u];
+                out.PT[part][r * 5 + s] = pr;
+            }
+        }
+        double W[3][3], C[5][5];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) W[i][j] = F.en[i] * F.dpx[part][j];
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) C[r][s] = 0.0;
+        accum_c3_tab(th, t, W, C);
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) out.CT[part][r * 5 + s] = C[r][s];
+    } else {
+        double G[5][5];
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) G[r][s] = 0.0;
+#pragma unroll
+        for (int I = 0; I < 3; I++)
+#pragma unroll
+            for (int p = 0; p < 17; p++) G[JPAT_R[I][p]][JPAT_S[I][p]] += F.en[I] * t.M1[I][p];
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) out.G[r * 5 + s] = G[r][s];
+        const double dtsq_2 = (dt * dt) / 2.0;
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) {
+            double acc = 0.0, aq = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                if (r > 0) acc += F.en[i] * t.P[i][r];
+                aq += F.en[i] * t.AD[i][r];
+            }
+            out.Q[r] = -acc + dtsq_2 * aq;
+        }
+    }
+}
+
+// Face point p's contribution for lane (N, M) from its matrices.
+__device__ __forceinline__ void face_contract(const FaceTab& F, const FaceMat& Mx, int N, int M, double dt,
                                               double s3, double R[5][5], double rhs[5]) {
     constexpr int NEQ = 5;
-    const double dtsq_2 = (dt * dt) / 2.0;
     int fN = -1, fM = -1;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
@@ -859,82 +939,34 @@ __device__ __forceinline__ void face_contract(const DevThermo& th, const FaceTab
     }
     if (fN < 0) return;   // this lane's row is not on the face
     const double phN = F.phib[fN];
-    const double onM = (fM >= 0) ? 1.0 : 0.0;
-    const double phM = (fM >= 0) ? F.phib[fM] : 0.0;
-    double dpM[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) dpM[i] = (fM >= 0) ? F.dpx[fM][i] : 0.0;
-    double wg[3], wr[3], wa[3], W[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        wg[i] = dt * phN * phM * F.en[i] * onM;
-        wr[i] = phN * F.en[i];
-        wa[i] = F.en[i] * onM;
-#pragma unroll
-        for (int j = 0; j < 3; j++) W[i][j] = dt * s3 * phN * F.en[i] * dpM[j] * onM;
-    }
-    const GPTab& t = F.t;
-#pragma unroll
-    for (int I = 0; I < 3; I++)
-#pragma unroll
-        for (int p = 0; p < 17; p++) R[JPAT_R[I][p]][JPAT_S[I][p]] += wg[I] * t.M1[I][p];
-    // R += cA (sum_i wa_i A_i)(sum_j dpM_j S_j), cA = -dt^2/2 phi_N, one row
-    // of the left factor at a time
-    const double cA = -dtsq_2 * phN;
-    double Sh[5][5];
-#pragma unroll
-    for (int u = 0; u < NEQ; u++)
-#pragma unroll
-        for (int s = 0; s < NEQ; s++)
-            Sh[u][s] = dpM[0] * t.Sf[0][u * 5 + s] + dpM[1] * t.Sf[1][u * 5 + s] + dpM[2] * t.Sf[2][u * 5 + s];
-#pragma unroll 1
-    for (int r = 0; r < NEQ; r++) {
-        double Ah[5];
-#pragma unroll
-        for (int u = 0; u < NEQ; u++)
-            Ah[u] = wa[0] * t.Af[0][r * 5 + u] + wa[1] * t.Af[1][r * 5 + u] + wa[2] * t.Af[2][r * 5 + u];
-        double pr[5];
-#pragma unroll
-        for (int s = 0; s < NEQ; s++) {
-            pr[s] = 0.0;
-#pragma unroll
-            for (int u = 0; u < NEQ; u++) pr[s] += Ah[u] * Sh[u][s];
-        }
-        // R[r][.] with a runtime r: select the row by unrolled comparisons
-#pragma unroll
-        for (int rr = 0; rr < NEQ; rr++)
-            if (rr == r) {
-#pragma unroll
-                for (int s = 0; s < NEQ; s++) R[rr][s] += cA * pr[s];
-            }
-    }
-    accum_c3_tab(th, t, W, R);
     if (M == 0) {
 #pragma unroll
-        for (int r = 0; r < NEQ; r++) {
-            double acc = 0.0, aq = 0.0;
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                if (r > 0) acc += wr[i] * t.P[i][r];
-                aq += wr[i] * t.AD[i][r];
-            }
-            rhs[r] += -acc + dtsq_2 * aq;
-        }
+        for (int r = 0; r < NEQ; r++) rhs[r] += phN * Mx.Q[r];
     }
+    if (fM < 0) return;   // column node off the face: no block terms
+    const double cg = dt * phN * F.phib[fM], cA = -(dt * dt) / 2.0 * phN, cc = dt * s3 * phN;
+    const double* PT = Mx.PT[fM];
+    const double* CT = Mx.CT[fM];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int s = 0; s < NEQ; s++) R[r][s] += cg * Mx.G[r * 5 + s] + cA * PT[r * 5 + s] + cc * CT[r * 5 + s];
 }
 
 // Boundary elements whose domain part is already in R64/rhs: face integrals
 // and ApplyBC on top.  One wavefront per listed slot (no hanging-node
 // elements: those are built whole by build3d_kernel).  Face points come in
-// build3d_faces' order (point-major over the faces with 0 < bc < 100), eight
-// at a time: lanes (point, part) for part < 4 form the tables, part 0..2 one
-// direction's Jacobian combinations each, part 3 the c_ij scalars and the
-// geometry.
+// build3d_faces' order (point-major over the faces with 0 < bc < 100), four
+// at a time: lanes (point, part), part < 4, form the point tables (part
+// 0..2 one direction's Jacobian combinations each, part 3 the c_ij scalars
+// and the geometry), then lanes (point, part), part < 5, the node-pair
+// matrices, then lane (N, M) adds them to its block.
 template <bool W32>
 __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
-    constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600;
+    constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600, PB = 4;
     __shared__ double s_nd[NNOD][ND_STRIDE];
-    __shared__ FaceTab s_ft[8];
+    __shared__ FaceTab s_ft[PB];
+    __shared__ FaceMat s_fm[PB];
     __shared__ int s_status;
     const int lane = threadIdx.x;
     if ((long)blockIdx.x >= A.n_bslots) return;
@@ -964,12 +996,11 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
     for (int fc = 0; fc < 6; fc++) nfa += (bcs[fc] != 0 && bcs[fc] < 100);
     const int npts = 4 * nfa;
     // R and rhs are read and written per batch of points, so that the point
-    // tables are formed with no node-pair block live (every element with at
-    // most two boundary faces takes one batch)
-    for (int b0 = 0; b0 < npts || b0 == 0; b0 += 8) {
-        const int nb = (npts - b0 < 8) ? npts - b0 : 8;
-        if (lane < 32 && (lane & 7) < nb) {
-            const int p = lane & 7, part = lane >> 3;
+    // tables are formed with no node-pair block live
+    for (int b0 = 0; b0 < npts || b0 == 0; b0 += PB) {
+        const int nb = (npts - b0 < PB) ? npts - b0 : PB;
+        const int p = lane & 7, part = lane >> 3;
+        if (p < nb && part < 4) {
             const int i = b0 + p, fgp = i / nfa, k = i - fgp * nfa;
             int fc = 0, seen = 0;
             for (int c = 0; c < 6; c++)
@@ -1003,6 +1034,8 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
             }
         }
         __syncthreads();
+        if (p < nb && part < 5 && s_ft[p].ok) face_mats(th, s_ft[p], part, A.dt, s_fm[p]);
+        __syncthreads();
         double R[NEQ][NEQ], rhs[NEQ];
 #pragma unroll
         for (int r = 0; r < NEQ; r++) {
@@ -1011,9 +1044,9 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
             for (int s = 0; s < NEQ; s++) R[r][s] = A.R64[blk_index<double>(e, (NEQ * N + r) * K + NEQ * M + s, KK)];
         }
 #pragma unroll 1
-        for (int p = 0; p < nb; p++)
-            if (s_ft[p].ok) face_contract(th, s_ft[p], N, M, A.dt, sp[2], R, rhs);
-        const bool last = b0 + 8 >= npts;
+        for (int q = 0; q < nb; q++)
+            if (s_ft[q].ok) face_contract(s_ft[q], s_fm[q], N, M, A.dt, sp[2], R, rhs);
+        const bool last = b0 + PB >= npts;
         if (last) apply_bc3(A, e, lane, R, rhs);
 #pragma unroll
         for (int r = 0; r < NEQ; r++)
