@@ -4,7 +4,7 @@
 //   cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed]
 //            [--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE]
 //            [--flatten FILE] [--adap parity|3d] [--tecplot N] [--tecplot-only]
-//            [--storage rows|ebe]
+//            [--storage rows|ebe] [--save N]
 //
 // Reads a reference binary deck (control, aero, binarynodes, binaryelements;
 // LoadBinary.h:156-245) into element/node objects shaped like the
@@ -30,13 +30,23 @@
 // that iteration reads and the element s1..s4 of that build (ele_data, as
 // main.cpp passes it); --tecplot-only writes <out>/tecplot_<iter>.plt of the
 // loaded deck without s1..s4 and without touching the device.
+// --save N writes a restartable deck <out>/<iter>/ (binarynodes,
+// binaryelements, adap, control with iter, aero) every N iterations before
+// the step and after the last one, as main.cpp:177 and :208 call
+// config->Save every 1000 (the default; 0 disables); running cfdv_run on
+// such a directory continues from its iteration.  <out>/time.txt gets
+// IterationTimer's lines (Utility/Timer.h:7-35: header, then "iter
+// run_time step_time t dt" per iteration), in wall-clock seconds.
 // --storage: the FP64 and mixed solvers (--mode fp64|mixed) multiply over
 // the element blocks summed into node rows (rows, the default) or element by
 // element (ebe); the reference solve always runs element by element.
 // A deck whose control has adap = 1 is read with its adap file
 // (LoadBinary.h:247-261: "elno adap refine_level" then the serialised Hnm
 // tensor, TensorStream.h:95-175, column-major).
+#include <sys/stat.h>
+
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -314,9 +324,13 @@ void write_tecplot(const Deck& d, const std::string& path, double time, const do
     if (!tout) throw std::runtime_error("cannot write " + path);
 }
 
-// SaveBinaryData (SaveBinary.h:34-168) for the refined deck, after NodeUnpack
-// (NodeUnpack.h:124-144): node x, rho, v, T; element node ids then bc codes
-void save_deck(const Deck& d, const std::string& dir, const std::string& src) {
+// SaveBinaryData (SaveBinary.h:34-168) after NodeUnpack (NodeUnpack.h:
+// 124-144): node x, rho, v, T; element node ids then bc codes; the adap file
+// of the adapted elements.  The control keeps the source deck's keys with
+// the new sizes, adap = 1 when any element hangs (always for a refined
+// deck: force_adap) and, for a checkpoint, iter = the iteration it starts.
+void save_deck(const Deck& d, const std::string& dir, const std::string& src, bool force_adap = true,
+               int iter = -1) {
     const int ndim = d.ndim, nnod = d.nnod, nface = ndim == 3 ? 6 : 4;
     const double Cv = thermo_cv(d.th);
     std::ofstream nf(dir + "/binarynodes", std::ios::binary);
@@ -346,7 +360,9 @@ void save_deck(const Deck& d, const std::string& dir, const std::string& src) {
     std::ifstream cin_(src + "/control");
     std::ofstream cf(dir + "/control");
     std::string line;
-    bool have_adap = false;
+    bool any_adap = force_adap;
+    for (const Element* e : d.elements) any_adap = any_adap || e->adap;
+    bool have_adap = false, have_iter = false;
     while (std::getline(cin_, line)) {
         const size_t eq = line.find('=');
         std::string key = eq == std::string::npos ? "" : line.substr(0, eq);
@@ -354,10 +370,12 @@ void save_deck(const Deck& d, const std::string& dir, const std::string& src) {
         key.erase(0, key.find_first_not_of(" \t"));
         if (key == "nodes") line = "nodes = " + std::to_string(d.nodes.size());
         else if (key == "elements") line = "elements = " + std::to_string(d.elements.size());
-        else if (key == "adap") { line = "adap = 1"; have_adap = true; }
+        else if (key == "adap") { line = any_adap ? "adap = 1" : "adap = 0"; have_adap = true; }
+        else if (key == "iter" && iter >= 0) { line = "iter = " + std::to_string(iter); have_iter = true; }
         cf << line << "\n";
     }
-    if (!have_adap) cf << "adap = 1\n";
+    if (!have_adap && any_adap) cf << "adap = 1\n";
+    if (!have_iter && iter >= 0) cf << "iter = " << iter << "\n";
     std::ifstream ain(src + "/aero");
     std::ofstream aout(dir + "/aero");
     aout << ain.rdbuf();
@@ -379,11 +397,12 @@ int main(int argc, char** argv) {
     if (argc < 2) {
         std::fprintf(stderr, "usage: cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed] "
                              "[--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE] [--flatten FILE] "
-                             "[--adap parity|3d] [--tecplot N] [--tecplot-only] [--storage rows|ebe]\n");
+                             "[--adap parity|3d] [--tecplot N] [--tecplot-only] [--storage rows|ebe] "
+                             "[--save N]\n");
         return 2;
     }
     std::string dir = argv[1], out = ".", dump, mode = "reference", flat, precond = "none", adap, storage = "rows";
-    int tecplot = 0;
+    int tecplot = 0, save_every = 1000;
     bool tecplot_only = false;
     int iters = -1;
     double rtol = 1e-8;
@@ -409,6 +428,7 @@ int main(int argc, char** argv) {
         else if (a == "--adap") adap = v;
         else if (a == "--tecplot") tecplot = std::atoi(v.c_str());
         else if (a == "--storage") storage = v;
+        else if (a == "--save") save_every = std::atoi(v.c_str());
         else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
@@ -482,8 +502,17 @@ int main(int argc, char** argv) {
         if (md != CFDV_SOLVE_REFERENCE) gmres.options().restarts = std::max(restart, 200);
         if (precond == "nodeblock") gmres.options().precond = CFDV_PRECOND_NODE_BLOCK;
         std::ofstream rms(out + "/RMSError.txt", std::ios::app);
-        double t = 0;
+        std::ofstream tfile(out + "/time.txt");
+        tfile << "Iteration, Run_Time, Time_Elapsed, Sim_Time, Step_dt" << std::endl;
+        auto checkpoint = [&](int it) {
+            const std::string sd = out + "/" + std::to_string(it);
+            ::mkdir(sd.c_str(), 0755);
+            save_deck(d, sd, dir, false, it);
+        };
+        double t = 0, run_time = 0;
         for (; iter < itermax; iter++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if (save_every > 0 && iter % save_every == 0) checkpoint(iter);
             gpu.set_state(d.nodes);
             const double dt = gpu.cfl(cfl);
             cfdv::FDVGalerkin fdv(gpu, dt);
@@ -511,7 +540,11 @@ int main(int argc, char** argv) {
                         st.rel_resid);
             std::fflush(stdout);
             t += dt;
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            run_time += el;
+            tfile << iter << " " << run_time << " " << el << " " << t << " " << dt << " " << std::endl;
         }
+        if (save_every > 0) checkpoint(iter);
         if (!dump.empty()) {
             std::ofstream f(dump, std::ios::binary);
             for (const Node* n : d.nodes) f.write((const char*)n->U.d.data(), (std::streamsize)(d.neqn * sizeof(double)));

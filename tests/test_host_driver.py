@@ -113,6 +113,50 @@ def test_driver_solvers_on_rows_and_elements(gpu_available, runner, tmp_path, mo
     assert np.linalg.norm(out["rows"] - out["ebe"]) <= 1e-7 * np.linalg.norm(ref)
 
 
+@pytest.mark.gpu
+def test_driver_checkpoint_restart_and_time_file(gpu_available, runner, tmp_path):
+    """--save 1 writes a restartable deck before every iteration and after
+    the last one (config->Save, main.cpp:177, 208); running the driver on the
+    iteration-1 checkpoint for one iteration lands where two uninterrupted
+    iterations land (the saved primitives round-trip U to rounding); time.txt
+    has IterationTimer's header and one line per iteration."""
+    deck = os.path.join(GOLD, "decks", "3dtestlagrange")
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    r = subprocess.run([runner, deck, "--iters", "2", "--save", "1", "--out", str(a), "--dump", str(a / "U.bin")],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout[-500:], r.stderr[-500:])
+    assert r.returncode == 0
+    for it in (0, 1, 2):
+        assert (a / str(it) / "binarynodes").exists() and (a / str(it) / "binaryelements").exists()
+        ctl = (a / str(it) / "control").read_text()
+        assert f"iter = {it}" in ctl
+    m0, _, U0, _ = M.read_deck(deck)
+    m2, _, U2d, _ = M.read_deck(str(a / "2"))
+    assert np.array_equal(m2.conn, m0.conn) and np.array_equal(m2.bc, m0.bc)
+    UA = np.fromfile(a / "U.bin", dtype=np.float64).reshape(U0.shape)
+    assert np.allclose(U2d, UA, rtol=1e-13, atol=1e-15)
+    r = subprocess.run([runner, str(a / "1"), "--iters", "1", "--save", "0", "--out", str(b),
+                        "--dump", str(b / "U.bin")], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-500:], r.stderr[-500:])
+    assert r.returncode == 0
+    UB = np.fromfile(b / "U.bin", dtype=np.float64).reshape(U0.shape)
+    _, _, U1d, _ = M.read_deck(str(a / "1"))
+    err = np.linalg.norm(UB - UA) / np.linalg.norm(UA - U1d)
+    print("restart vs uninterrupted, relative to the step's change:", err)
+    assert err < 1e-9
+    assert (b / "RMSError.txt").read_text().split("\t")[0] == "1"
+    lines = (a / "time.txt").read_text().splitlines()
+    assert lines[0] == "Iteration, Run_Time, Time_Elapsed, Sim_Time, Step_dt"
+    assert len(lines) == 3
+    rows = [[float(v) for v in ln.split()] for ln in lines[1:]]
+    assert [int(r_[0]) for r_ in rows] == [0, 1]
+    assert all(len(r_) == 5 and r_[1] >= r_[2] > 0 and r_[4] > 0 for r_ in rows)
+    # six significant digits, the stream default IterationTimer writes with
+    assert rows[1][3] == pytest.approx(rows[0][3] + rows[1][4], rel=1e-5)
+
+
 def _refined_deck(tmp_path):
     """a flat-plate deck refined twice on the host (3D node matching)"""
     from cfdv_amd import refine as PR
