@@ -1515,6 +1515,11 @@ int cfdv_build_vjp(cfdv_ctx* ctx, const double* lambda, const double* du, double
 
 // Y = K X (or K^T X) for 8 vectors at once with the FP32 blocks
 static void matvec_multi_dev(cfdv_ctx* ctx, const float* X, float* Y, bool tr) {
+    if (ctx->d_B32 && !tr) {   // over the assembled rows (K^T X stays element by element)
+        launch_rows_matvec8(ctx->ra, ctx->d_B32, X, Y, ctx->st);
+        ctx->check_launch();
+        return;
+    }
     vec_fill_f(Y, 0.0f, ctx->ndof * 8, ctx->st);
     for (int c = 0; c < ctx->ncolor; c++) {
         MatvecArgs a{ctx->ndim, ctx->nnod, ctx->neqn, ctx->cstart[c], ctx->cstart[c + 1], ctx->d_conn};
@@ -1526,7 +1531,8 @@ static void matvec_multi_dev(cfdv_ctx* ctx, const float* X, float* Y, bool tr) {
 int cfdv_matvec_multi(cfdv_ctx* ctx, const float* X, float* Y, int nvec, int transpose) {
     if (!ctx || !X || !Y || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_matvec_multi before cfdv_build");
     if (nvec != 8) return fail(ctx, CFDV_ERR_ARG, "cfdv_matvec_multi takes 8 vectors");
-    if (ctx->ndim != 3 || !ctx->d_R32) return fail(ctx, CFDV_ERR_ARG, "needs hex8 FP32 blocks");
+    if (ctx->ndim != 3 || !(ctx->d_R32 || (ctx->d_B32 && !transpose)))
+        return fail(ctx, CFDV_ERR_ARG, "needs hex8 FP32 blocks (or node rows for X -> K X)");
     GUARD_BEGIN
     const size_t n = (size_t)ctx->ndof * 8;
     float* dX = ctx->alloc<float>(n);
@@ -1545,7 +1551,7 @@ int cfdv_matvec_multi(cfdv_ctx* ctx, const float* X, float* Y, int nvec, int tra
 
 int cfdv_profile_matvec_multi(cfdv_ctx* ctx, int reps, double* ms) {
     if (!ctx || !ms || reps < 1 || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "profile before build");
-    if (ctx->ndim != 3 || !ctx->d_R32) return fail(ctx, CFDV_ERR_ARG, "needs hex8 FP32 blocks");
+    if (ctx->ndim != 3 || !(ctx->d_R32 || ctx->d_B32)) return fail(ctx, CFDV_ERR_ARG, "needs hex8 FP32 blocks");
     GUARD_BEGIN
     const size_t n = (size_t)ctx->ndof * 8;
     float* dX = ctx->alloc<float>(n);

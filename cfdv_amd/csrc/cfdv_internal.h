@@ -134,6 +134,8 @@ struct RowsArgs {
 // rows, FP64 vectors).  y is written, not accumulated.
 void launch_rows_matvec(const RowsArgs& a, const double* B64, const float* B32, const void* x, void* y, int prec,
                         hipStream_t st);
+// Y = K X for 8 vectors over the FP32 rows, X and Y [n_node * neqn][8]
+void launch_rows_matvec8(const RowsArgs& a, const float* B32, const float* X, float* Y, hipStream_t st);
 // rows (FP64 and/or FP32, either may be null) from the FP64 element blocks
 void launch_rows_assemble(const RowsArgs& a, const double* R64, double* B64, float* B32, hipStream_t st);
 // node-block inverses (as launch_nodeblock) from the diagonal slots of the

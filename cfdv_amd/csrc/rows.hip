@@ -99,6 +99,73 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
 }
 
+// Y = K X for 8 vectors over the FP32 rows: X, Y node-major [n_node * NEQ][8]
+// (the 8 values of a dof contiguous, 32 bytes).  One lane per row as in the
+// single product; each group's 16-byte block loads serve all 8 vectors, the
+// column's 8-wide values come in two 16-byte loads per dof.  The blocks
+// differ row by row, so there is no operand two rows share for a
+// matrix-core tile; at 4 flop per block byte the product stays HBM-bound on
+// the VALU.
+template <int NEQ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void rows_matvec8_kernel(
+    long n_node, long n_chunks, const long* __restrict__ goff, const int* __restrict__ cols,
+    const float* __restrict__ B, const float* __restrict__ X, float* __restrict__ Y) {
+    constexpr int BB = NEQ * NEQ, GV = 4 * BB, V = 4, NVEC = GV / V;
+    typedef typename V16<float>::type vec_t;
+    const long c = chunk_of_wave(n_chunks);
+    if (c >= n_chunks) return;
+    const int lane = threadIdx.x & 63;
+    const long g0 = goff[c], g1 = goff[c + 1];
+    float acc[NEQ][8];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+        for (int v = 0; v < 8; v++) acc[r][v] = 0.0f;
+    const vec_t* Bv = reinterpret_cast<const vec_t*>(B) + lane;
+    const vec_t* Xv = reinterpret_cast<const vec_t*>(X);
+    // per slot only the 16-byte vectors holding its NEQ^2 values are loaded
+    // (the vector shared by two slots is read twice, from cache)
+    constexpr int NW = (BB + 2 * V - 2) / V;
+    for (long g = g0; g < g1; g++) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int v0 = (t * BB) / V, off = t * BB - v0 * V;
+            vec_t bw[NW];
+#pragma unroll
+            for (int i = 0; i < NW; i++)
+                if (v0 + i < NVEC) bw[i] = __builtin_nontemporal_load(Bv + (size_t)(g * NVEC + v0 + i) * 64);
+            const int cj = cols[(g * 4 + t) * 64 + lane];
+            vec_t xv[NEQ][2];
+#pragma unroll
+            for (int q = 0; q < NEQ; q++) {
+                xv[q][0] = Xv[((size_t)cj * NEQ + q) * 2];
+                xv[q][1] = Xv[((size_t)cj * NEQ + q) * 2 + 1];
+            }
+#pragma unroll
+            for (int r = 0; r < NEQ; r++)
+#pragma unroll
+                for (int q = 0; q < NEQ; q++) {
+                    const int f = off + r * NEQ + q;
+                    const float b = bw[f / V][f % V];
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        acc[r][v] += b * xv[q][0][v];
+                        acc[r][4 + v] += b * xv[q][1][v];
+                    }
+                }
+        }
+    }
+    const long row = c * 64 + lane;
+    if (row < n_node) {
+        vec_t* Yv = reinterpret_cast<vec_t*>(Y);
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) {
+            Yv[((size_t)row * NEQ + r) * 2] = vec_t{acc[r][0], acc[r][1], acc[r][2], acc[r][3]};
+            Yv[((size_t)row * NEQ + r) * 2 + 1] = vec_t{acc[r][4], acc[r][5], acc[r][6], acc[r][7]};
+        }
+    }
+}
+
 // rows from the FP64 element blocks (chunked layout of blk_offset: the
 // entries of an element row come in 16-byte pairs).  A block row of NEQ
 // entries starting at column j0 = M NEQ is read as whole pairs: NEQ / 2 of
@@ -215,6 +282,15 @@ void launch_rows_matvec(const RowsArgs& a, const double* B64, const float* B32, 
         else L(float, double, double, 4, B32);
     }
 #undef L
+}
+
+void launch_rows_matvec8(const RowsArgs& a, const float* B32, const float* X, float* Y, hipStream_t st) {
+    if (a.n_chunks <= 0) return;
+    const dim3 grid(row_grid(a.n_chunks)), blk(256);
+    if (a.neqn == 5)
+        hipLaunchKernelGGL((rows_matvec8_kernel<5>), grid, blk, 0, st, a.n_node, a.n_chunks, a.goff, a.cols, B32, X, Y);
+    else
+        hipLaunchKernelGGL((rows_matvec8_kernel<4>), grid, blk, 0, st, a.n_node, a.n_chunks, a.goff, a.cols, B32, X, Y);
 }
 
 void launch_rows_assemble(const RowsArgs& a, const double* R64, double* B64, float* B32, hipStream_t st) {

@@ -54,13 +54,13 @@ def main(argv=None):
     out["build_wall_ms"] = (time.perf_counter() - t0) * 1e3
     ne, nn = mesh.n_elem, mesh.n_node
     k = mesh.k
+    nnzb = (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1)
     for prec in ((32, 64) if a.precision == "both" else (int(a.precision),)):
         ms = ctx.profile_matvec(prec, a.reps)
         w = prec // 8
         if rows:
             # node blocks of the structured node grid: (3nx+1)(3ny+1)(3nz+1),
             # 25 values and a column index each; x read and y written once
-            nnzb = (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1)
             nbytes = nnzb * (mesh.neqn ** 2 * w + 4) + nn * mesh.neqn * 2 * w
         else:
             nbytes = ne * k * k * w + nn * mesh.neqn * 2 * w
@@ -68,7 +68,10 @@ def main(argv=None):
         out[f"matvec{prec}_bytes"] = nbytes
         out[f"matvec{prec}_GBps"] = nbytes / (ms * 1e-3) / 1e9
     ms8 = ctx.profile_matvec_multi(a.reps)
-    b8 = ne * k * k * 4 + nn * mesh.neqn * 8 * 2 * 4
+    if rows:
+        b8 = nnzb * (mesh.neqn ** 2 * 4 + 4) + nn * mesh.neqn * 8 * 2 * 4
+    else:
+        b8 = ne * k * k * 4 + nn * mesh.neqn * 8 * 2 * 4
     out["matvec32x8_ms"] = ms8
     out["matvec32x8_bytes"] = b8
     out["matvec32x8_GBps"] = b8 / (ms8 * 1e-3) / 1e9

@@ -211,9 +211,12 @@ int cfdv_profile_matvec(cfdv_ctx *ctx, int precision, int reps, double *ms);
 int cfdv_step(cfdv_ctx *ctx, double cfl, const cfdv_solve_opts *opts, cfdv_step_stats *st);
 /* Y = K X (or K^T X) for 8 vectors in one pass over the FP32 blocks (no
    reference counterpart): X, Y host arrays [n_node*neqn][8] (the 8 values
-   of a dof contiguous); nvec must be 8; hex8 meshes with FP32 blocks */
+   of a dof contiguous); nvec must be 8; hex8 meshes.  K X runs over the
+   FP32 node rows when the context holds them, else element by element with
+   CFDV_BLOCKS_FP32; K^T X needs CFDV_BLOCKS_FP32 */
 int cfdv_matvec_multi(cfdv_ctx *ctx, const float *X, float *Y, int nvec, int transpose);
-/* average time of one 8-vector product (all colors), HIP events */
+/* average time of one 8-vector product K X (node rows, or EBE over all
+   colors), HIP events */
 int cfdv_profile_matvec_multi(cfdv_ctx *ctx, int reps, double *ms);
 
 /* ---- discrete adjoint of one step (no reference counterpart) ----------

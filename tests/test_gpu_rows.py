@@ -206,3 +206,26 @@ def test_warm_start_reaches_the_same_steps(gpu_available, mode):
     sw = warm.step(0.5, mode=mode, warm_start=True, **kw)
     sc = cold.step(0.5, mode=mode, **kw)
     assert np.array_equal(warm.get_state()[0], cold.get_state()[0])
+
+
+@pytest.mark.parametrize("case", ["box3d", "shuffled", "refined"])
+def test_rows_eight_vector_product(gpu_available, case):
+    """The 8-vector product over the FP32 rows is, vector by vector, the
+    single FP32 row product bit for bit (same sums in the same order), and
+    within 1e-6 of the oracle's EBE product; K^T X without FP32 element
+    blocks is refused."""
+    m, th, U = _case(case)
+    ctx = FDVContext(m, th, U, blocks=("fp64", "rows"))
+    ctx.build(ctx.cfl(0.5))
+    R, _, _ = ctx.blocks()
+    rng = np.random.default_rng(21)
+    X = rng.standard_normal((ctx.ndof, 8)).astype(np.float32)
+    Y = ctx.matvec_multi(X)
+    for v in range(8):
+        x = X[:, v].astype(np.float64)
+        y1 = ctx.matvec(x, precision=32)
+        assert np.array_equal(Y[:, v], y1.astype(np.float32))
+        yo = O.matvec(m, R, x)
+        assert np.linalg.norm(Y[:, v] - yo) <= 2e-6 * np.linalg.norm(yo)
+    with pytest.raises(CFDVError):
+        ctx.matvec_multi(X, transpose=True)
