@@ -37,6 +37,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+BUILD_FLOP_PER_ELEMENT = 0.70541e6
 
 # HBM bytes per full product measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # on MI355X, each scaled by the counter's reading for a plain 1 GiB copy
@@ -193,6 +195,7 @@ def main():
     ms64 = sum(s.ms_matvec64 for s in st_64)
     n64 = sum(s.n_matvec64 for s in st_64)
     ach32 = b32 * n32 / (ms32 * 1e-3) / 1e9 if ms32 > 0 else None
+    ms_build = float(np.mean([s.ms_build for s in st_mx]))
     ach64 = b64 * n64 / (ms64 * 1e-3) / 1e9 if ms64 > 0 else None
 
     cpu = None
@@ -246,6 +249,13 @@ def main():
                      "kernel": f"{kname}, {b32} algorithmic bytes per full product ({per})"},
         "roofline_fp64_matvec": {"achieved": ach64, "frac": (ach64 / HBM_PEAK_GBS) if ach64 else None,
                                  "bytes_per_matvec": b64, "traffic": traffic.get("fp64")},
+        # the element build (interior and face kernels): algorithmic FP64
+        # flops of the contraction form, 0.705 MFLOP per hex8 element
+        # (counted on the all-VALU kernel: 64 (2 FMA + MUL + ADD) per wave)
+        "roofline_build": {"bound": "fp64", "unit": "TFLOP/s", "peak": FP64_PEAK_TFS,
+                           "achieved": BUILD_FLOP_PER_ELEMENT * ne / (ms_build * 1e-3) / 1e12,
+                           "frac": BUILD_FLOP_PER_ELEMENT * ne / (ms_build * 1e-3) / 1e12 / FP64_PEAK_TFS,
+                           "ms": ms_build},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -4,10 +4,12 @@
 // The reference multiplies element by element (MPI_GMRES.h:442-566 sums
 // R_e x_e over the elements).  A hex8 element holds 64 blocks of 5x5; summed
 // per node pair an interior node has 27 blocks, so one product over the
-// assembled rows moves 27 * 25 values per node instead of 64 * 25 per
-// element: 2.4x fewer bytes for the same operator.  The rows are formed
-// after every build from the FP64 element blocks (each element block is read
-// once) and stored in FP64 and FP32.
+// assembled rows moves 27 * 25 values (and 27 column indices) per node
+// instead of 64 * 25 per element: 2.2x fewer bytes for the same operator.
+// The rows are formed after every build from the FP64 element blocks (each
+// element block is read once) and stored in FP64 and FP32.  Kernels here:
+// the assembly, K x (FP64, FP32, FP32 blocks with FP64 vectors), K X for 8
+// vectors, and the node-block inverses from the diagonal slots.
 //
 // Layout.  Rows come in chunks of 64 nodes (lane = node % 64).  Chunk c has
 // G_c groups of 4 slots (slot = one neighbour node of the row, ascending
