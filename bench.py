@@ -57,7 +57,9 @@ def log(*a):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
+    # a time loop of 10 steps from the initial state (the first solve cold,
+    # the rest warm-started)
+    p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--size", default="256x128x64")
     p.add_argument("--cfl", type=float, default=0.5)

@@ -229,3 +229,22 @@ def test_rows_eight_vector_product(gpu_available, case):
         assert np.linalg.norm(Y[:, v] - yo) <= 2e-6 * np.linalg.norm(yo)
     with pytest.raises(CFDVError):
         ctx.matvec_multi(X, transpose=True)
+
+
+def test_rows_with_an_unused_node(gpu_available):
+    """A node no element references gets a row of its own column only (a
+    zero block): K x is zero there, as element by element."""
+    m, th, U = M.mixed_bc_box(3, 3, 2, seed=3)
+    x = np.vstack([m.x, m.x[-1:] + 0.5])
+    d = np.vstack([m.dirichlet, np.zeros((1, m.neqn), m.dirichlet.dtype)])
+    m2 = M.Mesh(3, 8, 5, m.conn, x, m.bc, d)
+    U2 = np.vstack([U, U[-1:]])
+    rows = FDVContext(m2, th, U2, blocks=("fp64", "rows"))
+    ebe = FDVContext(m2, th, U2)
+    dt = rows.cfl(0.5)
+    rows.build(dt)
+    ebe.build(dt)
+    v = np.random.default_rng(5).standard_normal(m2.n_node * m2.neqn)
+    yr, ye = rows.matvec(v), ebe.matvec(v)
+    assert np.all(yr[-m2.neqn:] == 0.0)
+    assert np.linalg.norm(yr - ye) <= 1e-13 * np.linalg.norm(ye)
