@@ -35,6 +35,60 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// FDVParam (src/FDV/Function/FDVParam.h:15-111) of one hex8 element over
+// the 64 lanes of its wavefront, every lane active: lane l evaluates node
+// l % 8's Mach and Reynolds numbers, each octet reduces its extremes (lanes
+// 0-31 the Mach numbers, 32-63 the Reynolds numbers), and both halves run the
+// r / s tail on the same instructions (one pow instead of two).  Min and max
+// are exact, so for finite states s equals fdvparam<3, 8>'s bit for bit
+// (element3d.h), which evaluates the eight nodes one after the other on
+// every lane.
+__device__ void fdvparam_wave(const DevThermo& th, const double (*nd)[ND_STRIDE], int lane, double s[4],
+                              int* tie) {
+#pragma clang fp contract(off)
+    const int beta = lane & 7;
+    double dd = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) dd += nd[beta][ND_V + i] * nd[beta][ND_V + i];
+    const double V = sqrt(dd);
+    double T = nd[beta][ND_T];
+    if (T < 1.0E-12) T = 0;
+    const double a = sqrt(th.gamma * th.cgas * T);
+    double M = 0;
+    if (!(a < 1.0E-12)) M = V / a;
+    const double mu = calc_mu(th, T);
+    const double Re = (nd[beta][ND_RHO] * V) / mu;
+    const bool re_half = lane >= 32;
+    double hi = re_half ? Re : M, lo = hi;
+#pragma unroll
+    for (int d = 1; d < 8; d <<= 1) {
+        const double oh = __shfl_xor(hi, d, 64), ol = __shfl_xor(lo, d, 64);
+        hi = (hi < oh) ? oh : hi;
+        lo = (ol < lo) ? ol : lo;
+    }
+    const double eps = 0.0000001;
+    const double r = sqrt(hi * hi - lo * lo) / (lo + eps);
+    double sv = 0.0;
+    int t = 0;
+    if (r > 0.01) sv = (1.0 < r) ? 1.0 : r;
+    else if (r < 0.01 && lo >= 10E-10) sv = 0.0;
+    else if (re_half ? (lo <= 10E-10) : (lo < 10E-10)) sv = 1.0;
+    else t = 1;
+    const double pw = 0.5 * (1 + pow(sv, 0.2));
+    s[0] = readlane_f64(sv, 0);
+    s[1] = readlane_f64(pw, 0);
+    s[2] = readlane_f64(sv, 32);
+    s[3] = readlane_f64(pw, 32);
+    *tie = __builtin_amdgcn_readlane(t, 0) | __builtin_amdgcn_readlane(t, 32);
+}
+
 // eval1dboundary (src/FDV/Function/eval1dboundary.h:9-251)
 CFDV_HD bool eval1dboundary(int bc, GPState<2>& f, const double x0[2], const double x1[2],
                                const double en[2]) {
@@ -565,7 +619,7 @@ __device__ __forceinline__ void mfma_as_product(const GPTab* tabs, double dt, in
 }
 
 template <bool W64, bool W32>
-__global__ __launch_bounds__(64) void build3d_interior_kernel(BuildArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void build3d_interior_kernel(BuildArgs A) {
     // one wavefront (workgroup) per element slot
     constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600;
     __shared__ double s_nd[NNOD][ND_STRIDE];
@@ -605,17 +659,7 @@ __global__ __launch_bounds__(64) void build3d_interior_kernel(BuildArgs A) {
     __syncthreads();
     double sp[4];
     int tie = 0;
-    {
-        double vn[NNOD][3], Tn[NNOD], rhon[NNOD];
-#pragma unroll
-        for (int n = 0; n < NNOD; n++) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) vn[n][i] = s_nd[n][ND_V + i];
-            Tn[n] = s_nd[n][ND_T];
-            rhon[n] = s_nd[n][ND_RHO];
-        }
-        fdvparam<3, 8>(th, vn, Tn, rhon, sp, &tie);
-    }
+    fdvparam_wave(th, s_nd, lane, sp, &tie);
     // phase 1: lanes (part, gp) = (lane / 8, lane % 8), part < 4: the
     // Gauss-point state (evaluated by the four lanes of a point alike), then
     // direction 0, 1, 2 of the Jacobian combinations or the common part
@@ -978,17 +1022,7 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
     const DevThermo th = A.th;
     double sp[4];
     int tie = 0;
-    {
-        double vn[NNOD][3], Tn[NNOD], rhon[NNOD];
-#pragma unroll
-        for (int n = 0; n < NNOD; n++) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) vn[n][i] = s_nd[n][ND_V + i];
-            Tn[n] = s_nd[n][ND_T];
-            rhon[n] = s_nd[n][ND_RHO];
-        }
-        fdvparam<3, 8>(th, vn, Tn, rhon, sp, &tie);
-    }
+    fdvparam_wave(th, s_nd, lane, sp, &tie);
     const int N = lane >> 3, M = lane & 7;
     const int* bcs = A.bc + e * 6;
     int nfa = 0;
