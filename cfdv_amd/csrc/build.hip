@@ -371,13 +371,6 @@ constexpr int JPAT_R[3][17] = {{0, 1, 1, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 4, 
 constexpr int JPAT_S[3][17] = {{1, 0, 1, 2, 3, 4, 0, 1, 2, 0, 1, 3, 0, 1, 2, 3, 4},
                                {2, 0, 1, 2, 0, 1, 2, 3, 4, 0, 2, 3, 0, 1, 2, 3, 4},
                                {3, 0, 1, 3, 0, 2, 3, 0, 1, 2, 3, 4, 0, 1, 2, 3, 4}};
-// (r, s) is a structural nonzero of some direction
-CFDV_HD constexpr bool jnz(int r, int s) {
-    for (int I = 0; I < 3; I++)
-        for (int p = 0; p < 17; p++)
-            if (JPAT_R[I][p] == r && JPAT_S[I][p] == s) return true;
-    return false;
-}
 
 // Per Gauss point, what the node-pair contraction reads: the direction
 // combinations M1_I = s1 a_I + s3 b_I on their 17 nonzeros, A_I = a_I + b_I
@@ -393,32 +386,71 @@ struct GPTab {
     double c_mu, c_lambda, c_muR, c_rinv, c_denom, c_l, c_m, c_n, c_TERM1, c_hkocv, c_en;
 };
 
-// Phase 1b, direction I of one Gauss point (one lane)
-template <int I>
-CFDV_HD void gp_tab_dir(const DevThermo& th, const GPShared& g, const double sp[4], GPTab& t) {
+// Equation index x of direction 0 seen from direction I: momentum rows 1
+// and 1 + I swap (I = 0: identity); pi_ax does the same for the axes 0 and I.
+CFDV_HD constexpr int sig_eq(int I, int x) { return x == 1 ? 1 + I : (x == 1 + I ? 1 : x); }
+CFDV_HD constexpr int pi_ax(int I, int k) { return k == 0 ? I : (k == I ? 0 : k); }
+template <typename T>
+CFDV_HD T pick3(int I, T x0, T x1, T x2) { return I == 1 ? x1 : (I == 2 ? x2 : x0); }
+
+// Phase 1b, direction I (0..2, at run time) of one Gauss point (one lane),
+// so that the lanes of the three directions run the same instructions.
+// a_I and b_I are direction 0's formulas (calc_a3_dir<0>, calc_b3_dir<0>)
+// on the state with axes 0 and I swapped -- velocities, momenta, derivative
+// directions and tau -- whose entry (r, c) is entry (sig_eq(I, r),
+// sig_eq(I, c)) of a_I, b_I (the Jacobians are covariant under that swap,
+// NavierStokes3D.h:74-343).  Stored rotated, read with the swap folded into
+// the (compile-time) indices: M1[I][p] is entry (sig_eq(I, JPAT_R[0][p]),
+// sig_eq(I, JPAT_S[0][p])) of M1_I, AD[I][r] is row sig_eq(I, r) of A_I D.
+// Af and Sf, which the matrix-core product reads at lane-dependent indices,
+// are stored in their own orientation.
+CFDV_HD void gp_tab_dir_rot(const DevThermo& th, const GPShared& g, const double sp[4], int I, GPTab& t) {
+    const GPState<3>& f0 = g.f;
+    GPState<3> f;   // the members calc_a3_dir<0> and calc_b3_dir<0> read
+    f.E = f0.E;
+    f.mu = f0.mu;
+    f.k = f0.k;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        f.v[k] = pick3(I, f0.v[pi_ax(0, k)], f0.v[pi_ax(1, k)], f0.v[pi_ax(2, k)]);
+        f.tau[0][k] = pick3(I, f0.tau[0][pi_ax(0, k)], f0.tau[1][pi_ax(1, k)], f0.tau[2][pi_ax(2, k)]);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        f.U[q] = pick3(I, f0.U[sig_eq(0, q)], f0.U[sig_eq(1, q)], f0.U[sig_eq(2, q)]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (q == 4 && k > 0) continue;   // calc_b3_dir<0> reads dU[4][0] only
+            f.dU[q][k] = pick3(I, f0.dU[sig_eq(0, q)][pi_ax(0, k)], f0.dU[sig_eq(1, q)][pi_ax(1, k)],
+                               f0.dU[sig_eq(2, q)][pi_ax(2, k)]);
+        }
+    }
     double a[5][5], b[5][5];
-    calc_a3_dir<I>(th, g.f, a);
-    calc_b3_dir<I>(th, g.f, b);
+    calc_a3_dir<0>(th, f, a);
+    calc_b3_dir<0>(th, f, b);
     const double s1 = sp[0], s2 = sp[1], s3 = sp[2], s4 = sp[3];
 #pragma unroll
     for (int p = 0; p < 17; p++) {
-        const int r = JPAT_R[I][p], c = JPAT_S[I][p];
+        const int r = JPAT_R[0][p], c = JPAT_S[0][p];
         t.M1[I][p] = s1 * a[r][c] + s3 * b[r][c];
     }
+    const int sg[5] = {0, 1 + I, I == 1 ? 1 : 2, I == 2 ? 1 : 3, 4};   // sig_eq(I, .)
 #pragma unroll
     for (int r = 0; r < 5; r++)
 #pragma unroll
         for (int c = 0; c < 5; c++) {
-            t.Af[I][r * 5 + c] = a[r][c] + b[r][c];
-            t.Sf[I][r * 5 + c] = s2 * a[r][c] + s4 * b[r][c];
+            t.Af[I][sg[r] * 5 + sg[c]] = a[r][c] + b[r][c];
+            t.Sf[I][sg[r] * 5 + sg[c]] = s2 * a[r][c] + s4 * b[r][c];
         }
+    double D[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) D[c] = pick3(I, g.D[sig_eq(0, c)], g.D[sig_eq(1, c)], g.D[sig_eq(2, c)]);
 #pragma unroll
     for (int r = 0; r < 5; r++) {
         double q = 0.0;
 #pragma unroll
-        for (int c = 0; c < 5; c++) q += (a[r][c] + b[r][c]) * g.D[c];
+        for (int c = 0; c < 5; c++) q += (a[r][c] + b[r][c]) * D[c];
         t.AD[I][r] = q;
-        t.P[I][r] = g.P[I][r];
     }
 }
 
@@ -426,6 +458,10 @@ CFDV_HD void gp_tab_dir(const DevThermo& th, const GPShared& g, const double sp[
 // the scalars are accum_c3's preamble
 CFDV_HD void gp_tab_common(const DevThermo& th, const GPShared& g, GPTab& t) {
     t.wJ = g.wJ;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int r = 0; r < 5; r++) t.P[i][r] = g.P[i][r];
     const GPState<3>& f = g.f;
     const double mu = f.mu;
     const double lambda = -2. * mu / 3.;
@@ -533,7 +569,8 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs
 #pragma unroll
         for (int I = 0; I < 3; I++)
 #pragma unroll
-            for (int p = 0; p < 17; p++) R[JPAT_R[I][p]][JPAT_S[I][p]] += wg[I] * g.M1[I][p];
+            for (int p = 0; p < 17; p++)
+                R[sig_eq(I, JPAT_R[0][p])][sig_eq(I, JPAT_S[0][p])] += wg[I] * g.M1[I][p];
         // rhs: P and the A D product (contract3's AQ)
 #pragma unroll
         for (int r = 0; r < NEQ; r++) {
@@ -541,7 +578,7 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 if (r > 0) acc += wr[i] * g.P[i][r];
-                aq += wr[i] * g.AD[i][r];
+                aq += wr[i] * g.AD[i][sig_eq(i, r)];
             }
             rhs[r] += acc - dtsq_2 * aq;
         }
@@ -667,9 +704,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         const int gp = lane & 7, part = lane >> 3;
         GPShared g;
         gp_phase1(th, s_g, s_nd, gp, A.dt, g, part == 3 ? s_tab[gp].dphi : nullptr);
-        if (part == 0) gp_tab_dir<0>(th, g, sp, s_tab[gp]);
-        else if (part == 1) gp_tab_dir<1>(th, g, sp, s_tab[gp]);
-        else if (part == 2) gp_tab_dir<2>(th, g, sp, s_tab[gp]);
+        if (part < 3) gp_tab_dir_rot(th, g, sp, part, s_tab[gp]);
         else gp_tab_common(th, g, s_tab[gp]);
     }
     __syncthreads();
@@ -952,7 +987,8 @@ __device__ void face_mats(const DevThermo& th, const FaceTab& F, int part, doubl
 #pragma unroll
         for (int I = 0; I < 3; I++)
 #pragma unroll
-            for (int p = 0; p < 17; p++) G[JPAT_R[I][p]][JPAT_S[I][p]] += F.en[I] * t.M1[I][p];
+            for (int p = 0; p < 17; p++)
+                G[sig_eq(I, JPAT_R[0][p])][sig_eq(I, JPAT_S[0][p])] += F.en[I] * t.M1[I][p];
 #pragma unroll
         for (int r = 0; r < NEQ; r++)
 #pragma unroll
@@ -964,7 +1000,7 @@ __device__ void face_mats(const DevThermo& th, const FaceTab& F, int part, doubl
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 if (r > 0) acc += F.en[i] * t.P[i][r];
-                aq += F.en[i] * t.AD[i][r];
+                aq += F.en[i] * t.AD[i][sig_eq(i, r)];
             }
             out.Q[r] = -acc + dtsq_2 * aq;
         }
@@ -1061,9 +1097,7 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
                 if (st) atomicOr(&s_status, st);
             }
             if (ok) {
-                if (part == 0) gp_tab_dir<0>(th, g, sp, F.t);
-                else if (part == 1) gp_tab_dir<1>(th, g, sp, F.t);
-                else if (part == 2) gp_tab_dir<2>(th, g, sp, F.t);
+                if (part < 3) gp_tab_dir_rot(th, g, sp, part, F.t);
                 else gp_tab_common(th, g, F.t);
             }
         }
