@@ -373,17 +373,36 @@ constexpr int JPAT_S[3][17] = {{1, 0, 1, 2, 3, 4, 0, 1, 2, 0, 1, 3, 0, 1, 2, 3, 
                                {3, 0, 1, 3, 0, 2, 3, 0, 1, 2, 3, 4, 0, 1, 2, 3, 4}};
 
 // Per Gauss point, what the node-pair contraction reads: the direction
-// combinations M1_I = s1 a_I + s3 b_I on their 17 nonzeros, A_I = a_I + b_I
-// and S_I = s2 a_I + s4 b_I as full row-major 5x5 (operands of the matrix-core
-// product), the rhs pieces and the scalars of c_ij.
+// combinations M1_I = s1 a_I + s3 b_I on their 17 nonzeros (less the first,
+// a_I(0, 1 + I) = 1 and b_I(0, .) = 0 make it s1), A_I = a_I + b_I and
+// S_I = s2 a_I + s4 b_I as full row-major 5x5 (operands of the matrix-core
+// product), the rhs vectors PQ_I = P_I - dt^2/2 A_I D and the distinct
+// entries of the c_ij / Re blocks (CT_* below).
+enum {
+    CT_R0 = 0,    // [k] c_ii(1+k, 0), k == i:  mu_R Uc_k / rho^2
+    CT_M0 = 3,    // [k] c_ii(1+k, 0), k != i, and c_ij(1+j, 0) (k = i):  mu Uc_k / rho^2
+    CT_RR = 6,    // c_ii(1+k, 1+k), k == i:  -mu_R / rho
+    CT_MR = 7,    // c_ii(1+k, 1+k), k != i, and c_ij(1+j, 1+i):  -mu / rho
+    CT_40 = 8,    // [i] c_ii(4, 0)
+    CT_4R = 11,   // [k] c_ii(4, 1+k), k == i
+    CT_4M = 14,   // [k] c_ii(4, 1+k), k != i
+    CT_44 = 17,   // c_ii(4, 4)
+    CT_L0 = 18,   // [j] c_ij(1+i, 0):  lambda Uc_j / rho^2
+    CT_LR = 21,   // c_ij(1+i, 1+j):  -lambda / rho
+    CT_O40 = 22,  // [ct_pair(i, j)] c_ij(4, 0), i != j
+    CT_O4I = 28,  // [j] c_ij(4, 1+i)
+    CT_O4J = 31,  // [i] c_ij(4, 1+j)
+    CT_N = 34
+};
+// (i, j), i != j, numbered 0..5 in row-major order
+CFDV_HD constexpr int ct_pair(int i, int j) { return 2 * i + (j > i ? j - 1 : j); }
 struct GPTab {
     double dphi[8][3];
     double wJ;
-    double M1[3][17];
+    double M1[3][16];   // pattern slots 1..16
     double Af[3][25], Sf[3][25];
-    double P[3][5];     // P(i, r), r = 1..4
-    double AD[3][5];    // (A_i D)(r)
-    double c_mu, c_lambda, c_muR, c_rinv, c_denom, c_l, c_m, c_n, c_TERM1, c_hkocv, c_en;
+    double PQ[3][5];
+    double C[CT_N];
 };
 
 // Equation index x of direction 0 seen from direction I: momentum rows 1
@@ -401,10 +420,12 @@ CFDV_HD T pick3(int I, T x0, T x1, T x2) { return I == 1 ? x1 : (I == 2 ? x2 : x
 // sig_eq(I, c)) of a_I, b_I (the Jacobians are covariant under that swap,
 // NavierStokes3D.h:74-343).  Stored rotated, read with the swap folded into
 // the (compile-time) indices: M1[I][p] is entry (sig_eq(I, JPAT_R[0][p]),
-// sig_eq(I, JPAT_S[0][p])) of M1_I, AD[I][r] is row sig_eq(I, r) of A_I D.
+// sig_eq(I, JPAT_S[0][p])) of M1_I (slot p - 1), PQ[I][r] is row sig_eq(I, r)
+// of PQ_I.
 // Af and Sf, which the matrix-core product reads at lane-dependent indices,
 // are stored in their own orientation.
-CFDV_HD void gp_tab_dir_rot(const DevThermo& th, const GPShared& g, const double sp[4], int I, GPTab& t) {
+CFDV_HD void gp_tab_dir_rot(const DevThermo& th, const GPShared& g, const double sp[4], double dt, int I,
+                            GPTab& t) {
     const GPState<3>& f0 = g.f;
     GPState<3> f;   // the members calc_a3_dir<0> and calc_b3_dir<0> read
     f.E = f0.E;
@@ -430,9 +451,9 @@ CFDV_HD void gp_tab_dir_rot(const DevThermo& th, const GPShared& g, const double
     calc_b3_dir<0>(th, f, b);
     const double s1 = sp[0], s2 = sp[1], s3 = sp[2], s4 = sp[3];
 #pragma unroll
-    for (int p = 0; p < 17; p++) {
+    for (int p = 1; p < 17; p++) {
         const int r = JPAT_R[0][p], c = JPAT_S[0][p];
-        t.M1[I][p] = s1 * a[r][c] + s3 * b[r][c];
+        t.M1[I][p - 1] = s1 * a[r][c] + s3 * b[r][c];
     }
     const int sg[5] = {0, 1 + I, I == 1 ? 1 : 2, I == 2 ? 1 : 3, 4};   // sig_eq(I, .)
 #pragma unroll
@@ -445,89 +466,96 @@ CFDV_HD void gp_tab_dir_rot(const DevThermo& th, const GPShared& g, const double
     double D[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) D[c] = pick3(I, g.D[sig_eq(0, c)], g.D[sig_eq(1, c)], g.D[sig_eq(2, c)]);
+    const double dtsq_2 = (dt * dt) / 2.0;
 #pragma unroll
     for (int r = 0; r < 5; r++) {
         double q = 0.0;
 #pragma unroll
         for (int c = 0; c < 5; c++) q += (a[r][c] + b[r][c]) * D[c];
-        t.AD[I][r] = q;
+        const double P = r == 0 ? 0.0 : pick3(I, g.P[0][sig_eq(0, r)], g.P[1][sig_eq(1, r)], g.P[2][sig_eq(2, r)]);
+        t.PQ[I][r] = P - dtsq_2 * q;
     }
 }
 
-// Phase 1b, w det J and the c_ij scalars of one Gauss point (one lane);
-// the scalars are accum_c3's preamble
+// Phase 1b, w det J and the c_ij / Re table of one Gauss point (one lane):
+// accum_c3's coefficients (NavierStokes3D.h:347-477), each distinct value once
 CFDV_HD void gp_tab_common(const DevThermo& th, const GPShared& g, GPTab& t) {
     t.wJ = g.wJ;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int r = 0; r < 5; r++) t.P[i][r] = g.P[i][r];
     const GPState<3>& f = g.f;
+    const double reinv = 1.0 / th.creyn;
     const double mu = f.mu;
     const double lambda = -2. * mu / 3.;
+    const double mu_R = 2. * mu + lambda;
     const double rho = f.U[0];
     const double rinv = 1.0 / rho;
     const double denom = 1. / (rho * rho);
-    const double l = f.U[1], m = f.U[2], n = f.U[3], re = f.U[4];
+    const double Uc[3] = {f.U[1], f.U[2], f.U[3]};
+    const double re = f.U[4];
+    const double hkocv = f.k / th.Cv;
     const double TERM1 = f.k / th.Cv * denom;
-    t.c_mu = mu;
-    t.c_lambda = lambda;
-    t.c_muR = 2. * mu + lambda;
-    t.c_rinv = rinv;
-    t.c_denom = denom;
-    t.c_l = l;
-    t.c_m = m;
-    t.c_n = n;
-    t.c_TERM1 = TERM1;
-    t.c_hkocv = f.k / th.Cv;
-    t.c_en = TERM1 * (-re + (l * l + m * m + n * n) * rinv);
-}
-
-// accum_c3 (NavierStokes3D.h:347-477 contracted) from the tabulated scalars
-CFDV_HD void accum_c3_tab(const DevThermo& th, const GPTab& t, const double W[3][3], double out[5][5]) {
-    const double mu = t.c_mu, lambda = t.c_lambda, mu_R = t.c_muR;
-    const double rinv = t.c_rinv, denom = t.c_denom;
-    const double l = t.c_l, m = t.c_m, n = t.c_n;
-    const double hkocv = t.c_hkocv, TERM1 = t.c_TERM1, en = t.c_en;
-    const double reinv = 1.0 / th.creyn;
-    const double lr = l * rinv, mr = m * rinv, nr = n * rinv;
+    const double en = TERM1 * (-re + (Uc[0] * Uc[0] + Uc[1] * Uc[1] + Uc[2] * Uc[2]) * rinv);
+    const double Ur[3] = {Uc[0] * rinv, Uc[1] * rinv, Uc[2] * rinv};
+    double* C = t.C;
+    double cR0[3], cM0[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        cR0[k] = mu_R * Uc[k] * denom;
+        cM0[k] = mu * Uc[k] * denom;
+        C[CT_R0 + k] = cR0[k] * reinv;
+        C[CT_M0 + k] = cM0[k] * reinv;
+        C[CT_L0 + k] = lambda * Uc[k] * denom * reinv;
+    }
+    const double cRR = -mu_R * rinv, cMR = -mu * rinv, cLR = -lambda * rinv;
+    C[CT_RR] = cRR * reinv;
+    C[CT_MR] = cMR * reinv;
+    C[CT_LR] = cLR * reinv;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const double MUX = (i == 0) ? mu_R : mu, MUY = (i == 1) ? mu_R : mu, MUZ = (i == 2) ? mu_R : mu;
-        double c10 = MUX * l * denom, c11 = (-MUX * rinv);
-        double c20 = MUY * m * denom, c22 = (-MUY * rinv);
-        double c30 = MUZ * n * denom, c33 = (-MUZ * rinv);
-        const double c40 = (l * c10 + m * c20 + n * c30) * rinv - en;
-        const double c41 = lr * c11 + TERM1 * l;
-        const double c42 = mr * c22 + TERM1 * m;
-        const double c43 = nr * c33 + TERM1 * n;
-        const double c44 = (-hkocv * rinv);
-        const double wgt = W[i][i] * reinv;
-        out[1][0] += wgt * c10; out[1][1] += wgt * c11;
-        out[2][0] += wgt * c20; out[2][2] += wgt * c22;
-        out[3][0] += wgt * c30; out[3][3] += wgt * c33;
-        out[4][0] += wgt * c40; out[4][1] += wgt * c41; out[4][2] += wgt * c42;
-        out[4][3] += wgt * c43; out[4][4] += wgt * c44;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) acc += Uc[k] * (k == i ? cR0[k] : cM0[k]);
+        C[CT_40 + i] = (acc * rinv - en) * reinv;
+        C[CT_4R + i] = (Ur[i] * cRR + TERM1 * Uc[i]) * reinv;
+        C[CT_4M + i] = (Ur[i] * cMR + TERM1 * Uc[i]) * reinv;
+        C[CT_O4I + i] = Ur[i] * cMR * reinv;
+        C[CT_O4J + i] = Ur[i] * cLR * reinv;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            if (j != i) C[CT_O40 + ct_pair(i, j)] = (Ur[i] * (lambda * Uc[j] * denom) + Ur[j] * cM0[i]) * reinv;
     }
-    const double Uc[3] = {l, m, n};
-    const double Ur[3] = {lr, mr, nr};
+    C[CT_44] = -hkocv * rinv * reinv;
+}
+
+// accum_c3 (NavierStokes3D.h:347-477) contracted with the weights W from
+// the tabulated coefficients:  out[r][s] += sum_ij W[i][j] c_ij[r][s] / Re,
+// the diagonal blocks first, then the off-diagonal pairs, as accum_c3.
+CFDV_HD void accum_c3_ctab(const double* C, const double W[3][3], double out[5][5]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double w = W[i][i];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            out[1 + k][0] += w * C[(k == i ? CT_R0 : CT_M0) + k];
+            out[1 + k][1 + k] += w * C[k == i ? CT_RR : CT_MR];
+        }
+        out[4][0] += w * C[CT_40 + i];
+#pragma unroll
+        for (int k = 0; k < 3; k++) out[4][1 + k] += w * C[(k == i ? CT_4R : CT_4M) + k];
+        out[4][4] += w * C[CT_44];
+    }
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             if (i == j) continue;
-            const double wgt = W[i][j] * reinv;
-            const double c_i0 = lambda * Uc[j] * denom;
-            const double c_ij = (-lambda * rinv);
-            const double c_j0 = mu * Uc[i] * denom;
-            const double c_ji = (-mu * rinv);
-            out[1 + i][0] += wgt * c_i0;
-            out[1 + i][1 + j] += wgt * c_ij;
-            out[1 + j][0] += wgt * c_j0;
-            out[1 + j][1 + i] += wgt * c_ji;
-            out[4][0] += wgt * (Ur[i] * c_i0 + Ur[j] * c_j0);
-            out[4][1 + i] += wgt * (Ur[j] * c_ji);
-            out[4][1 + j] += wgt * (Ur[i] * c_ij);
+            const double w = W[i][j];
+            out[1 + i][0] += w * C[CT_L0 + j];
+            out[1 + i][1 + j] += w * C[CT_LR];
+            out[1 + j][0] += w * C[CT_M0 + i];
+            out[1 + j][1 + i] += w * C[CT_MR];
+            out[4][0] += w * C[CT_O40 + ct_pair(i, j)];
+            out[4][1 + i] += w * C[CT_O4I + j];
+            out[4][1 + j] += w * C[CT_O4J + i];
         }
 }
 
@@ -540,14 +568,13 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs
     constexpr int NEQ = 5;
     const int N = lane >> 3, M = lane & 7;
     const double s3 = sp[2];
-    const double dtsq_2 = (dt * dt) / 2.0;
 #pragma unroll
     for (int r = 0; r < NEQ; r++) {
         rhs[r] = 0.0;
 #pragma unroll
         for (int s = 0; s < NEQ; s++) R[r][s] = 0.0;
     }
-#pragma unroll 2
+#pragma unroll 1
     for (int gp = 0; gp < 8; gp++) {
         const GPTab& g = tabs[gp];
         const double PN = G.phi3[gp][N], PM = G.phi3[gp][M];
@@ -570,19 +597,16 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs
         for (int I = 0; I < 3; I++)
 #pragma unroll
             for (int p = 0; p < 17; p++)
-                R[sig_eq(I, JPAT_R[0][p])][sig_eq(I, JPAT_S[0][p])] += wg[I] * g.M1[I][p];
-        // rhs: P and the A D product (contract3's AQ)
+                R[sig_eq(I, JPAT_R[0][p])][sig_eq(I, JPAT_S[0][p])] += wg[I] * (p == 0 ? sp[0] : g.M1[I][p - 1]);
+        // rhs: P and the A D product (contract3's AQ), as PQ = P - dt^2/2 A D
 #pragma unroll
         for (int r = 0; r < NEQ; r++) {
-            double acc = 0.0, aq = 0.0;
+            double acc = 0.0;
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                if (r > 0) acc += wr[i] * g.P[i][r];
-                aq += wr[i] * g.AD[i][sig_eq(i, r)];
-            }
-            rhs[r] += acc - dtsq_2 * aq;
+            for (int i = 0; i < 3; i++) acc += wr[i] * g.PQ[i][sig_eq(i, r)];
+            rhs[r] += acc;
         }
-        accum_c3_tab(th, g, W, R);
+        accum_c3_ctab(g.C, W, R);
     }
 }
 
@@ -704,7 +728,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         const int gp = lane & 7, part = lane >> 3;
         GPShared g;
         gp_phase1(th, s_g, s_nd, gp, A.dt, g, part == 3 ? s_tab[gp].dphi : nullptr);
-        if (part < 3) gp_tab_dir_rot(th, g, sp, part, s_tab[gp]);
+        if (part < 3) gp_tab_dir_rot(th, g, sp, A.dt, part, s_tab[gp]);
         else gp_tab_common(th, g, s_tab[gp]);
     }
     __syncthreads();
@@ -751,7 +775,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 // hanging-node elements.
 // ---------------------------------------------------------------------------
 struct FaceTab {
-    GPTab t;             // M1, Af, Sf, AD, P and the c_ij scalars (dphi, wJ unused)
+    GPTab t;             // M1, Af, Sf, PQ and the c_ij table (dphi, wJ unused)
     double en[3];        // w0 w1 det J times the unit face normal (TFNorm3D)
     double phib[4];      // face shape functions at the point
     double dpx[4][3];    // their x derivatives
@@ -939,8 +963,8 @@ __device__ bool face_point(const DevThermo& th, const GaussTables& G, const doub
 }
 
 // Part `part` of face point F's node-pair matrices: part t < 4 forms PT[t]
-// and CT[t], part 4 forms G and Q.
-__device__ void face_mats(const DevThermo& th, const FaceTab& F, int part, double dt, FaceMat& out) {
+// and CT[t], part 4 forms G and Q; s1 is the first M1 slot of every direction.
+__device__ void face_mats(const FaceTab& F, int part, double s1, FaceMat& out) {
     constexpr int NEQ = 5;
     const GPTab& t = F.t;
     if (part < 4) {
@@ -973,7 +997,7 @@ __device__ void face_mats(const DevThermo& th, const FaceTab& F, int part, doubl
         for (int r = 0; r < NEQ; r++)
 #pragma unroll
             for (int s = 0; s < NEQ; s++) C[r][s] = 0.0;
-        accum_c3_tab(th, t, W, C);
+        accum_c3_ctab(t.C, W, C);
 #pragma unroll
         for (int r = 0; r < NEQ; r++)
 #pragma unroll
@@ -988,21 +1012,17 @@ __device__ void face_mats(const DevThermo& th, const FaceTab& F, int part, doubl
         for (int I = 0; I < 3; I++)
 #pragma unroll
             for (int p = 0; p < 17; p++)
-                G[sig_eq(I, JPAT_R[0][p])][sig_eq(I, JPAT_S[0][p])] += F.en[I] * t.M1[I][p];
+                G[sig_eq(I, JPAT_R[0][p])][sig_eq(I, JPAT_S[0][p])] += F.en[I] * (p == 0 ? s1 : t.M1[I][p - 1]);
 #pragma unroll
         for (int r = 0; r < NEQ; r++)
 #pragma unroll
             for (int s = 0; s < NEQ; s++) out.G[r * 5 + s] = G[r][s];
-        const double dtsq_2 = (dt * dt) / 2.0;
 #pragma unroll
         for (int r = 0; r < NEQ; r++) {
-            double acc = 0.0, aq = 0.0;
+            double acc = 0.0;
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                if (r > 0) acc += F.en[i] * t.P[i][r];
-                aq += F.en[i] * t.AD[i][sig_eq(i, r)];
-            }
-            out.Q[r] = -acc + dtsq_2 * aq;
+            for (int i = 0; i < 3; i++) acc += F.en[i] * t.PQ[i][sig_eq(i, r)];
+            out.Q[r] = -acc;
         }
     }
 }
@@ -1097,12 +1117,12 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
                 if (st) atomicOr(&s_status, st);
             }
             if (ok) {
-                if (part < 3) gp_tab_dir_rot(th, g, sp, part, F.t);
+                if (part < 3) gp_tab_dir_rot(th, g, sp, A.dt, part, F.t);
                 else gp_tab_common(th, g, F.t);
             }
         }
         __syncthreads();
-        if (p < nb && part < 5 && s_ft[p].ok) face_mats(th, s_ft[p], part, A.dt, s_fm[p]);
+        if (p < nb && part < 5 && s_ft[p].ok) face_mats(s_ft[p], part, sp[0], s_fm[p]);
         __syncthreads();
         double R[NEQ][NEQ], rhs[NEQ];
 #pragma unroll
