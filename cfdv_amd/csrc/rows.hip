@@ -194,32 +194,41 @@ __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const
             double acc[BB];
 #pragma unroll
             for (int i = 0; i < BB; i++) acc[i] = 0.0;
-            for (long p = coff[k]; p < coff[k + 1]; p++) {
-                const int code = contrib[p * 64 + lane];
-                if (code < 0) continue;
-                const long s = code / NN2;
-                const int nm = code % NN2, N = nm / NNOD, M = nm % NNOD;
-                const int j0 = M * NEQ, odd = j0 & 1;
-                // pair p of element s sits at (s / 64) 64 KK / 2 + p 64 + s % 64
-                const d2_t* Rs = Rv + (size_t)(s >> 6) * 64 * (KK / 2) + (s & 63);
+            // the next contributor's code is in flight with this one's block
+            // rows, and all NEQ x NP pair loads issue before the first add
+            const long pe = coff[k + 1];
+            long p = coff[k];
+            int code = p < pe ? contrib[p * 64 + lane] : -1;
+            for (; p < pe; p++) {
+                const int code_n = p + 1 < pe ? contrib[(p + 1) * 64 + lane] : -1;
+                if (code >= 0) {
+                    const long s = code / NN2;
+                    const int nm = code % NN2, N = nm / NNOD, M = nm % NNOD;
+                    const int j0 = M * NEQ, odd = j0 & 1;
+                    // pair p of element s sits at (s / 64) 64 KK / 2 + p 64 + s % 64
+                    const d2_t* Rs = Rv + (size_t)(s >> 6) * 64 * (KK / 2) + (s & 63);
+                    d2_t w[NEQ][NP];
 #pragma unroll
-                for (int r = 0; r < NEQ; r++) {
-                    const int p0 = ((N * NEQ + r) * K + j0 - odd) >> 1;
-                    double v[2 * NP];
+                    for (int r = 0; r < NEQ; r++) {
+                        const int p0 = ((N * NEQ + r) * K + j0 - odd) >> 1;
 #pragma unroll
-                    for (int i = 0; i < NP; i++) {
-                        const d2_t w = Rs[(size_t)(p0 + i) * 64];
-                        v[2 * i] = w[0];
-                        v[2 * i + 1] = w[1];
+                        for (int i = 0; i < NP; i++) w[r][i] = Rs[(size_t)(p0 + i) * 64];
                     }
-                    if constexpr (NEQ % 2 == 1) {
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                        for (int q = 0; q < NEQ; q++) acc[r * NEQ + q] += odd ? v[q + 1] : v[q];
-                    } else {   // j0 even: the pairs hold exactly the row
+                    for (int r = 0; r < NEQ; r++)
 #pragma unroll
-                        for (int q = 0; q < NEQ; q++) acc[r * NEQ + q] += v[q];
-                    }
+                        for (int q = 0; q < NEQ; q++) {
+                            const double ve = w[r][q >> 1][q & 1];
+                            if constexpr (NEQ % 2 == 1) {
+                                const double vo = w[r][(q + 1) >> 1][(q + 1) & 1];
+                                acc[r * NEQ + q] += odd ? vo : ve;
+                            } else {   // j0 even: the pairs hold exactly the row
+                                acc[r * NEQ + q] += ve;
+                            }
+                        }
                 }
+                code = code_n;
             }
 #pragma unroll
             for (int i = 0; i < BB; i++) {
