@@ -159,6 +159,7 @@ struct cfdv_ctx {
     double* d_B64 = nullptr;
     float* d_B32 = nullptr;
     bool rows_off = false;    // set while the reference solve runs (EBE only)
+    bool rows_tr = false;     // the rows hold K^T (assembled for K^T x)
     // discrete adjoint: per-slot tangents [n_pad][41], host-given vectors
     double *d_gelem = nullptr, *d_lam = nullptr, *d_duv = nullptr, *d_gU = nullptr, *d_g = nullptr;
     double last_dt = 0.0;
@@ -207,11 +208,20 @@ struct cfdv_ctx {
     }
     void check_launch() { HIP_CHECK_THROW(hipGetLastError()); }
 
-    // products K x use the assembled rows when the context holds them (not
-    // for K^T x, nor inside the reference solve)
+    // products K x and K^T x use the assembled rows when the context holds
+    // them (not inside the reference solve)
     bool use_rows(int prec, bool tr) const {
-        if (tr || rows_off) return false;
+        (void)tr;
+        if (rows_off) return false;
         return prec == 64 ? d_B64 != nullptr : d_B32 != nullptr;
+    }
+    // the rows hold K (tr false) or K^T (tr true): re-sum them from the FP64
+    // element blocks when the orientation changes
+    void orient_rows(bool tr) {
+        if (!d_B64 || rows_tr == tr) return;
+        launch_rows_assemble(ra, d_R64, tr, d_B64, d_B32, st);
+        check_launch();
+        rows_tr = tr;
     }
     // blocks available for a product of this precision
     bool have_blocks(int prec, bool tr) const {
@@ -228,6 +238,7 @@ struct cfdv_ctx {
         return evpool[ev_used++];
     }
     void matvec(const void* x, void* y, int prec, bool tr) {
+        if (use_rows(prec, tr)) orient_rows(tr);
         int mark = -1;
         if (timing) {
             mark = ev_used;
@@ -310,7 +321,7 @@ struct cfdv_ctx {
         }
         if (pinv_valid) return;
         HIP_CHECK_THROW(hipMemsetAsync(d_counters + 2, 0, sizeof(int), st));
-        if (d_B64) launch_nodeblock_rows(ra, d_diag, d_B64, d_P64, d_P32, d_counters + 2, st);
+        if (d_B64) launch_nodeblock_rows(ra, d_diag, d_B64, rows_tr, d_P64, d_P32, d_counters + 2, st);
         else launch_nodeblock(d_node_ptr, d_node_ent, d_R64, d_R32, nnod, neqn, n_node, d_P64, d_P32, d_counters + 2, st);
         check_launch();
         int singular = 0;
@@ -984,8 +995,7 @@ int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, 
     const long n = c->ndof;
     const bool tr = o->transpose != 0;
     if (!c->have_blocks(32, tr))
-        return fail(c, CFDV_ERR_ARG, tr ? "transposed mixed solve needs CFDV_BLOCKS_FP32"
-                                        : "mixed solve needs CFDV_BLOCKS_FP32 or CFDV_BLOCKS_ROWS");
+        return fail(c, CFDV_ERR_ARG, "mixed solve needs CFDV_BLOCKS_FP32 or CFDV_BLOCKS_ROWS");
     const int outer_prec = c->have_blocks(64, tr) ? 64 : 3;   // FP64 residual with FP64 (or widened FP32) blocks
     c->ensure_krylov32(o->inner_m);
     const bool pc = o->precond == CFDV_PRECOND_NODE_BLOCK;
@@ -1191,7 +1201,8 @@ int cfdv_build(cfdv_ctx* ctx, double dt) {
     a.n_aslots = ctx->n_aslots;
     launch_build(a, ctx->d_R64 != nullptr, ctx->d_R32 != nullptr, ctx->st);
     if (ctx->timing) HIP_CHECK_THROW(hipEventRecord(ctx->ev[5], ctx->st));
-    if (ctx->d_B64) launch_rows_assemble(ctx->ra, ctx->d_R64, ctx->d_B64, ctx->d_B32, ctx->st);
+    if (ctx->d_B64) launch_rows_assemble(ctx->ra, ctx->d_R64, false, ctx->d_B64, ctx->d_B32, ctx->st);
+    ctx->rows_tr = false;
     if (ctx->timing) HIP_CHECK_THROW(hipEventRecord(ctx->ev[6], ctx->st));
     ctx->check_launch();
     int cnt[2];
@@ -1515,7 +1526,8 @@ int cfdv_build_vjp(cfdv_ctx* ctx, const double* lambda, const double* du, double
 
 // Y = K X (or K^T X) for 8 vectors at once with the FP32 blocks
 static void matvec_multi_dev(cfdv_ctx* ctx, const float* X, float* Y, bool tr) {
-    if (ctx->d_B32 && !tr) {   // over the assembled rows (K^T X stays element by element)
+    if (ctx->d_B32) {   // over the assembled rows
+        ctx->orient_rows(tr);
         launch_rows_matvec8(ctx->ra, ctx->d_B32, X, Y, ctx->st);
         ctx->check_launch();
         return;
@@ -1531,8 +1543,8 @@ static void matvec_multi_dev(cfdv_ctx* ctx, const float* X, float* Y, bool tr) {
 int cfdv_matvec_multi(cfdv_ctx* ctx, const float* X, float* Y, int nvec, int transpose) {
     if (!ctx || !X || !Y || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_matvec_multi before cfdv_build");
     if (nvec != 8) return fail(ctx, CFDV_ERR_ARG, "cfdv_matvec_multi takes 8 vectors");
-    if (ctx->ndim != 3 || !(ctx->d_R32 || (ctx->d_B32 && !transpose)))
-        return fail(ctx, CFDV_ERR_ARG, "needs hex8 FP32 blocks (or node rows for X -> K X)");
+    if (ctx->ndim != 3 || !(ctx->d_R32 || ctx->d_B32))
+        return fail(ctx, CFDV_ERR_ARG, "needs hex8 FP32 blocks or node rows");
     GUARD_BEGIN
     const size_t n = (size_t)ctx->ndof * 8;
     float* dX = ctx->alloc<float>(n);
@@ -1551,7 +1563,8 @@ int cfdv_matvec_multi(cfdv_ctx* ctx, const float* X, float* Y, int nvec, int tra
 
 int cfdv_profile_matvec_multi(cfdv_ctx* ctx, int reps, double* ms) {
     if (!ctx || !ms || reps < 1 || !ctx->built) return fail(ctx, CFDV_ERR_ARG, "profile before build");
-    if (ctx->ndim != 3 || !(ctx->d_R32 || ctx->d_B32)) return fail(ctx, CFDV_ERR_ARG, "needs hex8 FP32 blocks");
+    if (ctx->ndim != 3 || !(ctx->d_R32 || ctx->d_B32))
+        return fail(ctx, CFDV_ERR_ARG, "needs hex8 FP32 blocks or node rows");
     GUARD_BEGIN
     const size_t n = (size_t)ctx->ndof * 8;
     float* dX = ctx->alloc<float>(n);

@@ -136,13 +136,15 @@ void launch_rows_matvec(const RowsArgs& a, const double* B64, const float* B32, 
                         hipStream_t st);
 // Y = K X for 8 vectors over the FP32 rows, X and Y [n_node * neqn][8]
 void launch_rows_matvec8(const RowsArgs& a, const float* B32, const float* X, float* Y, hipStream_t st);
-// rows (FP64 and/or FP32, either may be null) from the FP64 element blocks
-void launch_rows_assemble(const RowsArgs& a, const double* R64, double* B64, float* B32, hipStream_t st);
+// rows (FP64 and/or FP32, either may be null) from the FP64 element blocks:
+// those of K, or with tr those of K^T (the same slots and sums, each block
+// transposed), over which launch_rows_matvec then forms K^T x
+void launch_rows_assemble(const RowsArgs& a, const double* R64, bool tr, double* B64, float* B32, hipStream_t st);
 // node-block inverses (as launch_nodeblock) from the diagonal slots of the
-// FP64 rows (diag[n]: global slot index of node n's own column); the slot
-// sums are the element blocks' in element order, so P equals
-// launch_nodeblock's bit for bit
-void launch_nodeblock_rows(const RowsArgs& a, const long* diag, const double* B64, double* P64, float* P32,
+// FP64 rows (diag[n]: global slot index of node n's own column; tr: the
+// rows hold K^T); the slot sums are the element blocks' in element order,
+// so P equals launch_nodeblock's bit for bit
+void launch_nodeblock_rows(const RowsArgs& a, const long* diag, const double* B64, bool tr, double* P64, float* P32,
                            int* status, hipStream_t st);
 
 // ---- coloring (solve.hip) ----

@@ -173,8 +173,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 // entries starting at column j0 = M NEQ is read as whole pairs: NEQ / 2 of
 // them for even NEQ, (NEQ + 1) / 2 from the pair holding j0 for odd NEQ
 // (one entry of a neighbouring block read and dropped), so each lane's
-// loads are full 16-byte vectors.
-template <int NNOD, int NEQ>
+// loads are full 16-byte vectors.  TR: the rows of K^T -- slot (n, c) sums
+// the transposed blocks K_e(c, n), the same contributors with N and M
+// swapped.
+template <int NNOD, int NEQ, bool TR>
 __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const long* __restrict__ goff,
                                                             const long* __restrict__ coff,
                                                             const int* __restrict__ contrib,
@@ -203,7 +205,8 @@ __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const
                 const int code_n = p + 1 < pe ? contrib[(p + 1) * 64 + lane] : -1;
                 if (code >= 0) {
                     const long s = code / NN2;
-                    const int nm = code % NN2, N = nm / NNOD, M = nm % NNOD;
+                    const int nm = code % NN2;
+                    const int N = TR ? nm % NNOD : nm / NNOD, M = TR ? nm / NNOD : nm % NNOD;
                     const int j0 = M * NEQ, odd = j0 & 1;
                     // pair p of element s sits at (s / 64) 64 KK / 2 + p 64 + s % 64
                     const d2_t* Rs = Rv + (size_t)(s >> 6) * 64 * (KK / 2) + (s & 63);
@@ -220,11 +223,12 @@ __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const
 #pragma unroll
                         for (int q = 0; q < NEQ; q++) {
                             const double ve = w[r][q >> 1][q & 1];
+                            const int f = TR ? q * NEQ + r : r * NEQ + q;
                             if constexpr (NEQ % 2 == 1) {
                                 const double vo = w[r][(q + 1) >> 1][(q + 1) & 1];
-                                acc[r * NEQ + q] += odd ? vo : ve;
+                                acc[f] += odd ? vo : ve;
                             } else {   // j0 even: the pairs hold exactly the row
-                                acc[r * NEQ + q] += ve;
+                                acc[f] += ve;
                             }
                         }
                 }
@@ -240,8 +244,9 @@ __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const
     }
 }
 
-// node-block inverses from the diagonal slot of each FP64 row
-template <int NEQ>
+// node-block inverses of K's diagonal blocks from the diagonal slot of
+// each FP64 row (TR: the rows hold K^T, the slot is read transposed)
+template <int NEQ, bool TR>
 __global__ __launch_bounds__(256) void nodeblock_rows_kernel(long n_node, const long* __restrict__ diag,
                                                              const double* __restrict__ B64,
                                                              double* __restrict__ P64, float* __restrict__ P32,
@@ -256,19 +261,24 @@ __global__ __launch_bounds__(256) void nodeblock_rows_kernel(long n_node, const 
 #pragma unroll
     for (int r = 0; r < NEQ; r++)
 #pragma unroll
-        for (int c = 0; c < NEQ; c++) A[r][c] = B64[row_offset<double>(g, lane, t * BB + r * NEQ + c, GV)];
+        for (int c = 0; c < NEQ; c++)
+            A[r][c] = B64[row_offset<double>(g, lane, t * BB + (TR ? c * NEQ + r : r * NEQ + c), GV)];
     nodeblock_invert_store<NEQ>(A, n, P64, P32, status);
 }
 
-void launch_nodeblock_rows(const RowsArgs& a, const long* diag, const double* B64, double* P64, float* P32,
+void launch_nodeblock_rows(const RowsArgs& a, const long* diag, const double* B64, bool tr, double* P64, float* P32,
                            int* status, hipStream_t st) {
     const int nb = (int)((a.n_node + 255) / 256);
-    if (a.neqn == 5)
-        hipLaunchKernelGGL((nodeblock_rows_kernel<5>), dim3(nb), dim3(256), 0, st, a.n_node, diag, B64, P64, P32,
-                           status);
-    else
-        hipLaunchKernelGGL((nodeblock_rows_kernel<4>), dim3(nb), dim3(256), 0, st, a.n_node, diag, B64, P64, P32,
-                           status);
+#define L(NQ, T) \
+    hipLaunchKernelGGL((nodeblock_rows_kernel<NQ, T>), dim3(nb), dim3(256), 0, st, a.n_node, diag, B64, P64, P32, status)
+    if (a.neqn == 5) {
+        if (tr) L(5, true);
+        else L(5, false);
+    } else {
+        if (tr) L(4, true);
+        else L(4, false);
+    }
+#undef L
 }
 
 static int row_grid(long n_chunks) {
@@ -304,15 +314,20 @@ void launch_rows_matvec8(const RowsArgs& a, const float* B32, const float* X, fl
         hipLaunchKernelGGL((rows_matvec8_kernel<4>), grid, blk, 0, st, a.n_node, a.n_chunks, a.goff, a.cols, B32, X, Y);
 }
 
-void launch_rows_assemble(const RowsArgs& a, const double* R64, double* B64, float* B32, hipStream_t st) {
+void launch_rows_assemble(const RowsArgs& a, const double* R64, bool tr, double* B64, float* B32, hipStream_t st) {
     if (a.n_chunks <= 0) return;
     const dim3 grid(row_grid(a.n_chunks)), blk(256);
-    if (a.nnod == 8)
-        hipLaunchKernelGGL((rows_assemble_kernel<8, 5>), grid, blk, 0, st, a.n_chunks, a.goff, a.coff, a.contrib, R64,
-                           B64, B32);
-    else
-        hipLaunchKernelGGL((rows_assemble_kernel<4, 4>), grid, blk, 0, st, a.n_chunks, a.goff, a.coff, a.contrib, R64,
-                           B64, B32);
+#define L(NN, NQ, T)                                                                                     \
+    hipLaunchKernelGGL((rows_assemble_kernel<NN, NQ, T>), grid, blk, 0, st, a.n_chunks, a.goff, a.coff, a.contrib, \
+                       R64, B64, B32)
+    if (a.nnod == 8) {
+        if (tr) L(8, 5, true);
+        else L(8, 5, false);
+    } else {
+        if (tr) L(4, 4, true);
+        else L(4, 4, false);
+    }
+#undef L
 }
 
 }  // namespace cfdv

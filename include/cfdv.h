@@ -106,8 +106,10 @@ typedef struct {          /* the aero deck keys (MPIBinaryConfiguration.h:54-63)
    multiplies element by element, MPI_GMRES.h:442-566).  Needs
    CFDV_BLOCKS_FP64.  Products K x of the FP64 and mixed solves and of
    cfdv_matvec then use the rows (27 blocks per hex8 node instead of 64 per
-   element); K^T x and the reference solve stay element by element, so a
-   transposed mixed solve still needs CFDV_BLOCKS_FP32. */
+   element).  K^T x (transposed solves, the adjoint) uses them too: the
+   first K^T x after a K x re-sums the rows as those of K^T (one assembly)
+   and the first K x after a K^T x back.  The reference solve stays element
+   by element. */
 #define CFDV_BLOCKS_ROWS 4
 
 /* solver modes */
@@ -203,7 +205,7 @@ int cfdv_refine_flags(cfdv_ctx *ctx, uint8_t *flags, double *s1, int32_t *nflagg
 int cfdv_color(cfdv_ctx *ctx, int32_t *color, int32_t *ncolors);
 /* y = K x (or K^T x) with the built blocks; precision 64, 32 (FP32 blocks and
    vectors) or 3 (FP32 blocks, FP64 vectors); host arrays [n_node*neqn].
-   K x uses the assembled rows when the context holds them */
+   K x and K^T x use the assembled rows when the context holds them */
 int cfdv_matvec(cfdv_ctx *ctx, const double *x, double *y, int precision, int transpose);
 /* average time of one full K x (the rows, or EBE over all colors),
    measured with HIP events on the context stream; precision 64 or 32 */
@@ -211,9 +213,9 @@ int cfdv_profile_matvec(cfdv_ctx *ctx, int precision, int reps, double *ms);
 int cfdv_step(cfdv_ctx *ctx, double cfl, const cfdv_solve_opts *opts, cfdv_step_stats *st);
 /* Y = K X (or K^T X) for 8 vectors in one pass over the FP32 blocks (no
    reference counterpart): X, Y host arrays [n_node*neqn][8] (the 8 values
-   of a dof contiguous); nvec must be 8; hex8 meshes.  K X runs over the
-   FP32 node rows when the context holds them, else element by element with
-   CFDV_BLOCKS_FP32; K^T X needs CFDV_BLOCKS_FP32 */
+   of a dof contiguous); nvec must be 8; hex8 meshes.  K X and K^T X run
+   over the FP32 node rows when the context holds them, else element by
+   element with CFDV_BLOCKS_FP32 */
 int cfdv_matvec_multi(cfdv_ctx *ctx, const float *X, float *Y, int nvec, int transpose);
 /* average time of one 8-vector product K X (node rows, or EBE over all
    colors), HIP events */

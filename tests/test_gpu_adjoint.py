@@ -196,3 +196,22 @@ def test_build_vjp_on_hanging_node_elements(gpu_available):
         eo, fdo = _fd_best(lambda h: _phi_oracle(mesh, th, U + h * d, dt, lam, du), an)
         print("adapted dir", k, "adjoint", an, "fd(oracle)", fdo, "rel err", eo)
         assert eo <= 1e-6
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_step_gradient_on_node_rows(gpu_available, mode):
+    """The adjoint over the assembled rows (K^T lambda = g on the rows
+    re-summed as those of K^T, the forward step on K's rows) gives the
+    element-by-element adjoint's gradient, to the solver tolerance."""
+    mesh, th, U0 = _case("ramp")
+    ebe = FDVContext(mesh, th, U0)
+    rows = FDVContext(mesh, th, U0, blocks=("fp64", "rows"))
+    dt = ebe.cfl(0.5)
+    g = np.random.default_rng(5).standard_normal(ebe.ndof)
+    gU_e, gM_e, U1_e = step_gradient(ebe, U0, dt, g, rtol=1e-12)
+    gU_r, gM_r, U1_r = step_gradient(rows, U0, dt, g, rtol=1e-12, mode=mode)
+    eU = np.linalg.norm(gU_r - gU_e) / np.linalg.norm(gU_e)
+    eM = abs(gM_r - gM_e) / abs(gM_e)
+    e1 = np.linalg.norm(U1_r - U1_e) / np.linalg.norm(U1_e)
+    print(mode, "rows vs EBE: dJ/dU0 %.2e dJ/dM %.2e U1 %.2e" % (eU, eM, e1))
+    assert eU <= 1e-7 and eM <= 1e-7 and e1 <= 1e-10

@@ -15,7 +15,6 @@ context without rows), and K^T x stays on the element blocks.
 import numpy as np
 import pytest
 
-from cfdv_amd import _lib
 from cfdv_amd import mesh as M
 from cfdv_amd import refine as PR
 from cfdv_amd.fdv import CFDVError, FDVContext
@@ -98,9 +97,13 @@ def test_rows_product_parity(gpu_available, case):
     A = _dense(m, R)
     y3d = A.astype(np.float32).astype(np.float64) @ x
     assert np.linalg.norm(y3 - y3d) <= 1e-13 * nrm
-    # K^T x stays on the element blocks
+    # K^T x over the rows re-summed transposed (same slots, same element
+    # order), in FP64 and FP32; K x afterwards re-sums K's rows, bit for bit
     yt = ctx.matvec(x, transpose=True)
     assert np.linalg.norm(yt - O.matvec(m, R, x, transpose=True)) <= 1e-13 * nrm
+    y3t = ctx.matvec(x, precision=3, transpose=True)
+    assert np.linalg.norm(y3t - A.astype(np.float32).astype(np.float64).T @ x) <= 1e-13 * nrm
+    assert np.array_equal(ctx.matvec(x), y)
 
 
 @pytest.mark.parametrize("case", ["box3d", "lid", "shuffled", "refined"])
@@ -133,14 +136,19 @@ def test_rows_solves(gpu_available, case):
           "FP32 products", st.matvecs32, "FP64", st.matvecs64)
     assert st.matvecs32 > 0
     check(st, A, xs)
-    # the transposed FP64 solve runs on the element blocks
+    # the transposed solves run on the rows of K^T, the node-block
+    # preconditioner stays that of K (applied transposed)
     xt = np.linalg.solve(A.T, b)
     st, _ = ctx.solve("fp64", restarts=60, m=30, rtol=1e-10, transpose=True, precond="nodeblock")
     check(st, A.T, xt)
     assert nx > 0
-    # a transposed mixed solve needs the FP32 element blocks
-    st, rc = ctx.solve("mixed", restarts=5, m=30, rtol=1e-10, transpose=True, check=False)
-    assert rc == _lib.CFDV_ERR_ARG
+    st, _ = ctx.solve("mixed", restarts=20, m=30, rtol=1e-10, inner_m=20, inner_rtol=1e-4,
+                      inner_restarts=3, transpose=True, precond="nodeblock")
+    assert st.matvecs32 > 0
+    check(st, A.T, xt)
+    # and K x = b again after them
+    st, _ = ctx.solve("fp64", restarts=60, m=30, rtol=1e-10, precond="nodeblock")
+    check(st, A, xs)
 
 
 def test_reference_solve_stays_element_by_element(gpu_available):
@@ -212,8 +220,8 @@ def test_warm_start_reaches_the_same_steps(gpu_available, mode):
 def test_rows_eight_vector_product(gpu_available, case):
     """The 8-vector product over the FP32 rows is, vector by vector, the
     single FP32 row product bit for bit (same sums in the same order), and
-    within 1e-6 of the oracle's EBE product; K^T X without FP32 element
-    blocks is refused."""
+    within 1e-6 of the oracle's EBE product; likewise K^T X over the rows
+    re-summed transposed."""
     m, th, U = _case(case)
     ctx = FDVContext(m, th, U, blocks=("fp64", "rows"))
     ctx.build(ctx.cfl(0.5))
@@ -227,8 +235,13 @@ def test_rows_eight_vector_product(gpu_available, case):
         assert np.array_equal(Y[:, v], y1.astype(np.float32))
         yo = O.matvec(m, R, x)
         assert np.linalg.norm(Y[:, v] - yo) <= 2e-6 * np.linalg.norm(yo)
-    with pytest.raises(CFDVError):
-        ctx.matvec_multi(X, transpose=True)
+    Yt = ctx.matvec_multi(X, transpose=True)
+    for v in range(8):
+        x = X[:, v].astype(np.float64)
+        assert np.array_equal(Yt[:, v], ctx.matvec(x, precision=32, transpose=True).astype(np.float32))
+        yo = O.matvec(m, R, x, transpose=True)
+        assert np.linalg.norm(Yt[:, v] - yo) <= 2e-6 * np.linalg.norm(yo)
+    assert np.array_equal(ctx.matvec_multi(X), Y)
 
 
 def test_rows_with_an_unused_node(gpu_available):
