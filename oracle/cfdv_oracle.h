@@ -9,7 +9,7 @@
  * Every routine follows the reference's own loop and operand order (so that
  * it reproduces the reference's rounding as closely as plain C allows; build
  * with -ffp-contract=off) and cites the reference file:line it restates.
- * Reference quirks (SURVEY quirk ids Q1..Q16) are reproduced, not fixed.
+ * Reference quirks (numbered Q1..Q16 where they are reproduced) are kept, not fixed.
  *
  * Pinning: each function is checked in tests/test_oracle_*.py against closed
  * forms, invariants or finite differences (see the header comment of each
