@@ -6,8 +6,10 @@ Re=5e5, adiabatic wall, CFL 0.5, synthetic initial state (uniform
 freestream, wall at rest).  A step is one full FDV time step on the GPU:
 CFL, element build (FDVParam + domain + boundary + ApplyBC, FP64 arithmetic),
 GMRES to a 1e-8 relative true residual (both paths right-preconditioned by
-node-block Jacobi unless --precond none; each solve after the first starts
-from the previous step's du scaled by dt / dt_prev unless --warm-start 0),
+node-block Jacobi unless --precond none; each solve after the first two
+starts from dt (2 du_1 / dt_1 - du_2 / dt_2), the increment rate of the two
+previous steps extrapolated, the second from du_1 dt / dt_1; --warm-start 1
+uses the latter throughout, --warm-start 0 starts every solve from 0),
 update and RMS monitor.
 
 Two paths are timed from the same initial state:
@@ -72,10 +74,11 @@ def parse():
     p.add_argument("--inner-rtol", type=float, default=1e-4)
     p.add_argument("--precond", choices=["nodeblock", "none"], default="nodeblock")
     p.add_argument("--storage", choices=["rows", "ebe"], default="rows")
-    # each solve starts from the previous step's du scaled by dt / dt_prev
-    # (the first step after the initial state starts from 0); --warm-start 0
-    # starts every solve from 0 as the reference does
-    p.add_argument("--warm-start", type=int, default=1)
+    # 2: each solve starts from the increment rate of the two previous steps
+    # extrapolated (the first step after the initial state starts from 0,
+    # the second from du_1 dt / dt_1); 1: always du_1 dt / dt_1; 0: every
+    # solve from 0 as the reference does
+    p.add_argument("--warm-start", type=int, default=2, choices=[0, 1, 2])
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=32768)
     return p.parse_args()
@@ -96,7 +99,7 @@ def run_steps(ctx, U0, mode, n, args):
     for i in range(n):
         st = ctx.step(args.cfl, mode=mode, restarts=60 if mode == "fp64" else 20, m=args.m,
                       rtol=args.rtol, inner_m=args.inner_m, inner_rtol=args.inner_rtol,
-                      inner_restarts=3, precond=args.precond, warm_start=bool(args.warm_start))
+                      inner_restarts=3, precond=args.precond, warm_start=int(args.warm_start))
         out.append(st)
         log(f"  {mode} step {i}: build {st.ms_build:.1f} ms assemble {st.ms_assemble:.1f} ms "
             f"solve {st.ms_solve:.1f} ms "
@@ -232,7 +235,7 @@ def main():
                                f"{args.precond} right preconditioner on both paths",
                    "elements": ne, "nodes": nn, "colors": info["ncolors"],
                    "precond": args.precond, "inner_m": args.inner_m, "inner_rtol": args.inner_rtol,
-                   "storage": args.storage, "warm_start": bool(args.warm_start)},
+                   "storage": args.storage, "warm_start": int(args.warm_start)},
         "fp64_steps_per_s": fp64_sps,
         "mixed_cold_start": cold,
         "mixed_over_fp64": value / fp64_sps,

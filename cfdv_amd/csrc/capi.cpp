@@ -163,9 +163,11 @@ struct cfdv_ctx {
     // discrete adjoint: per-slot tangents [n_pad][41], host-given vectors
     double *d_gelem = nullptr, *d_lam = nullptr, *d_duv = nullptr, *d_gU = nullptr, *d_g = nullptr;
     double last_dt = 0.0;
-    // warm start: d_du keeps the last update's increment, built with prev_dt
-    bool have_prev = false;
-    double prev_dt = 0.0;
+    // warm start: d_du keeps the last update's increment, built with prev_dt;
+    // with warm_start 2, d_dup the one before (prev_dt2), d_dut is scratch
+    bool have_prev = false, have_prev2 = false, saved2 = false;
+    double prev_dt = 0.0, prev_dt2 = 0.0;
+    double *d_dup = nullptr, *d_dut = nullptr;
     double* d_r0 = nullptr;
     double *d_part = nullptr, *d_small = nullptr, *d_hcol = nullptr, *d_cflpart = nullptr;
     double* h_pin = nullptr;   // pinned staging for small host-to-device arrays
@@ -971,11 +973,37 @@ int solve_fp64_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, c
     return CFDV_OK;
 }
 
-// x0 = du of the last update scaled by dt / dt_prev, in place in d_du
+// x0 in d_du: warm_start 1, du of the last update scaled by dt / dt_prev;
+// warm_start 2, the increment rate extrapolated linearly from the last two
+// updates, x0 = dt (2 du_1 / dt_1 - du_2 / dt_2) (the first step after a
+// reset falls back to 1).  Mode 2 keeps du_1 in d_dup for the next step.
 bool warm_guess(cfdv_ctx* c, const cfdv_solve_opts* o) {
+    c->saved2 = false;
     if (!o->warm_start || o->transpose || !c->have_prev || !(c->prev_dt > 0) || !(c->last_dt > 0)) return false;
+    const long n = c->ndof;
+    if (o->warm_start == 2) {
+        if (!c->d_dup) {
+            c->d_dup = c->alloc<double>((size_t)n);
+            c->d_dut = c->alloc<double>((size_t)n);
+        }
+        if (c->have_prev2 && c->prev_dt2 > 0) {
+            vec_set(c->d_small + 13, c->prev_dt / (2.0 * c->last_dt), c->st);
+            vec_set(c->d_small + 14, c->last_dt / c->prev_dt2, c->st);
+            vec_scale_to(c->d_dut, c->d_du, c->d_small + 13, n, c->st);
+            vec_axpy(c->d_dut, c->d_dup, c->d_small + 14, -1.0, n, c->st);
+            std::swap(c->d_dup, c->d_du);   // d_dup: du_1; d_du: scratch
+            std::swap(c->d_du, c->d_dut);   // d_du: x0; d_dut: scratch
+        } else {
+            vec_copy(c->d_dup, c->d_du, n, c->st);
+            vec_set(c->d_small + 12, c->prev_dt / c->last_dt, c->st);
+            vec_scale_to(c->d_du, c->d_du, c->d_small + 12, n, c->st);
+        }
+        c->check_launch();
+        c->saved2 = true;
+        return true;
+    }
     vec_set(c->d_small + 12, c->prev_dt / c->last_dt, c->st);
-    vec_scale_to(c->d_du, c->d_du, c->d_small + 12, c->ndof, c->st);
+    vec_scale_to(c->d_du, c->d_du, c->d_small + 12, n, c->st);
     return true;
 }
 
@@ -1141,7 +1169,7 @@ int cfdv_set_state(cfdv_ctx* ctx, const double* U) {
     ctx->sync();
     ctx->built = false;
     ctx->pinv_valid = false;
-    ctx->have_prev = false;
+    ctx->have_prev = ctx->have_prev2 = false;
     return CFDV_OK;
     GUARD_END
 }
@@ -1329,6 +1357,9 @@ int cfdv_update(cfdv_ctx* ctx) {
     ctx->built = false;
     ctx->pinv_valid = false;
     ctx->have_du = false;
+    ctx->have_prev2 = ctx->saved2;   // d_dup holds the increment before d_du's
+    ctx->prev_dt2 = ctx->prev_dt;
+    ctx->saved2 = false;
     ctx->have_prev = true;   // d_du keeps the increment for a warm start
     ctx->prev_dt = ctx->last_dt;
     return CFDV_OK;
@@ -1449,7 +1480,7 @@ int cfdv_set_thermo(cfdv_ctx* ctx, const cfdv_thermo* thermo) {
     ctx->built = false;
     ctx->pinv_valid = false;
     ctx->have_du = false;
-    ctx->have_prev = false;
+    ctx->have_prev = ctx->have_prev2 = false;
     return CFDV_OK;
 }
 

@@ -140,10 +140,14 @@ typedef struct {
                           CFDV_PRECOND_NODE_BLOCK (right preconditioner; the
                           stopping test stays on the true residual)            */
     int warm_start;    /* FP64 and mixed modes, K du = b: start from the last
-                          update's du scaled by dt / dt_prev instead of 0 (the
-                          stopping test is unchanged, ||b - K du|| <= rtol
-                          ||b||); ignored for the first solve after create,
-                          cfdv_set_state or cfdv_set_thermo                  */
+                          update's du scaled by dt / dt_prev (1), or from
+                          dt (2 du_1 / dt_1 - du_2 / dt_2), the increment rate
+                          of the last two updates extrapolated (2; one more
+                          du kept on the device), instead of 0 (the stopping
+                          test is unchanged, ||b - K du|| <= rtol ||b||);
+                          ignored for the first solve after create,
+                          cfdv_set_state or cfdv_set_thermo (2: the second
+                          falls back to 1)                                   */
 } cfdv_solve_opts;
 
 #define CFDV_PRECOND_NONE 0

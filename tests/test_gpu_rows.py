@@ -182,21 +182,25 @@ def test_mixed_step_on_rows(gpu_available):
         FDVContext(m, th, U, blocks=("rows",))
 
 
+@pytest.mark.parametrize("ws", [1, 2])
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
-def test_warm_start_reaches_the_same_steps(gpu_available, mode):
-    """warm_start begins each solve from the last update's du scaled by
-    dt / dt_prev: three steps reach the same states as cold-started steps
-    (both stop at rtol 1e-10 of the true residual), the first solve after
-    create or set_state ignores the option (bit-identical to a cold one), and
-    the warm solves need fewer products."""
+def test_warm_start_reaches_the_same_steps(gpu_available, mode, ws):
+    """warm_start 1 begins each solve from the last update's du scaled by
+    dt / dt_prev, warm_start 2 from dt (2 du_1 / dt_1 - du_2 / dt_2): four
+    steps reach the same states as cold-started steps (both stop at rtol
+    1e-10 of the true residual), the first solve after create or set_state
+    ignores the option (bit-identical to a cold one), with 2 the second solve
+    falls back to 1 (bit-identical to it), and the warm solves need fewer
+    products."""
     m, th, U = M.compression_ramp(12, 5, 6)
     kw = dict(restarts=40, m=30, rtol=1e-10, inner_m=30, inner_rtol=1e-4, inner_restarts=3, precond="nodeblock")
     cold = FDVContext(m, th, U, blocks=("fp64", "rows"))
     warm = FDVContext(m, th, U, blocks=("fp64", "rows"))
+    one = FDVContext(m, th, U, blocks=("fp64", "rows"))
     nc = nw = 0
-    for k in range(3):
+    for k in range(4):
         sc = cold.step(0.5, mode=mode, **kw)
-        sw = warm.step(0.5, mode=mode, warm_start=True, **kw)
+        sw = warm.step(0.5, mode=mode, warm_start=ws, **kw)
         assert sc.solve.rel_resid <= 1e-10 and sw.solve.rel_resid <= 1e-10
         Uc, U0c = cold.get_state()
         Uw, U0w = warm.get_state()
@@ -205,13 +209,16 @@ def test_warm_start_reaches_the_same_steps(gpu_available, mode):
         else:
             nc += sc.solve.matvecs32 + sc.solve.matvecs64
             nw += sw.solve.matvecs32 + sw.solve.matvecs64
+        if k < 2:
+            one.step(0.5, mode=mode, warm_start=1, **kw)
+            assert np.array_equal(one.get_state()[0], Uw)
         assert np.linalg.norm((Uw - U0w) - (Uc - U0c)) <= 1e-7 * np.linalg.norm(Uc - U0c)
-    print(mode, "products over steps 2-3: cold", nc, "warm", nw)
+    print(mode, "warm_start", ws, "products over steps 2-4: cold", nc, "warm", nw)
     assert nw <= nc
-    # set_state forgets the increment: the next solve is cold again
+    # set_state forgets the increments: the next solve is cold again
     warm.set_state(U)
     cold.set_state(U)
-    sw = warm.step(0.5, mode=mode, warm_start=True, **kw)
+    sw = warm.step(0.5, mode=mode, warm_start=ws, **kw)
     sc = cold.step(0.5, mode=mode, **kw)
     assert np.array_equal(warm.get_state()[0], cold.get_state()[0])
 
