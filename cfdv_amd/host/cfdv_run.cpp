@@ -4,7 +4,7 @@
 //   cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed]
 //            [--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE]
 //            [--flatten FILE] [--adap parity|3d] [--tecplot N] [--tecplot-only]
-//            [--storage rows|ebe] [--save N]
+//            [--storage rows|ebe] [--save N] [--warm-start 0|1|2]
 //
 // Reads a reference binary deck (control, aero, binarynodes, binaryelements;
 // LoadBinary.h:156-245) into element/node objects shaped like the
@@ -40,6 +40,9 @@
 // --storage: the FP64 and mixed solvers (--mode fp64|mixed) multiply over
 // the element blocks summed into node rows (rows, the default) or element by
 // element (ebe); the reference solve always runs element by element.
+// --warm-start (fp64 and mixed modes; 0, the reference's zero start, by
+// default): 1 starts each solve from the last increment scaled by dt / dt_prev,
+// 2 from the last two increments' rate extrapolated (cfdv_solve_opts).
 // A deck whose control has adap = 1 is read with its adap file
 // (LoadBinary.h:247-261: "elno adap refine_level" then the serialised Hnm
 // tensor, TensorStream.h:95-175, column-major).
@@ -398,11 +401,11 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "usage: cfdv_run <deck_dir> [--iters N] [--mode reference|fp64|mixed] "
                              "[--rtol X] [--precond none|nodeblock] [--out DIR] [--dump FILE] [--flatten FILE] "
                              "[--adap parity|3d] [--tecplot N] [--tecplot-only] [--storage rows|ebe] "
-                             "[--save N]\n");
+                             "[--save N] [--warm-start 0|1|2]\n");
         return 2;
     }
     std::string dir = argv[1], out = ".", dump, mode = "reference", flat, precond = "none", adap, storage = "rows";
-    int tecplot = 0, save_every = 1000;
+    int tecplot = 0, save_every = 1000, warm = 0;
     bool tecplot_only = false;
     int iters = -1;
     double rtol = 1e-8;
@@ -429,6 +432,7 @@ int main(int argc, char** argv) {
         else if (a == "--tecplot") tecplot = std::atoi(v.c_str());
         else if (a == "--storage") storage = v;
         else if (a == "--save") save_every = std::atoi(v.c_str());
+        else if (a == "--warm-start") warm = std::atoi(v.c_str());
         else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
@@ -458,6 +462,7 @@ int main(int argc, char** argv) {
         const int itermax = iters >= 0 ? iter + iters : (int)d.control.at("itermax");
         const int md = mode == "fp64" ? CFDV_SOLVE_FP64 : mode == "mixed" ? CFDV_SOLVE_MIXED : CFDV_SOLVE_REFERENCE;
         if (storage != "rows" && storage != "ebe") throw std::runtime_error("--storage takes rows or ebe");
+        if (warm < 0 || warm > 2) throw std::runtime_error("--warm-start takes 0, 1 or 2");
         // the FP64 and mixed solvers multiply over assembled node rows unless
         // --storage ebe; the reference solve always multiplies element by
         // element
@@ -501,6 +506,7 @@ int main(int argc, char** argv) {
         cfdv::HipGMRES<Element, Node> gmres(gpu, restart, m, md, rtol);
         if (md != CFDV_SOLVE_REFERENCE) gmres.options().restarts = std::max(restart, 200);
         if (precond == "nodeblock") gmres.options().precond = CFDV_PRECOND_NODE_BLOCK;
+        if (md != CFDV_SOLVE_REFERENCE) gmres.options().warm_start = warm;
         std::ofstream rms(out + "/RMSError.txt", std::ios::app);
         std::ofstream tfile(out + "/time.txt");
         tfile << "Iteration, Run_Time, Time_Elapsed, Sim_Time, Step_dt" << std::endl;
@@ -510,10 +516,15 @@ int main(int argc, char** argv) {
             save_deck(d, sd, dir, false, it);
         };
         double t = 0, run_time = 0;
+        bool first = true;
         for (; iter < itermax; iter++) {
             const auto t0 = std::chrono::steady_clock::now();
             if (save_every > 0 && iter % save_every == 0) checkpoint(iter);
-            gpu.set_state(d.nodes);
+            // after an update the device holds the nodes' U already; with a
+            // warm start the upload is skipped, since cfdv_set_state forgets
+            // the increments the next solve starts from
+            if (first || warm == 0) gpu.set_state(d.nodes);
+            first = false;
             const double dt = gpu.cfl(cfl);
             cfdv::FDVGalerkin fdv(gpu, dt);
             fdv();

@@ -114,6 +114,35 @@ def test_driver_solvers_on_rows_and_elements(gpu_available, runner, tmp_path, mo
 
 
 @pytest.mark.gpu
+def test_driver_warm_start(gpu_available, runner, tmp_path):
+    """--warm-start 1 and 2 (fp64/mixed modes) change where each solve starts,
+    not where it stops: four mixed iterations on the ramp deck land on the
+    cold-started iterations' state to the solver tolerance.  Other values
+    are refused."""
+    name = "3dtestramp_new"
+    deck = os.path.join(GOLD, "decks", name)
+    U = {}
+    for ws in (0, 1, 2):
+        d = tmp_path / f"ws{ws}"
+        d.mkdir()
+        r = subprocess.run([runner, deck, "--iters", "4", "--mode", "mixed", "--rtol", "1e-10", "--precond",
+                            "nodeblock", "--warm-start", str(ws), "--out", str(d), "--dump", str(d / "U.bin")],
+                           capture_output=True, text=True, timeout=300)
+        print(ws, r.stdout[-300:], r.stderr[-300:])
+        assert r.returncode == 0
+        U[ws] = np.fromfile(d / "U.bin", dtype=np.float64)
+    g = np.load(os.path.join(GOLD, f"{name}.npz"))
+    dU = np.linalg.norm(U[0] - g["U0"].ravel())
+    for ws in (1, 2):
+        err = np.linalg.norm(U[ws] - U[0]) / dU
+        print("warm start", ws, "vs cold, relative to the 4-step change:", err)
+        assert 0 < err <= 1e-7      # a different start, the same stopping test
+    r = subprocess.run([runner, deck, "--iters", "1", "--mode", "mixed", "--warm-start", "3",
+                        "--out", str(tmp_path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "warm-start" in (r.stdout + r.stderr)
+
+
+@pytest.mark.gpu
 def test_driver_checkpoint_restart_and_time_file(gpu_available, runner, tmp_path):
     """--save 1 writes a restartable deck before every iteration and after
     the last one (config->Save, main.cpp:177, 208); running the driver on the
