@@ -76,9 +76,10 @@ def parse():
     p.add_argument("--storage", choices=["rows", "ebe"], default="rows")
     # 2: each solve starts from the increment rate of the two previous steps
     # extrapolated (the first step after the initial state starts from 0,
-    # the second from du_1 dt / dt_1); 1: always du_1 dt / dt_1; 0: every
-    # solve from 0 as the reference does
-    p.add_argument("--warm-start", type=int, default=2, choices=[0, 1, 2])
+    # the second from du_1 dt / dt_1); 3: quadratic in the rate of the last
+    # three; 1: always du_1 dt / dt_1; 0: every solve from 0 as the
+    # reference does
+    p.add_argument("--warm-start", type=int, default=2, choices=[0, 1, 2, 3])
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=32768)
     return p.parse_args()

@@ -164,10 +164,12 @@ struct cfdv_ctx {
     double *d_gelem = nullptr, *d_lam = nullptr, *d_duv = nullptr, *d_gU = nullptr, *d_g = nullptr;
     double last_dt = 0.0;
     // warm start: d_du keeps the last update's increment, built with prev_dt;
-    // with warm_start 2, d_dup the one before (prev_dt2), d_dut is scratch
-    bool have_prev = false, have_prev2 = false, saved2 = false;
-    double prev_dt = 0.0, prev_dt2 = 0.0;
-    double *d_dup = nullptr, *d_dut = nullptr;
+    // with warm_start 2 or 3, d_old[k] the k+1-th before it (old_dt[k];
+    // n_old of them valid), d_dut is scratch
+    bool have_prev = false;
+    int n_old = 0, n_saved = 0;
+    double prev_dt = 0.0, old_dt[2] = {0.0, 0.0};
+    double *d_old[2] = {nullptr, nullptr}, *d_dut = nullptr;
     double* d_r0 = nullptr;
     double *d_part = nullptr, *d_small = nullptr, *d_hcol = nullptr, *d_cflpart = nullptr;
     double* h_pin = nullptr;   // pinned staging for small host-to-device arrays
@@ -973,37 +975,46 @@ int solve_fp64_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, c
     return CFDV_OK;
 }
 
-// x0 in d_du: warm_start 1, du of the last update scaled by dt / dt_prev;
-// warm_start 2, the increment rate extrapolated linearly from the last two
-// updates, x0 = dt (2 du_1 / dt_1 - du_2 / dt_2) (the first step after a
-// reset falls back to 1).  Mode 2 keeps du_1 in d_dup for the next step.
+// x0 in d_du from the increment rates r_k = du_k / dt_k of the last updates
+// (du_1 the last): warm_start 1, x0 = dt r_1; 2, dt (2 r_1 - r_2); 3,
+// dt (3 r_1 - 3 r_2 + r_3) -- the rate extrapolated with a polynomial of
+// degree warm_start - 1, lowered while fewer updates are kept (after a
+// reset).  Modes 2 and 3 keep du_1 (and du_2) for the next step.
 bool warm_guess(cfdv_ctx* c, const cfdv_solve_opts* o) {
-    c->saved2 = false;
+    c->n_saved = 0;
     if (!o->warm_start || o->transpose || !c->have_prev || !(c->prev_dt > 0) || !(c->last_dt > 0)) return false;
     const long n = c->ndof;
-    if (o->warm_start == 2) {
-        if (!c->d_dup) {
-            c->d_dup = c->alloc<double>((size_t)n);
-            c->d_dut = c->alloc<double>((size_t)n);
-        }
-        if (c->have_prev2 && c->prev_dt2 > 0) {
-            vec_set(c->d_small + 13, c->prev_dt / (2.0 * c->last_dt), c->st);
-            vec_set(c->d_small + 14, c->last_dt / c->prev_dt2, c->st);
-            vec_scale_to(c->d_dut, c->d_du, c->d_small + 13, n, c->st);
-            vec_axpy(c->d_dut, c->d_dup, c->d_small + 14, -1.0, n, c->st);
-            std::swap(c->d_dup, c->d_du);   // d_dup: du_1; d_du: scratch
-            std::swap(c->d_du, c->d_dut);   // d_du: x0; d_dut: scratch
-        } else {
-            vec_copy(c->d_dup, c->d_du, n, c->st);
-            vec_set(c->d_small + 12, c->prev_dt / c->last_dt, c->st);
-            vec_scale_to(c->d_du, c->d_du, c->d_small + 12, n, c->st);
-        }
-        c->check_launch();
-        c->saved2 = true;
+    const int ws = std::min(std::max(o->warm_start, 1), 3);
+    if (ws == 1) {
+        vec_set(c->d_small + 12, c->prev_dt / c->last_dt, c->st);
+        vec_scale_to(c->d_du, c->d_du, c->d_small + 12, n, c->st);
         return true;
     }
-    vec_set(c->d_small + 12, c->prev_dt / c->last_dt, c->st);
-    vec_scale_to(c->d_du, c->d_du, c->d_small + 12, n, c->st);
+    const int keep = ws - 1;   // older increments kept for the next step
+    for (int k = 0; k < keep; k++)
+        if (!c->d_old[k]) c->d_old[k] = c->alloc<double>((size_t)n);
+    if (!c->d_dut) c->d_dut = c->alloc<double>((size_t)n);
+    const int order = std::min(ws, 1 + c->n_old);
+    static const double coef[3][3] = {{1, 0, 0}, {2, -1, 0}, {3, -3, 1}};
+    const double* cf = coef[order - 1];
+    vec_set(c->d_small + 12, c->prev_dt / (cf[0] * c->last_dt), c->st);
+    vec_scale_to(c->d_dut, c->d_du, c->d_small + 12, n, c->st);
+    for (int k = 1; k < order; k++) {
+        vec_set(c->d_small + 12 + k, c->last_dt / c->old_dt[k - 1], c->st);
+        vec_axpy(c->d_dut, c->d_old[k - 1], c->d_small + 12 + k, cf[k], n, c->st);
+    }
+    // shift the history: du_1 becomes the first older increment
+    double* spare = c->d_old[keep - 1];
+    for (int k = keep - 1; k > 0; k--) {
+        c->d_old[k] = c->d_old[k - 1];
+        c->old_dt[k] = c->old_dt[k - 1];
+    }
+    c->d_old[0] = c->d_du;
+    c->old_dt[0] = c->prev_dt;
+    c->d_du = c->d_dut;
+    c->d_dut = spare;
+    c->check_launch();
+    c->n_saved = std::min(c->n_old + 1, keep);
     return true;
 }
 
@@ -1169,7 +1180,8 @@ int cfdv_set_state(cfdv_ctx* ctx, const double* U) {
     ctx->sync();
     ctx->built = false;
     ctx->pinv_valid = false;
-    ctx->have_prev = ctx->have_prev2 = false;
+    ctx->have_prev = false;
+    ctx->n_old = 0;
     return CFDV_OK;
     GUARD_END
 }
@@ -1357,9 +1369,8 @@ int cfdv_update(cfdv_ctx* ctx) {
     ctx->built = false;
     ctx->pinv_valid = false;
     ctx->have_du = false;
-    ctx->have_prev2 = ctx->saved2;   // d_dup holds the increment before d_du's
-    ctx->prev_dt2 = ctx->prev_dt;
-    ctx->saved2 = false;
+    ctx->n_old = ctx->n_saved;   // d_old[] hold the increments before d_du's
+    ctx->n_saved = 0;
     ctx->have_prev = true;   // d_du keeps the increment for a warm start
     ctx->prev_dt = ctx->last_dt;
     return CFDV_OK;
@@ -1480,7 +1491,8 @@ int cfdv_set_thermo(cfdv_ctx* ctx, const cfdv_thermo* thermo) {
     ctx->built = false;
     ctx->pinv_valid = false;
     ctx->have_du = false;
-    ctx->have_prev = ctx->have_prev2 = false;
+    ctx->have_prev = false;
+    ctx->n_old = 0;
     return CFDV_OK;
 }
 

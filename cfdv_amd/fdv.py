@@ -144,9 +144,9 @@ class FDVContext:
               warm_start=False):
         """``precond``: "none" or "nodeblock" (node-block Jacobi, right-applied;
         FP64 and mixed modes).  ``warm_start`` (FP64 and mixed modes): 1 or
-        True, start from the last update's du scaled by dt / dt_prev; 2, from
-        dt (2 du_1 / dt_1 - du_2 / dt_2), the last two updates' increment
-        rate extrapolated."""
+        True, start from the last update's du scaled by dt / dt_prev; 2 or 3,
+        from the increment rate of the last two or three updates extrapolated
+        linearly or quadratically."""
         md = {"reference": _lib.CFDV_SOLVE_REFERENCE, "fp64": _lib.CFDV_SOLVE_FP64,
               "mixed": _lib.CFDV_SOLVE_MIXED}[mode]
         o = _lib.SolveOpts(md, restarts, m, rtol, inner_m, inner_rtol, inner_restarts,

@@ -140,14 +140,15 @@ typedef struct {
                           CFDV_PRECOND_NODE_BLOCK (right preconditioner; the
                           stopping test stays on the true residual)            */
     int warm_start;    /* FP64 and mixed modes, K du = b: start from the last
-                          update's du scaled by dt / dt_prev (1), or from
-                          dt (2 du_1 / dt_1 - du_2 / dt_2), the increment rate
-                          of the last two updates extrapolated (2; one more
-                          du kept on the device), instead of 0 (the stopping
-                          test is unchanged, ||b - K du|| <= rtol ||b||);
-                          ignored for the first solve after create,
-                          cfdv_set_state or cfdv_set_thermo (2: the second
-                          falls back to 1)                                   */
+                          update's du scaled by dt / dt_prev (1), or from the
+                          increment rate r_k = du_k / dt_k of the last updates
+                          extrapolated: dt (2 r_1 - r_2) (2) or
+                          dt (3 r_1 - 3 r_2 + r_3) (3), with one or two more
+                          du kept on the device; instead of 0 (the stopping
+                          test is unchanged, ||b - K du|| <= rtol ||b||).
+                          Ignored for the first solve after create,
+                          cfdv_set_state or cfdv_set_thermo; the next ones use
+                          the lower modes until enough updates are kept     */
 } cfdv_solve_opts;
 
 #define CFDV_PRECOND_NONE 0

@@ -182,16 +182,16 @@ def test_mixed_step_on_rows(gpu_available):
         FDVContext(m, th, U, blocks=("rows",))
 
 
-@pytest.mark.parametrize("ws", [1, 2])
+@pytest.mark.parametrize("ws", [1, 2, 3])
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
 def test_warm_start_reaches_the_same_steps(gpu_available, mode, ws):
     """warm_start 1 begins each solve from the last update's du scaled by
-    dt / dt_prev, warm_start 2 from dt (2 du_1 / dt_1 - du_2 / dt_2): four
-    steps reach the same states as cold-started steps (both stop at rtol
-    1e-10 of the true residual), the first solve after create or set_state
-    ignores the option (bit-identical to a cold one), with 2 the second solve
-    falls back to 1 (bit-identical to it), and the warm solves need fewer
-    products."""
+    dt / dt_prev, warm_start 2 from dt (2 r_1 - r_2), 3 from
+    dt (3 r_1 - 3 r_2 + r_3), r_k = du_k / dt_k: four steps reach the same
+    states as cold-started steps (both stop at rtol 1e-10 of the true
+    residual), the first solve after create or set_state ignores the option
+    (bit-identical to a cold one), the second falls back to 1 (bit-identical
+    to it), and the warm solves need fewer products."""
     m, th, U = M.compression_ramp(12, 5, 6)
     kw = dict(restarts=40, m=30, rtol=1e-10, inner_m=30, inner_rtol=1e-4, inner_restarts=3, precond="nodeblock")
     cold = FDVContext(m, th, U, blocks=("fp64", "rows"))
