@@ -265,6 +265,64 @@ def compression_ramp(nx=256, ny=128, nz=64, *, ratio=1.05, mach=6.0, reynolds=5e
     return mesh, th, U
 
 
+def cylinder_ogrid(nt=256, nr=128, nz=32, *, r0=0.5, r1=10.0, lz=1.0, ratio=1.04, mach=2.0,
+                   reynolds=1e4, twall_ratio=1.8, noise=0.0, seed=20261203) -> tuple:
+    """Cylinder in crossflow on a hex8 O-grid (x streamwise, z spanwise).
+
+    nt cells around the cylinder (periodic: no seam nodes, the last ring of
+    cells closes onto the first), nr cells radially from the wall r0 to the
+    farfield r1 (geometrically stretched by ``ratio``), nz spanwise over lz.
+    Element axes: xi radial, eta circumferential, zeta spanwise, which keeps
+    det J > 0.  Isothermal wall 1 at ``twall_ratio`` on the cylinder (faces
+    xi = -1), farfield -1 on the outer faces whose centre has x < 0 and
+    exit 8 on the rest, spanwise symmetry 2 (z-normal faces) at both ends;
+    uniform freestream u = 1 with the wall at rest.  Nodes are numbered
+    circumferential index fastest, then radial, then spanwise.
+    """
+    th_ = 2.0 * np.pi * np.arange(nt) / nt
+    rs = r0 + (r1 - r0) * _stretch(nr, ratio)
+    zs = np.linspace(0.0, lz, nz + 1)
+    X = np.empty((nz + 1, nr + 1, nt, 3))
+    X[..., 0] = rs[None, :, None] * np.cos(th_)[None, None, :]
+    X[..., 1] = rs[None, :, None] * np.sin(th_)[None, None, :]
+    X[..., 2] = zs[:, None, None]
+    x = X.reshape(-1, 3)
+
+    def node(i, j, k):
+        return (i % nt) + nt * (j + (nr + 1) * k)
+
+    j, i, k = np.meshgrid(np.arange(nr), np.arange(nt), np.arange(nz), indexing="ij")
+    j = j.transpose(2, 1, 0).ravel()      # element order: radial fastest
+    i = i.transpose(2, 1, 0).ravel()
+    k = k.transpose(2, 1, 0).ravel()
+    conn = np.stack([node(i, j, k), node(i, j + 1, k), node(i + 1, j + 1, k), node(i + 1, j, k),
+                     node(i, j, k + 1), node(i, j + 1, k + 1), node(i + 1, j + 1, k + 1),
+                     node(i + 1, j, k + 1)], axis=1).astype(np.int32)
+    ne = conn.shape[0]
+    bc = np.zeros((ne, 6), dtype=np.int32)
+    bc[k == 0, 0] = 2
+    bc[k == nz - 1, 5] = 2
+    bc[j == 0, 4] = 1
+    outer = j == nr - 1
+    xc = 0.5 * (np.cos(th_[i]) + np.cos(th_[(i + 1) % nt])) * r1
+    bc[outer, 2] = np.where(xc[outer] < 0.0, -1, 8)
+    th = Thermo(gamma=1.4, mach=mach, reynolds=reynolds, prandtl=0.72, tinf=216.6666,
+                twall=twall_ratio, adiabatic=False)
+    d = dirichlet_flags(3, 5, x.shape[0], conn, bc, False)
+    nn = x.shape[0]
+    rho = 1.0 + density_noise(nn, seed, noise) if noise else np.ones(nn)
+    v = np.zeros((nn, 3))
+    v[:, 0] = 1.0
+    T = np.ones(nn)
+    wall = np.zeros(nn, dtype=bool)
+    wall[conn[bc[:, 4] == 1][:, FACE_NODES_3D[4]].ravel()] = True
+    v[wall] = 0.0
+    T[wall] = twall_ratio
+    U = conservative(rho, v, T, th.Cv)
+    mesh = Mesh(3, 8, 5, conn, x, bc, d, shape=(nr, nt, nz))
+    return mesh, th, U
+
+
 def lid_cavity(n=32, *, mach=0.1, reynolds=100.0) -> tuple:
     """2D lid-driven cavity on the unit square, uniform n x n quad4.
 

@@ -38,6 +38,8 @@ def _case(name):
         return M.compression_ramp(6, 4, 3)
     if name == "plate":
         return M.flat_plate(6, 4, 3)
+    if name == "cylinder":
+        return M.cylinder_ogrid(12, 4, 2)
     raise KeyError(name)
 
 
@@ -104,7 +106,7 @@ def test_build_vjp_against_finite_differences(gpu_available, case):
     assert eM <= 1e-6
 
 
-@pytest.mark.parametrize("case", ["box3d", "ramp"])
+@pytest.mark.parametrize("case", ["box3d", "ramp", "cylinder"])
 def test_step_gradient_16_directions(gpu_available, case):
     mesh, th, U0 = _case(case)
     ctx = FDVContext(mesh, th, U0)

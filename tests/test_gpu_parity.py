@@ -27,6 +27,8 @@ def _case(name):
         return M.compression_ramp(12, 5, 6)
     if name == "lid":
         return M.lid_cavity(8)
+    if name == "cylinder":
+        return M.cylinder_ogrid(12, 4, 2)
     raise KeyError(name)
 
 
@@ -36,7 +38,7 @@ def _rel_fro(a, b):
     return num / np.maximum(den, 1e-300)
 
 
-@pytest.mark.parametrize("case", ["box3d", "box3d_adiabatic", "plate", "ramp", "lid"])
+@pytest.mark.parametrize("case", ["box3d", "box3d_adiabatic", "plate", "ramp", "lid", "cylinder"])
 def test_build_parity(gpu_available, case):
     m, th, U = _case(case)
     ctx = FDVContext(m, th, U, blocks=("fp64", "fp32"))
@@ -149,7 +151,7 @@ def test_hip_gmres_interface(gpu_available):
     assert st.iters > 0
 
 
-@pytest.mark.parametrize("case", ["box3d", "ramp", "plate", "lid"])
+@pytest.mark.parametrize("case", ["box3d", "ramp", "plate", "lid", "cylinder"])
 def test_refine_flags_bit_exact(gpu_available, case):
     m, th, U = _case(case)
     ctx = FDVContext(m, th, U)
