@@ -44,6 +44,34 @@ def step_gradient(ctx: FDVContext, U0, dt, g, mode="fp64", rtol=1e-12, precond="
     return np.asarray(g).reshape(n, -1) + gU, gM, U1.reshape(n, -1)
 
 
+def multi_step_gradient(ctx: FDVContext, U0, dts, g, mode="fp64", rtol=1e-12, precond="nodeblock"):
+    """Gradient of J = g^T U_N after N = len(dts) steps at the given dt's,
+    with respect to U0 and the Mach number; returns (dJ/dU0 [n_node][neqn],
+    dJ/dM, U_N).
+
+    Checkpointed: the forward sweep keeps only the N states U_0..U_{N-1}
+    (host arrays); the reverse sweep re-runs step n from U_n and applies its
+    adjoint, g_n = dJ/dU_n = g_{n+1} + d/dU_n [lam_n^T (b - K du)] with
+    K^T lam_n = g_{n+1}, and dJ/dM sums the steps' Mach terms.  One
+    forward solve and one forward plus one adjoint solve per step in
+    reverse: 2N forward and N adjoint solves in all.
+    """
+    n = ctx.mesh.n_node
+    states = []
+    U = np.asarray(U0, dtype=np.float64).reshape(n, -1)
+    for dt in dts:
+        states.append(U)
+        U1, _ = step_fixed_dt(ctx, U, dt, mode=mode, rtol=rtol, precond=precond)
+        U = U1.reshape(n, -1)
+    gk = np.asarray(g, dtype=np.float64).ravel()
+    gM = 0.0
+    for Un, dt in zip(reversed(states), reversed(list(dts))):
+        gU, gMn, _ = step_gradient(ctx, Un, dt, gk, mode=mode, rtol=rtol, precond=precond)
+        gk = gU.ravel()
+        gM += gMn
+    return gk.reshape(n, -1), gM, U
+
+
 def wall_temperature_gradient(dJdU0, U0, Cv, wall_nodes):
     """dJ/dT_n at the given nodes, T = (E/rho - |v|^2/2)/Cv with rho and the
     velocity held: dU_E/dT = rho Cv (NodeUnpack.h:124-144 inverted)."""

@@ -24,7 +24,7 @@ import numpy as np
 import pytest
 
 from cfdv_amd import mesh as M
-from cfdv_amd.adjoint import step_fixed_dt, step_gradient, wall_temperature_gradient
+from cfdv_amd.adjoint import multi_step_gradient, step_fixed_dt, step_gradient, wall_temperature_gradient
 from cfdv_amd.fdv import FDVContext
 from oracle import oracle as O
 
@@ -217,3 +217,44 @@ def test_step_gradient_on_node_rows(gpu_available, mode):
     e1 = np.linalg.norm(U1_r - U1_e) / np.linalg.norm(U1_e)
     print(mode, "rows vs EBE: dJ/dU0 %.2e dJ/dM %.2e U1 %.2e" % (eU, eM, e1))
     assert eU <= 1e-7 and eM <= 1e-7 and e1 <= 1e-10
+
+
+def test_multi_step_gradient(gpu_available):
+    """The checkpointed adjoint of three steps (dJ/dU0 and dJ/dM of
+    J = g^T U3 at fixed dt's) against central differences of the three
+    forward steps on 6 random directions and the Mach number.  The case is
+    the perturbed box, whose FDVParam ratios r stay clear of the 0.01
+    switch (FDVParam.h:15-111) over the three steps; from a uniform
+    freestream the developing flow brings some element's r to the switch,
+    where J jumps and no difference quotient converges."""
+    mesh, th, U0 = _case("box3d")
+    ctx = FDVContext(mesh, th, U0)
+    dt0 = ctx.cfl(0.5)
+    dts = [dt0, 0.9 * dt0, 1.1 * dt0]
+    g = np.random.default_rng(17).standard_normal(ctx.ndof)
+    gU0, gM, U3 = multi_step_gradient(ctx, U0, dts, g, rtol=1e-13)
+
+    def J(U, mach=th.mach):
+        ctx.set_thermo(dataclasses.replace(th, mach=mach))
+        Uk = np.asarray(U, dtype=np.float64)
+        for dt in dts:
+            Uk, _ = step_fixed_dt(ctx, Uk, dt, rtol=1e-13)
+        return g @ Uk
+
+    # J = g^T U0 + g^T (U3 - U0): differences of the second term only
+    def Jd(U, mach=th.mach):
+        return J(U, mach) - g @ np.asarray(U).ravel()
+
+    rng = np.random.default_rng(3)
+    scale = np.abs(U0).max(axis=0)
+    gd = gU0 - g.reshape(gU0.shape)
+    for k in range(6):
+        d = rng.standard_normal(U0.shape) * scale
+        an = np.sum(gd * d)
+        err, fds = _fd_best(lambda h: Jd(U0 + h * d), an)
+        print("direction", k, "adjoint", an, "fd", fds, "rel err", err)
+        assert err <= 1e-5
+    errM, fdM = _fd_best(lambda h: Jd(U0, th.mach * (1 + h)), gM * th.mach)
+    ctx.set_thermo(th)
+    print("Mach adjoint", gM, "fd", [v / th.mach for v in fdM], "rel err", errM)
+    assert errM <= 1e-5
