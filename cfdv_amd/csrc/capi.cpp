@@ -1324,6 +1324,7 @@ int cfdv_solve(cfdv_ctx* ctx, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
         return fail(ctx, CFDV_ERR_ARG, "unknown preconditioner");
     if (o->precond != CFDV_PRECOND_NONE && o->mode == CFDV_SOLVE_REFERENCE)
         return fail(ctx, CFDV_ERR_ARG, "the reference solve is unpreconditioned");
+    if (o->warm_start < 0 || o->warm_start > 3) return fail(ctx, CFDV_ERR_ARG, "warm_start takes 0 to 3");
     std::memset(s, 0, sizeof(*s));
     GUARD_BEGIN
     switch (o->mode) {

@@ -203,6 +203,9 @@ def test_preconditioner_rejected_in_reference_mode(gpu_available):
     ctx.build(ctx.cfl(0.5))
     _, rc = ctx.solve("reference", restarts=1, m=10, precond="nodeblock", check=False)
     assert rc == -1
+    # warm_start takes 0 to 3
+    _, rc = ctx.solve("fp64", restarts=1, m=10, warm_start=4, check=False)
+    assert rc == -1
 
 
 def _adapted_box(seed=4):
