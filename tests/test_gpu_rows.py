@@ -268,3 +268,23 @@ def test_rows_with_an_unused_node(gpu_available):
     yr, ye = rows.matvec(v), ebe.matvec(v)
     assert np.all(yr[-m2.neqn:] == 0.0)
     assert np.linalg.norm(yr - ye) <= 1e-13 * np.linalg.norm(ye)
+
+
+def test_preconditioner_from_transposed_rows(gpu_available):
+    """The node-block preconditioner formed while the rows hold K^T (a K^T x
+    came first after the build) is K's, read transposed from the diagonal
+    slots: the transposed solve is bit-identical to one whose
+    preconditioner was formed from K's rows."""
+    m, th, U = _case("box3d")
+    a = FDVContext(m, th, U, blocks=("fp64", "rows"))
+    b = FDVContext(m, th, U, blocks=("fp64", "rows"))
+    dt = a.cfl(0.5)
+    a.build(dt)
+    b.build(dt)
+    x = np.random.default_rng(9).standard_normal(a.ndof)
+    a.matvec(x, transpose=True)           # a: rows now hold K^T, P not formed yet
+    b.solve("fp64", restarts=60, m=30, rtol=1e-10, precond="nodeblock")   # b: P formed from K's rows
+    sa, _ = a.solve("fp64", restarts=60, m=30, rtol=1e-10, transpose=True, precond="nodeblock")
+    sb, _ = b.solve("fp64", restarts=60, m=30, rtol=1e-10, transpose=True, precond="nodeblock")
+    assert sa.iters == sb.iters
+    assert np.array_equal(a.du(), b.du())
