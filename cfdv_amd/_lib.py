@@ -44,7 +44,7 @@ EXPORTS = ["cfdv_create", "cfdv_destroy", "cfdv_last_error", "cfdv_mesh_info", "
            "cfdv_set_thermo", "cfdv_adjoint_solve", "cfdv_build_vjp", "cfdv_matvec_multi",
            "cfdv_profile_matvec_multi", "cfdv_refine_create", "cfdv_refine_destroy",
            "cfdv_refine_last_error", "cfdv_refine_list", "cfdv_refine_apply", "cfdv_refine_size",
-           "cfdv_refine_get"]
+           "cfdv_refine_get", "cfdv_reference_trace"]
 
 
 class Mesh(C.Structure):
@@ -111,6 +111,8 @@ def load() -> C.CDLL:
     L.cfdv_get_rhs_vector.argtypes = [vp, dp]
     L.cfdv_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.cfdv_get_du.argtypes = [vp, dp]
+    L.cfdv_reference_trace.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp, dp,
+                                       C.POINTER(C.c_int32)]
     L.cfdv_update.argtypes = [vp]
     L.cfdv_rms.argtypes = [vp, dp]
     L.cfdv_refine_flags.argtypes = [vp, C.POINTER(C.c_uint8), dp, C.POINTER(C.c_int32),

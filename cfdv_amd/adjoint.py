@@ -74,7 +74,7 @@ def multi_step_gradient(ctx: FDVContext, U0, dts, g, mode="fp64", rtol=1e-12, pr
 
 def wall_temperature_gradient(dJdU0, U0, Cv, wall_nodes):
     """dJ/dT_n at the given nodes, T = (E/rho - |v|^2/2)/Cv with rho and the
-    velocity held: dU_E/dT = rho Cv (NodeUnpack.h:124-144 inverted)."""
+    velocity held: dU_E/dT = rho Cv (NodeUnpack.h:9-29 inverted)."""
     U0 = np.asarray(U0)
     rho = U0[wall_nodes, 0]
     return dJdU0[wall_nodes, -1] * rho * Cv

@@ -798,7 +798,7 @@ struct FaceMat {
     double Q[5];
 };
 
-// State of face point fgp on face fc (TFNorm3D, calc_TF_norm3D.h:143-301, the
+// State of face point fgp on face fc (TFNorm3D, calc_TF_norm3D.h:9-167, the
 // face interpolation and eval2dboundary's edits, FDVGalerkin.h:455-533),
 // as build3d_faces computes it, into g (f, D, P, SPhi) and the geometry.
 // Returns false (and sets *status bit 0) for an unknown bc code.
@@ -822,7 +822,7 @@ __device__ bool face_point(const DevThermo& th, const GaussTables& G, const doub
     phib[2] = 0.25 * (1 + z0) * (1 + z1);
     phib[3] = 0.25 * (1 - z0) * (1 + z1);
     // the Lambda/Mu magnitude is recomputed after each component
-    // (calc_TF_norm3D.h:208-212)
+    // (calc_TF_norm3D.h:74-78)
     double Lm[3], Mu[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) { Lm[i] = nl[1][i] - nl[0][i]; Mu[i] = nl[3][i] - nl[0][i]; }
@@ -1367,8 +1367,9 @@ CFDV_HD void build2d_elem(const BuildArgs& A, const GaussTables& G, long e, int 
             const double S = f.v[0] * f.v[0] + f.v[1] * f.v[1];
             const double R_xi = f.U[0] * v_xi * h_xi / f.mu;
             const double R_eta = f.U[0] * v_eta * h_eta / f.mu;
-            const double al_xi = (R_xi == 0) ? 0.0 : cosh(R_xi / 2.) / sinh(R_xi / 2.) - 2. / R_xi;
-            const double al_eta = (R_eta == 0) ? 0.0 : cosh(R_eta / 2.) / sinh(R_eta / 2.) - 2. / R_eta;
+            // coth by the host libm's method (libm_cosh / libm_sinh), bit for bit
+            const double al_xi = (R_xi == 0) ? 0.0 : libm_cosh(R_xi / 2.) / libm_sinh(R_xi / 2.) - 2. / R_xi;
+            const double al_eta = (R_eta == 0) ? 0.0 : libm_cosh(R_eta / 2.) / libm_sinh(R_eta / 2.) - 2. / R_eta;
             const double taun = (S == 0) ? 0.0 : (1. / 4.) * (al_xi * h_xi * v_xi + al_eta * h_eta * v_eta) / S;
             double Av[2] = {0, 0};
 #pragma unroll
@@ -1729,7 +1730,7 @@ void launch_flags(int ndim, const DevThermo& th, const int* conn, const double* 
 }
 
 // ---------------------------------------------------------------------------
-// CFL (src/FDV/Function/MPI_CFL.h:170-217): per-element dxele, then a
+// CFL (src/FDV/Function/MPI_CFL.h:9-56): per-element dxele, then a
 // block min; min is order-independent, so the result is exact.
 // ---------------------------------------------------------------------------
 template <int NDIM, int NNOD>
@@ -1773,7 +1774,7 @@ void launch_cfl(int ndim, const DevThermo& th, const int* conn, const double* U,
         hipLaunchKernelGGL((cfl_kernel<2, 4>), dim3(nblocks), dim3(256), 0, st, th, conn, U, cleng, n_elem_pad, part);
 }
 
-// CalcLength (src/FDV/Function/CalcLength.h:256-332): area by 2-point Gauss,
+// CalcLength (src/FDV/Function/CalcLength.h:34-110): area by 2-point Gauss,
 // cleng = area^(1/ndim)
 template <int NDIM, int NNOD>
 __global__ void length_kernel(const int* __restrict__ conn, const double* __restrict__ x, long n_elem_pad,

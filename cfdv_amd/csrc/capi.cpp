@@ -20,9 +20,10 @@ using namespace cfdv;
 namespace {
 
 constexpr int NBLK_RED = 1024;  // blocks of the reduction kernels (partials: NBLK_RED x 32 doubles)
+constexpr int MAX_M1 = 501;     // Krylov dimension m <= 500: H holds (m+1) x m
 
 // 2-point Gauss-Legendre by Newton iteration, the rule of
-// GaussQuad::get_gauss (src/FiniteElement/GaussQuad.h:258-301).
+// GaussQuad::get_gauss (src/FiniteElement/GaussQuad.h:116-159).
 void gauss2(double w[2], double z[2]) {
     const unsigned N = 2;
     const double eps = 3.E-14;
@@ -167,12 +168,28 @@ struct cfdv_ctx {
     // with warm_start 2 or 3, d_old[k] the k+1-th before it (old_dt[k];
     // n_old of them valid), d_dut is scratch
     bool have_prev = false;
+    bool warm_consumed = false;   // a solve of the current build formed the warm start
+    bool warm_had = false;        // ... and it is kept in d_x0
+    double* d_x0 = nullptr;
     int n_old = 0, n_saved = 0;
     double prev_dt = 0.0, old_dt[2] = {0.0, 0.0};
     double *d_old[2] = {nullptr, nullptr}, *d_dut = nullptr;
     double* d_r0 = nullptr;
     double *d_part = nullptr, *d_small = nullptr, *d_hcol = nullptr, *d_cflpart = nullptr;
-    double* h_pin = nullptr;   // pinned staging for small host-to-device arrays
+    // GMRES least squares on the device: Hessenberg H ((m+1) x m, m <= 500),
+    // then g, cs, sn; a copy of H for the reference mode's rotations; the
+    // gate of the Arnoldi step kernels ([0]) and the reference breakdown
+    // flag ([1]); per-step records in host-visible pinned memory (h_steps,
+    // device address d_steps) and two events that mark the steps
+    double *d_hess = nullptr, *d_hessr = nullptr;
+    int* d_gate = nullptr;
+    GmresStep *h_steps = nullptr, *d_steps = nullptr;
+    hipEvent_t ev_step[2] = {nullptr, nullptr};
+    // the last reference-mode solve, per restart: H before the rotations
+    // ((m+1) x m, column-major), the rotated g (m+1) and the Arnoldi steps
+    int trace_m = 0;
+    std::vector<double> trace_H, trace_g;
+    std::vector<int> trace_mm;
     int *d_counters = nullptr;     // [0] ties, [1] status, [2] singular node block
     hipEvent_t ev[8];
     std::vector<void*> allocs;
@@ -186,7 +203,9 @@ struct cfdv_ctx {
     ~cfdv_ctx() {
         for (auto& e : evpool) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
-        if (h_pin) (void)hipHostFree(h_pin);
+        if (h_steps) (void)hipHostFree(h_steps);
+        for (auto& e : ev_step)
+            if (e) (void)hipEventDestroy(e);
         if (st) {
             for (auto& e : ev) (void)hipEventDestroy(e);
             if (ev_sync) (void)hipEventDestroy(ev_sync);
@@ -205,10 +224,13 @@ struct cfdv_ctx {
     hipEvent_t ev_sync = nullptr;
     void sync() {
         HIP_CHECK_THROW(hipEventRecord(ev_sync, st));
-        hipError_t e;
-        while ((e = hipEventQuery(ev_sync)) == hipErrorNotReady) {
+        wait_event(ev_sync);
+    }
+    static void wait_event(hipEvent_t e) {
+        hipError_t r;
+        while ((r = hipEventQuery(e)) == hipErrorNotReady) {
         }
-        HIP_CHECK_THROW(e);
+        HIP_CHECK_THROW(r);
     }
     void check_launch() { HIP_CHECK_THROW(hipGetLastError()); }
 
@@ -241,7 +263,9 @@ struct cfdv_ctx {
         }
         return evpool[ev_used++];
     }
-    void matvec(const void* x, void* y, int prec, bool tr) {
+    // gate: a device flag that turns the product into a no-op when set
+    // (the Arnoldi step enqueued after the last one of a GMRES cycle)
+    void matvec(const void* x, void* y, int prec, bool tr, const int* gate = nullptr) {
         if (use_rows(prec, tr)) orient_rows(tr);
         int mark = -1;
         if (timing) {
@@ -249,13 +273,13 @@ struct cfdv_ctx {
             HIP_CHECK_THROW(hipEventRecord(next_event(), st));
         }
         if (use_rows(prec, tr)) {
-            launch_rows_matvec(ra, d_B64, d_B32, x, y, prec, st);
+            launch_rows_matvec(ra, d_B64, d_B32, x, y, prec, st, gate);
         } else {
-            if (prec == 32) vec_fill_f((float*)y, 0.0f, ndof, st);
-            else vec_fill((double*)y, 0.0, ndof, st);
+            if (prec == 32) vec_fill_f((float*)y, 0.0f, ndof, st, gate);
+            else vec_fill((double*)y, 0.0, ndof, st, gate);
             for (int c = 0; c < ncolor; c++) {
                 MatvecArgs a{ndim, nnod, neqn, cstart[c], cstart[c + 1], d_conn};
-                launch_matvec_color(a, d_R64, d_R32, x, y, prec, tr, st);
+                launch_matvec_color(a, d_R64, d_R32, x, y, prec, tr, st, gate);
             }
         }
         check_launch();
@@ -263,6 +287,10 @@ struct cfdv_ctx {
             HIP_CHECK_THROW(hipEventRecord(next_event(), st));
             mv_marks.push_back({mark, prec});
         }
+    }
+    // forget the last product's timing marks (a gated product that did nothing)
+    void drop_last_matvec_mark() {
+        if (timing && !mv_marks.empty()) mv_marks.pop_back();
     }
     // sum of the recorded matvec intervals (ms) per precision; call after a sync
     void collect_matvec_times(double* ms64, double* ms32, int* n64, int* n32) {
@@ -679,8 +707,14 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
     c->d_small = c->alloc<double>(1024);
     c->d_hcol = c->alloc<double>(2048);
     c->d_cflpart = c->alloc<double>((np + 255) / 256);
-    HIP_CHECK_THROW(hipHostMalloc((void**)&c->h_pin, sizeof(double) * 1024, hipHostMallocDefault));
     c->d_counters = c->alloc<int>(4);
+    c->d_hess = c->alloc<double>((size_t)MAX_M1 * (MAX_M1 - 1) + 3 * (size_t)MAX_M1);
+    c->d_hessr = c->alloc<double>((size_t)MAX_M1 * (MAX_M1 - 1));
+    c->d_gate = c->alloc<int>(4);
+    HIP_CHECK_THROW(hipHostMalloc((void**)&c->h_steps, sizeof(GmresStep) * MAX_M1,
+                                  hipHostMallocMapped | hipHostMallocCoherent));
+    HIP_CHECK_THROW(hipHostGetDevicePointer((void**)&c->d_steps, c->h_steps, 0));
+    for (auto& e : c->ev_step) HIP_CHECK_THROW(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // CalcLength once (main.cpp:120-121)
     launch_length(c->ndim, c->d_conn, c->d_x, np, c->d_area, c->d_cleng, c->st);
     c->check_launch();
@@ -690,31 +724,40 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
 // ---------------------------------------------------------------------------
 // GMRES variants
 // ---------------------------------------------------------------------------
-// MPI_GMRES::iterate at one rank (MPI_GMRES.h:125-220), quirks Q2-Q4.
+// MPI_GMRES::iterate at one rank (MPI_GMRES.h:125-220), quirks Q2-Q4.  The
+// Hessenberg matrix, the early-break estimate, the rotations and the back
+// substitution run on the device (launch_ref_arnoldi, launch_ref_finish);
+// the host reads one GmresStep per Arnoldi step, as the reference's loop
+// tests its estimate before the next product.
 int solve_reference(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
-    const int m_ = o->m;
+    const int m_ = o->m, ldh = m_ + 1;
     const long n = c->ndof;
     c->ensure_krylov(m_);
     double* V = c->d_V;
     const long ldv = c->ldv();
-    std::vector<double> h((size_t)(m_ + 1) * m_), g(m_ + 1), y(m_);
-    auto H = [&](int i, int j) -> double& { return h[i + (size_t)(m_ + 1) * j]; };
+    double* H = c->d_hess;
+    double* g = H + (size_t)ldh * m_;
+    double* y = c->d_small + 16;
+    c->trace_m = m_;
+    c->trace_H.assign((size_t)o->restarts * ldh * m_, 0.0);
+    c->trace_g.assign((size_t)o->restarts * ldh, 0.0);
+    c->trace_mm.assign(o->restarts, 0);
     vec_fill(c->d_du, 0.0, n, c->st);
+    HIP_CHECK_THROW(hipMemsetAsync(c->d_gate + 1, 0, sizeof(int), c->st));
     launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
     const double bnorm = c->norm(c->d_b);
     const double tol = bnorm * 1.0E-9;
     s->bnorm = bnorm;
     for (int gloop = 0; gloop < o->restarts; gloop++) {
         int mm = m_;
-        std::fill(h.begin(), h.end(), 0.0);
-        std::fill(g.begin(), g.end(), 0.0);
-        std::fill(y.begin(), y.end(), 0.0);
+        // clear() (:821-831): h, g, y start from zero every restart
+        vec_fill(H, 0.0, (long)ldh * m_, c->st);
         c->matvec(c->d_du, c->d_y, 64, false);
         s->matvecs64++;
         vec_sub(c->d_r, c->d_b, c->d_y, n, c->st);
         const double mag_r = c->norm(c->d_r);
-        g[0] = mag_r;
         if (!(mag_r > 0)) s->breakdown = 1;
+        launch_gmres_start(g, m_, mag_r, c->d_gate, c->st);
         // v_0 = r / mag_r
         vec_set(c->d_small + 8, mag_r, c->st);
         vec_scale_inv<double>(V, c->d_r, c->d_small + 8, n, c->st);
@@ -729,66 +772,36 @@ int solve_reference(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) 
                 vec_axpy(w, V + (size_t)i * ldv, c->d_hcol + i, -1.0, n, c->st);
             }
             c->multidot<double>(w, 1, w, c->d_hcol + j + 1);
-            std::vector<double> col(j + 2);
-            HIP_CHECK_THROW(hipMemcpyAsync(col.data(), c->d_hcol, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, c->st));
-            c->sync();
-            for (int i = 0; i <= j; i++) H(i, j) = col[i];
-            H(j + 1, j) = std::sqrt(col[j + 1]);
-            if (!(H(j + 1, j) > 0)) s->breakdown = 1;
-            vec_set(c->d_small + 9, H(j + 1, j), c->st);
+            launch_ref_arnoldi(H, ldh, c->d_hcol, j, mag_r, tol, c->d_small + 9, c->d_gate + 1, c->d_steps, c->st);
             vec_scale_inv<double>(V + (size_t)(j + 1) * ldv, w, c->d_small + 9, n, c->st);
-            // rotate_and_check (:696-730): c, s from the unrotated h (Q3)
-            const int nn = j + 1;
-            std::vector<double> hh((size_t)(nn + 1) * nn), gg(nn + 1, 0.0);
-            gg[0] = g[0];
-            for (int i = 0; i < nn + 1; i++)
-                for (int jj = 0; jj < nn; jj++) hh[i + (size_t)(nn + 1) * jj] = H(i, jj);
-            for (int i = 0; i < nn; i++) {
-                const double mag = std::sqrt(H(i, i) * H(i, i) + H(i + 1, i) * H(i + 1, i));
-                const double cc = H(i, i) / mag, ss = H(i + 1, i) / mag;
-                for (int jj = 0; jj < nn; jj++) {
-                    const double h0 = hh[i + (size_t)(nn + 1) * jj], h1 = hh[i + 1 + (size_t)(nn + 1) * jj];
-                    hh[i + (size_t)(nn + 1) * jj] = cc * h0 + ss * h1;
-                    hh[i + 1 + (size_t)(nn + 1) * jj] = -ss * h0 + cc * h1;
-                }
-                gg[i + 1] = -ss * gg[i];
-                gg[i] = cc * gg[i];
-            }
-            if (std::fabs(gg[nn]) < tol) {
+            c->check_launch();
+            HIP_CHECK_THROW(hipEventRecord(c->ev_step[0], c->st));
+            cfdv_ctx::wait_event(c->ev_step[0]);
+            if (c->h_steps[j].stop) {
                 mm = j + 1;
                 break;
             }
         }
-        // rotate (:733-752)
-        for (int i = 0; i < mm; i++) {
-            const double mag = std::sqrt(H(i, i) * H(i, i) + H(i + 1, i) * H(i + 1, i));
-            const double cc = H(i, i) / mag, ss = H(i + 1, i) / mag;
-            for (int jj = 0; jj < mm; jj++) {
-                const double h0 = H(i, jj), h1 = H(i + 1, jj);
-                H(i, jj) = cc * h0 + ss * h1;
-                H(i + 1, jj) = -ss * h0 + cc * h1;
-            }
-            g[i + 1] = -ss * g[i];
-            g[i] = cc * g[i];
-        }
-        s->est_resid = std::fabs(g[mm]);
-        // solve (:756-780): the back substitution reads h transposed (Q2)
-        for (int j = mm - 1; j >= 0; j--) {
-            y[j] = g[j];
-            for (int i = mm - 1; i >= 0; i--)
-                if (i != j) y[j] -= H(i, j) * y[i];
-            y[j] /= H(j, j);
-        }
-        std::memcpy(c->h_pin, y.data(), sizeof(double) * mm);
-        HIP_CHECK_THROW(hipMemcpyAsync(c->d_small + 16, c->h_pin, sizeof(double) * mm, hipMemcpyHostToDevice, c->st));
-        vec_combine<double>(c->d_du, V, ldv, mm, c->d_small + 16, n, c->st);
+        launch_ref_finish(H, c->d_hessr, ldh, g, mm, y, c->st);
+        vec_combine<double>(c->d_du, V, ldv, mm, y, n, c->st);
+        c->check_launch();
+        HIP_CHECK_THROW(hipMemcpyAsync(c->trace_H.data() + (size_t)gloop * ldh * m_, H, sizeof(double) * ldh * m_,
+                                       hipMemcpyDeviceToHost, c->st));
+        HIP_CHECK_THROW(hipMemcpyAsync(c->trace_g.data() + (size_t)gloop * ldh, g, sizeof(double) * ldh,
+                                       hipMemcpyDeviceToHost, c->st));
+        c->sync();
+        c->trace_mm[gloop] = mm;
+        s->est_resid = std::fabs(c->trace_g[(size_t)gloop * ldh + mm]);
         s->restarts++;   // Q4: no early exit from the restart loop
     }
+    int brk = 0;
+    HIP_CHECK_THROW(hipMemcpyAsync(&brk, c->d_gate + 1, sizeof(int), hipMemcpyDeviceToHost, c->st));
     // final true residual for reporting
     c->matvec(c->d_du, c->d_y, 64, false);
     vec_sub(c->d_r, c->d_b, c->d_y, n, c->st);
     s->resid = c->norm(c->d_r);
     s->rel_resid = bnorm > 0 ? s->resid / bnorm : 0.0;
+    if (brk) s->breakdown = 1;
     c->have_du = true;
     if (s->breakdown) return CFDV_ERR_BREAKDOWN;
     if (!std::isfinite(s->resid)) return CFDV_ERR_NAN;
@@ -802,13 +815,25 @@ int solve_reference(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) 
 // right-preconditioned: K Pinv u = rhs, x = Pinv u, so the Givens estimate
 // is still the true residual; z is a work vector for Pinv V_j.  Returns
 // Arnoldi iterations; *conv set when the Givens estimate met tol_abs.
+//
+// The Hessenberg column, its rotations, g and the back substitution stay on
+// the device (launch_givens_step, launch_backsub).  The host enqueues step
+// j + 1 before it reads step j's GmresStep record, so the stream never waits
+// for the host; every kernel of a step is gated on the device flag that
+// step j's rotation sets when the cycle ends there, so the step enqueued
+// after the last one does no work.
 template <typename T>
 int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, int maxrestart, double tol_abs,
                double rnorm0, cfdv_solve_stats* s, bool* conv, T* V, T* w, const T* Pinv, T* z, bool cgs2) {
     const long n = c->ndof;
     const long ldv = c->ldv();
-    std::vector<double> h((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m);
-    auto H = [&](int i, int j) -> double& { return h[i + (size_t)(m + 1) * j]; };
+    const int ldh = m + 1;
+    double* H = c->d_hess;
+    double* g = H + (size_t)ldh * m;
+    double* cs = g + ldh;
+    double* sn = cs + m;
+    double* y = c->d_small + 16;
+    int* gate = c->d_gate;
     int iters = 0;
     *conv = false;
     double beta = rnorm0;
@@ -840,20 +865,16 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
             vec_set(c->d_small + 8, beta, c->st);
             vec_scale_inv<T>(V, rhs, c->d_small + 8, n, c->st);
         }
-        std::fill(h.begin(), h.end(), 0.0);
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
+        launch_gmres_start(g, m, beta, gate, c->st);
         int jdone = 0;
-        bool brk = false;
+        bool early = false;   // the end of the cycle was read one step late
         for (int j = 0; j < m; j++) {
             if (Pinv) {
-                vec_nodeblock_apply<T, T>(Pinv, V + (size_t)j * ldv, z, c->neqn, c->n_node, false, tr, c->st);
-                c->matvec(z, w, prec, tr);
+                vec_nodeblock_apply<T, T>(Pinv, V + (size_t)j * ldv, z, c->neqn, c->n_node, false, tr, c->st, gate);
+                c->matvec(z, w, prec, tr, gate);
             } else {
-                c->matvec(V + (size_t)j * ldv, w, prec, tr);
+                c->matvec(V + (size_t)j * ldv, w, prec, tr, gate);
             }
-            if (prec == 64) s->matvecs64++; else s->matvecs32++;
-            iters++;
             // Classical Gram-Schmidt.  CGS2 (two passes: h1 = V^T w,
             // w -= V h1, h2 = V^T w, w -= V h2, |w|^2) keeps the basis
             // orthogonal to working precision; CGS1 (h1 = V^T w,
@@ -861,117 +882,116 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
             // FP32 inner solve, whose result the FP64 refinement checks and
             // corrects.  Up to 32 basis vectors each pass is one fused kernel.
             const int nv = j + 1;
-            std::vector<double> col(nv), col2(nv, 0.0);
-            double nrm2;
             if (nv <= 32) {
-                vec_cgs<T>(0, w, V, ldv, nv, nullptr, n, c->d_part, NBLK_RED, c->d_hcol, c->st);
+                vec_cgs<T>(0, w, V, ldv, nv, nullptr, n, c->d_part, NBLK_RED, c->d_hcol, c->st, gate);
                 if (cgs2)
-                    vec_cgs<T>(1, w, V, ldv, nv, c->d_hcol, n, c->d_part, NBLK_RED, c->d_hcol + 1024, c->st);
+                    vec_cgs<T>(1, w, V, ldv, nv, c->d_hcol, n, c->d_part, NBLK_RED, c->d_hcol + 1024, c->st, gate);
                 vec_cgs<T>(2, w, V, ldv, nv, cgs2 ? c->d_hcol + 1024 : c->d_hcol, n, c->d_part, NBLK_RED,
-                           c->d_hcol + 2040, c->st);
+                           c->d_hcol + 2040, c->st, gate);
             } else {
-                vec_multidot<T>(V, ldv, nv, w, n, c->d_part, c->d_hcol, NBLK_RED, c->st);
-                vec_multi_axpy<T>(w, V, ldv, nv, c->d_hcol, 1.0, n, c->st);
+                vec_multidot<T>(V, ldv, nv, w, n, c->d_part, c->d_hcol, NBLK_RED, c->st, gate);
+                vec_multi_axpy<T>(w, V, ldv, nv, c->d_hcol, 1.0, n, c->st, gate);
                 if (cgs2) {
-                    vec_multidot<T>(V, ldv, nv, w, n, c->d_part, c->d_hcol + 1024, NBLK_RED, c->st);
-                    vec_multi_axpy<T>(w, V, ldv, nv, c->d_hcol + 1024, 1.0, n, c->st);
+                    vec_multidot<T>(V, ldv, nv, w, n, c->d_part, c->d_hcol + 1024, NBLK_RED, c->st, gate);
+                    vec_multi_axpy<T>(w, V, ldv, nv, c->d_hcol + 1024, 1.0, n, c->st, gate);
                 }
-                c->multidot<T>(w, 1, w, c->d_hcol + 2040);
+                vec_multidot<T>(w, ldv, 1, w, n, c->d_part, c->d_hcol + 2040, NBLK_RED, c->st, gate);
             }
-            HIP_CHECK_THROW(hipMemcpyAsync(col.data(), c->d_hcol, sizeof(double) * nv, hipMemcpyDeviceToHost, c->st));
-            if (cgs2)
-                HIP_CHECK_THROW(hipMemcpyAsync(col2.data(), c->d_hcol + 1024, sizeof(double) * nv, hipMemcpyDeviceToHost, c->st));
-            HIP_CHECK_THROW(hipMemcpyAsync(&nrm2, c->d_hcol + 2040, sizeof(double), hipMemcpyDeviceToHost, c->st));
-            c->sync();
-            for (int i = 0; i <= j; i++) H(i, j) = col[i] + col2[i];
-            const double hn = std::sqrt(nrm2);
-            H(j + 1, j) = hn;
-            if (!std::isfinite(hn)) { s->breakdown = 1; brk = true; }
-            if (hn > 0 && !brk) {
-                vec_set(c->d_small + 9, H(j + 1, j), c->st);
-                vec_scale_inv<T>(V + (size_t)(j + 1) * ldv, w, c->d_small + 9, n, c->st);
-            } else {
-                brk = true;   // happy breakdown or failure: the space is invariant
-            }
-            for (int i = 0; i < j; i++) {
-                const double h0 = H(i, j), h1 = H(i + 1, j);
-                H(i, j) = cs[i] * h0 + sn[i] * h1;
-                H(i + 1, j) = -sn[i] * h0 + cs[i] * h1;
-            }
-            const double mag = std::sqrt(H(j, j) * H(j, j) + H(j + 1, j) * H(j + 1, j));
-            cs[j] = H(j, j) / mag;
-            sn[j] = H(j + 1, j) / mag;
-            H(j, j) = mag;
-            H(j + 1, j) = 0.0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
+            launch_givens_step(H, ldh, g, cs, sn, c->d_hcol, cgs2 ? c->d_hcol + 1024 : nullptr, c->d_hcol + 2040, j,
+                               tol_abs, c->d_small + 9, gate, c->d_steps, c->st);
+            vec_scale_inv<T>(V + (size_t)(j + 1) * ldv, w, c->d_small + 9, n, c->st, gate);
+            c->check_launch();
+            HIP_CHECK_THROW(hipEventRecord(c->ev_step[j & 1], c->st));
+            if (prec == 64) s->matvecs64++; else s->matvecs32++;
+            iters++;
             jdone = j + 1;
-            s->est_resid = std::fabs(g[j + 1]);
-            if (s->est_resid <= tol_abs || brk) break;
+            if (j > 0) {
+                cfdv_ctx::wait_event(c->ev_step[(j - 1) & 1]);
+                if (c->h_steps[j - 1].stop) {
+                    // step j was enqueued behind the last step and did nothing
+                    if (prec == 64) s->matvecs64--; else s->matvecs32--;
+                    iters--;
+                    jdone = j;
+                    early = true;
+                    c->drop_last_matvec_mark();
+                    break;
+                }
+            }
         }
-        for (int j = jdone - 1; j >= 0; j--) {
-            y[j] = g[j];
-            for (int i = j + 1; i < jdone; i++) y[j] -= H(j, i) * y[i];
-            y[j] /= H(j, j);
-        }
-        std::memcpy(c->h_pin, y.data(), sizeof(double) * jdone);
-        HIP_CHECK_THROW(hipMemcpyAsync(c->d_small + 16, c->h_pin, sizeof(double) * jdone, hipMemcpyHostToDevice, c->st));
-        c->sync();
+        if (!early) cfdv_ctx::wait_event(c->ev_step[(jdone - 1) & 1]);
+        const GmresStep last = c->h_steps[jdone - 1];
+        s->est_resid = last.est;
+        if (last.fail) s->breakdown = 1;
+        launch_backsub(H, ldh, g, jdone, y, c->st);
         if (Pinv) {
             // x += Pinv (V y), the combination formed in FP64
             vec_fill(c->d_t64, 0.0, n, c->st);
-            vec_combine<T>(c->d_t64, V, ldv, jdone, c->d_small + 16, n, c->st);
+            vec_combine<T>(c->d_t64, V, ldv, jdone, y, n, c->st);
             vec_nodeblock_apply<double, T>(Pinv, c->d_t64, x, c->neqn, c->n_node, true, tr, c->st);
         } else {
-            vec_combine<T>(x, V, ldv, jdone, c->d_small + 16, n, c->st);
+            vec_combine<T>(x, V, ldv, jdone, y, n, c->st);
         }
+        c->check_launch();
         s->restarts++;
-        if (s->est_resid <= tol_abs) { *conv = true; break; }
-        if (brk) break;
+        if (last.est <= tol_abs) { *conv = true; break; }
+        if (last.stop) break;   // invariant subspace or failure
     }
     return iters;
 }
 
 // K x = b (K^T x = b with o->transpose) for device b, x (both [ndof]).
 // warm: x holds a starting guess x0 on entry; K d = b - K x0 is solved to
-// the same absolute tolerance rtol ||b|| and x = x0 + d.
+// the same absolute tolerance rtol ||b|| and x = x0 + d.  The Givens
+// estimate ends a GMRES cycle; the true residual b - K x decides: while it is
+// above rtol ||b|| the solve continues with K d = b - K x from the current x,
+// within the same total of o->restarts restarts, and fails with NOCONV when
+// they run out.
 int solve_fp64_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, const double* b, double* x,
                    bool warm = false) {
     const long n = c->ndof;
     if (!c->have_blocks(64, o->transpose != 0)) return fail(c, CFDV_ERR_ARG, "FP64 solve needs CFDV_BLOCKS_FP64");
     c->ensure_krylov(o->m);
+    if (!c->d_r0) c->d_r0 = c->alloc<double>((size_t)n);
     const double bnorm = c->norm(b);
     s->bnorm = bnorm;
     bool conv = false;
     const bool pc = o->precond == CFDV_PRECOND_NODE_BLOCK;
     if (pc) c->ensure_precond();
     const bool tr = o->transpose != 0;
-    if (!warm) {
-        vec_fill(x, 0.0, n, c->st);
-        if (bnorm > 0)
-            s->iters += gmres_core<double>(c, 64, tr, b, x, o->m, o->restarts, o->rtol * bnorm, bnorm, s, &conv,
-                                           c->d_V, c->d_w, pc ? c->d_P64 : nullptr, pc ? c->d_z64 : nullptr, true);
-    } else {
-        if (!c->d_r0) c->d_r0 = c->alloc<double>((size_t)n);
+    const double tol = o->rtol * bnorm;
+    // r0 = b - K x and its norm (the product is counted by the caller when
+    // a further cycle uses it)
+    auto residual = [&]() {
         c->matvec(x, c->d_y, 64, tr);
-        s->matvecs64++;
         vec_sub(c->d_r0, b, c->d_y, n, c->st);
-        const double r0 = c->norm(c->d_r0);
-        if (r0 > o->rtol * bnorm) {
-            vec_fill(c->d_d64, 0.0, n, c->st);
-            s->iters += gmres_core<double>(c, 64, tr, c->d_r0, c->d_d64, o->m, o->restarts, o->rtol * bnorm, r0, s,
-                                           &conv, c->d_V, c->d_w, pc ? c->d_P64 : nullptr, pc ? c->d_z64 : nullptr,
-                                           true);
-            vec_set(c->d_small + 10, 1.0, c->st);
-            vec_axpy(x, c->d_d64, c->d_small + 10, 1.0, n, c->st);
+        return c->norm(c->d_r0);
+    };
+    double rnorm = bnorm;
+    if (warm) {
+        rnorm = residual();
+        s->matvecs64++;
+    } else {
+        vec_fill(x, 0.0, n, c->st);
+        if (bnorm > 0) {
+            s->iters += gmres_core<double>(c, 64, tr, b, x, o->m, o->restarts, tol, bnorm, s, &conv, c->d_V, c->d_w,
+                                           pc ? c->d_P64 : nullptr, pc ? c->d_z64 : nullptr, true);
+            rnorm = residual();
         }
     }
-    c->matvec(x, c->d_y, 64, o->transpose != 0);
-    vec_sub(c->d_r, b, c->d_y, n, c->st);
-    s->resid = c->norm(c->d_r);
+    while (rnorm > tol && std::isfinite(rnorm) && s->restarts < o->restarts) {
+        if (!warm) s->matvecs64++;   // this residual product starts a further cycle
+        warm = false;
+        vec_fill(c->d_d64, 0.0, n, c->st);
+        s->iters += gmres_core<double>(c, 64, tr, c->d_r0, c->d_d64, o->m, o->restarts - s->restarts, tol, rnorm, s,
+                                       &conv, c->d_V, c->d_w, pc ? c->d_P64 : nullptr, pc ? c->d_z64 : nullptr, true);
+        vec_set(c->d_small + 10, 1.0, c->st);
+        vec_axpy(x, c->d_d64, c->d_small + 10, 1.0, n, c->st);
+        rnorm = residual();
+    }
+    s->resid = rnorm;
     s->rel_resid = bnorm > 0 ? s->resid / bnorm : 0.0;
     if (!std::isfinite(s->resid)) return fail(c, CFDV_ERR_NAN, "NaN in the FP64 solve");
-    if (s->rel_resid > o->rtol * 10 && bnorm > 0) return fail(c, CFDV_ERR_NOCONV, "FP64 GMRES did not converge");
+    if (s->rel_resid > o->rtol && bnorm > 0) return fail(c, CFDV_ERR_NOCONV, "FP64 GMRES did not converge");
     return CFDV_OK;
 }
 
@@ -980,14 +1000,27 @@ int solve_fp64_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, c
 // dt (3 r_1 - 3 r_2 + r_3) -- the rate extrapolated with a polynomial of
 // degree warm_start - 1, lowered while fewer updates are kept (after a
 // reset).  Modes 2 and 3 keep du_1 (and du_2) for the next step.
+// The first solve of a build forms the guess (and shifts the history); a
+// repeated solve of the same build starts from the same guess, kept in
+// d_x0, so it returns the same du and leaves the history as one solve does.
 bool warm_guess(cfdv_ctx* c, const cfdv_solve_opts* o) {
+    const long n = c->ndof;
+    if (c->warm_consumed) {
+        if (!c->warm_had) return false;
+        vec_copy(c->d_du, c->d_x0, n, c->st);
+        return true;
+    }
+    c->warm_consumed = true;
+    c->warm_had = false;
     c->n_saved = 0;
     if (!o->warm_start || o->transpose || !c->have_prev || !(c->prev_dt > 0) || !(c->last_dt > 0)) return false;
-    const long n = c->ndof;
+    if (!c->d_x0) c->d_x0 = c->alloc<double>((size_t)n);
     const int ws = std::min(std::max(o->warm_start, 1), 3);
     if (ws == 1) {
         vec_set(c->d_small + 12, c->prev_dt / c->last_dt, c->st);
         vec_scale_to(c->d_du, c->d_du, c->d_small + 12, n, c->st);
+        vec_copy(c->d_x0, c->d_du, n, c->st);
+        c->warm_had = true;
         return true;
     }
     const int keep = ws - 1;   // older increments kept for the next step
@@ -1013,12 +1046,15 @@ bool warm_guess(cfdv_ctx* c, const cfdv_solve_opts* o) {
     c->old_dt[0] = c->prev_dt;
     c->d_du = c->d_dut;
     c->d_dut = spare;
+    vec_copy(c->d_x0, c->d_du, n, c->st);
     c->check_launch();
     c->n_saved = std::min(c->n_old + 1, keep);
+    c->warm_had = true;
     return true;
 }
 
 int solve_fp64(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
+    if (!c->have_blocks(64, o->transpose != 0)) return fail(c, CFDV_ERR_ARG, "FP64 solve needs CFDV_BLOCKS_FP64");
     launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
     const bool warm = warm_guess(c, o);
     const int rc = solve_fp64_rhs(c, o, s, c->d_b, c->d_du, warm);
@@ -1092,6 +1128,8 @@ int solve_mixed_rhs(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s, 
 }
 
 int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
+    if (!c->have_blocks(32, o->transpose != 0))
+        return fail(c, CFDV_ERR_ARG, "mixed solve needs CFDV_BLOCKS_FP32 or CFDV_BLOCKS_ROWS");
     launch_compose(c->d_node_ptr, c->d_node_ent, c->d_rhs, c->k, c->neqn, c->n_node, c->d_b, c->st);
     const bool warm = warm_guess(c, o);
     const int rc = solve_mixed_rhs(c, o, s, c->d_b, c->d_du, warm);
@@ -1243,6 +1281,8 @@ int cfdv_build(cfdv_ctx* ctx, double dt) {
     if (ctx->timing) HIP_CHECK_THROW(hipEventRecord(ctx->ev[5], ctx->st));
     if (ctx->d_B64) launch_rows_assemble(ctx->ra, ctx->d_R64, false, ctx->d_B64, ctx->d_B32, ctx->st);
     ctx->rows_tr = false;
+    ctx->pinv_valid = false;   // the node-block inverses belong to the previous blocks
+    ctx->warm_consumed = false;
     if (ctx->timing) HIP_CHECK_THROW(hipEventRecord(ctx->ev[6], ctx->st));
     ctx->check_launch();
     int cnt[2];
@@ -1350,6 +1390,18 @@ int cfdv_solve(cfdv_ctx* ctx, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
         return fail(ctx, CFDV_ERR_ARG, "unknown solve mode");
     }
     GUARD_END
+}
+
+int cfdv_reference_trace(cfdv_ctx* ctx, int32_t* m, int32_t* n_restarts, double* H, double* g, int32_t* mm) {
+    if (!ctx) return CFDV_ERR_ARG;
+    if (ctx->trace_mm.empty()) return fail(ctx, CFDV_ERR_ARG, "no reference-mode solve to trace");
+    const size_t nr = ctx->trace_mm.size();
+    if (m) *m = ctx->trace_m;
+    if (n_restarts) *n_restarts = (int32_t)nr;
+    if (H) std::memcpy(H, ctx->trace_H.data(), sizeof(double) * ctx->trace_H.size());
+    if (g) std::memcpy(g, ctx->trace_g.data(), sizeof(double) * ctx->trace_g.size());
+    if (mm) std::memcpy(mm, ctx->trace_mm.data(), sizeof(int32_t) * nr);
+    return CFDV_OK;
 }
 
 int cfdv_get_du(cfdv_ctx* ctx, double* du) {
@@ -1634,6 +1686,15 @@ int cfdv_step(cfdv_ctx* ctx, double cfl, const cfdv_solve_opts* o, cfdv_step_sta
     if (!ctx || !o || !st) return CFDV_ERR_ARG;
     std::memset(st, 0, sizeof(*st));
     GUARD_BEGIN
+    // product timing is on for this step only, whichever way it returns
+    struct TimingOff {
+        cfdv_ctx* c;
+        ~TimingOff() {
+            c->timing = false;
+            c->mv_marks.clear();
+            c->ev_used = 0;
+        }
+    } timing_guard{ctx};
     ctx->timing = true;
     ctx->mv_marks.clear();
     ctx->ev_used = 0;
@@ -1645,7 +1706,7 @@ int cfdv_step(cfdv_ctx* ctx, double cfl, const cfdv_solve_opts* o, cfdv_step_sta
     if (rc) return rc;
     HIP_CHECK_THROW(hipEventRecord(ctx->ev[2], ctx->st));
     rc = cfdv_solve(ctx, o, &st->solve);
-    ctx->timing = false;
+    ctx->timing = false;   // update and RMS are not products
     if (rc) return rc;
     HIP_CHECK_THROW(hipEventRecord(ctx->ev[3], ctx->st));
     rc = cfdv_update(ctx);

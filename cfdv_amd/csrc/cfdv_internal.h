@@ -68,8 +68,11 @@ struct MatvecArgs {
     const int* conn;          // [n_elem_pad][nnod]
 };
 // y += sum over elements of one color of R_e x_e (or R_e^T x_e)
+// Kernels that take a gate (a device int) return at once when *gate != 0:
+// the Arnoldi step kernels of the restarted GMRES, launched one step ahead of
+// the host's convergence test, do no work after the step that converged.
 void launch_matvec_color(const MatvecArgs& a, const double* R64, const float* R32, const void* x, void* y,
-                         int prec, bool transpose, hipStream_t st);
+                         int prec, bool transpose, hipStream_t st, const int* gate = nullptr);
 // hex8 EBE product with 8 right-hand sides, FP32 blocks/vectors, vectors
 // [n_node*neqn][8]; one color range per launch, Y accumulated
 void launch_matvec_multi_color(const MatvecArgs& a, const float* R32, const float* X, float* Y, bool tr,
@@ -79,9 +82,9 @@ void launch_compose(const int* node_ptr, const int* node_ent, const double* rhs,
 
 // vector kernels; all reductions write per-block partials in a fixed order
 // and finish with a single-block reduction, so results are deterministic.
-void vec_fill(double* y, double v, long n, hipStream_t st);
+void vec_fill(double* y, double v, long n, hipStream_t st, const int* gate = nullptr);
 void vec_set(double* p, double v, hipStream_t st);   // *p = v, stream-ordered
-void vec_fill_f(float* y, float v, long n, hipStream_t st);
+void vec_fill_f(float* y, float v, long n, hipStream_t st, const int* gate = nullptr);
 void vec_copy(double* y, const double* x, long n, hipStream_t st);
 void vec_axpy(double* y, const double* x, const double* alpha_dev, double scale, long n, hipStream_t st);
 void vec_sub(double* r, const double* b, const double* y, long n, hipStream_t st);  // r = b - y
@@ -92,18 +95,19 @@ void vec_update_state(double* U, double* U0, const double* du, long n, hipStream
 // dots: out[j] = <V_j, w> for j < nv (V rows of length n, stride ldv)
 template <typename T>
 void vec_multidot(const T* V, long ldv, int nv, const T* w, long n, double* part, double* out, int nblk,
-                  hipStream_t st);
+                  hipStream_t st, const int* gate = nullptr);
 // Classical Gram-Schmidt passes on 16-byte loads (V_j = V + j ldv, ldv a
 // multiple of 64, nv <= 32): mode 0 out[j] = <V_j, w>; mode 1 w -= sum h_j
 // V_j then out[j] = <V_j, w>; mode 2 w -= sum h_j V_j then out[0] = <w, w>.
 // part holds nblk * nv doubles.
 template <typename T>
 void vec_cgs(int mode, T* w, const T* V, long ldv, int nv, const double* h, long n, double* part, int nblk,
-             double* out, hipStream_t st);
+             double* out, hipStream_t st, const int* gate = nullptr);
 template <typename T>
-void vec_multi_axpy(T* w, const T* V, long ldv, int nv, const double* h, double sign, long n, hipStream_t st);
+void vec_multi_axpy(T* w, const T* V, long ldv, int nv, const double* h, double sign, long n, hipStream_t st,
+                    const int* gate = nullptr);
 template <typename T>
-void vec_scale_inv(T* y, const T* x, const double* denom_dev, long n, hipStream_t st);
+void vec_scale_inv(T* y, const T* x, const double* denom_dev, long n, hipStream_t st, const int* gate = nullptr);
 template <typename T>
 void vec_combine(double* x, const T* V, long ldv, int nv, const double* y, long n, hipStream_t st);
 // node-block Jacobi: Pinv per node (entry r neqn + c), FP64 and FP32 copies,
@@ -115,8 +119,39 @@ void launch_nodeblock(const int* node_ptr, const int* node_ent, const double* R6
 // z = Pinv v (accumulate: z += Pinv v; trans: Pinv^T)
 template <typename TV, typename TP>
 void vec_nodeblock_apply(const TP* P, const TV* v, TV* z, int neqn, long n_node, bool accumulate, bool trans,
-                         hipStream_t st);
+                         hipStream_t st, const int* gate = nullptr);
 void vec_sumsq_eqn(const double* U, int neqn, long n_node, double* part, double* out, int nblk, hipStream_t st);
+
+// ---- GMRES least squares on the device (solve.hip) ----
+// H: (m+1) x m Hessenberg, column-major, leading dimension ldh = m+1; g: the
+// rotated right-hand side [m+1]; cs, sn: Givens rotations [m]; y [m].
+// Per Arnoldi step j the device writes a GmresStep to host-visible memory.
+struct GmresStep {
+    double est;   // |g_{j+1}|: the residual estimate after step j
+    int stop;     // the cycle ends after step j (converged, invariant space or failure)
+    int fail;     // |w| not finite
+};
+// g = (beta, 0, ..., 0), *gate = 0 (start of a cycle)
+void launch_gmres_start(double* g, int m, double beta, int* gate, hipStream_t st);
+// step j: column j of H = h1 (+ h2) with H(j+1, j) = sqrt(*nrm2), rotated by the
+// previous rotations; rotation j; g rotated; *hn_out = sqrt(*nrm2); *gate and
+// out[j].stop = est <= tol, or |w| == 0, or |w| not finite.  Gated itself.
+void launch_givens_step(double* H, int ldh, double* g, double* cs, double* sn, const double* h1, const double* h2,
+                        const double* nrm2, int j, double tol, double* hn_out, int* gate, GmresStep* out,
+                        hipStream_t st);
+// y = R^-1 g for the leading n x n upper triangle of the rotated H
+void launch_backsub(const double* H, int ldh, const double* g, int n, double* y, hipStream_t st);
+// The reference GMRES (MPI_GMRES.h at one rank, quirks Q2-Q3) with H on the
+// device: column j of H from the modified Gram-Schmidt dots hc[0..j] and
+// |w|^2 = hc[j+1] (:154-181; *breakdown = 1 when |w| is not positive), then
+// rotate_and_check's estimate (:696-730, angles from the unrotated H):
+// out[j].est = |g0 prod_i (-s_i)|, out[j].stop = est < tol; *hn_out = |w|.
+void launch_ref_arnoldi(double* H, int ldh, const double* hc, int j, double g0, double tol, double* hn_out,
+                        int* breakdown, GmresStep* out, hipStream_t st);
+// rotate (:733-752) on a copy Hr of H's leading (mm+1) x mm block and g in
+// place, then the back substitution reading h transposed (:756-780, Q2)
+// into y [mm]
+void launch_ref_finish(const double* H, double* Hr, int ldh, double* g, int mm, double* y, hipStream_t st);
 void min_reduce(const double* part, int n, double* out, hipStream_t st);
 
 // ---- assembled node rows (rows.hip) ----
@@ -133,7 +168,7 @@ struct RowsArgs {
 // y = K x over the rows; prec 64 (FP64 rows and vectors), 32 (FP32), 3 (FP32
 // rows, FP64 vectors).  y is written, not accumulated.
 void launch_rows_matvec(const RowsArgs& a, const double* B64, const float* B32, const void* x, void* y, int prec,
-                        hipStream_t st);
+                        hipStream_t st, const int* gate = nullptr);
 // Y = K X for 8 vectors over the FP32 rows, X and Y [n_node * neqn][8]
 void launch_rows_matvec8(const RowsArgs& a, const float* B32, const float* X, float* Y, hipStream_t st);
 // rows (FP64 and/or FP32, either may be null) from the FP64 element blocks:

@@ -55,7 +55,7 @@ CFDV_HD void fdvparam(const TH& th, const S vn[][NDIM], const S* Tn, const S* rh
     s[3] = 0.5 * (1 + pow(s[2], 0.2));
 }
 
-// NodeUnpack (src/FDV/Function/NodeUnpack.h:124-144) for one node
+// NodeUnpack (src/FDV/Function/NodeUnpack.h:9-29) for one node
 template <int NDIM, typename TH, typename S>
 CFDV_HD void unpack(const TH& th, const S* U, S& rho, S v[NDIM], S& E, S& T) {
     rho = U[0];
@@ -268,7 +268,7 @@ CFDV_HD void build3d_faces(const BuildArgs& A, const TH& th, const GaussTables& 
                 }
                 if (Hn) {
                     // face corners from the Hnm-mapped element nodes
-                    // (calc_TF_norm3D.h:169-200)
+                    // (calc_TF_norm3D.h:35-66)
                     for (int t = 0; t < 4; t++)
 #pragma unroll
                         for (int i = 0; i < 3; i++) {
@@ -277,7 +277,7 @@ CFDV_HD void build3d_faces(const BuildArgs& A, const TH& th, const GaussTables& 
                             nl[t][i] = acc;
                         }
                 }
-                // TFNorm3D (calc_TF_norm3D.h:143-301), including the Lambda/Mu
+                // TFNorm3D (calc_TF_norm3D.h:9-167), including the Lambda/Mu
                 // magnitude recomputed after each component (:208-212)
                 double phib[4];
                 phib[0] = 0.25 * (1 - z0) * (1 - z1);

@@ -87,6 +87,93 @@ CFDV_HD S calc_k(const TH& th, const S& mu) {
     return mu / ((th.gamma - 1.0) * (th.cmach * th.cmach) * th.Pr);
 }
 
+// coth(x) = cosh(x)/sinh(x) as the reference's host libm (glibc 2.35)
+// evaluates it, for the quad4 numerical diffusion: alpha = coth(R/2) - 2/R
+// (FDVGalerkin.h:46-50, 308-316) cancels as R -> 0, so the reference's
+// blocks carry the last bits of its cosh and sinh.  The method is fdlibm's
+// (Sun Microsystems, 1993): expm1 by reduction to |r| <= ln2/2 with
+// k = round(x/ln2) and a rational approximation in r^2 whose polynomial is
+// evaluated in glibc's grouping R1 + h2 R2 + h4 R3; then
+// sinh(x) = sign(x)/2 (2t - t^2/(t+1)) for |x| < 1, sign(x)/2 (t + t/(t+1))
+// for |x| < 22, and cosh(x) = 1 + t^2/(2(1+t)) for |x| < ln2/2, with
+// t = expm1(|x|).  Measured bit-identical to glibc 2.35's expm1, sinh and
+// cosh on 25 million random arguments in those ranges.  Elsewhere the device
+// libm is used: there |R| > ln 2 and the subtraction no longer cancels.
+CFDV_HD double fdlibm_expm1(double x) {
+#pragma clang fp contract(off)
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    const double invln2 = 1.44269504088896338700e+00;
+    const double Q1 = -3.33333333333331316428e-02, Q2 = 1.58730158725481460165e-03,
+                 Q3 = -7.93650757867487942473e-05, Q4 = 4.00821782732936239552e-06,
+                 Q5 = -2.01099218183624371326e-07;
+    const unsigned long long bits = __builtin_bit_cast(unsigned long long, x);
+    const unsigned hx = (unsigned)(bits >> 32) & 0x7fffffffu;
+    const bool neg = (bits >> 63) != 0;
+    if (hx >= 0x4043687Au) return exp(x) - 1.0;   // |x| >= 38.8: not reached by the callers
+    int k = 0;
+    double c = 0.0;
+    if (hx > 0x3fd62e42u) {   // |x| > ln2/2: reduce
+        double hi, lo;
+        if (hx < 0x3FF0A2B2u) {   // |x| < 3 ln2 / 2
+            if (!neg) { hi = x - ln2_hi; lo = ln2_lo; k = 1; }
+            else { hi = x + ln2_hi; lo = -ln2_lo; k = -1; }
+        } else {
+            k = (int)(invln2 * x + (neg ? -0.5 : 0.5));
+            const double t = k;
+            hi = x - t * ln2_hi;
+            lo = t * ln2_lo;
+        }
+        x = hi - lo;
+        c = (hi - x) - lo;
+    } else if (hx < 0x3c900000u) {   // |x| < 2^-54
+        return x;
+    }
+    const double hfx = 0.5 * x;
+    const double hxs = x * hfx;
+    const double R1 = 1.0 + hxs * Q1, h2 = hxs * hxs;
+    const double R2 = Q2 + hxs * Q3, h4 = h2 * h2;
+    const double R3 = Q4 + hxs * Q5;
+    const double r1 = R1 + h2 * R2 + h4 * R3;
+    const double t = 3.0 - r1 * hfx;
+    double e = hxs * ((r1 - t) / (6.0 - x * t));
+    if (k == 0) return x - (x * e - hxs);
+    e = (x * (e - c) - c);
+    e -= hxs;
+    if (k == -1) return 0.5 * (x - e) - 0.5;
+    if (k == 1) {
+        if (x < -0.25) return -2.0 * (e - (x + 0.5));
+        return 1.0 + 2.0 * (x - e);
+    }
+    const double twopk = __builtin_bit_cast(double, (unsigned long long)(0x3ff + k) << 52);
+    if (k <= -2 || k > 56) {
+        const double y = 1.0 - (e - x);
+        return y * twopk - 1.0;
+    }
+    if (k < 20) {
+        const double o = __builtin_bit_cast(double, (unsigned long long)(0x3ff00000u - (0x200000u >> k)) << 32);
+        return (o - (e - x)) * twopk;
+    }
+    const double o = __builtin_bit_cast(double, (unsigned long long)(0x3ff - k) << 52);
+    return ((x - (e + o)) + 1.0) * twopk;
+}
+CFDV_HD double libm_sinh(double x) {
+#pragma clang fp contract(off)
+    const double ax = fabs(x), h = x < 0 ? -0.5 : 0.5;
+    if (!(ax < 22.0)) return sinh(x);
+    if (ax < 0x1p-28) return x;
+    const double t = fdlibm_expm1(ax);
+    if (ax < 1.0) return h * (2.0 * t - t * t / (t + 1.0));
+    return h * (t + t / (t + 1.0));
+}
+CFDV_HD double libm_cosh(double x) {
+#pragma clang fp contract(off)
+    const double ax = fabs(x);
+    if (!(ax < __builtin_bit_cast(double, 0x3fd62e4300000000ull))) return cosh(x);   // ~ln2/2
+    if (ax < 0x1p-55) return 1.0;
+    const double t = fdlibm_expm1(ax), w = 1.0 + t;
+    return 1.0 + (t * t) / (w + w);
+}
+
 // Gauss-point state (Flow.h:9-62 with Tau)
 template <int NDIM, typename S = double> struct GPState {
     static constexpr int NEQ = Dims<NDIM>::NEQ;

@@ -61,9 +61,10 @@ __device__ __forceinline__ long chunk_of_wave(long n_chunks) {
 template <typename TB, typename TV, typename TA, int NEQ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void rows_matvec_kernel(
     long n_node, long n_chunks, const long* __restrict__ goff, const int* __restrict__ cols,
-    const TB* __restrict__ B, const TV* __restrict__ x, TV* __restrict__ y) {
+    const TB* __restrict__ B, const TV* __restrict__ x, TV* __restrict__ y, const int* __restrict__ gate) {
     constexpr int BB = NEQ * NEQ, GV = 4 * BB, V = 16 / (int)sizeof(TB), NVEC = GV / V;
     typedef typename V16<TB>::type vec_t;
+    if (gate && *gate) return;
     const long c = chunk_of_wave(n_chunks);
     if (c >= n_chunks) return;
     const int lane = threadIdx.x & 63;
@@ -287,12 +288,12 @@ static int row_grid(long n_chunks) {
 }
 
 void launch_rows_matvec(const RowsArgs& a, const double* B64, const float* B32, const void* x, void* y, int prec,
-                        hipStream_t st) {
+                        hipStream_t st, const int* gate) {
     if (a.n_chunks <= 0) return;
     const dim3 grid(row_grid(a.n_chunks)), blk(256);
 #define L(TB, TV, TA, NQ, BP)                                                                              \
     hipLaunchKernelGGL((rows_matvec_kernel<TB, TV, TA, NQ>), grid, blk, 0, st, a.n_node, a.n_chunks, a.goff, \
-                       a.cols, BP, (const TV*)x, (TV*)y)
+                       a.cols, BP, (const TV*)x, (TV*)y, gate)
     if (a.neqn == 5) {
         if (prec == 64) L(double, double, double, 5, B64);
         else if (prec == 32) L(float, float, float, 5, B32);

@@ -26,9 +26,10 @@ template <> struct Vec16<float> { typedef v4f_t type; };
 template <typename TB, typename TV, typename TA, int NNOD, int NEQ, bool TR>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) void matvec_kernel(long e0, long e1, const int* __restrict__ conn,
                                                      const TB* __restrict__ R, const TV* __restrict__ x,
-                                                     TV* __restrict__ y) {
+                                                     TV* __restrict__ y, const int* __restrict__ gate) {
     constexpr int K = NNOD * NEQ, KK = K * K, V = 16 / (int)sizeof(TB), GPR = K / V;   // vector groups per row
     typedef typename Vec16<TB>::type vec_t;
+    if (gate && *gate) return;
     const long e = e0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= e1) return;
     const int* ce = conn + e * NNOD;
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 }
 
 void launch_matvec_color(const MatvecArgs& a, const double* R64, const float* R32, const void* x, void* y,
-                         int prec, bool tr, hipStream_t st) {
+                         int prec, bool tr, hipStream_t st, const int* gate) {
     const long n = a.e_end - a.e_begin;
     if (n <= 0) return;
     const int bs = 128;
@@ -107,7 +108,7 @@ void launch_matvec_color(const MatvecArgs& a, const double* R64, const float* R3
 #define L(TB, TV, TA, NN, NQ, T)                                                                      \
     hipLaunchKernelGGL((matvec_kernel<TB, TV, TA, NN, NQ, T>), dim3(nb), dim3(bs), 0, st, a.e_begin, \
                        a.e_end, a.conn, (const TB*)(prec == 64 ? (const void*)R64 : (const void*)R32),  \
-                       (const TV*)x, (TV*)y)
+                       (const TV*)x, (TV*)y, gate)
     if (a.ndim == 3) {
         if (prec == 64) { if (tr) L(double, double, double, 8, 5, true); else L(double, double, double, 8, 5, false); }
         else if (prec == 32) { if (tr) L(float, float, float, 8, 5, true); else L(float, float, float, 8, 5, false); }
@@ -265,10 +266,12 @@ static inline int grid_for(long n, int bs = 256, int cap = 8192) {
     return (int)nb;
 }
 
-__global__ void fill_kernel(double* y, double v, long n) {
+__global__ void fill_kernel(double* y, double v, long n, const int* gate) {
+    if (gate && *gate) return;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = v;
 }
-__global__ void fill_f_kernel(float* y, float v, long n) {
+__global__ void fill_f_kernel(float* y, float v, long n, const int* gate) {
+    if (gate && *gate) return;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = v;
 }
 __global__ void copy_kernel(double* y, const double* x, long n) {
@@ -307,11 +310,11 @@ __global__ void update_kernel(double* U, double* U0, const double* du, long n) {
 
 __global__ void set_kernel(double* p, double v) { *p = v; }
 void vec_set(double* p, double v, hipStream_t st) { hipLaunchKernelGGL(set_kernel, dim3(1), dim3(1), 0, st, p, v); }
-void vec_fill(double* y, double v, long n, hipStream_t st) {
-    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, v, n);
+void vec_fill(double* y, double v, long n, hipStream_t st, const int* gate) {
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, v, n, gate);
 }
-void vec_fill_f(float* y, float v, long n, hipStream_t st) {
-    hipLaunchKernelGGL(fill_f_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, v, n);
+void vec_fill_f(float* y, float v, long n, hipStream_t st, const int* gate) {
+    hipLaunchKernelGGL(fill_f_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, v, n, gate);
 }
 void vec_copy(double* y, const double* x, long n, hipStream_t st) {
     hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, x, n);
@@ -338,7 +341,9 @@ void vec_update_state(double* U, double* U0, const double* du, long n, hipStream
 // multi-dot: part[blk*nv + j] = sum over this block's slice of V_j . w
 template <typename T, int NV>
 __global__ __launch_bounds__(256) void multidot_kernel(const T* __restrict__ V, long ldv, int nv,
-                                                       const T* __restrict__ w, long n, double* __restrict__ part) {
+                                                       const T* __restrict__ w, long n, double* __restrict__ part,
+                                                       const int* __restrict__ gate) {
+    if (gate && *gate) return;
     __shared__ double sm[NV][256 / 64];
     double acc[NV];
 #pragma unroll
@@ -368,7 +373,9 @@ __global__ __launch_bounds__(256) void multidot_kernel(const T* __restrict__ V, 
 // out[j] = sum over blocks of part[blk*nv + j], fixed order
 // out[j] = sum_b part[b nv + j]: one block per j, each thread a strided
 // partial sum, then a fixed tree
-__global__ void finish_dot_kernel(const double* __restrict__ part, int nblk, int nv, double* __restrict__ out) {
+__global__ void finish_dot_kernel(const double* __restrict__ part, int nblk, int nv, double* __restrict__ out,
+                                  const int* __restrict__ gate) {
+    if (gate && *gate) return;
     __shared__ double sm[256];
     const int j = blockIdx.x;
     double acc = 0.0;
@@ -384,20 +391,22 @@ __global__ void finish_dot_kernel(const double* __restrict__ part, int nblk, int
 
 template <typename T>
 void vec_multidot(const T* V, long ldv, int nv, const T* w, long n, double* part, double* out, int nblk,
-                  hipStream_t st) {
+                  hipStream_t st, const int* gate) {
     for (int j0 = 0; j0 < nv; j0 += 16) {
         const int c = (nv - j0 < 16) ? nv - j0 : 16;
         if (c <= 1)
-            hipLaunchKernelGGL((multidot_kernel<T, 1>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part);
+            hipLaunchKernelGGL((multidot_kernel<T, 1>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part, gate);
         else if (c <= 4)
-            hipLaunchKernelGGL((multidot_kernel<T, 4>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part);
+            hipLaunchKernelGGL((multidot_kernel<T, 4>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part, gate);
         else
-            hipLaunchKernelGGL((multidot_kernel<T, 16>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part);
-        hipLaunchKernelGGL(finish_dot_kernel, dim3(c), dim3(256), 0, st, part, nblk, c, out + j0);
+            hipLaunchKernelGGL((multidot_kernel<T, 16>), dim3(nblk), dim3(256), 0, st, V + (size_t)j0 * ldv, ldv, c, w, n, part, gate);
+        hipLaunchKernelGGL(finish_dot_kernel, dim3(c), dim3(256), 0, st, part, nblk, c, out + j0, gate);
     }
 }
-template void vec_multidot<double>(const double*, long, int, const double*, long, double*, double*, int, hipStream_t);
-template void vec_multidot<float>(const float*, long, int, const float*, long, double*, double*, int, hipStream_t);
+template void vec_multidot<double>(const double*, long, int, const double*, long, double*, double*, int, hipStream_t,
+                                   const int*);
+template void vec_multidot<float>(const float*, long, int, const float*, long, double*, double*, int, hipStream_t,
+                                  const int*);
 
 // ---------------------------------------------------------------------------
 // Classical Gram-Schmidt kernels on 16-byte vector loads (V_j = V + j ldv,
@@ -438,7 +447,9 @@ __device__ __forceinline__ void block_partials(double (&acc)[NV], int nv, int st
 
 template <typename T, int NV, int MODE>   // MODE 0: dot, 1: axpy + dot, 2: axpy + norm
 __global__ __launch_bounds__(256) void cgs_kernel(T* __restrict__ w, const T* __restrict__ V, long ldv, int nv,
-                                                  const double* __restrict__ h, long n, double* __restrict__ part) {
+                                                  const double* __restrict__ h, long n, double* __restrict__ part,
+                                                  const int* __restrict__ gate) {
+    if (gate && *gate) return;
     constexpr int L = cgs_lanes<T, NV, MODE>();
     typedef typename VecT<T, L>::type vt;
     double hh[NV];
@@ -532,8 +543,9 @@ __global__ __launch_bounds__(256) void cgs_kernel(T* __restrict__ w, const T* __
 
 template <typename T>
 static void launch_cgs(int mode, T* w, const T* V, long ldv, int nv, const double* h, long n, double* part,
-                       int nblk, double* out, hipStream_t st) {
-#define CGS(NVV, MODE) hipLaunchKernelGGL((cgs_kernel<T, NVV, MODE>), dim3(nblk), dim3(256), 0, st, w, V, ldv, nv, h, n, part)
+                       int nblk, double* out, hipStream_t st, const int* gate) {
+#define CGS(NVV, MODE) \
+    hipLaunchKernelGGL((cgs_kernel<T, NVV, MODE>), dim3(nblk), dim3(256), 0, st, w, V, ldv, nv, h, n, part, gate)
 #define CGS_NV(MODE)                                   \
     if (nv <= 1) CGS(1, MODE);                         \
     else if (nv <= 4) CGS(4, MODE);                    \
@@ -546,24 +558,26 @@ static void launch_cgs(int mode, T* w, const T* V, long ldv, int nv, const doubl
 #undef CGS_NV
 #undef CGS
     const int cnt = mode == 2 ? 1 : nv;
-    hipLaunchKernelGGL(finish_dot_kernel, dim3(cnt), dim3(256), 0, st, part, nblk, cnt, out);
+    hipLaunchKernelGGL(finish_dot_kernel, dim3(cnt), dim3(256), 0, st, part, nblk, cnt, out, gate);
 }
 
 template <typename T>
 void vec_cgs(int mode, T* w, const T* V, long ldv, int nv, const double* h, long n, double* part, int nblk,
-             double* out, hipStream_t st) {
+             double* out, hipStream_t st, const int* gate) {
     if (nv > 32) throw std::runtime_error("vec_cgs: at most 32 basis vectors per call");
-    launch_cgs<T>(mode, w, V, ldv, nv, h, n, part, nblk, out, st);
+    launch_cgs<T>(mode, w, V, ldv, nv, h, n, part, nblk, out, st, gate);
 }
 template void vec_cgs<double>(int, double*, const double*, long, int, const double*, long, double*, int, double*,
-                              hipStream_t);
+                              hipStream_t, const int*);
 template void vec_cgs<float>(int, float*, const float*, long, int, const double*, long, double*, int, double*,
-                             hipStream_t);
+                             hipStream_t, const int*);
 
 // w -= sign * sum_j h[j] V_j  (h on device)
 template <typename T>
 __global__ __launch_bounds__(256) void multi_axpy_kernel(T* __restrict__ w, const T* __restrict__ V, long ldv, int nv,
-                                                         const double* __restrict__ h, double sign, long n) {
+                                                         const double* __restrict__ h, double sign, long n,
+                                                         const int* __restrict__ gate) {
+    if (gate && *gate) return;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         double acc = 0.0;
         for (int j = 0; j < nv; j++) acc += h[j] * (double)V[(size_t)j * ldv + i];
@@ -571,24 +585,28 @@ __global__ __launch_bounds__(256) void multi_axpy_kernel(T* __restrict__ w, cons
     }
 }
 template <typename T>
-void vec_multi_axpy(T* w, const T* V, long ldv, int nv, const double* h, double sign, long n, hipStream_t st) {
-    hipLaunchKernelGGL((multi_axpy_kernel<T>), dim3(grid_for(n)), dim3(256), 0, st, w, V, ldv, nv, h, sign, n);
+void vec_multi_axpy(T* w, const T* V, long ldv, int nv, const double* h, double sign, long n, hipStream_t st,
+                    const int* gate) {
+    hipLaunchKernelGGL((multi_axpy_kernel<T>), dim3(grid_for(n)), dim3(256), 0, st, w, V, ldv, nv, h, sign, n, gate);
 }
-template void vec_multi_axpy<double>(double*, const double*, long, int, const double*, double, long, hipStream_t);
-template void vec_multi_axpy<float>(float*, const float*, long, int, const double*, double, long, hipStream_t);
+template void vec_multi_axpy<double>(double*, const double*, long, int, const double*, double, long, hipStream_t,
+                                     const int*);
+template void vec_multi_axpy<float>(float*, const float*, long, int, const double*, double, long, hipStream_t,
+                                    const int*);
 
 template <typename T>
-__global__ void scale_inv_kernel(T* y, const T* x, const double* d, long n) {
+__global__ void scale_inv_kernel(T* y, const T* x, const double* d, long n, const int* gate) {
+    if (gate && *gate) return;
     const double dd = *d;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         y[i] = (T)((double)x[i] / dd);
 }
 template <typename T>
-void vec_scale_inv(T* y, const T* x, const double* d, long n, hipStream_t st) {
-    hipLaunchKernelGGL((scale_inv_kernel<T>), dim3(grid_for(n)), dim3(256), 0, st, y, x, d, n);
+void vec_scale_inv(T* y, const T* x, const double* d, long n, hipStream_t st, const int* gate) {
+    hipLaunchKernelGGL((scale_inv_kernel<T>), dim3(grid_for(n)), dim3(256), 0, st, y, x, d, n, gate);
 }
-template void vec_scale_inv<double>(double*, const double*, const double*, long, hipStream_t);
-template void vec_scale_inv<float>(float*, const float*, const double*, long, hipStream_t);
+template void vec_scale_inv<double>(double*, const double*, const double*, long, hipStream_t, const int*);
+template void vec_scale_inv<float>(float*, const float*, const double*, long, hipStream_t, const int*);
 
 // x += sum_j y[j] V_j
 template <typename T>
@@ -606,6 +624,145 @@ void vec_combine(double* x, const T* V, long ldv, int nv, const double* y, long 
 }
 template void vec_combine<double>(double*, const double*, long, int, const double*, long, hipStream_t);
 template void vec_combine<float>(double*, const float*, long, int, const double*, long, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// GMRES least squares on the device, in FP64 whatever the basis precision.
+// One workgroup; the serial parts run on lane 0 (at most a few hundred
+// flops per Arnoldi step).  The host reads only the per-step GmresStep
+// records, from host-visible memory, one step behind the stream.
+// ---------------------------------------------------------------------------
+__global__ void gmres_start_kernel(double* __restrict__ g, int m, double beta, int* __restrict__ gate) {
+    for (int i = threadIdx.x; i <= m; i += blockDim.x) g[i] = (i == 0) ? beta : 0.0;
+    if (threadIdx.x == 0) *gate = 0;
+}
+void launch_gmres_start(double* g, int m, double beta, int* gate, hipStream_t st) {
+    hipLaunchKernelGGL(gmres_start_kernel, dim3(1), dim3(64), 0, st, g, m, beta, gate);
+}
+
+__global__ void givens_step_kernel(double* __restrict__ H, int ldh, double* __restrict__ g, double* __restrict__ cs,
+                                   double* __restrict__ sn, const double* __restrict__ h1,
+                                   const double* __restrict__ h2, const double* __restrict__ nrm2, int j,
+                                   double tol, double* __restrict__ hn_out, int* __restrict__ gate,
+                                   GmresStep* __restrict__ out) {
+    if (*gate) return;
+    double* Hj = H + (size_t)ldh * j;
+    for (int i = threadIdx.x; i <= j; i += blockDim.x) Hj[i] = h2 ? h1[i] + h2[i] : h1[i];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double hn = sqrt(*nrm2);
+    const int fail = !isfinite(hn);
+    Hj[j + 1] = hn;
+    for (int i = 0; i < j; i++) {
+        const double a = Hj[i], b = Hj[i + 1];
+        Hj[i] = cs[i] * a + sn[i] * b;
+        Hj[i + 1] = -sn[i] * a + cs[i] * b;
+    }
+    const double mag = sqrt(Hj[j] * Hj[j] + Hj[j + 1] * Hj[j + 1]);
+    cs[j] = Hj[j] / mag;
+    sn[j] = Hj[j + 1] / mag;
+    Hj[j] = mag;
+    Hj[j + 1] = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    const double est = fabs(g[j + 1]);
+    const int stop = (est <= tol) || fail || !(hn > 0);
+    *hn_out = hn;
+    *gate = stop;
+    out[j].est = est;
+    out[j].stop = stop;
+    out[j].fail = fail;
+    __threadfence_system();
+}
+void launch_givens_step(double* H, int ldh, double* g, double* cs, double* sn, const double* h1, const double* h2,
+                        const double* nrm2, int j, double tol, double* hn_out, int* gate, GmresStep* out,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(givens_step_kernel, dim3(1), dim3(64), 0, st, H, ldh, g, cs, sn, h1, h2, nrm2, j, tol, hn_out,
+                       gate, out);
+}
+
+__global__ void backsub_kernel(const double* __restrict__ H, int ldh, const double* __restrict__ g, int n,
+                               double* __restrict__ y) {
+    if (threadIdx.x != 0) return;
+    for (int j = n - 1; j >= 0; j--) {
+        double v = g[j];
+        for (int i = j + 1; i < n; i++) v -= H[j + (size_t)ldh * i] * y[i];
+        y[j] = v / H[j + (size_t)ldh * j];
+    }
+}
+void launch_backsub(const double* H, int ldh, const double* g, int n, double* y, hipStream_t st) {
+    hipLaunchKernelGGL(backsub_kernel, dim3(1), dim3(64), 0, st, H, ldh, g, n, y);
+}
+
+// The reference's arithmetic, unfused as the host compiler leaves it.
+__global__ void ref_arnoldi_kernel(double* __restrict__ H, int ldh, const double* __restrict__ hc, int j, double g0,
+                                   double tol, double* __restrict__ hn_out, int* __restrict__ breakdown,
+                                   GmresStep* __restrict__ out) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    double* Hj = H + (size_t)ldh * j;
+    for (int i = 0; i <= j; i++) Hj[i] = hc[i];
+    const double hn = sqrt(hc[j + 1]);
+    Hj[j + 1] = hn;
+    if (!(hn > 0)) *breakdown = 1;
+    *hn_out = hn;
+    // gg[i+1] = -s_i gg[i] with c_i, s_i from the unrotated H (Q3)
+    double gg = g0;
+    for (int i = 0; i <= j; i++) {
+        const double hii = H[i + (size_t)ldh * i], hi1 = H[i + 1 + (size_t)ldh * i];
+        const double mag = sqrt(hii * hii + hi1 * hi1);
+        const double ss = hi1 / mag;
+        gg = -ss * gg;
+    }
+    const double est = fabs(gg);
+    out[j].est = est;
+    out[j].stop = est < tol;
+    out[j].fail = 0;
+    __threadfence_system();
+}
+void launch_ref_arnoldi(double* H, int ldh, const double* hc, int j, double g0, double tol, double* hn_out,
+                        int* breakdown, GmresStep* out, hipStream_t st) {
+    hipLaunchKernelGGL(ref_arnoldi_kernel, dim3(1), dim3(64), 0, st, H, ldh, hc, j, g0, tol, hn_out, breakdown, out);
+}
+
+__global__ void ref_finish_kernel(const double* __restrict__ H, double* __restrict__ Hr, int ldh,
+                                  double* __restrict__ g, int mm, double* __restrict__ y) {
+#pragma clang fp contract(off)
+    __shared__ double cssn[2];
+    for (int jj = 0; jj < mm; jj++)
+        for (int i = threadIdx.x; i <= mm; i += blockDim.x) Hr[i + (size_t)ldh * jj] = H[i + (size_t)ldh * jj];
+    __syncthreads();
+    for (int i = 0; i < mm; i++) {
+        if (threadIdx.x == 0) {
+            const double hii = Hr[i + (size_t)ldh * i], hi1 = Hr[i + 1 + (size_t)ldh * i];
+            const double mag = sqrt(hii * hii + hi1 * hi1);
+            cssn[0] = hii / mag;
+            cssn[1] = hi1 / mag;
+        }
+        __syncthreads();
+        const double cc = cssn[0], ss = cssn[1];
+        for (int jj = threadIdx.x; jj < mm; jj += blockDim.x) {
+            const double h0 = Hr[i + (size_t)ldh * jj], h1 = Hr[i + 1 + (size_t)ldh * jj];
+            Hr[i + (size_t)ldh * jj] = cc * h0 + ss * h1;
+            Hr[i + 1 + (size_t)ldh * jj] = -ss * h0 + cc * h1;
+        }
+        if (threadIdx.x == 0) {
+            g[i + 1] = -ss * g[i];
+            g[i] = cc * g[i];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    for (int j = 0; j < mm; j++) y[j] = 0.0;
+    for (int j = mm - 1; j >= 0; j--) {
+        double v = g[j];
+        for (int i = mm - 1; i >= 0; i--)
+            if (i != j) v -= Hr[i + (size_t)ldh * j] * y[i];
+        y[j] = v / Hr[j + (size_t)ldh * j];
+    }
+}
+void launch_ref_finish(const double* H, double* Hr, int ldh, double* g, int mm, double* y, hipStream_t st) {
+    hipLaunchKernelGGL(ref_finish_kernel, dim3(1), dim3(64), 0, st, H, Hr, ldh, g, mm, y);
+}
 
 // ---------------------------------------------------------------------------
 // Node-block Jacobi preconditioner.  P_n = sum over the (element, local node)
@@ -664,7 +821,8 @@ void launch_nodeblock(const int* node_ptr, const int* node_ent, const double* R6
 template <typename TV, typename TP, int NEQ>
 __global__ __launch_bounds__(256) void nodeblock_apply_kernel(const TP* __restrict__ P, const TV* __restrict__ v,
                                                               TV* __restrict__ z, long n_node, int accumulate,
-                                                              int trans) {
+                                                              int trans, const int* __restrict__ gate) {
+    if (gate && *gate) return;
     for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < n_node; n += (long)gridDim.x * blockDim.x) {
         TV x[NEQ];
 #pragma unroll
@@ -682,21 +840,21 @@ __global__ __launch_bounds__(256) void nodeblock_apply_kernel(const TP* __restri
 
 template <typename TV, typename TP>
 void vec_nodeblock_apply(const TP* P, const TV* v, TV* z, int neqn, long n_node, bool accumulate, bool trans,
-                         hipStream_t st) {
+                         hipStream_t st, const int* gate) {
     const int nb = grid_for(n_node);
     if (neqn == 5)
         hipLaunchKernelGGL((nodeblock_apply_kernel<TV, TP, 5>), dim3(nb), dim3(256), 0, st, P, v, z, n_node,
-                           (int)accumulate, (int)trans);
+                           (int)accumulate, (int)trans, gate);
     else
         hipLaunchKernelGGL((nodeblock_apply_kernel<TV, TP, 4>), dim3(nb), dim3(256), 0, st, P, v, z, n_node,
-                           (int)accumulate, (int)trans);
+                           (int)accumulate, (int)trans, gate);
 }
 template void vec_nodeblock_apply<double, double>(const double*, const double*, double*, int, long, bool, bool,
-                                                  hipStream_t);
+                                                  hipStream_t, const int*);
 template void vec_nodeblock_apply<float, float>(const float*, const float*, float*, int, long, bool, bool,
-                                                hipStream_t);
+                                                hipStream_t, const int*);
 template void vec_nodeblock_apply<double, float>(const float*, const double*, double*, int, long, bool, bool,
-                                                 hipStream_t);
+                                                 hipStream_t, const int*);
 
 // per-equation sums of squares of a node array (MPI_RMSError.h:19-26)
 __global__ void sumsq_eqn_kernel(const double* __restrict__ U, int neqn, long n_node, double* __restrict__ part) {
@@ -718,7 +876,7 @@ __global__ void sumsq_eqn_kernel(const double* __restrict__ U, int neqn, long n_
 }
 void vec_sumsq_eqn(const double* U, int neqn, long n_node, double* part, double* out, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(sumsq_eqn_kernel, dim3(nblk), dim3(256), 0, st, U, neqn, n_node, part);
-    hipLaunchKernelGGL(finish_dot_kernel, dim3(neqn), dim3(256), 0, st, part, nblk, neqn, out);
+    hipLaunchKernelGGL(finish_dot_kernel, dim3(neqn), dim3(256), 0, st, part, nblk, neqn, out, nullptr);
 }
 
 __global__ void min_reduce_kernel(const double* part, int n, double* out) {

@@ -162,6 +162,19 @@ class FDVContext:
         self._check(self.L.cfdv_get_du(self.h, _dp(d)))
         return d
 
+    def reference_trace(self):
+        """Per restart of the last reference-mode solve (MPI_GMRES.h:125-220):
+        H [restarts][m+1][m] before the rotations, the rotated g
+        [restarts][m+1] and the Arnoldi steps taken [restarts]."""
+        m, nr = C.c_int32(), C.c_int32()
+        self._check(self.L.cfdv_reference_trace(self.h, C.byref(m), C.byref(nr), None, None, None))
+        H = np.zeros((nr.value, m.value, m.value + 1))
+        g = np.zeros((nr.value, m.value + 1))
+        mm = np.zeros(nr.value, dtype=np.int32)
+        self._check(self.L.cfdv_reference_trace(self.h, None, None, _dp(H), _dp(g),
+                                                mm.ctypes.data_as(C.POINTER(C.c_int32))))
+        return H.transpose(0, 2, 1).copy(), g, mm
+
     def update(self):
         self._check(self.L.cfdv_update(self.h))
 

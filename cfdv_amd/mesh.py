@@ -508,7 +508,7 @@ def write_deck(path: str, mesh: Mesh, th: Thermo, rho, v, T, control_extra=None)
 
 
 def primitives(U: np.ndarray, ndim: int, Cv: float, gamma: float) -> dict:
-    """NodeUnpack (src/FDV/Function/NodeUnpack.h:124-144), vectorised."""
+    """NodeUnpack (src/FDV/Function/NodeUnpack.h:9-29), vectorised."""
     rho = U[:, 0]
     v = U[:, 1:1 + ndim] / U[:, :1]
     vs = np.zeros(U.shape[0])

@@ -12,18 +12,18 @@
  *   cfdv_create/destroy  Thermo::set (FDV/Thermo.h:22-33) + the mesh held by
  *                        LoadBinaryData (Utility/IO/ASCIItoBinary/LoadBinary.h:
  *                        156-318: nodes, elements, faces, Dirichlet flags)
- *                        and CalcLength (FDV/Function/CalcLength.h:256-288,
+ *                        and CalcLength (FDV/Function/CalcLength.h:34-110,
  *                        main.cpp:120-121)
  *   cfdv_set_state       node U after LoadBinary::loadNodes (LoadBinary.h:
  *                        174-200) or any later state
- *   cfdv_cfl             CFL (FDV/Function/MPI_CFL.h:170-217, main.cpp:179)
+ *   cfdv_cfl             CFL (FDV/Function/MPI_CFL.h:9-56, main.cpp:179)
  *   cfdv_build           for_each(ClearElement), for_each(NodeUnpack),
  *                        for_each(FDVGalerkin), for_each(ApplyBC)
  *                        (main.cpp:169-194; FDV/FDVGalerkin.h:121-154,
  *                        FDV/Function/ApplyBC.h:14-133)
- *   cfdv_get_blocks      Element::R, Element::rhs (FDV/Element.h:227-228)
+ *   cfdv_get_blocks      Element::R, Element::rhs (FDV/Element.h:33-34)
  *   cfdv_solve           GMRESbase::iterate (Utility/Solvers/GMRESbase.h:
- *                        71-72) as implemented by MPI_GMRES::iterate
+ *                        16-17) as implemented by MPI_GMRES::iterate
  *                        (Utility/Solvers/MPI_GMRES.h:125-220)
  *   cfdv_update          GMRESbase::update / MPI_GMRES::update
  *                        (MPI_GMRES.h:224-301)
@@ -201,6 +201,14 @@ int cfdv_get_blocks32(cfdv_ctx *ctx, float *R);
 int cfdv_get_rhs_vector(cfdv_ctx *ctx, double *b);             /* assembled b */
 int cfdv_solve(cfdv_ctx *ctx, const cfdv_solve_opts *opts, cfdv_solve_stats *st);
 int cfdv_get_du(cfdv_ctx *ctx, double *du);
+/* The last CFDV_SOLVE_REFERENCE solve of the context, per restart r <
+   *n_restarts, for parity checks upstream of its back substitution (no
+   reference counterpart; the values are those of MPI_GMRES::iterate,
+   MPI_GMRES.h:125-220): H [r][m][m+1] the Hessenberg matrix as the Arnoldi
+   process left it, before the rotations (H(i,j) at [r][j][i]); g [r][m+1] the
+   right-hand side after rotate (:733-752); mm [r] the Arnoldi steps taken.
+   Any output may be NULL (call with NULLs first for m and n_restarts). */
+int cfdv_reference_trace(cfdv_ctx *ctx, int32_t *m, int32_t *n_restarts, double *H, double *g, int32_t *mm);
 int cfdv_update(cfdv_ctx *ctx);
 int cfdv_rms(cfdv_ctx *ctx, double *urms);
 /* flags [n_elem] = s1 > 0.8; ties = elements where r == 0.01 exactly left

@@ -100,7 +100,7 @@ double orc_calc_k(const orc_thermo *th, double mu)
 }
 
 /* ------------------------------------------------------------------------ */
-/* Gauss-Legendre by Newton (src/FiniteElement/GaussQuad.h:258-301)         */
+/* Gauss-Legendre by Newton (src/FiniteElement/GaussQuad.h:116-159)         */
 /* ------------------------------------------------------------------------ */
 void orc_gauss(int Ni, double *w, double *zeta)
 {
@@ -275,7 +275,7 @@ static double mag3(const double *t, int n)
 double orc_tfnorm3d(const double *nodeloc, const double *w, const double *zeta,
                     double *phibou, double *en, double *dshpdx)
 {
-    /* TFNorm3D, calc_TF_norm3D.h:143-301 */
+    /* TFNorm3D, calc_TF_norm3D.h:9-167 */
     phibou[0] = 0.25 * (1 - zeta[0]) * (1 - zeta[1]);
     phibou[1] = 0.25 * (1 + zeta[0]) * (1 - zeta[1]);
     phibou[2] = 0.25 * (1 + zeta[0]) * (1 + zeta[1]);
@@ -341,7 +341,7 @@ double orc_tfnorm3d(const double *nodeloc, const double *w, const double *zeta,
 void orc_node_unpack(const orc_thermo *th, int ndim, const double *U,
                      orc_prim *n)
 {
-    /* NodeUnpack.h:124-158 */
+    /* NodeUnpack.h:9-43 */
     memset(n, 0, sizeof(*n));
     n->rho = U[0];
     double vsqr = 0;
@@ -373,7 +373,7 @@ static void elem_coords(const orc_mesh *m, int e, double *xn)
 
 void orc_calc_length(const orc_mesh *m, double *area, double *cleng)
 {
-    /* CalcLength.h:256-332 */
+    /* CalcLength.h:34-110 */
     double gw[2], gz[2];
     orc_gauss(2, gw, gz);
     for (int e = 0; e < m->n_elem; e++) {
@@ -408,7 +408,7 @@ void orc_calc_length(const orc_mesh *m, double *area, double *cleng)
 int orc_cfl(const orc_thermo *th, const orc_mesh *m, const orc_prim *prim,
             const double *cleng, double cfl, double *dt)
 {
-    /* CFL, MPI_CFL.h:170-217 (literal 1.4: quirk Q12) */
+    /* CFL, MPI_CFL.h:9-56 (literal 1.4: quirk Q12) */
     double dx_dv = 1000;
     int set = 0;
     for (int e = 0; e < m->n_elem; e++) {
@@ -1512,7 +1512,7 @@ static void eval_boundary(bctx *c, const double *w, const double *zeta, int bc,
     for (int i = 0; i < nb; i++) gn[i] = gconn[n[i]];
     double phibou[4], en[3], dpidx[12] = {0};
     if (nd == 2) {
-        /* TFNorm2D, TestFunctions1D.h:243-252 */
+        /* TFNorm2D, TestFunctions1D.h:73-82 */
         const double *x0 = c->m->x + (size_t)gn[0] * 2, *x1 = c->m->x + (size_t)gn[1] * 2;
         phibou[0] = 0.5 * (1 - zeta[0]);
         phibou[1] = 0.5 * (1 + zeta[0]);
@@ -1534,7 +1534,7 @@ static void eval_boundary(bctx *c, const double *w, const double *zeta, int bc,
             for (int N = 0; N < 4; N++)
                 for (int j = 0; j < 3; j++) nodeloc[N * 3 + j] = c->m->x[(size_t)gn[N] * 3 + j];
         } else {
-            /* calc_TF_norm3D.h:169-200 */
+            /* calc_TF_norm3D.h:35-66 */
             const double *Hnm = c->m->hnm + (size_t)c->e * 64;
             double allnode[8][3] = {{0}};
             memset(nodeloc, 0, sizeof(nodeloc));
@@ -1754,6 +1754,13 @@ static double vdot(size_t n, const double *a, const double *b)
 int orc_gmres_quirk(const orc_mesh *m, const double *R, const double *rhs,
                     int restart, int mdim, double *du, orc_gmres_stats *st)
 {
+    return orc_gmres_quirk_trace(m, R, rhs, restart, mdim, du, st, NULL, NULL, NULL);
+}
+
+int orc_gmres_quirk_trace(const orc_mesh *m, const double *R, const double *rhs,
+                          int restart, int mdim, double *du, orc_gmres_stats *st,
+                          double *Htr, double *gtr, int *mmtr)
+{
     /* MPI_GMRES::iterate (MPI_GMRES.h:125-220) at one rank */
     size_t size = (size_t)m->n_node * m->neqn;
     int m_ = mdim, mm = mdim;
@@ -1828,6 +1835,8 @@ int orc_gmres_quirk(const orc_mesh *m, const double *R, const double *rhs,
                 }
             }
         }
+        if (Htr) memcpy(Htr + (size_t)gloop * (m_ + 1) * m_, h, sizeof(double) * (m_ + 1) * m_);
+        if (mmtr) mmtr[gloop] = mm;
         /* rotate (:733-752) */
         for (int i = 0; i < mm; i++) {
             double mag = sqrt(H(i, i) * H(i, i) + H(i + 1, i) * H(i + 1, i));
@@ -1841,6 +1850,7 @@ int orc_gmres_quirk(const orc_mesh *m, const double *R, const double *rhs,
             g[i] = cc * g[i];
         }
         st->last_resid = fabs(g[mm]);
+        if (gtr) memcpy(gtr + (size_t)gloop * (m_ + 1), g, sizeof(double) * (m_ + 1));
         /* solve (:756-780): h read transposed in the back substitution (Q2) */
         for (int j = mm - 1; j >= 0; j--) {
             y[j] = g[j];

@@ -87,7 +87,7 @@ double orc_jacobian3d(const double *dpxi, const double *xn /*[8][3]*/,
                       double *jac, double *dphi);
 double orc_jacobian2d(const double *dpxi, const double *xn /*[4][2]*/,
                       double *jac, double *dphi);
-/* Face test functions/normal (calc_TF_norm3D.h:143-301); nodeloc [4][3] */
+/* Face test functions/normal (calc_TF_norm3D.h:9-167); nodeloc [4][3] */
 double orc_tfnorm3d(const double *nodeloc, const double *w, const double *zeta,
                     double *phibou, double *en, double *dpidx /*[4][3]*/);
 
@@ -150,6 +150,14 @@ void orc_matvec_t(const orc_mesh *m, const double *R, const double *x,
 /* MPI_GMRES::iterate at one rank (MPI_GMRES.h:125-220), quirks Q2-Q4 */
 int orc_gmres_quirk(const orc_mesh *m, const double *R, const double *rhs,
                     int restart, int mdim, double *du, orc_gmres_stats *st);
+/* The same, recording per restart the Hessenberg matrix as the Arnoldi
+   process leaves it (before the rotations, Htr [restart][(mdim+1)*mdim],
+   H(i,j) at i + (mdim+1) j), the rotated right-hand side g (gtr
+   [restart][mdim+1]) and the number of Arnoldi steps (mmtr [restart]);
+   any of the three may be NULL. */
+int orc_gmres_quirk_trace(const orc_mesh *m, const double *R, const double *rhs,
+                          int restart, int mdim, double *du, orc_gmres_stats *st,
+                          double *Htr, double *gtr, int *mmtr);
 /* Textbook restarted GMRES(m): Givens, correct back substitution, exit at
    ||b - A du|| <= rtol ||b|| or maxrestart restarts.  transpose != 0 solves
    with the transpose operator. */

@@ -108,6 +108,8 @@ def lib() -> C.CDLL:
         L.orc_matvec_t.argtypes = [C.POINTER(MeshS), dp, dp, dp]
         L.orc_gmres_quirk.argtypes = [C.POINTER(MeshS), dp, dp, C.c_int, C.c_int, dp,
                                       C.POINTER(GmresStats)]
+        L.orc_gmres_quirk_trace.argtypes = [C.POINTER(MeshS), dp, dp, C.c_int, C.c_int, dp,
+                                            C.POINTER(GmresStats), dp, dp, ip]
         L.orc_gmres_true.argtypes = [C.POINTER(MeshS), dp, dp, C.c_int, C.c_int, C.c_double,
                                      C.c_int, dp, C.POINTER(GmresStats)]
         L.orc_update.argtypes = [C.c_int, dp, dp, dp]
@@ -268,6 +270,25 @@ def gmres_quirk(mesh, R, rhs, restart, mdim):
     if rc:
         raise RuntimeError("orc_gmres_quirk failed")
     return du, st
+
+
+def gmres_quirk_trace(mesh, R, rhs, restart, mdim):
+    """gmres_quirk plus, per restart, the Hessenberg matrix as the Arnoldi
+    process leaves it (H [restart][mdim+1][mdim], before the rotations), the
+    rotated right-hand side g [restart][mdim+1] and the Arnoldi step count."""
+    om = OMesh(mesh)
+    du = np.zeros(mesh.n_node * mesh.neqn)
+    st = GmresStats()
+    Rc = _cm(R)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    H = np.zeros((restart, mdim, mdim + 1))
+    g = np.zeros((restart, mdim + 1))
+    mm = np.zeros(restart, dtype=np.int32)
+    rc = lib().orc_gmres_quirk_trace(om.ref, _p(Rc), _p(rhs), restart, mdim, _p(du), C.byref(st),
+                                     _p(H), _p(g), mm.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc:
+        raise RuntimeError("orc_gmres_quirk_trace failed")
+    return du, st, H.transpose(0, 2, 1).copy(), g, mm
 
 
 def gmres_true(mesh, R, b, mdim, maxrestart, rtol, transpose=False):

@@ -34,7 +34,16 @@
    backstep_refined_new (control, aero, gzipped binary records) and the
    reference's own output on it at t = 0, tecplot_0.plt (gzipped).
 
-Run from the repository root:  python tests/golden/make_golden.py [--refine | --tecplot]
+5. Configuration C1 (``--c1`` writes only this, ``lid32_c1.npz``): the 2D
+   lid-driven cavity on 32 x 32 quad4 (cfdv_amd.mesh.lid_cavity(32): M 0.1,
+   Re 100, Pr 0.72, gamma 1.4, at rest with rho = T = 1), 100 time steps at
+   CFL 0.5 with the reference GMRES as the Lid deck runs it (gmresiter 20,
+   gmresrestart 1): U after every step, dt, Arnoldi steps, RMS rows, and per
+   step how far du moves under 1e-16 relative noise on the element blocks
+   and rhs (as for the decks above).
+
+Run from the repository root:
+    python tests/golden/make_golden.py [--refine | --tecplot | --c1]
 """
 from __future__ import annotations
 
@@ -184,9 +193,46 @@ def make_refine(name):
     print(f"  -> {os.path.getsize(path)} bytes")
 
 
+C1_STEPS = 100
+
+
+def make_c1():
+    mesh, th, U = M.lid_cavity(32)
+    cfl, mdim, restart = 0.5, 20, 1
+    Us, dts, its, rms, sens = [U], [], [], [], []
+    rng = np.random.default_rng(20261204)
+    Uc = U
+    for k in range(C1_STEPS):
+        dt = O.cfl(th, mesh, Uc, cfl)
+        R, rhs, _ = O.build(th, mesh, Uc, dt)
+        du_a, _ = O.gmres_quirk(mesh, R, rhs, restart, mdim)
+        spread = 0.0
+        for noise_R, noise_rhs in ((1e-16, 0.0), (0.0, 1e-16), (1e-16, 1e-16)):
+            Rp = R * (1 + noise_R * rng.standard_normal(R.shape))
+            rp = rhs * (1 + noise_rhs * rng.standard_normal(rhs.shape))
+            du_b, _ = O.gmres_quirk(mesh, Rp, rp, restart, mdim)
+            spread = max(spread, np.linalg.norm(du_b - du_a) / np.linalg.norm(du_a))
+        U1, U0, st = O.step(th, mesh, Uc, cfl=cfl, restart=restart, mdim=mdim, mode=0)
+        assert np.array_equal(Uc.ravel() + du_a, U1.ravel())
+        Us.append(U1)
+        dts.append(st.dt)
+        its.append(st.gm.iters)
+        rms.append(np.array(st.urms)[:mesh.neqn])
+        sens.append(spread)
+        Uc = U1
+    path = os.path.join(HERE, "lid32_c1.npz")
+    np.savez_compressed(path, cfl=cfl, mdim=mdim, restart=restart, U=np.array(Us), dt=np.array(dts),
+                        iters=np.array(its), rms=np.array(rms), sens=np.array(sens))
+    print(f"C1 lid 32x32: {C1_STEPS} steps, dt {dts[0]:.4e}..{dts[-1]:.4e}, iters {min(its)}..{max(its)}, "
+          f"sensitivity {min(sens):.1e}..{max(sens):.1e} -> {os.path.getsize(path)} bytes")
+
+
 def main():
     if "--tecplot" in sys.argv:
         copy_tecplot_fixture()
+        return
+    if "--c1" in sys.argv:
+        make_c1()
         return
     if "--refine" not in sys.argv:
         if os.path.isdir(SRC):

@@ -39,3 +39,22 @@ def test_point_tables_match_the_per_direction_formulas(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_quad4_cosh_sinh_equal_the_host_libm(tmp_path):
+    """The quad4 numerical diffusion's coth (FDVGalerkin.h:46-50) on the GPU
+    uses fdlibm_expm1 / libm_sinh / libm_cosh (fdv_device.h); compiled for
+    the host they equal this host's libm bit for bit on 4 million arguments,
+    so the device blocks carry the same last bits as the oracle's."""
+    hipcc = _hipcc()
+    if hipcc is None:
+        pytest.skip("hipcc not found")
+    exe = str(tmp_path / "libm_coth_check")
+    src = os.path.join(HERE, "host", "libm_coth_check.hip")
+    r = subprocess.run([hipcc, "--offload-arch=gfx950", "-O2", "-std=c++17", "-Xclang", "-target-feature",
+                        "-Xclang", "-packed-fp32-ops", "-Wno-unused-result", src, "-o", exe],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr

@@ -2,9 +2,9 @@
 coloring and refinement flags.
 
 Each check is an independent statement of what the reference routine must
-produce: closed forms on boxes (CalcLength.h:256-332, MPI_CFL.h:170-217),
+produce: closed forms on boxes (CalcLength.h:34-110, MPI_CFL.h:9-56),
 hand-computed FDVParam values (FDVParam.h:15-111), the dense assembled
-operator for the element-by-element products (GMRES.h:460-530), a dense
+operator for the element-by-element products (GMRES.h:183-228), a dense
 solve for textbook GMRES, the one-step minimal-residual closed form and the
 diagonal back substitution of the reference's GMRES (MPI_GMRES.h:125-220,
 756-780), and brute-force first-fit coloring.
@@ -68,7 +68,7 @@ def test_calc_length_ramp_volume_sums_to_domain():
 
 def test_cfl_closed_form_and_literal_gamma():
     """dt = CFL * min_e cleng_e / (|v| + sqrt(1.4 cgas T)); the sound speed
-    uses the literal 1.4 even when gamma differs (MPI_CFL.h:197)."""
+    uses the literal 1.4 even when gamma differs (MPI_CFL.h:38)."""
     hs = (0.3, 0.2, 0.25)
     m = _plain_box(3, 2, 2, hs)
     for gamma in (1.4, 1.3):
