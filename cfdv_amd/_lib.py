@@ -44,7 +44,7 @@ EXPORTS = ["cfdv_create", "cfdv_destroy", "cfdv_last_error", "cfdv_mesh_info", "
            "cfdv_set_thermo", "cfdv_adjoint_solve", "cfdv_build_vjp", "cfdv_matvec_multi",
            "cfdv_profile_matvec_multi", "cfdv_refine_create", "cfdv_refine_destroy",
            "cfdv_refine_last_error", "cfdv_refine_list", "cfdv_refine_apply", "cfdv_refine_size",
-           "cfdv_refine_get", "cfdv_reference_trace"]
+           "cfdv_refine_get", "cfdv_reference_trace", "cfdv_refine_flags_noalloc"]
 
 
 class Mesh(C.Structure):
@@ -117,6 +117,8 @@ def load() -> C.CDLL:
     L.cfdv_rms.argtypes = [vp, dp]
     L.cfdv_refine_flags.argtypes = [vp, C.POINTER(C.c_uint8), dp, C.POINTER(C.c_int32),
                                     C.POINTER(C.c_int32)]
+    L.cfdv_refine_flags_noalloc.argtypes = [vp, C.POINTER(C.c_uint8), dp, C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_int32)]
     L.cfdv_color.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cfdv_matvec.argtypes = [vp, dp, dp, C.c_int, C.c_int]
     L.cfdv_profile_matvec.argtypes = [vp, C.c_int, C.c_int, dp]

@@ -18,6 +18,9 @@
 #include "element3d.h"
 #include "fdv_device.h"
 
+#include <mutex>
+#include <vector>
+
 namespace cfdv {
 
 __constant__ GaussTables c_gauss;
@@ -1727,6 +1730,51 @@ void launch_flags(int ndim, const DevThermo& th, const int* conn, const double* 
         hipLaunchKernelGGL((flags_kernel<3, 8>), dim3(nb), dim3(256), 0, st, th, conn, U, perm_inv, n_elem, flags, s1, ties);
     else
         hipLaunchKernelGGL((flags_kernel<2, 4>), dim3(nb), dim3(256), 0, st, th, conn, U, perm_inv, n_elem, flags, s1, ties);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free flag pass (cfdv_refine_flags_noalloc): a
+// file-scope device array that every GPU the library runs on holds until the
+// process ends.  In host code `scratch` only names the symbol; kernels get
+// the device address that flags_scratch() resolves with hipGetSymbolAddress
+// once per device, with that device current.  Calls on one GPU share the
+// array: overlapping calls race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[FLAGS_SCRATCH_BYTES];
+
+void* flags_scratch() {
+    static std::mutex mu;
+    static std::vector<void*> addr;
+    int dev = 0;
+    HIP_CHECK_THROW(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)addr.size() <= dev) addr.resize(dev + 1, nullptr);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        HIP_CHECK_THROW(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) returned a null address");
+        addr[dev] = p;
+    }
+    return addr[dev];
+}
+
+__global__ void zero_ints_kernel(int* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+void launch_flags_scratch(int ndim, const DevThermo& th, const int* conn, const double* U, const int* perm_inv,
+                          long n_elem, void* scratch_dev, bool want_s1, hipStream_t st, uint8_t** flags_dev,
+                          double** s1_dev, int** ties_dev) {
+    if (n_elem < 0 || n_elem > FLAGS_NOALLOC_MAX_ELEM) throw std::invalid_argument("flag scratch too small");
+    unsigned char* base = static_cast<unsigned char*>(scratch_dev);
+    int* counters = reinterpret_cast<int*>(base);
+    uint8_t* f = base + 256;
+    double* s1 = want_s1 ? reinterpret_cast<double*>(base + 256 + ((n_elem + 255) / 256) * 256) : nullptr;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, st, counters, 4);
+    launch_flags(ndim, th, conn, U, perm_inv, n_elem, f, s1, counters, st);
+    *flags_dev = f;
+    *s1_dev = s1;
+    *ties_dev = counters;
 }
 
 // ---------------------------------------------------------------------------

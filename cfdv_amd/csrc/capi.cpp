@@ -167,6 +167,8 @@ struct cfdv_ctx {
     // warm start: d_du keeps the last update's increment, built with prev_dt;
     // with warm_start 2 or 3, d_old[k] the k+1-th before it (old_dt[k];
     // n_old of them valid), d_dut is scratch
+    void* d_flag_scratch = nullptr;   // this device's flag scratch (cfdv_refine_flags_noalloc)
+    std::string flag_scratch_err;
     bool have_prev = false;
     bool warm_consumed = false;   // a solve of the current build formed the warm start
     bool warm_had = false;        // ... and it is kept in d_x0
@@ -1181,6 +1183,11 @@ int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flag
         for (auto& e : ctx->ev) HIP_CHECK_THROW(hipEventCreate(&e));
         HIP_CHECK_THROW(hipEventCreateWithFlags(&ctx->ev_sync, hipEventDisableTiming));
         upload_gauss_tables(make_tables());
+        try {
+            ctx->d_flag_scratch = flags_scratch();
+        } catch (const std::exception& ex) {
+            ctx->flag_scratch_err = ex.what();   // reported by cfdv_refine_flags_noalloc
+        }
         setup(ctx, mesh);
     } catch (const std::invalid_argument& ex) {
         delete ctx;
@@ -1462,6 +1469,38 @@ int cfdv_refine_flags(cfdv_ctx* ctx, uint8_t* flags, double* s1, int32_t* nflagg
     if (d_s1) { ctx->allocs.pop_back(); (void)hipFree(d_s1); }
     ctx->allocs.pop_back();
     (void)hipFree(d_f);
+    if (nflagged) {
+        long cnt = 0;
+        for (long e = 0; e < ctx->n_elem; e++) cnt += flags[e];
+        *nflagged = (int32_t)cnt;
+    }
+    if (nties) *nties = ties;
+    return CFDV_OK;
+    GUARD_END
+}
+
+int cfdv_refine_flags_noalloc(cfdv_ctx* ctx, uint8_t* flags, double* s1, int32_t* nflagged, int32_t* nties) {
+    if (!ctx || !flags) return CFDV_ERR_ARG;
+    if (ctx->n_elem > CFDV_FLAGS_NOALLOC_MAX_ELEM)
+        return fail(ctx, CFDV_ERR_ARG,
+                    "cfdv_refine_flags_noalloc: " + std::to_string(ctx->n_elem) +
+                        " elements exceed CFDV_FLAGS_NOALLOC_MAX_ELEM (" +
+                        std::to_string((long)CFDV_FLAGS_NOALLOC_MAX_ELEM) + "); use cfdv_refine_flags");
+    if (!ctx->d_flag_scratch)
+        return fail(ctx, CFDV_ERR_HIP, "cfdv_refine_flags_noalloc: no flag scratch on this device: " +
+                                           ctx->flag_scratch_err);
+    GUARD_BEGIN
+    uint8_t* d_f = nullptr;
+    double* d_s1 = nullptr;
+    int* d_ties = nullptr;
+    launch_flags_scratch(ctx->ndim, ctx->th, ctx->d_conn, ctx->d_U, ctx->d_perm_inv, ctx->n_elem,
+                         ctx->d_flag_scratch, s1 != nullptr, ctx->st, &d_f, &d_s1, &d_ties);
+    ctx->check_launch();
+    int ties = 0;
+    HIP_CHECK_THROW(hipMemcpyAsync(flags, d_f, ctx->n_elem, hipMemcpyDeviceToHost, ctx->st));
+    if (s1) HIP_CHECK_THROW(hipMemcpyAsync(s1, d_s1, sizeof(double) * ctx->n_elem, hipMemcpyDeviceToHost, ctx->st));
+    HIP_CHECK_THROW(hipMemcpyAsync(&ties, d_ties, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
     if (nflagged) {
         long cnt = 0;
         for (long e = 0; e < ctx->n_elem; e++) cnt += flags[e];

@@ -56,6 +56,24 @@ void launch_column_sum(const double* a, long n, int stride, int offset, double* 
 void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st);
 void launch_flags(int ndim, const DevThermo& th, const int* conn, const double* U, const int* perm_inv,
                   long n_elem, uint8_t* flags, double* s1, int* ties, hipStream_t st);
+// The allocation-free flag pass works in a fixed device array (build.hip):
+// FLAGS_SCRATCH_BYTES per GPU, held for the life of the process; layout per
+// call: [0, 256) counters, n_elem flag bytes at 256, n_elem doubles of s1
+// after the flags rounded up to 256 bytes.
+constexpr size_t FLAGS_SCRATCH_BYTES = (size_t)32 << 20;
+constexpr long FLAGS_NOALLOC_MAX_ELEM = 3700000;
+static_assert(256 + ((FLAGS_NOALLOC_MAX_ELEM + 255) / 256) * 256 + 8 * FLAGS_NOALLOC_MAX_ELEM <=
+                  (long)FLAGS_SCRATCH_BYTES,
+              "flag scratch too small for FLAGS_NOALLOC_MAX_ELEM");
+// device address of the scratch array on the current device (resolved once
+// per device with hipGetSymbolAddress); throws if it cannot be resolved
+void* flags_scratch();
+// the flag pass into the scratch at scratch_dev (n_elem <=
+// FLAGS_NOALLOC_MAX_ELEM): counters cleared by a kernel, flags and (want_s1)
+// s1 written there; returns their device addresses and the ties counter's
+void launch_flags_scratch(int ndim, const DevThermo& th, const int* conn, const double* U, const int* perm_inv,
+                          long n_elem, void* scratch_dev, bool want_s1, hipStream_t st, uint8_t** flags_dev,
+                          double** s1_dev, int** ties_dev);
 void launch_cfl(int ndim, const DevThermo& th, const int* conn, const double* U, const double* cleng,
                 long n_elem_pad, double* part, int nblocks, hipStream_t st);
 void launch_length(int ndim, const int* conn, const double* x, long n_elem_pad, double* area, double* cleng,

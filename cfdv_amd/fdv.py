@@ -202,6 +202,16 @@ class FDVContext:
                                              _dp(s1), C.byref(nf), C.byref(nt)))
         return f, s1, nt.value
 
+    def refine_flags_noalloc(self, want_s1=True):
+        """refine_flags through cfdv_refine_flags_noalloc (no device
+        allocation: the library's fixed per-GPU scratch)."""
+        f = np.zeros(self.mesh.n_elem, dtype=np.uint8)
+        s1 = np.zeros(self.mesh.n_elem) if want_s1 else None
+        nf, nt = C.c_int32(), C.c_int32()
+        self._check(self.L.cfdv_refine_flags_noalloc(self.h, f.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                     _dp(s1) if want_s1 else None, C.byref(nf), C.byref(nt)))
+        return f, s1, nt.value, nf.value
+
     def color(self):
         c = np.zeros(self.mesh.n_elem, dtype=np.int32)
         n = C.c_int32()

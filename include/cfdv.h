@@ -215,6 +215,17 @@ int cfdv_rms(cfdv_ctx *ctx, double *urms);
    s1/s3 unassigned in the reference (quirk Q11) */
 int cfdv_refine_flags(cfdv_ctx *ctx, uint8_t *flags, double *s1, int32_t *nflagged,
                       int32_t *nties);
+/* cfdv_refine_flags with no device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The flags, s1 and the tie counter live in a fixed 32 MiB
+   device array that the library holds on each GPU until the process ends;
+   meshes above CFDV_FLAGS_NOALLOC_MAX_ELEM elements are rejected with
+   CFDV_ERR_ARG.  Every call on one GPU uses that GPU's array, so calls that
+   overlap on one GPU (several contexts or threads) race. */
+#define CFDV_FLAGS_NOALLOC_MAX_ELEM 3700000
+int cfdv_refine_flags_noalloc(cfdv_ctx *ctx, uint8_t *flags, double *s1, int32_t *nflagged,
+                              int32_t *nties);
 int cfdv_color(cfdv_ctx *ctx, int32_t *color, int32_t *ncolors);
 /* y = K x (or K^T x) with the built blocks; precision 64, 32 (FP32 blocks and
    vectors) or 3 (FP32 blocks, FP64 vectors); host arrays [n_node*neqn].
