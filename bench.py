@@ -41,6 +41,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 BUILD_FLOP_PER_ELEMENT = 0.70541e6
+REFINE_NODE_TOL = 1e-9
 
 # HBM bytes per full product measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # on MI355X, each scaled by the counter's reading for a plain 1 GiB copy
@@ -80,6 +81,11 @@ def parse():
     # three; 1: always du_1 dt / dt_1; 0: every solve from 0 as the
     # reference does
     p.add_argument("--warm-start", type=int, default=2, choices=[0, 1, 2, 3])
+    # sensor-driven refinement before the timed steps: passes of the GPU
+    # flag pass (s1 > 0.8) + host AdaptiveRefineList / MeshRefine2D
+    # (every coordinate compared for repeated nodes), each on the mesh and
+    # state the previous pass left
+    p.add_argument("--refine-levels", type=int, default=2)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=32768)
     return p.parse_args()
@@ -105,6 +111,65 @@ def run_steps(ctx, U0, mode, n, args):
         log(f"  {mode} step {i}: build {st.ms_build:.1f} ms assemble {st.ms_assemble:.1f} ms "
             f"solve {st.ms_solve:.1f} ms "
             f"mv64 {st.solve.matvecs64} mv32 {st.solve.matvecs32} rel {st.solve.rel_resid:.2e}")
+    return out
+
+
+def refine_mesh(mesh, th, U, levels):
+    """C3's refinement: `levels` --adap passes (main.cpp:143-157), each the
+    GPU flag pass s1 > 0.8 (RefineListAdaptive.h:28-38) on the current mesh
+    and state, then the host's AdaptiveRefineList + MeshRefine2D (refine.cpp)
+    comparing every coordinate for repeated nodes.  Returns the refined
+    mesh and state and what each pass did."""
+    from cfdv_amd import refine as PR
+    from cfdv_amd.fdv import FDVContext
+    cur, Uc, U0c, lv = mesh, U, None, None
+    info = {"elements": [int(mesh.n_elem)], "nodes": [int(mesh.n_node)], "flagged": [],
+            "refined": [], "flags_ms": [], "host_list_s": [], "host_apply_s": []}
+    for p in range(levels):
+        ctx = FDVContext(cur, th, Uc, blocks=("fp32",))
+        sync()
+        t0 = time.perf_counter()
+        flags, _, _ = ctx.refine_flags()
+        info["flags_ms"].append((time.perf_counter() - t0) * 1e3)
+        ctx.close()
+        del ctx
+        r = PR.RefinableMesh(cur, Uc, U0c, lv)
+        t0 = time.perf_counter()
+        el, rl = r.refine_list(flags)
+        t1 = time.perf_counter()
+        # repeated nodes are interpolated to the same coordinates up to
+        # rounding: the reference's absolute 1e-3 (MeshRefine2D.h:337-361)
+        # would merge distinct nodes of the level-2 wall cells (2.8e-4 high)
+        r.apply(el, rl, PR.MODE_3D, REFINE_NODE_TOL)
+        t2 = time.perf_counter()
+        cur, Uc, U0c, lv = r.get()
+        del r
+        info["flagged"].append(int(flags.sum()))
+        info["refined"].append(int(len(el)))
+        info["host_list_s"].append(t1 - t0)
+        info["host_apply_s"].append(t2 - t1)
+        info["elements"].append(int(cur.n_elem))
+        info["nodes"].append(int(cur.n_node))
+        log(f"refine pass {p}: {int(flags.sum())} flagged, {len(el)} refined (levels {rl.tolist()}) -> "
+            f"{cur.n_elem} elements, {cur.n_node} nodes; host list {t1 - t0:.1f} s, apply {t2 - t1:.1f} s")
+    info["adap_elements"] = int(cur.adap.sum()) if cur.adap is not None else 0
+    info["levels"] = np.bincount(lv).tolist()
+    info["mesh"], info["U"] = cur, Uc
+    return info
+
+
+def field_errors(mesh, U0, U64, Umx):
+    """Mixed minus FP64 after the same steps from the same state, per field
+    (rho, momentum, energy), relative to the FP64 path's change of that
+    field: ||Umx_f - U64_f|| / ||U64_f - U0_f||"""
+    nd = mesh.ndim
+    groups = {"rho": [0], "momentum": list(range(1, 1 + nd)), "energy": [1 + nd]}
+    out = {}
+    for name, cols in groups.items():
+        d = np.linalg.norm(Umx[:, cols] - U64[:, cols])
+        ch = np.linalg.norm(U64[:, cols] - U0[:, cols])
+        out[name] = float(d / ch) if ch > 0 else None
+        out[name + "_max_abs"] = float(np.abs(Umx[:, cols] - U64[:, cols]).max())
     return out
 
 
@@ -148,12 +213,16 @@ def main():
     t0 = time.time()
     mesh, th, U0 = M.compression_ramp(nx, ny, nz)
     log(f"mesh {nx}x{ny}x{nz}: {mesh.n_elem} elements, {mesh.n_node} nodes ({time.time() - t0:.1f}s)")
+    base_elem, base_node = mesh.n_elem, mesh.n_node
+    refine_info = refine_mesh(mesh, th, U0, args.refine_levels) if args.refine_levels > 0 else None
+    if refine_info is not None:
+        mesh, U0 = refine_info.pop("mesh"), refine_info.pop("U")
     t0 = time.time()
     ctx = FDVContext(mesh, th, U0, blocks=("fp64", "rows") if args.storage == "rows" else ("fp64", "fp32"))
     info = ctx.info()
     log(f"context: {info} ({time.time() - t0:.1f}s)")
 
-    results = {}
+    results, finals = {}, {}
     for mode in ("fp64", "mixed"):
         log(f"{mode}: warmup {args.warmup}")
         run_steps(ctx, U0, mode, args.warmup, args)
@@ -163,6 +232,9 @@ def main():
         sync()
         el = time.perf_counter() - t0
         results[mode] = (el, sts)
+        finals[mode] = ctx.get_state()[0]
+    ferr = field_errors(mesh, U0, finals["fp64"], finals["mixed"])
+    log(f"mixed vs fp64 after {args.steps} steps: {ferr}")
 
     # the mixed path again with every solve started from 0, for comparison
     cold = None
@@ -183,10 +255,10 @@ def main():
 
     ne, nn = mesh.n_elem, mesh.n_node
     if args.storage == "rows":
-        # one block per node pair sharing an element: on the structured
-        # (nx+1)(ny+1)(nz+1) node grid, (3nx+1)(3ny+1)(3nz+1) blocks; each
-        # block's 25 values plus its column index, x read and y written once
-        nnzb = (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1)
+        # one block per node pair sharing an element (on the unrefined
+        # structured grid (3nx+1)(3ny+1)(3nz+1) of them); each block's 25
+        # values plus its column index, x read and y written once
+        nnzb, _ = ctx.rows_info()
         b32 = nnzb * (25 * 4 + 4) + nn * 5 * 2 * 4
         b64 = nnzb * (25 * 8 + 4) + nn * 5 * 2 * 8
         kname = "rows_matvec_kernel<float,float,float,5> (assembled node rows, FP32)"
@@ -216,7 +288,14 @@ def main():
                           f"({det['t_matvec_sample_s'] * 1e3:.1f} ms), scaled to {ne} elements "
                           f"with {mv_per_step} matvecs per step (the FP64 GPU path's count)")}
 
-    traffic = MEASURED_TRAFFIC.get((args.storage, args.size), {})
+    traffic = MEASURED_TRAFFIC.get((args.storage, args.size), {}) if refine_info is None else {}
+    wl = (f"3D hypersonic 15-deg compression ramp hex8 {nx}x{ny}x{nz} ({base_elem} elements, "
+          f"{base_node} nodes)")
+    if refine_info is not None:
+        wl += (f", sensor-refined (s1 > 0.8) {args.refine_levels} levels to {ne} elements, {nn} nodes "
+               f"({refine_info['adap_elements']} with hanging nodes)")
+    wl += (f", M=6, Re=5e5, adiabatic, CFL {args.cfl}, GMRES({args.m}) to rel. residual {args.rtol}, "
+           f"{args.precond} right preconditioner on both paths")
     line = {
         "metric": "FDV steps/s at 2.1M hex8: FP32+FP64-refine vs FP64 GPU, equal 1e-8 residual",
         "value": value,
@@ -228,12 +307,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 blocks/matvec + f64 build, residual, Givens, update",
+        "dtype": "f32 blocks/matvec + f64 build, residual, Givens on GPU, update",
         "data": "synthetic (generated ramp mesh and freestream initial state)",
-        "config": {"workload": f"3D hypersonic 15-deg compression ramp hex8 {nx}x{ny}x{nz} "
-                               f"({ne} elements, {nn} nodes), M=6, Re=5e5, adiabatic, "
-                               f"CFL {args.cfl}, GMRES({args.m}) to rel. residual {args.rtol}, "
-                               f"{args.precond} right preconditioner on both paths",
+        "config": {"workload": wl, "base_elements": base_elem, "base_nodes": base_node,
                    "elements": ne, "nodes": nn, "colors": info["ncolors"],
                    "precond": args.precond, "inner_m": args.inner_m, "inner_rtol": args.inner_rtol,
                    "storage": args.storage, "warm_start": int(args.warm_start)},
@@ -242,6 +318,10 @@ def main():
         "mixed_over_fp64": value / fp64_sps,
         "rel_resid_mixed": max(s.solve.rel_resid for s in st_mx),
         "rel_resid_fp64": max(s.solve.rel_resid for s in st_64),
+        # mixed minus FP64 after the timed steps from the same state, per
+        # field, relative to the FP64 path's change of the field
+        "mixed_vs_fp64_fields": ferr,
+        "refine": refine_info,
         "ms_build": float(np.mean([s.ms_build for s in st_mx])),
         "ms_assemble": float(np.mean([s.ms_assemble for s in st_mx])),
         "ms_solve_mixed": float(np.mean([s.ms_solve for s in st_mx])),

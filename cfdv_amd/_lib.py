@@ -44,7 +44,7 @@ EXPORTS = ["cfdv_create", "cfdv_destroy", "cfdv_last_error", "cfdv_mesh_info", "
            "cfdv_set_thermo", "cfdv_adjoint_solve", "cfdv_build_vjp", "cfdv_matvec_multi",
            "cfdv_profile_matvec_multi", "cfdv_refine_create", "cfdv_refine_destroy",
            "cfdv_refine_last_error", "cfdv_refine_list", "cfdv_refine_apply", "cfdv_refine_size",
-           "cfdv_refine_get", "cfdv_reference_trace", "cfdv_refine_flags_noalloc"]
+           "cfdv_refine_get", "cfdv_reference_trace", "cfdv_refine_flags_noalloc", "cfdv_rows_info"]
 
 
 class Mesh(C.Structure):
@@ -102,6 +102,7 @@ def load() -> C.CDLL:
     L.cfdv_last_error.restype = C.c_char_p
     L.cfdv_mesh_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+    L.cfdv_rows_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.cfdv_set_state.argtypes = [vp, dp]
     L.cfdv_get_state.argtypes = [vp, dp, dp]
     L.cfdv_cfl.argtypes = [vp, C.c_double, dp]

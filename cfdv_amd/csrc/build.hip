@@ -219,6 +219,12 @@ template <typename T>
 CFDV_HD size_t blk_index(long e, int q, int KK) {
     return blk_offset<T>(e, q, KK);
 }
+// entry q = i k + j of element e's FP64 block in the context's layout
+template <int NNOD, int NEQ>
+__device__ __forceinline__ size_t r64_at(const BuildArgs& A, long e, int q) {
+    constexpr int K = NNOD * NEQ;
+    return A.r64_pairs ? pair_offset<NNOD, NEQ>(e, q) : blk_offset<double>(e, q, K * K);
+}
 
 // ---------------------------------------------------------------------------
 // Interior hex8 elements (no face with a bc code, no constrained dof): the
@@ -695,7 +701,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     if (e >= A.n_elem_pad) return;   // whole workgroup leaves together
     const int* cn = A.conn + e * NNOD;
     const bool real = cn[0] >= 0;
-    bool boundary = false;
+    bool boundary = false, hang = false;
     if (real) {
         const int* bcs = A.bc + e * 6;
         boundary = A.cmask[e] != 0ULL;
@@ -704,21 +710,40 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         // with FP64 blocks, a boundary element's domain part is written here
         // (FP64 R and rhs) and build3d_face_kernel adds faces and ApplyBC;
         // without, the per-lane kernel builds it whole.  Hanging-node
-        // elements are built whole by the per-lane kernel.
-        if (A.adap && A.adap[e]) return;
+        // elements without faces are built here on Hnm-mapped tables;
+        // those with faces are built whole by the per-lane kernel.
+        hang = A.adap && A.adap[e];
+        if (hang && boundary) return;
         if (boundary && !W64) return;
     } else {   // padding slot of a color range: zero block
         if (W64)
-            for (int q = lane; q < KK; q += 64) A.R64[blk_index<double>(e, q, KK)] = 0.0;
+            for (int q = lane; q < KK; q += 64) A.R64[r64_at<8, 5>(A, e, q)] = 0.0;
         if (W32)
             for (int q = lane; q < KK; q += 64) A.R32[blk_index<float>(e, q, KK)] = 0.0f;
         if (lane < K) A.rhs[e * K + lane] = 0.0;
         return;
     }
     if (lane < NNOD) load_node3d(A, cn[lane], s_nd[lane]);
-    s_g.phi3[lane >> 3][lane & 7] = c_gauss.phi3[lane >> 3][lane & 7];
+    if (!hang) {
+        s_g.phi3[lane >> 3][lane & 7] = c_gauss.phi3[lane >> 3][lane & 7];
 #pragma unroll
-    for (int j = 0; j < 3; j++) s_g.dpxi3[lane >> 3][lane & 7][j] = c_gauss.dpxi3[lane >> 3][lane & 7][j];
+        for (int j = 0; j < 3; j++) s_g.dpxi3[lane >> 3][lane & 7][j] = c_gauss.dpxi3[lane >> 3][lane & 7][j];
+    } else {
+        // adap() (MeshRefine2D.h:877-900): Phi <- Phi Hnm, dPxi <- dPxi Hnm,
+        // lane (gp, M) one column, summed over N in order
+        const int gp = lane >> 3, m = lane & 7;
+        const double* Hn = A.hnm + e * 64;
+        double ph = 0.0, dp[3] = {0.0, 0.0, 0.0};
+        for (int n = 0; n < 8; n++) {
+            const double h = Hn[n + 8 * m];
+            ph += c_gauss.phi3[gp][n] * h;
+#pragma unroll
+            for (int j = 0; j < 3; j++) dp[j] += c_gauss.dpxi3[gp][n][j] * h;
+        }
+        s_g.phi3[gp][m] = ph;
+#pragma unroll
+        for (int j = 0; j < 3; j++) s_g.dpxi3[gp][m][j] = dp[j];
+    }
     if (lane < 8) s_g.w3[lane] = c_gauss.w3[lane];
     __syncthreads();
     double sp[4];
@@ -753,7 +778,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 #pragma unroll
         for (int s = 0; s < NEQ; s++) {
             const int q = (NEQ * N + r) * K + NEQ * M + s;
-            if (W64) A.R64[blk_index<double>(e, q, KK)] = R[r][s];
+            if (W64) A.R64[r64_at<8, 5>(A, e, q)] = R[r][s];
             if (W32 && !boundary) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
         }
     if (M == 0) {
@@ -1132,7 +1157,7 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
         for (int r = 0; r < NEQ; r++) {
             rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
 #pragma unroll
-            for (int s = 0; s < NEQ; s++) R[r][s] = A.R64[blk_index<double>(e, (NEQ * N + r) * K + NEQ * M + s, KK)];
+            for (int s = 0; s < NEQ; s++) R[r][s] = A.R64[r64_at<8, 5>(A, e, (NEQ * N + r) * K + NEQ * M + s)];
         }
 #pragma unroll 1
         for (int q = 0; q < nb; q++)
@@ -1144,7 +1169,7 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
 #pragma unroll
             for (int s = 0; s < NEQ; s++) {
                 const int q = (NEQ * N + r) * K + NEQ * M + s;
-                A.R64[blk_index<double>(e, q, KK)] = R[r][s];
+                A.R64[r64_at<8, 5>(A, e, q)] = R[r][s];
                 if (W32 && last) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
             }
         if (M == 0) {
@@ -1177,7 +1202,7 @@ __global__ __launch_bounds__(256) void build3d_kernel(BuildArgs A, const int* li
     if (!in_range) return;
     if (!real) {   // padding slot of a color range: zero block
         if (W64)
-            for (int q = lane; q < KK; q += 64) A.R64[blk_index<double>(e, q, KK)] = 0.0;
+            for (int q = lane; q < KK; q += 64) A.R64[r64_at<8, 5>(A, e, q)] = 0.0;
         if (W32)
             for (int q = lane; q < KK; q += 64) A.R32[blk_index<float>(e, q, KK)] = 0.0f;
         if (lane < K) A.rhs[e * K + lane] = 0.0;
@@ -1194,7 +1219,7 @@ __global__ __launch_bounds__(256) void build3d_kernel(BuildArgs A, const int* li
 #pragma unroll
         for (int s = 0; s < NEQ; s++) {
             const int q = (NEQ * N + r) * K + NEQ * M + s;
-            if (W64) A.R64[blk_index<double>(e, q, KK)] = R[r][s];
+            if (W64) A.R64[r64_at<8, 5>(A, e, q)] = R[r][s];
             if (W32) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
         }
     if (M == 0) {
@@ -1625,7 +1650,7 @@ __global__ __launch_bounds__(256) void build2d_kernel(BuildArgs A) {
     const int* cn = A.conn + e * NNOD;
     if (cn[0] < 0) {
         for (int q = l16; q < KK; q += 16) {
-            if (W64) A.R64[blk_index<double>(e, q, KK)] = 0.0;
+            if (W64) A.R64[r64_at<4, 4>(A, e, q)] = 0.0;
             if (W32) A.R32[blk_index<float>(e, q, KK)] = 0.0f;
         }
         A.rhs[e * K + l16] = 0.0;
@@ -1640,7 +1665,7 @@ __global__ __launch_bounds__(256) void build2d_kernel(BuildArgs A) {
 #pragma unroll
         for (int s = 0; s < NEQ; s++) {
             const int q = (NEQ * N + r) * K + NEQ * M + s;
-            if (W64) A.R64[blk_index<double>(e, q, KK)] = R[r][s];
+            if (W64) A.R64[r64_at<4, 4>(A, e, q)] = R[r][s];
             if (W32) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
         }
     if (M == 0) {

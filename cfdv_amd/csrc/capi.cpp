@@ -127,6 +127,7 @@ struct cfdv_ctx {
     DevThermo th{};
     hipStream_t st = nullptr;
     int block_flags = 0;
+    int r64_pairs = 0;   // FP64 element blocks in pair_offset order (contexts with rows)
     bool built = false, have_du = false;
 
     int *d_conn = nullptr, *d_bc = nullptr, *d_perm_inv = nullptr;
@@ -154,7 +155,7 @@ struct cfdv_ctx {
     bool pinv_valid = false;
     // assembled node rows (CFDV_BLOCKS_ROWS), formed after each build
     RowsArgs ra{};
-    long n_groups = 0;
+    long n_groups = 0, n_row_blocks = 0;
     long *d_goff = nullptr, *d_coff = nullptr, *d_diag = nullptr;
     int *d_cols = nullptr, *d_contrib = nullptr;
     double* d_B64 = nullptr;
@@ -280,7 +281,7 @@ struct cfdv_ctx {
             if (prec == 32) vec_fill_f((float*)y, 0.0f, ndof, st, gate);
             else vec_fill((double*)y, 0.0, ndof, st, gate);
             for (int c = 0; c < ncolor; c++) {
-                MatvecArgs a{ndim, nnod, neqn, cstart[c], cstart[c + 1], d_conn};
+                MatvecArgs a{ndim, nnod, neqn, cstart[c], cstart[c + 1], d_conn, r64_pairs};
                 launch_matvec_color(a, d_R64, d_R32, x, y, prec, tr, st, gate);
             }
         }
@@ -399,6 +400,7 @@ DevThermo make_thermo(const cfdv_thermo* t) {
 struct RowsHost {
     std::vector<long> goff, coff, diag;
     std::vector<int> cols, contrib;
+    long n_blocks = 0;   // node pairs sharing an element (the real blocks)
 };
 
 RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* conn, const std::vector<int>& perm_inv,
@@ -450,6 +452,7 @@ RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* 
     // chunks and groups
     const long nch = (nnode + 63) / 64;
     RowsHost h;
+    h.n_blocks = (long)adj.size();
     std::vector<long>& goff = h.goff;
     goff.assign(nch + 1, 0);
     for (long ch = 0; ch < nch; ch++) {
@@ -505,6 +508,7 @@ void setup_rows(cfdv_ctx* c, const cfdv_mesh* m, const std::vector<int>& ptr, co
     const auto &goff = h.goff, &coff = h.coff;
     const auto &cols = h.cols, &contrib = h.contrib;
     c->n_groups = ng;
+    c->n_row_blocks = h.n_blocks;
     c->d_goff = c->alloc<long>(nch + 1);
     c->d_cols = c->alloc<int>(cols.size());
     c->d_coff = c->alloc<long>(coff.size());
@@ -518,7 +522,7 @@ void setup_rows(cfdv_ctx* c, const cfdv_mesh* m, const std::vector<int>& ptr, co
     const size_t nval = (size_t)ng * 4 * c->neqn * c->neqn * 64;
     c->d_B64 = c->alloc<double>(nval);
     c->d_B32 = c->alloc<float>(nval);
-    c->ra = RowsArgs{nnode, nch, c->nnod, c->neqn, c->d_goff, c->d_cols, c->d_coff, c->d_contrib};
+    c->ra = RowsArgs{nnode, nch, c->nnod, c->neqn, c->d_goff, c->d_cols, c->d_coff, c->d_contrib, c->r64_pairs};
 }
 
 void setup(cfdv_ctx* c, const cfdv_mesh* m) {
@@ -643,12 +647,14 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
     if (c->ndim == 3)
         for (long s = 0; s < np; s++) {
             if (c->perm[s] < 0) continue;
-            if (!adap_s.empty() && adap_s[s]) {
-                aslots.push_back((int)s);
-                continue;
-            }
             bool b = cmask[s] != 0ULL;
             for (int f = 0; f < nf; f++) b = b || bc_s[s * nf + f] != 0;
+            if (!adap_s.empty() && adap_s[s]) {
+                // hanging-node elements with faces: whole element per lane;
+                // without, the interior kernel (Hnm-mapped tables)
+                if (b) aslots.push_back((int)s);
+                continue;
+            }
             if (b) bslots.push_back((int)s);
         }
     c->n_bslots = (long)bslots.size();
@@ -1178,6 +1184,7 @@ int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flag
         ctx->n_node = mesh->n_node;
         ctx->ndof = mesh->n_node * mesh->neqn;
         ctx->block_flags = block_flags;
+        ctx->r64_pairs = (block_flags & CFDV_BLOCKS_ROWS) ? 1 : 0;
         ctx->th = make_thermo(thermo);
         HIP_CHECK_THROW(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
         for (auto& e : ctx->ev) HIP_CHECK_THROW(hipEventCreate(&e));
@@ -1215,6 +1222,13 @@ int cfdv_mesh_info(const cfdv_ctx* ctx, int64_t* n_elem, int64_t* n_node, int* n
     if (n_node) *n_node = ctx->n_node;
     if (ncolors) *ncolors = ctx->ncolor;
     if (n_slots) *n_slots = ctx->n_pad;
+    return CFDV_OK;
+}
+
+int cfdv_rows_info(const cfdv_ctx* ctx, int64_t* n_blocks, int64_t* n_slots) {
+    if (!ctx) return CFDV_ERR_ARG;
+    if (n_blocks) *n_blocks = ctx->d_B64 ? ctx->n_row_blocks : 0;
+    if (n_slots) *n_slots = ctx->d_B64 ? ctx->n_groups * 4 * 64 : 0;
     return CFDV_OK;
 }
 
@@ -1284,6 +1298,7 @@ int cfdv_build(cfdv_ctx* ctx, double dt) {
     a.hnm = ctx->d_hnm;
     a.aslots = ctx->d_aslots;
     a.n_aslots = ctx->n_aslots;
+    a.r64_pairs = ctx->r64_pairs;
     launch_build(a, ctx->d_R64 != nullptr, ctx->d_R32 != nullptr, ctx->st);
     if (ctx->timing) HIP_CHECK_THROW(hipEventRecord(ctx->ev[5], ctx->st));
     if (ctx->d_B64) launch_rows_assemble(ctx->ra, ctx->d_R64, false, ctx->d_B64, ctx->d_B32, ctx->st);
@@ -1316,7 +1331,9 @@ int cfdv_get_blocks(cfdv_ctx* ctx, double* R, double* rhs, double* s) {
             const long sl = ctx->perm_inv[e];
             for (int i = 0; i < k; i++)
                 for (int j = 0; j < k; j++)
-                    R[(size_t)e * kk + i + (size_t)k * j] = h[blk_offset<double>(sl, i * k + j, (int)kk)];
+                    R[(size_t)e * kk + i + (size_t)k * j] =
+                        h[!ctx->r64_pairs ? blk_offset<double>(sl, i * k + j, (int)kk)
+                          : ctx->ndim == 3 ? pair_offset<8, 5>(sl, i * k + j) : pair_offset<4, 4>(sl, i * k + j)];
         }
     }
     if (rhs || s) {
@@ -1669,7 +1686,7 @@ static void matvec_multi_dev(cfdv_ctx* ctx, const float* X, float* Y, bool tr) {
     }
     vec_fill_f(Y, 0.0f, ctx->ndof * 8, ctx->st);
     for (int c = 0; c < ctx->ncolor; c++) {
-        MatvecArgs a{ctx->ndim, ctx->nnod, ctx->neqn, ctx->cstart[c], ctx->cstart[c + 1], ctx->d_conn};
+        MatvecArgs a{ctx->ndim, ctx->nnod, ctx->neqn, ctx->cstart[c], ctx->cstart[c + 1], ctx->d_conn, 0};
         launch_matvec_multi_color(a, ctx->d_R32, X, Y, tr, ctx->st);
     }
     ctx->check_launch();

@@ -29,7 +29,8 @@ struct BuildArgs {
     const double* x;          // [n_node][ndim]
     const double* U;          // [n_node][neqn]
     double dt;
-    double* R64;              // chunked SoA blocks or nullptr
+    double* R64;              // chunked SoA blocks (pair_offset layout with
+                              // r64_pairs) or nullptr
     float* R32;
     double* rhs;              // [n_elem_pad][k]
     double* svals;            // [n_elem_pad][4] or nullptr
@@ -41,6 +42,7 @@ struct BuildArgs {
     const double* hnm;        // [n_elem_pad][64] Hnm(N, M) at N + 8 M (adapted slots)
     const int* aslots;        // 3D adapted slots (full per-lane kernel), ascending
     long n_aslots;
+    int r64_pairs;            // R64 in pair_offset order (contexts with rows)
 };
 
 void upload_gauss_tables(const GaussTables& g);
@@ -84,6 +86,7 @@ struct MatvecArgs {
     int ndim, nnod, neqn;
     long e_begin, e_end;      // color range (multiple of 64)
     const int* conn;          // [n_elem_pad][nnod]
+    int r64_pairs;            // FP64 blocks in pair_offset order
 };
 // y += sum over elements of one color of R_e x_e (or R_e^T x_e)
 // Kernels that take a gate (a device int) return at once when *gate != 0:
@@ -182,6 +185,7 @@ struct RowsArgs {
     const int* cols;       // [n_groups * 4][64] column node of each slot
     const long* coff;      // [n_groups * 4 + 1] contributor rows per slot
     const int* contrib;    // [coff[end]][64] element block codes, -1 = none
+    int r64_pairs;         // the FP64 element blocks are in pair_offset order
 };
 // y = K x over the rows; prec 64 (FP64 rows and vectors), 32 (FP32), 3 (FP32
 // rows, FP64 vectors).  y is written, not accumulated.

@@ -60,6 +60,18 @@ CFDV_HD size_t blk_offset(long e, int q, int KK) {
     return (size_t)(e >> 6) * 64 * KK + (size_t)(q - r) * 64 + (size_t)(e & 63) * V + r;
 }
 
+// FP64 element blocks of contexts with assembled node rows: element e's
+// k x k block as NNOD^2 contiguous NEQ x NEQ node-pair blocks, pair (N, M)
+// at (N NNOD + M) NEQ^2, row-major inside (R(i, j), i = N NEQ + r,
+// j = M NEQ + s): the row assembly reads each pair block as one contiguous
+// run, whatever the element order around a node.
+template <int NNOD, int NEQ>
+CFDV_HD size_t pair_offset(long e, int q) {
+    constexpr int K = NNOD * NEQ;
+    const int i = q / K, j = q % K;
+    return (size_t)e * K * K + (size_t)(((i / NEQ) * NNOD + j / NEQ) * NEQ * NEQ + (i % NEQ) * NEQ + j % NEQ);
+}
+
 template <int NDIM> struct Dims;
 template <> struct Dims<3> { static constexpr int NNOD = 8, NEQ = 5, NB = 4, NFACE = 6, NGP = 8, NFGP = 4; };
 template <> struct Dims<2> { static constexpr int NNOD = 4, NEQ = 4, NB = 2, NFACE = 4, NGP = 4, NFGP = 2; };

@@ -169,15 +169,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     }
 }
 
-// rows from the FP64 element blocks (chunked layout of blk_offset: the
-// entries of an element row come in 16-byte pairs).  A block row of NEQ
+// rows from the FP64 element blocks.  PAIRS (contexts with rows): each
+// node-pair block is NEQ^2 contiguous doubles (pair_offset), read whole.
+// Otherwise the chunked layout of blk_offset (the entries of an element row
+// come in 16-byte pairs).  A block row of NEQ
 // entries starting at column j0 = M NEQ is read as whole pairs: NEQ / 2 of
 // them for even NEQ, (NEQ + 1) / 2 from the pair holding j0 for odd NEQ
 // (one entry of a neighbouring block read and dropped), so each lane's
 // loads are full 16-byte vectors.  TR: the rows of K^T -- slot (n, c) sums
 // the transposed blocks K_e(c, n), the same contributors with N and M
 // swapped.
-template <int NNOD, int NEQ, bool TR>
+template <int NNOD, int NEQ, bool TR, bool PAIRS = false>
 __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const long* __restrict__ goff,
                                                             const long* __restrict__ coff,
                                                             const int* __restrict__ contrib,
@@ -204,7 +206,21 @@ __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const
             int code = p < pe ? contrib[p * 64 + lane] : -1;
             for (; p < pe; p++) {
                 const int code_n = p + 1 < pe ? contrib[(p + 1) * 64 + lane] : -1;
-                if (code >= 0) {
+                if (PAIRS && code >= 0) {
+                    // pair (N, M) of element s: BB contiguous doubles
+                    const long s = code / NN2;
+                    const int nm = code % NN2;
+                    const int N = TR ? nm % NNOD : nm / NNOD, M = TR ? nm / NNOD : nm % NNOD;
+                    const double* Rp = R + (size_t)s * KK + (size_t)(N * NNOD + M) * BB;
+                    double w[BB];
+#pragma unroll
+                    for (int i = 0; i < BB; i++) w[i] = Rp[i];
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int r = 0; r < NEQ; r++)
+#pragma unroll
+                        for (int q = 0; q < NEQ; q++) acc[TR ? q * NEQ + r : r * NEQ + q] += w[r * NEQ + q];
+                } else if (code >= 0) {
                     const long s = code / NN2;
                     const int nm = code % NN2;
                     const int N = TR ? nm % NNOD : nm / NNOD, M = TR ? nm / NNOD : nm % NNOD;
@@ -318,15 +334,15 @@ void launch_rows_matvec8(const RowsArgs& a, const float* B32, const float* X, fl
 void launch_rows_assemble(const RowsArgs& a, const double* R64, bool tr, double* B64, float* B32, hipStream_t st) {
     if (a.n_chunks <= 0) return;
     const dim3 grid(row_grid(a.n_chunks)), blk(256);
-#define L(NN, NQ, T)                                                                                     \
-    hipLaunchKernelGGL((rows_assemble_kernel<NN, NQ, T>), grid, blk, 0, st, a.n_chunks, a.goff, a.coff, a.contrib, \
+#define L(NN, NQ, T, P)                                                                                        \
+    hipLaunchKernelGGL((rows_assemble_kernel<NN, NQ, T, P>), grid, blk, 0, st, a.n_chunks, a.goff, a.coff, a.contrib, \
                        R64, B64, B32)
     if (a.nnod == 8) {
-        if (tr) L(8, 5, true);
-        else L(8, 5, false);
+        if (a.r64_pairs) { if (tr) L(8, 5, true, true); else L(8, 5, false, true); }
+        else { if (tr) L(8, 5, true, false); else L(8, 5, false, false); }
     } else {
-        if (tr) L(4, 4, true);
-        else L(4, 4, false);
+        if (a.r64_pairs) { if (tr) L(4, 4, true, true); else L(4, 4, false, true); }
+        else { if (tr) L(4, 4, true, false); else L(4, 4, false, false); }
     }
 #undef L
 }

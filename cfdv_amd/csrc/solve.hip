@@ -23,7 +23,7 @@ template <typename T> struct Vec16;
 template <> struct Vec16<double> { typedef v2d_t type; };
 template <> struct Vec16<float> { typedef v4f_t type; };
 
-template <typename TB, typename TV, typename TA, int NNOD, int NEQ, bool TR>
+template <typename TB, typename TV, typename TA, int NNOD, int NEQ, bool TR, bool PAIRS = false>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) void matvec_kernel(long e0, long e1, const int* __restrict__ conn,
                                                      const TB* __restrict__ R, const TV* __restrict__ x,
                                                      TV* __restrict__ y, const int* __restrict__ gate) {
@@ -34,6 +34,61 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     if (e >= e1) return;
     const int* ce = conn + e * NNOD;
     if (ce[0] < 0) return;
+    if constexpr (PAIRS) {
+        // blocks in node-pair order (pair_offset): scalar loads, the same
+        // products summed in the same order as the chunked path
+        const TB* Re = R + (size_t)e * KK;
+        if (!TR) {
+            TA xv[K];
+#pragma unroll
+            for (int m = 0; m < NNOD; m++) {
+                const int gm = ce[m];
+#pragma unroll
+                for (int q = 0; q < NEQ; q++) xv[m * NEQ + q] = (TA)x[(size_t)gm * NEQ + q];
+            }
+            for (int n = 0; n < NNOD; n++) {
+                const int gn = ce[n];
+#pragma unroll
+                for (int r = 0; r < NEQ; r++) {
+                    TA acc = 0;
+#pragma unroll
+                    for (int m = 0; m < NNOD; m++)
+#pragma unroll
+                        for (int s = 0; s < NEQ; s++)
+                            acc += (TA)Re[(n * NNOD + m) * NEQ * NEQ + r * NEQ + s] * xv[m * NEQ + s];
+                    TV* yp = y + (size_t)gn * NEQ + r;
+                    *yp = (TV)((TA)*yp + acc);
+                }
+            }
+        } else {
+            TA yv[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) yv[j] = 0;
+            for (int n = 0; n < NNOD; n++) {
+                const int gn = ce[n];
+                TA xi[NEQ];
+#pragma unroll
+                for (int r = 0; r < NEQ; r++) xi[r] = (TA)x[(size_t)gn * NEQ + r];
+#pragma unroll
+                for (int r = 0; r < NEQ; r++)
+#pragma unroll
+                    for (int m = 0; m < NNOD; m++)
+#pragma unroll
+                        for (int s = 0; s < NEQ; s++)
+                            yv[m * NEQ + s] += (TA)Re[(n * NNOD + m) * NEQ * NEQ + r * NEQ + s] * xi[r];
+            }
+#pragma unroll
+            for (int m = 0; m < NNOD; m++) {
+                const int gm = ce[m];
+#pragma unroll
+                for (int q = 0; q < NEQ; q++) {
+                    TV* yp = y + (size_t)gm * NEQ + q;
+                    *yp = (TV)((TA)*yp + yv[m * NEQ + q]);
+                }
+            }
+        }
+        return;
+    }
     const vec_t* Rb = reinterpret_cast<const vec_t*>(R + (size_t)(e >> 6) * 64 * KK) + (e & 63);
     if (!TR) {
         // y_block(n) = sum_j R(5n+r, j) x_j : all k inputs in registers, one
@@ -109,6 +164,15 @@ void launch_matvec_color(const MatvecArgs& a, const double* R64, const float* R3
     hipLaunchKernelGGL((matvec_kernel<TB, TV, TA, NN, NQ, T>), dim3(nb), dim3(bs), 0, st, a.e_begin, \
                        a.e_end, a.conn, (const TB*)(prec == 64 ? (const void*)R64 : (const void*)R32),  \
                        (const TV*)x, (TV*)y, gate)
+#define LP(NN, NQ, T)                                                                                      \
+    hipLaunchKernelGGL((matvec_kernel<double, double, double, NN, NQ, T, true>), dim3(nb), dim3(bs), 0, st, \
+                       a.e_begin, a.e_end, a.conn, R64, (const double*)x, (double*)y, gate)
+    if (prec == 64 && a.r64_pairs) {
+        if (a.ndim == 3) { if (tr) LP(8, 5, true); else LP(8, 5, false); }
+        else { if (tr) LP(4, 4, true); else LP(4, 4, false); }
+        return;
+    }
+#undef LP
     if (a.ndim == 3) {
         if (prec == 64) { if (tr) L(double, double, double, 8, 5, true); else L(double, double, double, 8, 5, false); }
         else if (prec == 32) { if (tr) L(float, float, float, 8, 5, true); else L(float, float, float, 8, 5, false); }

@@ -108,6 +108,12 @@ class FDVContext:
                                           C.byref(ns)))
         return {"n_elem": ne.value, "n_node": nn.value, "ncolors": nc.value, "n_slots": ns.value}
 
+    def rows_info(self):
+        """(node blocks, stored block slots) of the assembled rows"""
+        nb, ns = C.c_int64(), C.c_int64()
+        self._check(self.L.cfdv_rows_info(self.h, C.byref(nb), C.byref(ns)))
+        return nb.value, ns.value
+
     # ---- step pieces ----
     def cfl(self, cfl: float) -> float:
         dt = C.c_double()

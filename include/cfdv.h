@@ -189,6 +189,11 @@ const char *cfdv_last_error(const cfdv_ctx *ctx);
 int cfdv_mesh_info(const cfdv_ctx *ctx, int64_t *n_elem, int64_t *n_node, int *ncolors,
                    int64_t *n_slots);
 
+/* the assembled node rows (CFDV_BLOCKS_ROWS; zeros without): n_blocks node
+   pairs that share an element (the neqn x neqn blocks a product reads),
+   n_slots stored block slots including the padding of 64-row chunks (no
+   reference counterpart; measurement hook) */
+int cfdv_rows_info(const cfdv_ctx *ctx, int64_t *n_blocks, int64_t *n_slots);
 int cfdv_set_state(cfdv_ctx *ctx, const double *U);            /* [n_node][neqn] */
 int cfdv_get_state(cfdv_ctx *ctx, double *U, double *U0);      /* either may be NULL */
 int cfdv_cfl(cfdv_ctx *ctx, double cfl, double *dt);
