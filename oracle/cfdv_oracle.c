@@ -47,10 +47,18 @@
  *  orc_color_greedy*         brute-force first fit, validity
  *  orc_refine_flags          s1 > 0.8 of orc_fdvparam
  *  orc_update, orc_rms       closed forms
- * parity unpinned: orc_eval1dboundary (2D face edits) and the 2D
- *  discontinuity-capturing coefficients are reached only through the 2D
- *  column-sum/conservation invariants, which they leave unchanged; their
- *  values follow the source by reading only.
+ *  orc_eval1dboundary        per-code closed forms of the edits, including
+ *                            the sequential projections (v_1 - (v.n) n_1^3)
+ *  orc_psi                   numerical diffusion / discontinuity capturing:
+ *                            closed form on a rectangle (h_eta sign, coth,
+ *                            1/4, tau_b floor) with and without dV
+ *  3D domain assembly        fluid-at-rest closed form (c_ij and a_i terms)
+ *                            and a linear field against a numpy quadrature of
+ *                            the FDV Galerkin weak form (b terms, s1..s4)
+ *  Built with -DORC_MUTATE=k the oracle carries a planted mistake (k = 1:
+ *  s1/s3 swapped in g; 2: sign of dt s3 c_ij; 3: dt for dt^2/2 on the a, b
+ *  products; 4: the tau_b floor dropped; 5: h_eta's sign): test only, to
+ *  show each pin above fails on it.
  */
 #include "cfdv_oracle.h"
 
@@ -1263,7 +1271,11 @@ static void sum_domain_terms(bctx *c)
     for (int i = 0; i < nd; i++)
         for (int r = 0; r < nq; r++)
             for (int s = 0; s < nq; s++) {
+#if ORC_MUTATE == 1   /* planted mistake: s1 and s3 swapped in g */
+                c->g[i][r][s] -= dt * (c->s3 * c->a[i][r][s] + c->s1 * c->b[i][r][s]);
+#else
                 c->g[i][r][s] -= dt * (c->s1 * c->a[i][r][s] + c->s3 * c->b[i][r][s]);
+#endif
                 c->nd[i][r][s] += dt * (c->s1 * c->a[i][r][s] + c->s3 * c->b[i][r][s]);
                 for (int t = 0; t < nq; t++)
                     c->g[i][r][s] -= dtsq_2 * (c->s2 * (c->d[r][t] * c->a[i][t][s]) +
@@ -1287,10 +1299,20 @@ static void sum_domain_terms(bctx *c)
         for (int j = 0; j < nd; j++)
             for (int r = 0; r < nq; r++)
                 for (int s = 0; s < nq; s++) {
+#if ORC_MUTATE == 2   /* planted mistake: the sign of the dt s3 c_ij term */
+                    c->h[i][j][r][s] += (dt * c->s3 * c->c[i][j][r][s]);
+#else
                     c->h[i][j][r][s] -= (dt * c->s3 * c->c[i][j][r][s]);
+#endif
+#if ORC_MUTATE == 3   /* planted mistake: dt where dt^2/2 belongs on the a, b products */
+                    for (int t = 0; t < nq; t++)
+                        c->h[i][j][r][s] -= (-dt * (c->s2 * (c->a[i][r][t] * c->a[j][t][s] + c->b[i][r][t] * c->a[j][t][s]) +
+                                                    c->s4 * ((c->a[i][r][t] * c->b[j][t][s] + c->b[i][r][t] * c->b[j][t][s]))));
+#else
                     for (int t = 0; t < nq; t++)
                         c->h[i][j][r][s] -= (-dtsq_2 * (c->s2 * (c->a[i][r][t] * c->a[j][t][s] + c->b[i][r][t] * c->a[j][t][s]) +
                                                         c->s4 * ((c->a[i][r][t] * c->b[j][t][s] + c->b[i][r][t] * c->b[j][t][s]))));
+#endif
                 }
     for (int beta = 0; beta < nn; beta++)
         for (int r = 0; r < nq; r++) {
@@ -1347,32 +1369,14 @@ static void adap_transform(int nn, int nd, double *Phi, double *dPxi,
     memcpy(dPxi, dH, sizeof(double) * nn * nd);
 }
 
-/* domain (:157-403) at one Gauss point */
-static void domain_gp(bctx *c, const double *w, const double *zeta)
+/* Numerical diffusion (2D) and discontinuity capturing of one Gauss point,
+   FDVGalerkin.h:225-387: Psi[N] += (tau + tau_b) v . dPhi_N; xn [nnod][nd]
+   node coordinates, zeta the point, dPhi [nnod][nd] at the point.  In 3D
+   tau = 0 and so Psi = 0 (Q8). */
+void orc_psi(int nd, int nn, const orc_flow *f, const double *xn, const double *zeta, const double *dPhi,
+             double *Psi)
 {
-    int nd = c->ndim, nn = c->nnod;
-    const int *gn = c->m->conn + (size_t)c->e * nn;
-    double xn[MAXN * MAXD];
-    elem_coords(c->m, c->e, xn);
-    int adapted = c->m->adap && c->m->adap[c->e];
-    const double *Hnm = adapted ? c->m->hnm + (size_t)c->e * nn * nn : NULL;
-
-    double Phi[MAXN], dPxi[MAXN * MAXD], dPhi[MAXN * MAXD], J;
-    if (nd == 2) {
-        orc_shape2d(zeta, Phi, dPxi);
-        if (adapted) adap_transform(nn, nd, Phi, dPxi, Hnm);
-        J = orc_jacobian2d(dPxi, xn, NULL, dPhi);
-    } else {
-        orc_shape3d(zeta, Phi, dPxi);
-        if (adapted) adap_transform(nn, nd, Phi, dPxi, Hnm);
-        J = orc_jacobian3d(dPxi, xn, NULL, dPhi);
-    }
-
-    flow_sum(c, gn, nn, Phi, dPhi);
-    orc_tau(nd, c->flow.mu, c->flow.dV, c->flow.tau);
-    orc_flow *f = &c->flow;
-
-    double Psi[MAXN] = {0};
+    for (int N = 0; N < nn; N++) Psi[N] = 0.0;
     double tau = 0;
     if (nd == 2) {
         /* numerical diffusion, :225-332 */
@@ -1408,7 +1412,11 @@ static void domain_gp(bctx *c, const double *w, const double *zeta)
         if (R_eta == 0) alpha_eta = 0;
         else alpha_eta = coth(R_eta / 2.) - 2. / R_eta;
         if (S == 0) tau = 0;
+#if ORC_MUTATE == 5   /* planted mistake: the element height h_eta with the other sign */
+        else tau = (1. / 4.) * (alpha_xi * h_xi * v_xi - alpha_eta * h_eta * v_eta) / S;
+#else
         else tau = (1. / 4.) * (alpha_xi * h_xi * v_xi + alpha_eta * h_eta * v_eta) / S;
+#endif
         for (int N = 0; N < nn; N++)
             for (int i = 0; i < nd; i++) Psi[N] += tau * f->v[i] * dPhi[N * nd + i];
     }
@@ -1420,11 +1428,44 @@ static void domain_gp(bctx *c, const double *w, const double *zeta)
         double gamma = mag3(f->v, nd) * mag3(A, nd);
         double tau_b = 0;
         for (int j = 0; j < nd; j++) tau_b += tau * f->v[j] * A[j] / gamma;
+#if ORC_MUTATE == 4   /* planted mistake: the discontinuity-capturing floor dropped */
+        (void)0;
+#else
         if ((tau_b - tau) < 0) tau_b = tau;
+#endif
         if (gamma == 0) tau_b = 0;
         for (int N = 0; N < nn; N++)
             for (int i = 0; i < nd; i++) Psi[N] += tau_b * f->v[i] * dPhi[N * nd + i];
     }
+}
+
+/* domain (:157-403) at one Gauss point */
+static void domain_gp(bctx *c, const double *w, const double *zeta)
+{
+    int nd = c->ndim, nn = c->nnod;
+    const int *gn = c->m->conn + (size_t)c->e * nn;
+    double xn[MAXN * MAXD];
+    elem_coords(c->m, c->e, xn);
+    int adapted = c->m->adap && c->m->adap[c->e];
+    const double *Hnm = adapted ? c->m->hnm + (size_t)c->e * nn * nn : NULL;
+
+    double Phi[MAXN], dPxi[MAXN * MAXD], dPhi[MAXN * MAXD], J;
+    if (nd == 2) {
+        orc_shape2d(zeta, Phi, dPxi);
+        if (adapted) adap_transform(nn, nd, Phi, dPxi, Hnm);
+        J = orc_jacobian2d(dPxi, xn, NULL, dPhi);
+    } else {
+        orc_shape3d(zeta, Phi, dPxi);
+        if (adapted) adap_transform(nn, nd, Phi, dPxi, Hnm);
+        J = orc_jacobian3d(dPxi, xn, NULL, dPhi);
+    }
+
+    flow_sum(c, gn, nn, Phi, dPhi);
+    orc_tau(nd, c->flow.mu, c->flow.dV, c->flow.tau);
+    orc_flow *f = &c->flow;
+
+    double Psi[MAXN] = {0};
+    orc_psi(nd, nn, f, xn, zeta, dPhi, Psi);
     calc_abcd(c);
     sum_fgb(c, gn, nn);
     sum_domain_terms(c);

@@ -116,6 +116,11 @@ void orc_calc_c(const orc_thermo *th, int ndim, const orc_flow *f,
 void orc_calc_G(int ndim, const double tau[3][3], const double *v,
                 const double *dT, double HK, double Re, double *G /*[ndim][neqn]*/);
 void orc_calc_F2d(const double *U, double p, double *F /*[2][4]*/);
+/* numerical diffusion + discontinuity capturing of one Gauss point
+   (FDVGalerkin.h:225-387): Psi [nnod]; xn [nnod][nd] element nodes, zeta
+   the point, dPhi [nnod][nd] at the point (Psi = 0 in 3D) */
+void orc_psi(int nd, int nn, const orc_flow *f, const double *xn, const double *zeta, const double *dPhi,
+             double *Psi);
 /* Boundary edits of the Gauss-point state; returns 0 or -1 (unknown bc) */
 int  orc_eval2dboundary(const orc_thermo *th, int bc, orc_flow *f,
                         const double *en);

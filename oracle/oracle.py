@@ -70,57 +70,80 @@ def lib() -> C.CDLL:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build_lib()
-        L = C.CDLL(path)
-        L.orc_thermo_set.argtypes = [C.POINTER(Thermo)] + [C.c_double] * 6 + [C.c_int]
-        L.orc_calc_mu.restype = C.c_double
-        L.orc_calc_mu.argtypes = [C.POINTER(Thermo), C.c_double]
-        L.orc_calc_k.restype = C.c_double
-        L.orc_calc_k.argtypes = [C.POINTER(Thermo), C.c_double]
-        L.orc_gauss.argtypes = [C.c_int, dp, dp]
-        L.orc_shape3d.argtypes = [dp, dp, dp]
-        L.orc_shape2d.argtypes = [dp, dp, dp]
-        L.orc_jacobian3d.restype = C.c_double
-        L.orc_jacobian3d.argtypes = [dp, dp, dp, dp]
-        L.orc_jacobian2d.restype = C.c_double
-        L.orc_jacobian2d.argtypes = [dp, dp, dp, dp]
-        L.orc_tfnorm3d.restype = C.c_double
-        L.orc_tfnorm3d.argtypes = [dp, dp, dp, dp, dp, dp]
-        L.orc_node_unpack_all.argtypes = [C.POINTER(Thermo), C.c_int, C.c_int, C.c_int, dp,
-                                          C.POINTER(Prim)]
-        L.orc_calc_length.argtypes = [C.POINTER(MeshS), dp, dp]
-        L.orc_cfl.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), C.POINTER(Prim), dp,
-                              C.c_double, dp]
-        L.orc_fdvparam.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), C.POINTER(Prim),
-                                   C.c_int, dp]
-        L.orc_tau.argtypes = [C.c_int, C.c_double, dp, dp]
-        for f in ("orc_calc_a", "orc_calc_b", "orc_calc_c"):
-            getattr(L, f).argtypes = [C.POINTER(Thermo), C.c_int, C.POINTER(Flow), dp]
-        L.orc_calc_G.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp]
-        L.orc_calc_F2d.argtypes = [dp, C.c_double, dp]
-        L.orc_eval2dboundary.argtypes = [C.POINTER(Thermo), C.c_int, C.POINTER(Flow), dp]
-        L.orc_build_element.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), dp,
-                                        C.POINTER(Prim), C.c_int, C.c_double, dp, dp, dp]
-        L.orc_build.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), dp, C.POINTER(Prim),
-                                C.c_double, dp, dp, dp]
-        L.orc_apply_bc.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), dp, dp]
-        L.orc_compose_rhs.argtypes = [C.POINTER(MeshS), dp, dp]
-        L.orc_matvec.argtypes = [C.POINTER(MeshS), dp, dp, dp]
-        L.orc_matvec_t.argtypes = [C.POINTER(MeshS), dp, dp, dp]
-        L.orc_gmres_quirk.argtypes = [C.POINTER(MeshS), dp, dp, C.c_int, C.c_int, dp,
-                                      C.POINTER(GmresStats)]
-        L.orc_gmres_quirk_trace.argtypes = [C.POINTER(MeshS), dp, dp, C.c_int, C.c_int, dp,
-                                            C.POINTER(GmresStats), dp, dp, ip]
-        L.orc_gmres_true.argtypes = [C.POINTER(MeshS), dp, dp, C.c_int, C.c_int, C.c_double,
-                                     C.c_int, dp, C.POINTER(GmresStats)]
-        L.orc_update.argtypes = [C.c_int, dp, dp, dp]
-        L.orc_rms.argtypes = [C.c_int, C.c_int, dp, dp, dp]
-        L.orc_refine_flags.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), C.POINTER(Prim), up]
-        L.orc_color_greedy.argtypes = [C.POINTER(MeshS), ip]
-        L.orc_color_greedy_keys.argtypes = [C.POINTER(MeshS), C.POINTER(C.c_uint64), ip]
-        L.orc_step.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), dp, dp,
-                               C.POINTER(StepOpts), C.POINTER(StepStats)]
-        _LIB = L
+        _LIB = _bind(C.CDLL(path))
     return _LIB
+
+
+class using_library:
+    """Run the oracle functions on another build of cfdv_oracle.c (tests that
+    plant a mistake with -DORC_MUTATE=k):  with using_library(path): ..."""
+
+    def __init__(self, path):
+        self.L = _bind(C.CDLL(path))
+
+    def __enter__(self):
+        global _LIB
+        self.prev, _LIB = lib(), self.L
+        return self.L
+
+    def __exit__(self, *exc):
+        global _LIB
+        _LIB = self.prev
+        return False
+
+
+def _bind(L: C.CDLL) -> C.CDLL:
+    L.orc_thermo_set.argtypes = [C.POINTER(Thermo)] + [C.c_double] * 6 + [C.c_int]
+    L.orc_calc_mu.restype = C.c_double
+    L.orc_calc_mu.argtypes = [C.POINTER(Thermo), C.c_double]
+    L.orc_calc_k.restype = C.c_double
+    L.orc_calc_k.argtypes = [C.POINTER(Thermo), C.c_double]
+    L.orc_gauss.argtypes = [C.c_int, dp, dp]
+    L.orc_shape3d.argtypes = [dp, dp, dp]
+    L.orc_shape2d.argtypes = [dp, dp, dp]
+    L.orc_jacobian3d.restype = C.c_double
+    L.orc_jacobian3d.argtypes = [dp, dp, dp, dp]
+    L.orc_jacobian2d.restype = C.c_double
+    L.orc_jacobian2d.argtypes = [dp, dp, dp, dp]
+    L.orc_tfnorm3d.restype = C.c_double
+    L.orc_tfnorm3d.argtypes = [dp, dp, dp, dp, dp, dp]
+    L.orc_node_unpack_all.argtypes = [C.POINTER(Thermo), C.c_int, C.c_int, C.c_int, dp,
+                                      C.POINTER(Prim)]
+    L.orc_calc_length.argtypes = [C.POINTER(MeshS), dp, dp]
+    L.orc_cfl.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), C.POINTER(Prim), dp,
+                          C.c_double, dp]
+    L.orc_fdvparam.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), C.POINTER(Prim),
+                               C.c_int, dp]
+    L.orc_tau.argtypes = [C.c_int, C.c_double, dp, dp]
+    for f in ("orc_calc_a", "orc_calc_b", "orc_calc_c"):
+        getattr(L, f).argtypes = [C.POINTER(Thermo), C.c_int, C.POINTER(Flow), dp]
+    L.orc_calc_G.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp]
+    L.orc_calc_F2d.argtypes = [dp, C.c_double, dp]
+    L.orc_eval2dboundary.argtypes = [C.POINTER(Thermo), C.c_int, C.POINTER(Flow), dp]
+    L.orc_build_element.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), dp,
+                                    C.POINTER(Prim), C.c_int, C.c_double, dp, dp, dp]
+    L.orc_build.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), dp, C.POINTER(Prim),
+                            C.c_double, dp, dp, dp]
+    L.orc_apply_bc.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), dp, dp]
+    L.orc_compose_rhs.argtypes = [C.POINTER(MeshS), dp, dp]
+    L.orc_matvec.argtypes = [C.POINTER(MeshS), dp, dp, dp]
+    L.orc_matvec_t.argtypes = [C.POINTER(MeshS), dp, dp, dp]
+    L.orc_gmres_quirk.argtypes = [C.POINTER(MeshS), dp, dp, C.c_int, C.c_int, dp,
+                                  C.POINTER(GmresStats)]
+    L.orc_gmres_quirk_trace.argtypes = [C.POINTER(MeshS), dp, dp, C.c_int, C.c_int, dp,
+                                        C.POINTER(GmresStats), dp, dp, ip]
+    L.orc_gmres_true.argtypes = [C.POINTER(MeshS), dp, dp, C.c_int, C.c_int, C.c_double,
+                                 C.c_int, dp, C.POINTER(GmresStats)]
+    L.orc_update.argtypes = [C.c_int, dp, dp, dp]
+    L.orc_rms.argtypes = [C.c_int, C.c_int, dp, dp, dp]
+    L.orc_refine_flags.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), C.POINTER(Prim), up]
+    L.orc_color_greedy.argtypes = [C.POINTER(MeshS), ip]
+    L.orc_color_greedy_keys.argtypes = [C.POINTER(MeshS), C.POINTER(C.c_uint64), ip]
+    L.orc_step.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), dp, dp,
+                           C.POINTER(StepOpts), C.POINTER(StepStats)]
+    L.orc_psi.argtypes = [C.c_int, C.c_int, C.POINTER(Flow), dp, dp, dp, dp]
+    L.orc_eval1dboundary.argtypes = [C.c_int, C.POINTER(Flow), dp, dp, dp]
+    return L
 
 
 def _p(a, t=dp):
@@ -391,6 +414,38 @@ def calc_abc(th, ndim, f: Flow):
     lib().orc_calc_c(C.byref(t), ndim, C.byref(f), _p(c))
     return (a.reshape(ndim, neqn, neqn), b.reshape(ndim, neqn, neqn),
             c.reshape(ndim, ndim, neqn, neqn))
+
+
+def calc_G(ndim, tau3, v, dT, k, creyn):
+    """G [ndim][neqn] (NavierStokes3D::calc_G / NavierStokes2D) from the
+    Gauss-point tau, dT, k and a node's velocity"""
+    neqn = ndim + 2
+    t = np.ascontiguousarray(tau3, dtype=np.float64).reshape(3, 3)
+    vv = np.zeros(3)
+    vv[:ndim] = v
+    dd = np.zeros(3)
+    dd[:ndim] = dT
+    G = np.zeros(ndim * neqn)
+    lib().orc_calc_G(ndim, _p(t), _p(vv), _p(dd), float(k), float(creyn), _p(G))
+    return G.reshape(ndim, neqn)
+
+
+def psi(nd, f: Flow, xn, zeta, dphi):
+    """Psi [nnod] of one Gauss point (orc_psi)"""
+    xn = np.ascontiguousarray(xn, dtype=np.float64)
+    z = np.ascontiguousarray(zeta, dtype=np.float64)
+    d = np.ascontiguousarray(dphi, dtype=np.float64)
+    nn = xn.shape[0]
+    out = np.zeros(nn)
+    lib().orc_psi(nd, nn, C.byref(f), _p(xn), _p(z), _p(d), _p(out))
+    return out
+
+
+def eval1dboundary(bc, f: Flow, xn0, xn1, en):
+    """orc_eval1dboundary on f in place; returns its status (0 or -1)"""
+    return lib().orc_eval1dboundary(bc, C.byref(f), _p(np.ascontiguousarray(xn0, dtype=np.float64)),
+                                    _p(np.ascontiguousarray(xn1, dtype=np.float64)),
+                                    _p(np.ascontiguousarray(en, dtype=np.float64)))
 
 
 def tau(ndim, mu, dV):

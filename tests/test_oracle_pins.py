@@ -1,0 +1,346 @@
+"""Pins of the oracle's element terms that the invariants of
+test_oracle_element.py cannot see (each fails on a planted mistake, shown
+by test_each_pin_catches_its_planted_mistake):
+
+1. Fluid at rest, uniform T, on a box (FDVParam gives s1 = s2 = s3 = s4 = 1,
+   the viscous Jacobians b_i vanish, c_ij do not).  The FDV Galerkin
+   domain block has the closed form
+       R = M (x) I - dt sum_i C^i (x) a_i + sum_ij K^ij (x) (-dt c_ij + dt^2/2 a_i a_j)
+   with the exact box integrals M_NM = int Phi_N Phi_M,
+   C^i_NM = int dPhi_N/dx_i Phi_M, K^ij_NM = int dPhi_N/dx_i dPhi_M/dx_j
+   (the 2-point Gauss rule is exact for them); rhs = 0.
+2. A linear primitive field on the same box (s1 != s3 and b_i != 0): the
+   oracle's block and rhs against a direct numpy quadrature of the FDV
+   Galerkin weak form (Chung's FDV equations, integrated by parts),
+       R_NM = sum_gp wJ [Phi_N Phi_M I - dt dPhi_N,i (s1 a_i + s3 b_i) Phi_M
+              - dt s3 dPhi_N,i c_ij dPhi_M,j
+              + dt^2/2 dPhi_N,i (a_i + b_i)(s2 a_j + s4 b_j) dPhi_M,j],
+       rhs_N = sum_gp wJ [dt dPhi_N,i G_i - dt^2/2 dPhi_N,i (a_i + b_i) sum_j dG_j/dx_j],
+   with the Gauss-point state, a, b, c and the nodal G as the reference forms
+   them (FDVGalerkin::sum :761-797, sumFGB :801-811: each node's G from the
+   point's tau, dT, k and that node's velocity; F = 0 in 3D, quirk Q1).
+3. eval1dboundary (eval1dboundary.h:9-251) per bc code, against closed forms
+   of its edits -- including that its "v_n = 0" is two sequential updates,
+   so v_1 loses (v.n) n_1^3, not (v.n) n_1.
+4. The quad4 numerical diffusion and discontinuity capturing
+   (FDVGalerkin.h:225-387) on a rectangle: e_xi, e_eta the axes,
+   h_xi = dx, h_eta = -dy (quirk Q9), tau = (alpha_xi dx u - alpha_eta dy w)
+   / (4 |v|^2) with alpha = coth(R/2) - 2/R, and tau_b = tau whenever
+   v . dV v != 0 (v . A <= |v||A| puts it under the floor), else 0:
+   Psi_N = (tau + tau_b) v . grad Phi_N.
+"""
+import ctypes as C
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from cfdv_amd import mesh as M
+from oracle import oracle as O
+from test_oracle_element import CORNER, _1d, _box_matrices, _box_mesh
+
+HS = (0.3, 0.2, 0.25)
+DT = 0.017
+TH = M.Thermo(gamma=1.4, mach=2.0, reynolds=500.0)
+CENTER = 13   # the element of the 3 x 3 x 3 box with no face
+
+
+def _box_first_derivative(hs):
+    """C^i_NM = int dPhi_N/dx_i Phi_M on a box"""
+    ms = [_1d(h) for h in hs]
+    Ci = np.ones((3, 8, 8))
+    for N in range(8):
+        for Mi in range(8):
+            for d in range(3):
+                a, b = CORNER[N, d], CORNER[Mi, d]
+                m1, _, c1 = ms[d]
+                for i in range(3):
+                    Ci[i, N, Mi] *= c1[a, b] if d == i else m1[a, b]
+    return Ci
+
+
+def _mu_k(T):
+    t = O.thermo(TH)
+    mu = O.lib().orc_calc_mu(C.byref(t), T)
+    return mu, O.lib().orc_calc_k(C.byref(t), mu)
+
+
+def rest_error():
+    """Relative error of the oracle's block against the fluid-at-rest closed
+    form (and |rhs|)."""
+    mesh = _box_mesh(HS)
+    n = mesh.n_node
+    U = M.conservative(np.full(n, 1.1), np.zeros((n, 3)), np.full(n, 1.3), TH.Cv)
+    R, rhs, s = O.build(TH, mesh, U, DT, apply_bc=False)
+    assert np.array_equal(s[CENTER], [1.0, 1.0, 1.0, 1.0])
+    p = O.prim_arrays(O.unpack(TH, mesh, U), 3)
+    mu, k = _mu_k(p["T"][0])
+    f = O.flow_struct(p["E"][0], p["T"][0], np.zeros(3), np.zeros(3), U[0], np.zeros((5, 3)),
+                      np.zeros((3, 3)), np.zeros(3), k, mu, np.zeros((3, 3)))
+    a, b, c = O.calc_abc(TH, 3, f)
+    assert np.abs(b).max() == 0.0 and np.abs(c).max() > 0.0
+    Mm, K = _box_matrices(HS)
+    Ci = _box_first_derivative(HS)
+    expect = np.kron(Mm, np.eye(5))
+    for i in range(3):
+        expect -= DT * np.kron(Ci[i], a[i])
+        for j in range(3):
+            expect += np.kron(K[i, j], -DT * c[i, j] + DT * DT / 2.0 * (a[i] @ a[j]))
+    err = np.abs(R[CENTER] - expect).max() / np.abs(expect).max()
+    return err, np.abs(rhs[CENTER]).max()
+
+
+def _linear_state(mesh):
+    x, y, z = mesh.x[:, 0], mesh.x[:, 1], mesh.x[:, 2]
+    rho = 1.1 + 0.3 * x - 0.2 * y + 0.25 * z
+    v = np.stack([0.8 + 0.9 * x - 0.4 * z, 0.3 + 0.6 * y, -0.2 + 0.5 * x + 0.3 * y], axis=1)
+    T = 1.3 + 0.4 * x + 0.2 * y - 0.3 * z
+    return M.conservative(rho, v, T, TH.Cv)
+
+
+def weak_form_error():
+    """Relative errors of the oracle's block and rhs against the numpy
+    quadrature of the FDV Galerkin weak form on a linear field."""
+    mesh = _box_mesh(HS)
+    U = _linear_state(mesh)
+    R, rhs, s = O.build(TH, mesh, U, DT, apply_bc=False)
+    s1, s2, s3, s4 = s[CENTER]
+    assert 0.0 < s1 < 1.0 and 0.0 < s3 < 1.0 and abs(s1 - s3) > 0.01
+    nodes = mesh.conn[CENTER]
+    p = O.prim_arrays(O.unpack(TH, mesh, U), 3)
+    xN, UN = mesh.x[nodes], U[nodes]
+    vN, TN, EN = p["v"][nodes], p["T"][nodes], p["E"][nodes]
+    sig = 2.0 * CORNER - 1.0
+    h = np.array(HS)
+    z0 = 1.0 / np.sqrt(3.0)
+    wJ = np.prod(h / 2.0)
+    Rx = np.zeros((40, 40))
+    rx = np.zeros(40)
+    bnorm = 0.0
+    for za in (-z0, z0):
+        for zb in (-z0, z0):
+            for zc in (-z0, z0):
+                zeta = np.array([za, zb, zc])
+                fac = 1.0 + sig * zeta                       # [8][3]
+                Phi = np.prod(fac, axis=1) / 8.0
+                dPhi = np.empty((8, 3))
+                for d in range(3):
+                    others = [q for q in range(3) if q != d]
+                    dPhi[:, d] = sig[:, d] * fac[:, others[0]] * fac[:, others[1]] / 8.0 * 2.0 / h[d]
+                T = Phi @ TN
+                mu, k = _mu_k(T)
+                dV = vN.T @ dPhi                             # dV[i][j] = d v_i / d x_j
+                tau3 = O.tau(3, mu, dV)
+                f = O.flow_struct(Phi @ EN, T, Phi @ vN, Phi @ xN, Phi @ UN, UN.T @ dPhi, dV,
+                                  TN @ dPhi, k, mu, tau3)
+                a, b, c = O.calc_abc(TH, 3, f)
+                bnorm = max(bnorm, np.abs(b).max())
+                G = np.array([O.calc_G(3, tau3, vN[beta], TN @ dPhi, k, TH.reynolds) for beta in range(8)])
+                Gi = np.einsum("b,bir->ir", Phi, G)            # G_i at the point
+                D = np.einsum("bj,bjr->r", dPhi, G)            # sum_j dG_j/dx_j
+                blk = np.kron(np.outer(Phi, Phi), np.eye(5))
+                for i in range(3):
+                    blk += np.kron(np.outer(dPhi[:, i], Phi), -DT * (s1 * a[i] + s3 * b[i]))
+                    for j in range(3):
+                        hij = -DT * s3 * c[i, j] + DT * DT / 2.0 * ((a[i] + b[i]) @ (s2 * a[j] + s4 * b[j]))
+                        blk += np.kron(np.outer(dPhi[:, i], dPhi[:, j]), hij)
+                Rx += wJ * blk
+                for i in range(3):
+                    rx += wJ * np.kron(dPhi[:, i], DT * Gi[i] - DT * DT / 2.0 * ((a[i] + b[i]) @ D))
+    assert bnorm > 0.0
+    eR = np.abs(R[CENTER] - Rx).max() / np.abs(Rx).max()
+    er = np.abs(rhs[CENTER] - rx).max() / np.abs(rx).max()
+    return eR, er
+
+
+def test_fluid_at_rest_closed_form():
+    err, rhs = rest_error()
+    print(f"fluid at rest: block rel err {err:.2e}, |rhs| {rhs:.1e}")
+    assert err < 1e-12 and rhs < 1e-18   # gradients of a uniform field: rounding only
+
+
+def test_linear_field_against_the_weak_form():
+    eR, er = weak_form_error()
+    print(f"linear field: block rel err {eR:.2e}, rhs rel err {er:.2e}")
+    assert eR < 1e-12 and er < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# eval1dboundary
+# ---------------------------------------------------------------------------
+def _flow2d(rng):
+    v = rng.uniform(-1, 1, 2)
+    dV = rng.uniform(-1, 1, (2, 2))
+    dT = rng.uniform(-1, 1, 2)
+    dU = rng.uniform(-1, 1, (4, 2))
+    mu = 0.7
+    f = O.flow_struct(2.0, 1.2, v, np.zeros(2), rng.uniform(0.5, 1.5, 4), dU, dV, dT, 0.3, mu)
+    return f, dict(v=v.copy(), dV=dV.copy(), dT=dT.copy(), dU=dU.copy(), mu=mu)
+
+
+def _seq_project(w, n):
+    """w - (w.n) n as eval1dboundary writes it: component 0, then 1 with the
+    dot product recomputed on the updated vector"""
+    w = w.copy()
+    w[0] -= (w @ n) * n[0]
+    w[1] -= (w @ n) * n[1]
+    return w
+
+
+def _expected_1d(bc, s, en, x0, x1):
+    v, dV, dT, dU = s["v"].copy(), s["dV"].copy(), s["dT"].copy(), s["dU"].copy()
+    n = en / np.linalg.norm(en)
+    if bc in (2, 12):
+        v = _seq_project(v, n)
+        dT = _seq_project(dT, n)
+    elif bc in (1, 51):
+        dT = _seq_project(dT, n)
+    elif bc == -1:
+        dV[:, :] = 0.0
+        dT[0] = 0.0
+    elif bc == 6:
+        dV[:, :] = 0.0
+        dT[0] = 0.0
+        dU[:, 0] = 0.0
+    elif bc in (-2, 7):
+        dV[:, 1] = 0.0
+        dT[1] = 0.0
+        dU[:, 1] = 0.0
+    elif bc in (4, 8):
+        if bc == 4:
+            v[:] = 0.0
+            nn = en.copy()                       # the raw normal, not unit
+        else:
+            al = 10 * (3.14159 / 180.0)
+            nn = np.array([np.cos(al), np.sin(al)]) * abs(x1[1] - x0[1])
+        dV = dV - np.outer(dV @ nn, nn)
+        dT = dT - (dT @ nn) * nn
+        dU = dU - np.outer(dU @ nn, nn)
+    elif bc == -12:
+        dV[:, 1] = 0.0
+        dT[1] = 0.0
+        v[1] = 0.0
+        dU[:, 1] = 0.0
+    elif bc in (11, 21):
+        v[:] = 0.0
+        dV[:, 0] = 0.0
+        dT[1] = 0.0
+        dU[:, 1] = 0.0
+    tau = s["mu"] * (dV + dV.T) - 2.0 / 3.0 * s["mu"] * np.trace(dV) * np.eye(2)
+    if bc in (2, 12, 7):
+        tau[0, 1] = tau[1, 0] = 0.0
+    if bc == 6:
+        tau[0, 0] = tau[1, 0] = 0.0
+    return v, dV, dT, dU, tau
+
+
+def _got_1d(f):
+    v = np.array([f.v[i] for i in range(2)])
+    dV = np.array([[f.dV[i][j] for j in range(2)] for i in range(2)])
+    dT = np.array([f.dT[i] for i in range(2)])
+    dU = np.array([[f.dU[i][j] for j in range(2)] for i in range(4)])
+    tau = np.array([[f.tau[i][j] for j in range(2)] for i in range(2)])
+    return v, dV, dT, dU, tau
+
+
+CODES_1D = (2, 12, 1, 51, -1, 6, -2, 7, 4, 8, -12, 11, 21)
+
+
+def eval1d_error():
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    x0, x1 = np.array([0.2, 0.1]), np.array([0.5, 0.45])
+    for bc in CODES_1D:
+        for _ in range(4):
+            f, s = _flow2d(rng)
+            en = rng.uniform(-1, 1, 2) * 0.3
+            assert O.eval1dboundary(bc, f, x0, x1, en) == 0
+            for g, e in zip(_got_1d(f), _expected_1d(bc, s, en, x0, x1)):
+                worst = max(worst, np.abs(g - e).max())
+    f, _ = _flow2d(rng)
+    assert O.eval1dboundary(99, f, x0, x1, np.array([0.0, 1.0])) == -1
+    return worst
+
+
+def test_eval1dboundary_closed_forms():
+    err = eval1d_error()
+    print(f"eval1dboundary: worst abs err {err:.1e} over {len(CODES_1D)} codes")
+    assert err < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# quad4 numerical diffusion and discontinuity capturing
+# ---------------------------------------------------------------------------
+def psi_error():
+    dx, dy = 0.07, 0.04
+    xn = np.array([[0.0, 0.0], [dx, 0.0], [dx, dy], [0.0, dy]])
+    sig = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], dtype=float)
+    worst = 0.0
+    rng = np.random.default_rng(5)
+    for case in range(6):
+        zeta = rng.uniform(-0.6, 0.6, 2)
+        u, w = rng.uniform(0.2, 1.5) * rng.choice([-1, 1]), rng.uniform(0.2, 1.5) * rng.choice([-1, 1])
+        rho, mu = rng.uniform(0.8, 1.2), rng.uniform(0.05, 0.3)
+        fac = 1.0 + sig * zeta
+        dphi = np.stack([sig[:, 0] * fac[:, 1] / 4.0 * 2.0 / dx, sig[:, 1] * fac[:, 0] / 4.0 * 2.0 / dy], axis=1)
+        dV = np.zeros((2, 2)) if case % 2 == 0 else rng.uniform(-1, 1, (2, 2))
+        f = O.flow_struct(2.0, 1.0, np.array([u, w]), np.zeros(2), np.array([rho, rho * u, rho * w, 2.0]),
+                          np.zeros((4, 2)), dV, np.zeros(2), 0.1, mu)
+        got = O.psi(2, f, xn, zeta, dphi)
+
+        def alpha(R):
+            return 1.0 / np.tanh(R / 2.0) - 2.0 / R
+        h_xi, h_eta = dx, -dy
+        R_xi, R_eta = rho * u * h_xi / mu, rho * w * h_eta / mu
+        tau = 0.25 * (alpha(R_xi) * h_xi * u + alpha(R_eta) * h_eta * w) / (u * u + w * w)
+        A = dV @ np.array([u, w])
+        tau_b = 0.0 if np.linalg.norm(A) == 0.0 else tau
+        expect = (tau + tau_b) * (dphi @ np.array([u, w]))
+        worst = max(worst, np.abs(got - expect).max() / np.abs(expect).max())
+    return worst
+
+
+def test_quad4_numerical_diffusion_closed_form():
+    err = psi_error()
+    print(f"quad4 Psi: worst rel err {err:.1e}")
+    assert err < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# every pin fails on the mistake planted for it
+# ---------------------------------------------------------------------------
+def _pins():
+    return {
+        "rest": lambda: rest_error()[0] > 1e-12,
+        "weak": lambda: max(weak_form_error()) > 1e-12,
+        "psi": lambda: psi_error() > 1e-12,
+    }
+
+
+MUTATIONS = {1: ("s1 and s3 swapped in g", ["weak"]),
+             2: ("sign of dt s3 c_ij in h", ["rest", "weak"]),
+             3: ("dt for dt^2/2 on the a, b products", ["rest", "weak"]),
+             4: ("discontinuity-capturing floor dropped", ["psi"]),
+             5: ("h_eta with the other sign", ["psi"])}
+
+
+def test_each_pin_catches_its_planted_mistake(tmp_path):
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = os.path.join(os.path.dirname(O.__file__), "cfdv_oracle.c")
+    pins = _pins()
+    for k, (what, caught_by) in MUTATIONS.items():
+        so = str(tmp_path / f"liboracle_m{k}.so")
+        subprocess.run([cc, "-O2", "-std=c99", "-fPIC", "-ffp-contract=off", "-shared", f"-DORC_MUTATE={k}",
+                        src, "-o", so, "-lm"], check=True)
+        with O.using_library(so):
+            for name in caught_by:
+                try:
+                    failed = pins[name]()
+                except AssertionError:
+                    failed = True   # a precondition of the pin no longer holds
+                print(f"mutation {k} ({what}): pin '{name}' {'fails' if failed else 'PASSES'}")
+                assert failed, f"pin '{name}' does not see mutation {k} ({what})"
