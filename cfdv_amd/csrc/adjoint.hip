@@ -9,38 +9,506 @@
 namespace cfdv {
 
 // ---------------------------------------------------------------------------
-// Build tangents for the discrete adjoint (hex8).  For one element slot and
-// one batch of ND of its 41 inputs (U of its 8 nodes x 5 equations, then
-// the Mach number of the Thermo constants), the derivatives of
+// Build tangents for the discrete adjoint (hex8).  For element e, with the
+// adjoint lam and the update du held fixed, the derivatives of
 //   phi_e = sum_N lam_N^T (rhs_N - sum_M R_NM du_M)
 // with R, rhs the element's blocks after faces and ApplyBC at the fixed dt
-// (FDVGalerkin.h:121-154 + ApplyBC.h:14-133, differentiated in forward
-// mode through the same per-lane code as build3d_kernel).  gelem[slot][k]
-// receives d phi_e / d input k.
+// (FDVGalerkin.h:121-154 + ApplyBC.h:14-133) with respect to its 41 inputs
+// (U of its 8 nodes x 5 equations, then the Mach number).
+//
+// R and rhs are sums over Gauss points of products of a test function of
+// node N, a test function of node M and a matrix of the Gauss-point state
+// (the factorised contraction of contract3/accum_c3 and the rhs pieces of
+// build3d_elem).  With lam and du fixed, the node sums therefore fold into
+// test-function-weighted vectors at each Gauss point,
+//   LD_i = sum_N dPhi_N/dx_i lam_N,  uP = sum_M Phi_M du_M,
+//   uD_j = sum_M dPhi_M/dx_j du_M     (faces: Lf = sum_t Phi_t lam_t and the
+//                                      face nodes' Phi_t, dPhi_t/dx_j),
+// and each Gauss point adds one scalar of its state (A_i = a_i + b_i,
+// S_j = s2 a_j + s4 b_j, P_i and D the rhs pieces):
+//   domain  wJ (sum_i LD_i.P_i - dt^2/2 sum_i LD_i.A_i D)
+//           + dt wJ sum_i LD_i^T (s1 a_i + s3 b_i) uP
+//           - dt^2/2 wJ (sum_i LD_i^T A_i)(sum_j S_j uD_j)
+//           + dt s3 wJ sum_ij LD_i^T c_ij uD_j
+//   face    - sum_i en_i Lf.P_i + dt^2/2 sum_i en_i Lf.A_i D
+//           - dt sum_i en_i Lf^T (s1 a_i + s3 b_i) uP
+//           + dt^2/2 (sum_i en_i Lf^T A_i)(sum_j S_j uD_j)
+//           - dt s3 sum_ij en_i Lf^T c_ij uD_j.
+// The identity term of R (wJ Phi_N Phi_M) depends on neither U nor the Mach
+// number and is left out; rows that ApplyBC replaces by constants enter
+// with their lam entries zeroed.  One wavefront per element: the geometry
+// and the folded vectors are staged in LDS once, then lane k < 41 evaluates
+// phi_e in dual numbers seeded along input k, so the tangent costs one
+// Gauss-point evaluation per direction instead of one per node pair.
 // ---------------------------------------------------------------------------
-template <int ND>
+constexpr int VJP_NIN = 41;
+// primal Gauss-point state in LDS: E, T, v, U, dU, dV, dT as FDVGalerkin::sum
+// interpolates them, then sum Phi dT/dMach and sum dPhi dT/dMach
+enum { GS_E = 0, GS_T = 1, GS_V = 2, GS_U = 5, GS_DU = 10, GS_DV = 25, GS_DT = 34, GS_TM = 37, GS_DTM = 38,
+       GS_N = 41 };
+
+struct VjpLds {
+    double nd[8][ND_STRIDE];   // node rows: x, U, v, E, T, rho
+    double tT[8];              // dT/dMach of each node
+    double lam[8][5];          // lam with ApplyBC rows zeroed
+    double du[8][5];
+    double ph[8][8];           // [gp][n] Phi (Hnm-mapped on adapted elements)
+    double dph[8][8][3];       // [gp][n][i] dPhi/dx_i
+    double wJ[8], sph[8], sd[8][3];
+    double cv[8][35];          // [gp]: LD_0..2 (15), uP (5), uD_0..2 (15)
+    int fcode[24];             // face item (fgp, fc) = (j / 6, j % 6): bc or 0
+    double fph[24][4], fdp[24][4][3], fen[24][3], fsph[24], fsd[24][3];
+    double fcv[24][25];        // [j]: Lf (5), uP (5), uD_0..2 (15)
+    double gs[8][GS_N];        // primal Gauss-point state of the domain points
+    double fgs[24][GS_N];      // and of the face points (raw sums, before the bc edits)
+};
+
+// Register-level optimisation barrier on the fields the flux Jacobians read
+// (x is re-issued after dep, so work on x cannot start before dep exists)
+CFDV_HD void opaque(double& x, double dep) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(x) : "v"(dep));
+#else
+    (void)x;
+    (void)dep;
+#endif
+}
+CFDV_HD void opaque(Dual<1>& x, double dep) {
+    opaque(x.v, dep);
+    opaque(x.d[0], dep);
+}
+template <typename S>
+CFDV_HD void opaque(GPState<3, S>& f, double dep) {
+    opaque(f.E, dep);
+    opaque(f.k, dep);
+    opaque(f.mu, dep);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        opaque(f.v[i], dep);
+#pragma unroll
+        for (int j = 0; j < 3; j++) opaque(f.tau[i][j], dep);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        opaque(f.U[q], dep);
+#pragma unroll
+        for (int j = 0; j < 3; j++) opaque(f.dU[q][j], dep);
+    }
+}
+
+// One direction I of the Gauss-point contraction, folded: L^T a_I, L^T b_I
+// against uP and D, S_I uD.
+template <int I, typename TH, typename S>
+CFDV_HD void vjp_dir(const TH& th, const GPState<3, S>& f0, const double* L, const double* uP, const double* uD,
+                     const S Dv[5], const S& s1, const S& s2, const S& s3, const S& s4, double w, S& t1, S LA[5],
+                     S Sv[5], S& tD) {
+    // a private copy of the state, opaque to the optimiser and issued after
+    // the previous direction's last sum, so that no subexpression of one
+    // direction is shared with (and kept live for) another and directions
+    // do not interleave
+    GPState<3, S> f = f0;
+    opaque(f, value_of(tD) + tD.d[0]);
+    S la[5], lb[5], qa = 0.0, qb = 0.0;
+    {
+        S a[5][5];
+        calc_a3_dir<I>(th, f, a);
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+            S acc = 0.0;
+#pragma unroll
+            for (int r = 0; r < 5; r++) acc += L[r] * a[r][s];
+            la[s] = acc;
+            qa += acc * uP[s];
+        }
+#pragma unroll
+        for (int r = 0; r < 5; r++) {
+            S acc = 0.0;
+#pragma unroll
+            for (int s = 0; s < 5; s++) acc += a[r][s] * uD[s];
+            Sv[r] += s2 * acc;
+        }
+    }
+    {
+        S b[5][5];
+        calc_b3_dir<I>(th, f, b);
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+            S acc = 0.0;
+#pragma unroll
+            for (int r = 0; r < 5; r++) acc += L[r] * b[r][s];
+            lb[s] = acc;
+            qb += acc * uP[s];
+        }
+#pragma unroll
+        for (int r = 0; r < 5; r++) {
+            S acc = 0.0;
+#pragma unroll
+            for (int s = 0; s < 5; s++) acc += b[r][s] * uD[s];
+            Sv[r] += s4 * acc;
+        }
+    }
+    t1 += w * (s1 * qa + s3 * qb);
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+        const S A_ = la[s] + lb[s];
+        LA[s] += w * A_;
+        tD += w * (A_ * Dv[s]);
+    }
+}
+
+// sum_ij w_i L_i^T c_ij u_j (NavierStokes3D::calc_c, NavierStokes3D.h:347-477;
+// the same entries as accum_c3)
+template <typename TH, typename S>
+CFDV_HD S vjp_c3(const TH& th, const GPState<3, S>& f, const double* const L[3], const double w[3],
+                 const double* const u[3]) {
+    const S mu = f.mu;
+    const S lambda = -2. * mu / 3.;
+    const S mu_R = 2. * mu + lambda;
+    const S rho = f.U[0];
+    const S rinv = 1.0 / rho;
+    const S denom = 1. / (rho * rho);
+    const S Uc[3] = {f.U[1], f.U[2], f.U[3]};
+    const S hkocv = f.k / th.Cv;
+    const S TERM1 = f.k / th.Cv * denom;
+    const S Ur[3] = {Uc[0] * rinv, Uc[1] * rinv, Uc[2] * rinv};
+    const S en = TERM1 * (-f.U[4] + (Uc[0] * Uc[0] + Uc[1] * Uc[1] + Uc[2] * Uc[2]) * rinv);
+    S acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        // diagonal block c_ii: mu_R on row 1+i, mu on the other two
+        const double* Li = L[i];
+        const double* ui = u[i];
+        S c0[3], row4 = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const S md = (d == i) ? mu_R : mu;
+            c0[d] = md * Uc[d] * denom;             // c(1+d, 0)
+            const S cdd = -md * rinv;               // c(1+d, 1+d)
+            acc += w[i] * (Li[1 + d] * (c0[d] * ui[0] + cdd * ui[1 + d]));
+            row4 += (Ur[d] * cdd + TERM1 * Uc[d]) * ui[1 + d];
+        }
+        row4 += ((Uc[0] * c0[0] + Uc[1] * c0[1] + Uc[2] * c0[2]) * rinv - en) * ui[0];
+        row4 += (-hkocv * rinv) * ui[4];
+        acc += w[i] * (Li[4] * row4);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            if (i == j) continue;
+            const double* Li = L[i];
+            const double* uj = u[j];
+            const S c_i0 = lambda * Uc[j] * denom;
+            const S c_ij = (-lambda * rinv);
+            const S c_j0 = mu * Uc[i] * denom;
+            const S c_ji = (-mu * rinv);
+            S t = Li[1 + i] * (c_i0 * uj[0] + c_ij * uj[1 + j]);
+            t += Li[1 + j] * (c_j0 * uj[0] + c_ji * uj[1 + i]);
+            t += Li[4] * ((Ur[i] * c_i0 + Ur[j] * c_j0) * uj[0] + (Ur[j] * c_ji) * uj[1 + i] +
+                          (Ur[i] * c_ij) * uj[1 + j]);
+            acc += w[i] * t;
+        }
+    return acc * (1.0 / th.creyn);
+}
+
+// Gauss-point state in dual numbers: the primal sums g (GS_ layout) and the
+// tangent along one input.  Along U(nk, q) only node nk's test function
+// (phk, dk) carries the tangent row tr of node nk (U, v, E, T, rho); along
+// the Mach number (mach) only T and dT move, by g's dT/dMach sums.
+CFDV_HD void gp_state(GPState<3, Dual<1>>& f, const double* g, const double tr[ND_STRIDE - ND_U], double phk,
+                      const double dk[3], bool mach) {
+    typedef Dual<1> D;
+    auto mk = [](double v, double d) { D r(v); r.d[0] = d; return r; };
+    f.E = mk(g[GS_E], phk * tr[ND_E - ND_U]);
+    f.T = mk(g[GS_T], phk * tr[ND_T - ND_U] + (mach ? g[GS_TM] : 0.0));
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        f.v[i] = mk(g[GS_V + i], phk * tr[ND_V - ND_U + i]);
+        f.dT[i] = mk(g[GS_DT + i], dk[i] * tr[ND_T - ND_U] + (mach ? g[GS_DTM + i] : 0.0));
+#pragma unroll
+        for (int j = 0; j < 3; j++) f.dV[i][j] = mk(g[GS_DV + 3 * i + j], dk[j] * tr[ND_V - ND_U + i]);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        f.U[q] = mk(g[GS_U + q], phk * tr[q]);
+#pragma unroll
+        for (int j = 0; j < 3; j++) f.dU[q][j] = mk(g[GS_DU + 3 * q + j], dk[j] * tr[q]);
+    }
+}
+
 __global__ __launch_bounds__(64) void build3d_vjp_kernel(BuildArgs A, GaussTables G, const double* __restrict__ lam,
-                                                         const double* __restrict__ du, int batch,
+                                                         const double* __restrict__ du,
                                                          double* __restrict__ gelem) {
-    typedef Dual<ND> D;
-    constexpr int NNOD = 8, NEQ = 5, NIN = 41;
-    // node rows as raw LDS (Dual has constructors, which __shared__ forbids)
-    __shared__ double s_raw[NNOD * ND_STRIDE * (ND + 1)];
-    __shared__ double s_red[ND][64];
-    D (*s_nd)[ND_STRIDE] = reinterpret_cast<D (*)[ND_STRIDE]>(s_raw);
+    typedef Dual<1> D;
+    constexpr int NNOD = 8, NEQ = 5;
+    __shared__ VjpLds sm;
     const int lane = threadIdx.x;
     const long e = blockIdx.x;
     if (e >= A.n_elem_pad) return;
     const int* cn = A.conn + e * NNOD;
     if (cn[0] < 0) {   // padding slot
-        for (int j = lane; j < ND; j += 64) {
-            const int k = batch * ND + j;
-            if (k < NIN) gelem[e * NIN + k] = 0.0;
-        }
+        if (lane < VJP_NIN) gelem[e * VJP_NIN + lane] = 0.0;
         return;
     }
-    // Thermo with d/dMach seeded where the batch holds input 40:
-    // Cv = 1/(gamma (gamma-1) M^2), cgas = 1/(gamma M^2), cmach = M
+    const int FN[6][4] = {{3, 2, 1, 0}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {6, 7, 4, 5}};
+    const double* Hn = (A.adap && A.adap[e]) ? A.hnm + e * 64 : nullptr;
+    const int* bcs = A.bc + e * 6;
+    const double dt = A.dt, dtsq_2 = (dt * dt) / 2.0;
+    const double reinv = 1.0 / A.th.creyn;
+
+    // ---- stage 1: node rows, lam (ApplyBC rows zeroed), du
+    if (lane < NNOD) {
+        load_node3d(A, cn[lane], sm.nd[lane]);
+        // T = e / Cv with Cv = 1/(gamma (gamma-1) M^2)
+        ThermoT<Dual<1>> tm;
+        tm.Cv = Dual<1>(A.th.Cv);
+        tm.Cv.d[0] = -2.0 * A.th.Cv / A.th.cmach;
+        Dual<1> Ul[NEQ], rho, v[3], E, T;
+#pragma unroll
+        for (int q = 0; q < NEQ; q++) Ul[q] = Dual<1>(sm.nd[lane][ND_U + q]);
+        unpack<3>(tm, Ul, rho, v, E, T);
+        sm.tT[lane] = T.d[0];
+    }
+    if (lane < NNOD * NEQ) {
+        const int n = lane / NEQ, q = lane % NEQ;
+        const size_t g = (size_t)cn[n] * NEQ + q;
+        sm.lam[n][q] = ((A.cmask[e] >> lane) & 1ULL) ? 0.0 : lam[g];
+        sm.du[n][q] = du[g];
+    }
+    __syncthreads();
+
+    // ---- stage 2: domain Gauss-point geometry (lanes 0-7), face Gauss-point
+    // geometry (lanes 8-31, item j = lane - 8)
+    if (lane < 8) {
+        const int gp = lane;
+        double phg[8], dpg[8][3];
+        if (Hn) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                phg[m] = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; j++) dpg[m][j] = 0.0;
+            }
+            for (int n = 0; n < 8; n++)
+                for (int m = 0; m < 8; m++) {
+                    const double h = Hn[n + 8 * m];
+                    phg[m] += G.phi3[gp][n] * h;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) dpg[m][j] += G.dpxi3[gp][n][j] * h;
+                }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                phg[m] = G.phi3[gp][m];
+#pragma unroll
+                for (int j = 0; j < 3; j++) dpg[m][j] = G.dpxi3[gp][m][j];
+            }
+        }
+        double jac[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int n = 0; n < 8; n++) acc += dpg[n][j] * sm.nd[n][ND_X + i];
+                jac[i][j] = acc;
+            }
+        const double J = jac[0][0] * jac[1][1] * jac[2][2] + jac[0][1] * jac[1][2] * jac[2][0] +
+                         jac[0][2] * jac[1][0] * jac[2][1] - jac[0][2] * jac[1][1] * jac[2][0] -
+                         jac[0][1] * jac[1][0] * jac[2][2] - jac[0][0] * jac[1][2] * jac[2][1];
+        double ji[3][3];
+        ji[0][0] = jac[1][1] * jac[2][2] - jac[1][2] * jac[2][1];
+        ji[0][1] = -(jac[1][0] * jac[2][2] - jac[1][2] * jac[2][0]);
+        ji[0][2] = jac[1][0] * jac[2][1] - jac[1][1] * jac[2][0];
+        ji[1][0] = -(jac[0][1] * jac[2][2] - jac[0][2] * jac[2][1]);
+        ji[1][1] = jac[0][0] * jac[2][2] - jac[0][2] * jac[2][0];
+        ji[1][2] = -(jac[0][0] * jac[2][1] - jac[0][1] * jac[2][0]);
+        ji[2][0] = jac[0][1] * jac[1][2] - jac[0][2] * jac[1][1];
+        ji[2][1] = -(jac[0][0] * jac[1][2] - jac[0][2] * jac[1][0]);
+        ji[2][2] = jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0];
+        const double Jinv = 1.0 / J;
+        double SPhi = 0.0, SD[3] = {0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * dpg[n][j]);
+                sm.dph[gp][n][i] = acc;
+                SD[i] += acc;
+            }
+            sm.ph[gp][n] = phg[n];
+            SPhi += phg[n];
+        }
+        sm.wJ[gp] = G.w3[gp] * J;
+        sm.sph[gp] = SPhi;
+#pragma unroll
+        for (int i = 0; i < 3; i++) sm.sd[gp][i] = SD[i];
+    } else if (lane < 32) {
+        const int j = lane - 8, fgp = j / 6, fc = j % 6;
+        const int bc = bcs[fc];
+        const bool on = (bc != 0 && bc < 100);
+        sm.fcode[j] = on ? bc : 0;
+        if (on) {
+            const double w0 = G.wf[fgp][0], w1 = G.wf[fgp][1];
+            const double z0 = G.zf[fgp][0], z1 = G.zf[fgp][1];
+            int fl[4];
+            double nl[4][3];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                fl[t] = FN[fc][t];
+#pragma unroll
+                for (int i = 0; i < 3; i++) nl[t][i] = sm.nd[fl[t]][ND_X + i];
+            }
+            double phib[4];
+            phib[0] = 0.25 * (1 - z0) * (1 - z1);
+            phib[1] = 0.25 * (1 + z0) * (1 - z1);
+            phib[2] = 0.25 * (1 + z0) * (1 + z1);
+            phib[3] = 0.25 * (1 - z0) * (1 + z1);
+            if (Hn) {
+                // face corners and phibou from the Hnm-mapped element nodes
+                // (calc_TF_norm3D.h:35-66, MeshRefine2D.h:903-952)
+                for (int t = 0; t < 4; t++)
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        double acc = 0.0;
+                        for (int m = 0; m < 8; m++) acc += Hn[fl[t] + 8 * m] * sm.nd[m][ND_X + i];
+                        nl[t][i] = acc;
+                    }
+                double pt[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int n = 0; n < 4; n++)
+                    for (int m = 0; m < 4; m++) pt[m] += phib[n] * Hn[fl[n] + 8 * fl[m]];
+#pragma unroll
+                for (int t = 0; t < 4; t++) phib[t] = pt[t];
+            }
+            // TFNorm3D (calc_TF_norm3D.h:9-167)
+            double Lm[3], Mu[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) { Lm[i] = nl[1][i] - nl[0][i]; Mu[i] = nl[3][i] - nl[0][i]; }
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                Lm[i] /= sqrt(Lm[0] * Lm[0] + Lm[1] * Lm[1] + Lm[2] * Lm[2]);
+                Mu[i] /= sqrt(Mu[0] * Mu[0] + Mu[1] * Mu[1] + Mu[2] * Mu[2]);
+            }
+            double en_[3], ga[3];
+            en_[0] = Lm[1] * Mu[2] - Lm[2] * Mu[1];
+            en_[1] = Lm[2] * Mu[0] - Lm[0] * Mu[2];
+            en_[2] = Lm[0] * Mu[1] - Lm[1] * Mu[0];
+            ga[0] = en_[1] * Lm[2] - en_[2] * Lm[1];
+            ga[1] = en_[2] * Lm[0] - en_[0] * Lm[2];
+            ga[2] = en_[0] * Lm[1] - en_[1] * Lm[0];
+            double dp[4][2];
+            dp[0][0] = -0.25 * (1 - z1); dp[1][0] = 0.25 * (1 - z1);
+            dp[2][0] = 0.25 * (1 + z1);  dp[3][0] = -0.25 * (1 + z1);
+            dp[0][1] = -0.25 * (1 - z0); dp[1][1] = -0.25 * (1 + z0);
+            dp[2][1] = 0.25 * (1 + z0);  dp[3][1] = 0.25 * (1 - z0);
+            double dxd[3][2];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int jj = 0; jj < 2; jj++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) acc += dp[t][jj] * nl[t][i];
+                    dxd[i][jj] = acc;
+                }
+            double dtr[2][2];
+#pragma unroll
+            for (int jj = 0; jj < 2; jj++) {
+                dtr[0][jj] = Lm[0] * dxd[0][jj] + Lm[1] * dxd[1][jj] + Lm[2] * dxd[2][jj];
+                dtr[1][jj] = ga[0] * dxd[0][jj] + ga[1] * dxd[1][jj] + ga[2] * dxd[2][jj];
+            }
+            const double detjac = dtr[0][0] * dtr[1][1] - dtr[1][0] * dtr[0][1];
+            const double djinv = 1. / detjac;
+#pragma unroll
+            for (int i = 0; i < 3; i++) sm.fen[j][i] = en_[i] * w0 * w1 * detjac;
+            double Sph = 0.0, Sdp[3] = {0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const double d0 = (dtr[1][1] * dp[t][0] - dtr[1][0] * dp[t][1]) * djinv;
+                const double d1 = (-dtr[0][1] * dp[t][0] + dtr[0][0] * dp[t][1]) * djinv;
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const double dpx = Lm[i] * d0 + ga[i] * d1;
+                    sm.fdp[j][t][i] = dpx;
+                    Sdp[i] += dpx;
+                }
+                sm.fph[j][t] = phib[t];
+                Sph += phib[t];
+            }
+            sm.fsph[j] = Sph;
+#pragma unroll
+            for (int i = 0; i < 3; i++) sm.fsd[j][i] = Sdp[i];
+        }
+    }
+    __syncthreads();
+
+    // ---- stage 3: the folded lam/du vectors
+    for (int it = lane; it < 8 * 35; it += 64) {
+        const int gp = it / 35, c = it % 35;
+        double acc = 0.0;
+        if (c < 15) {
+            const int i = c / 5, r = c % 5;
+            for (int n = 0; n < 8; n++) acc += sm.dph[gp][n][i] * sm.lam[n][r];
+        } else if (c < 20) {
+            const int r = c - 15;
+            for (int n = 0; n < 8; n++) acc += sm.ph[gp][n] * sm.du[n][r];
+        } else {
+            const int jj = (c - 20) / 5, s = (c - 20) % 5;
+            for (int n = 0; n < 8; n++) acc += sm.dph[gp][n][jj] * sm.du[n][s];
+        }
+        sm.cv[gp][c] = acc;
+    }
+    for (int it = lane; it < 24 * 25; it += 64) {
+        const int j = it / 25, c = it % 25;
+        if (!sm.fcode[j]) continue;
+        const int fc = j % 6;
+        double acc = 0.0;
+        if (c < 5) {
+            for (int t = 0; t < 4; t++) acc += sm.fph[j][t] * sm.lam[FN[fc][t]][c];
+        } else if (c < 10) {
+            for (int t = 0; t < 4; t++) acc += sm.fph[j][t] * sm.du[FN[fc][t]][c - 5];
+        } else {
+            const int jj = (c - 10) / 5, s = (c - 10) % 5;
+            for (int t = 0; t < 4; t++) acc += sm.fdp[j][t][jj] * sm.du[FN[fc][t]][s];
+        }
+        sm.fcv[j][c] = acc;
+    }
+    // primal Gauss-point states, summed in node order
+    for (int it = lane; it < 32 * GS_N; it += 64) {
+        const int p = it / GS_N, c = it % GS_N;
+        const bool face = p >= 8;
+        const int j = p - 8;
+        if (face && !sm.fcode[j]) continue;
+        const int nt = face ? 4 : 8;
+        double acc = 0.0;
+        for (int t = 0; t < nt; t++) {
+            const int n = face ? FN[j % 6][t] : t;
+            const double ph = face ? sm.fph[j][t] : sm.ph[p][t];
+            const double* dp = face ? sm.fdp[j][t] : sm.dph[p][t];
+            const double* r = sm.nd[n];
+            double term;
+            if (c == GS_E) term = ph * r[ND_E];
+            else if (c == GS_T) term = ph * r[ND_T];
+            else if (c < GS_U) term = ph * r[ND_V + c - GS_V];
+            else if (c < GS_DU) term = ph * r[ND_U + c - GS_U];
+            else if (c < GS_DV) term = dp[(c - GS_DU) % 3] * r[ND_U + (c - GS_DU) / 3];
+            else if (c < GS_DT) term = dp[(c - GS_DV) % 3] * r[ND_V + (c - GS_DV) / 3];
+            else if (c < GS_TM) term = dp[c - GS_DT] * r[ND_T];
+            else if (c == GS_TM) term = ph * sm.tT[n];
+            else term = dp[c - GS_DTM] * sm.tT[n];
+            acc += term;
+        }
+        if (face) sm.fgs[j][c] = acc;
+        else sm.gs[p][c] = acc;
+    }
+    __syncthreads();
+    if (lane >= VJP_NIN) return;
+
+    // ---- lane k: phi_e in dual numbers along input k
+    const int k = lane;
     ThermoT<D> th;
     th.gamma = A.th.gamma; th.gamm1 = A.th.gamm1; th.gm1d2 = A.th.gm1d2;
     th.gamm3 = A.th.gamm3; th.gm3d2 = A.th.gm3d2; th.Pr = A.th.Pr; th.csuth = A.th.csuth;
@@ -48,58 +516,215 @@ __global__ __launch_bounds__(64) void build3d_vjp_kernel(BuildArgs A, GaussTable
     th.Cv = D(A.th.Cv);
     th.cgas = D(A.th.cgas);
     th.cmach = D(A.th.cmach);
-    {
-        const int jm = NIN - 1 - batch * ND;
-        if (jm >= 0 && jm < ND) {
-            th.Cv.d[jm] = -2.0 * A.th.Cv / A.th.cmach;
-            th.cgas.d[jm] = -2.0 * A.th.cgas / A.th.cmach;
-            th.cmach.d[jm] = 1.0;
-        }
-    }
-    if (lane < NNOD) {
-        const int g = cn[lane];
-        D Ul[NEQ];
+    // tangents of the node rows: node nk's U, v, E, T, rho along U(nk, qk),
+    // or every node's T along the Mach number (Cv = 1/(gamma (gamma-1) M^2),
+    // cgas = 1/(gamma M^2), cmach = M)
+    const int nk = (k < NNOD * NEQ) ? k / NEQ : -1;
+    double tr[ND_STRIDE - ND_U];
+#pragma unroll
+    for (int c = 0; c < ND_STRIDE - ND_U; c++) tr[c] = 0.0;
+    if (nk >= 0) {
+        D Ul[NEQ], rho, v[3], E, T;
 #pragma unroll
         for (int q = 0; q < NEQ; q++) {
-            Ul[q] = D(A.U[(size_t)g * NEQ + q]);
-            const int j = lane * NEQ + q - batch * ND;
-            if (j >= 0 && j < ND) Ul[q].d[j] = 1.0;
+            Ul[q] = D(sm.nd[nk][ND_U + q]);
+            if (q == k % NEQ) Ul[q].d[0] = 1.0;
+            tr[q] = Ul[q].d[0];
         }
-        load_node3d_t(th, A.x + (size_t)g * 3, Ul, s_nd[lane]);
-    }
-    __syncthreads();
-    const int N = lane >> 3, M = lane & 7;
-    D R[NEQ][NEQ], rhs[NEQ], sp[4];
-    int tie = 0, status = 0;
-    build3d_elem(A, th, G, s_nd, e, lane, R, rhs, sp, &tie, &status);
-    // this lane's share of phi_e
-    const int gN = cn[N], gM = cn[M];
-    D c(0.0);
+        unpack<3>(th, Ul, rho, v, E, T);
 #pragma unroll
-    for (int r = 0; r < NEQ; r++) {
-        D t = (M == 0) ? rhs[r] : D(0.0);
-#pragma unroll
-        for (int s = 0; s < NEQ; s++) t -= R[r][s] * du[(size_t)gM * NEQ + s];
-        c += t * lam[(size_t)gN * NEQ + r];
+        for (int i = 0; i < 3; i++) tr[ND_V - ND_U + i] = v[i].d[0];
+        tr[ND_E - ND_U] = E.d[0];
+        tr[ND_T - ND_U] = T.d[0];
+        tr[ND_RHO - ND_U] = rho.d[0];
+    } else {
+        th.Cv.d[0] = -2.0 * A.th.Cv / A.th.cmach;
+        th.cgas.d[0] = -2.0 * A.th.cgas / A.th.cmach;
+        th.cmach.d[0] = 1.0;
     }
-    // fixed-order reduction over the 64 lanes
+    const bool mach = (nk < 0);
+    // node n's field fld in dual numbers (n may be a runtime index)
+    auto node = [&](int n, int fld) -> D {
+        D r(sm.nd[n][fld]);
+        double t = (n == nk) ? tr[fld - ND_U] : 0.0;
+        if (fld == ND_T && mach) t += sm.tT[n];
+        r.d[0] = t;
+        return r;
+    };
+
+    D sp[4];
+    {
+        D vn[NNOD][3], Tn[NNOD], rhon[NNOD];
 #pragma unroll
-    for (int j = 0; j < ND; j++) s_red[j][lane] = c.d[j];
-    __syncthreads();
-    if (lane < ND) {
-        const int k = batch * ND + lane;
-        double acc = 0.0;
-        for (int t = 0; t < 64; t++) acc += s_red[lane][t];
-        if (k < NIN) gelem[e * NIN + k] = acc;
+        for (int n = 0; n < NNOD; n++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) vn[n][i] = node(n, ND_V + i);
+            Tn[n] = node(n, ND_T);
+            rhon[n] = node(n, ND_RHO);
+        }
+        int tie = 0;
+        fdvparam<3, 8>(th, vn, Tn, rhon, sp, &tie);
     }
+    const D s1 = sp[0], s2 = sp[1], s3 = sp[2], s4 = sp[3];
+    D phi = 0.0;
+
+    // ---- domain: 8 Gauss points
+#pragma unroll 1
+    for (int gp = 0; gp < 8; gp++) {
+        GPState<3, D> f;
+        {
+            double phk = 0.0, dk[3] = {0.0, 0.0, 0.0};
+            if (!mach) {
+                phk = sm.ph[gp][nk];
+#pragma unroll
+                for (int i = 0; i < 3; i++) dk[i] = sm.dph[gp][nk][i];
+            }
+            gp_state(f, sm.gs[gp], tr, phk, dk, mach);
+        }
+        f.mu = calc_mu(th, f.T);
+        f.k = calc_k(th, f.mu);
+        calc_tau<3>(f);
+        const double SPhi = sm.sph[gp];
+        const double SD[3] = {sm.sd[gp][0], sm.sd[gp][1], sm.sd[gp][2]};
+        const double* cv = sm.cv[gp];
+        D Dv[5];
+        Dv[0] = 0.0;
+#pragma unroll
+        for (int s = 1; s < 4; s++) {
+            D acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) acc += (-f.tau[j][s - 1] * reinv) * SD[j];
+            Dv[s] = acc;
+        }
+        {
+            D acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                D td = 0.0;
+#pragma unroll
+                for (int kk = 0; kk < 3; kk++) td += f.tau[j][kk] * f.dV[kk][j];
+                acc += (-td - f.k * f.dT[j] * SD[j]) * reinv;
+            }
+            Dv[4] = acc;
+        }
+        // sum_i LD_i . P_i
+        D pterm = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double* Li = cv + 5 * i;
+#pragma unroll
+            for (int r = 1; r < 4; r++) pterm += Li[r] * (dt * (-f.tau[i][r - 1] * reinv) * SPhi);
+            D tv = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) tv += f.tau[i][kk] * f.v[kk];
+            pterm += Li[4] * (dt * ((-tv - f.k * f.dT[i] * SPhi) * reinv));
+        }
+        D t1 = 0.0, tD = 0.0, LA[5], Sv[5];
+#pragma unroll
+        for (int r = 0; r < 5; r++) { LA[r] = 0.0; Sv[r] = 0.0; }
+        vjp_dir<0>(th, f, cv + 0, cv + 15, cv + 20, Dv, s1, s2, s3, s4, 1.0, t1, LA, Sv, tD);
+        vjp_dir<1>(th, f, cv + 5, cv + 15, cv + 25, Dv, s1, s2, s3, s4, 1.0, t1, LA, Sv, tD);
+        vjp_dir<2>(th, f, cv + 10, cv + 15, cv + 30, Dv, s1, s2, s3, s4, 1.0, t1, LA, Sv, tD);
+        const double* const Lc[3] = {cv + 0, cv + 5, cv + 10};
+        const double* const uc[3] = {cv + 20, cv + 25, cv + 30};
+        const double w1[3] = {1.0, 1.0, 1.0};
+        const D c3 = vjp_c3(th, f, Lc, w1, uc);
+        D lasv = 0.0;
+#pragma unroll
+        for (int r = 0; r < 5; r++) lasv += LA[r] * Sv[r];
+        const double wJ = sm.wJ[gp];
+        phi += wJ * (pterm - dtsq_2 * tD) + (dt * wJ) * t1 - (dtsq_2 * wJ) * lasv + (dt * wJ) * (s3 * c3);
+    }
+
+    // ---- faces with 0 < bc < 100: 4 Gauss points each
+#pragma unroll 1
+    for (int j = 0; j < 24; j++) {
+        const int bc = sm.fcode[j];
+        if (!bc) continue;
+        const int fc = j % 6;
+        GPState<3, D> f;
+        {
+            double phk = 0.0, dk[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                if (FN[fc][t] == nk) {
+                    phk = sm.fph[j][t];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) dk[i] = sm.fdp[j][t][i];
+                }
+            gp_state(f, sm.fgs[j], tr, phk, dk, mach);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int jj = 0; jj < 3; jj++) f.tau[i][jj] = 0;
+        }
+        D vph[3], dVp[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            vph[i] = f.v[i];
+#pragma unroll
+            for (int jj = 0; jj < 3; jj++) dVp[i][jj] = f.dV[i][jj];
+        }
+        f.mu = calc_mu(th, f.T);
+        f.k = calc_k(th, f.mu);
+        const double en[3] = {sm.fen[j][0], sm.fen[j][1], sm.fen[j][2]};
+        if (!eval2dboundary(th, bc, f, en)) continue;
+        const double Sph = sm.fsph[j];
+        const double Sdp[3] = {sm.fsd[j][0], sm.fsd[j][1], sm.fsd[j][2]};
+        const double* cv = sm.fcv[j];
+        D Dv[5];
+        Dv[0] = 0.0;
+#pragma unroll
+        for (int s = 1; s < 4; s++) {
+            D acc = 0.0;
+#pragma unroll
+            for (int jj = 0; jj < 3; jj++) acc += (-f.tau[jj][s - 1] * reinv) * Sdp[jj];
+            Dv[s] = acc;
+        }
+        {
+            D acc = 0.0;
+#pragma unroll
+            for (int jj = 0; jj < 3; jj++) {
+                D td = 0.0;
+#pragma unroll
+                for (int kk = 0; kk < 3; kk++) td += f.tau[jj][kk] * dVp[kk][jj];
+                acc += (-td - f.k * f.dT[jj] * Sdp[jj]) * reinv;
+            }
+            Dv[4] = acc;
+        }
+        // sum_i en_i Lf . P_i
+        D pterm = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            D pi = 0.0;
+#pragma unroll
+            for (int r = 1; r < 4; r++) pi += cv[r] * (dt * (-f.tau[i][r - 1] * reinv) * Sph);
+            D tv = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) tv += f.tau[i][kk] * vph[kk];
+            pi += cv[4] * (dt * ((-tv - f.k * f.dT[i] * Sph) * reinv));
+            pterm += en[i] * pi;
+        }
+        D t1 = 0.0, tD = 0.0, LA[5], Sv[5];
+#pragma unroll
+        for (int r = 0; r < 5; r++) { LA[r] = 0.0; Sv[r] = 0.0; }
+        vjp_dir<0>(th, f, cv, cv + 5, cv + 10, Dv, s1, s2, s3, s4, en[0], t1, LA, Sv, tD);
+        vjp_dir<1>(th, f, cv, cv + 5, cv + 15, Dv, s1, s2, s3, s4, en[1], t1, LA, Sv, tD);
+        vjp_dir<2>(th, f, cv, cv + 5, cv + 20, Dv, s1, s2, s3, s4, en[2], t1, LA, Sv, tD);
+        const double* const Lc[3] = {cv, cv, cv};
+        const double* const uc[3] = {cv + 10, cv + 15, cv + 20};
+        const D c3 = vjp_c3(th, f, Lc, en, uc);
+        D lasv = 0.0;
+#pragma unroll
+        for (int r = 0; r < 5; r++) lasv += LA[r] * Sv[r];
+        phi += -pterm + dtsq_2 * tD - dt * t1 + dtsq_2 * lasv - dt * (s3 * c3);
+    }
+    gelem[e * VJP_NIN + k] = phi.d[0];
 }
 
 void launch_build_vjp(const BuildArgs& a, const GaussTables& G, const double* lam, const double* du, double* gelem,
                       hipStream_t st) {
-    constexpr int ND = 8;
-    for (int b = 0; b * ND < 41; b++)
-        hipLaunchKernelGGL((build3d_vjp_kernel<ND>), dim3((int)a.n_elem_pad), dim3(64), 0, st, a, G, lam, du, b,
-                           gelem);
+    hipLaunchKernelGGL(build3d_vjp_kernel, dim3((int)a.n_elem_pad), dim3(64), 0, st, a, G, lam, du, gelem);
 }
 
 // gU[n][q] = sum over the (slot, local node l) pairs of node n of
