@@ -41,6 +41,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 BUILD_FLOP_PER_ELEMENT = 0.70541e6
+# repeated nodes of a refinement are interpolated to the same coordinates up
+# to rounding; the reference's absolute 1e-3 (MeshRefine2D.h:337-361) would
+# merge distinct nodes of the level-2 wall cells (2.8e-4 high)
 REFINE_NODE_TOL = 1e-9
 
 # HBM bytes per full product measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
@@ -114,65 +117,6 @@ def run_steps(ctx, U0, mode, n, args):
     return out
 
 
-def refine_mesh(mesh, th, U, levels):
-    """C3's refinement: `levels` --adap passes (main.cpp:143-157), each the
-    GPU flag pass s1 > 0.8 (RefineListAdaptive.h:28-38) on the current mesh
-    and state, then the host's AdaptiveRefineList + MeshRefine2D (refine.cpp)
-    comparing every coordinate for repeated nodes.  Returns the refined
-    mesh and state and what each pass did."""
-    from cfdv_amd import refine as PR
-    from cfdv_amd.fdv import FDVContext
-    cur, Uc, U0c, lv = mesh, U, None, None
-    info = {"elements": [int(mesh.n_elem)], "nodes": [int(mesh.n_node)], "flagged": [],
-            "refined": [], "flags_ms": [], "host_list_s": [], "host_apply_s": []}
-    for p in range(levels):
-        ctx = FDVContext(cur, th, Uc, blocks=("fp32",))
-        sync()
-        t0 = time.perf_counter()
-        flags, _, _ = ctx.refine_flags()
-        info["flags_ms"].append((time.perf_counter() - t0) * 1e3)
-        ctx.close()
-        del ctx
-        r = PR.RefinableMesh(cur, Uc, U0c, lv)
-        t0 = time.perf_counter()
-        el, rl = r.refine_list(flags)
-        t1 = time.perf_counter()
-        # repeated nodes are interpolated to the same coordinates up to
-        # rounding: the reference's absolute 1e-3 (MeshRefine2D.h:337-361)
-        # would merge distinct nodes of the level-2 wall cells (2.8e-4 high)
-        r.apply(el, rl, PR.MODE_3D, REFINE_NODE_TOL)
-        t2 = time.perf_counter()
-        cur, Uc, U0c, lv = r.get()
-        del r
-        info["flagged"].append(int(flags.sum()))
-        info["refined"].append(int(len(el)))
-        info["host_list_s"].append(t1 - t0)
-        info["host_apply_s"].append(t2 - t1)
-        info["elements"].append(int(cur.n_elem))
-        info["nodes"].append(int(cur.n_node))
-        log(f"refine pass {p}: {int(flags.sum())} flagged, {len(el)} refined (levels {rl.tolist()}) -> "
-            f"{cur.n_elem} elements, {cur.n_node} nodes; host list {t1 - t0:.1f} s, apply {t2 - t1:.1f} s")
-    info["adap_elements"] = int(cur.adap.sum()) if cur.adap is not None else 0
-    info["levels"] = np.bincount(lv).tolist()
-    info["mesh"], info["U"] = cur, Uc
-    return info
-
-
-def field_errors(mesh, U0, U64, Umx):
-    """Mixed minus FP64 after the same steps from the same state, per field
-    (rho, momentum, energy), relative to the FP64 path's change of that
-    field: ||Umx_f - U64_f|| / ||U64_f - U0_f||"""
-    nd = mesh.ndim
-    groups = {"rho": [0], "momentum": list(range(1, 1 + nd)), "energy": [1 + nd]}
-    out = {}
-    for name, cols in groups.items():
-        d = np.linalg.norm(Umx[:, cols] - U64[:, cols])
-        ch = np.linalg.norm(U64[:, cols] - U0[:, cols])
-        out[name] = float(d / ch) if ch > 0 else None
-        out[name + "_max_abs"] = float(np.abs(Umx[:, cols] - U64[:, cols]).max())
-    return out
-
-
 def cpu_baseline(mesh, th, U, n_matvec, sample, seed=7):
     """Times the CPU oracle on a random sample of elements; returns s/step."""
     from oracle import oracle as O
@@ -207,16 +151,21 @@ def main():
               "is built and measured on a single MI355X", file=sys.stderr)
         sys.exit(2)
     from cfdv_amd import mesh as M
-    from cfdv_amd.fdv import FDVContext
+    from cfdv_amd.fdv import FDVContext, field_differences
 
     nx, ny, nz = (int(v) for v in args.size.split("x"))
     t0 = time.time()
     mesh, th, U0 = M.compression_ramp(nx, ny, nz)
     log(f"mesh {nx}x{ny}x{nz}: {mesh.n_elem} elements, {mesh.n_node} nodes ({time.time() - t0:.1f}s)")
     base_elem, base_node = mesh.n_elem, mesh.n_node
-    refine_info = refine_mesh(mesh, th, U0, args.refine_levels) if args.refine_levels > 0 else None
-    if refine_info is not None:
-        mesh, U0 = refine_info.pop("mesh"), refine_info.pop("U")
+    refine_info = None
+    if args.refine_levels > 0:
+        from cfdv_amd.refine import sensor_refine
+        mesh, U0, refine_info = sensor_refine(mesh, th, U0, args.refine_levels, tol=REFINE_NODE_TOL)
+        for p in range(args.refine_levels):
+            log(f"refine pass {p}: {refine_info['flagged'][p]} flagged, {refine_info['refined'][p]} refined -> "
+                f"{refine_info['elements'][p + 1]} elements, {refine_info['nodes'][p + 1]} nodes; host list "
+                f"{refine_info['host_list_s'][p]:.1f} s, apply {refine_info['host_apply_s'][p]:.1f} s")
     t0 = time.time()
     ctx = FDVContext(mesh, th, U0, blocks=("fp64", "rows") if args.storage == "rows" else ("fp64", "fp32"))
     info = ctx.info()
@@ -233,7 +182,7 @@ def main():
         el = time.perf_counter() - t0
         results[mode] = (el, sts)
         finals[mode] = ctx.get_state()[0]
-    ferr = field_errors(mesh, U0, finals["fp64"], finals["mixed"])
+    ferr = field_differences(mesh.ndim, U0, finals["fp64"], finals["mixed"])
     log(f"mixed vs fp64 after {args.steps} steps: {ferr}")
 
     # the mixed path again with every solve started from 0, for comparison

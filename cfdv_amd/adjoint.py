@@ -44,32 +44,66 @@ def step_gradient(ctx: FDVContext, U0, dt, g, mode="fp64", rtol=1e-12, precond="
     return np.asarray(g).reshape(n, -1) + gU, gM, U1.reshape(n, -1)
 
 
-def multi_step_gradient(ctx: FDVContext, U0, dts, g, mode="fp64", rtol=1e-12, precond="nodeblock"):
-    """Gradient of J = g^T U_N after N = len(dts) steps at the given dt's,
-    with respect to U0 and the Mach number; returns (dJ/dU0 [n_node][neqn],
-    dJ/dM, U_N).
+def multi_step_gradient(ctx: FDVContext, U0, dts, g=None, mode="fp64", rtol=1e-12, precond="nodeblock",
+                        functional=None, progress=None):
+    """Gradient of J after N = len(dts) steps at the given dt's with respect
+    to U0 and the Mach number: J = g^T U_N, or J = functional(U_N, M) (an
+    object with value_and_grad(U, mach) -> (J, dJ/dU, dJ/dM), such as
+    functionals.WallFunctional).  Returns (dJ/dU0 [n_node][neqn], dJ/dM,
+    U_N), and J as a fourth value with a functional.
 
     Checkpointed: the forward sweep keeps only the N states U_0..U_{N-1}
     (host arrays); the reverse sweep re-runs step n from U_n and applies its
     adjoint, g_n = dJ/dU_n = g_{n+1} + d/dU_n [lam_n^T (b - K du)] with
-    K^T lam_n = g_{n+1}, and dJ/dM sums the steps' Mach terms.  One
-    forward solve and one forward plus one adjoint solve per step in
-    reverse: 2N forward and N adjoint solves in all.
+    K^T lam_n = g_{n+1}, and dJ/dM sums the steps' Mach terms (plus the
+    functional's own).  One forward solve and one forward plus one adjoint
+    solve per step in reverse: 2N forward and N adjoint solves in all.
+    ``progress(phase, k)`` is called after each step of either sweep.
     """
+    if (g is None) == (functional is None):
+        raise ValueError("give either g or functional")
     n = ctx.mesh.n_node
     states = []
     U = np.asarray(U0, dtype=np.float64).reshape(n, -1)
-    for dt in dts:
+    for k, dt in enumerate(dts):
         states.append(U)
         U1, _ = step_fixed_dt(ctx, U, dt, mode=mode, rtol=rtol, precond=precond)
         U = U1.reshape(n, -1)
-    gk = np.asarray(g, dtype=np.float64).ravel()
+        if progress:
+            progress("forward", k)
     gM = 0.0
-    for Un, dt in zip(reversed(states), reversed(list(dts))):
+    J = None
+    if functional is not None:
+        J, gU_N, gM = functional.value_and_grad(U, ctx.thermo.mach)
+        gk = gU_N.ravel()
+    else:
+        gk = np.asarray(g, dtype=np.float64).ravel()
+    for k, (Un, dt) in enumerate(zip(reversed(states), reversed(list(dts)))):
         gU, gMn, _ = step_gradient(ctx, Un, dt, gk, mode=mode, rtol=rtol, precond=precond)
         gk = gU.ravel()
         gM += gMn
+        if progress:
+            progress("reverse", len(dts) - 1 - k)
+    if functional is not None:
+        return gk.reshape(n, -1), gM, U, J
     return gk.reshape(n, -1), gM, U
+
+
+def functional_after_steps(ctx: FDVContext, U0, dts, functional, mach=None, mode="fp64", rtol=1e-12,
+                           precond="nodeblock", progress=None):
+    """functional(U_N, M) after N steps at fixed dt's from U0, with the Thermo
+    Mach number set to ``mach`` for the steps and the functional (the
+    context keeps that Mach number afterwards)."""
+    if mach is not None:
+        ctx.set_thermo(dataclasses.replace(ctx.thermo, mach=float(mach)))
+    n = ctx.mesh.n_node
+    U = np.asarray(U0, dtype=np.float64).reshape(n, -1)
+    for k, dt in enumerate(dts):
+        U1, _ = step_fixed_dt(ctx, U, dt, mode=mode, rtol=rtol, precond=precond)
+        U = U1.reshape(n, -1)
+        if progress:
+            progress("forward", k)
+    return functional.value(U, ctx.thermo.mach)
 
 
 def wall_temperature_gradient(dJdU0, U0, Cv, wall_nodes):

@@ -283,6 +283,21 @@ class FDVContext:
         return gU.reshape(self.mesh.n_node, self.mesh.neqn), gM.value
 
 
+def field_differences(ndim, U0, Ua, Ub):
+    """Per field group (rho, momentum, energy) the difference of two states
+    reached from U0, ||Ub_f - Ua_f|| / ||Ua_f - U0_f|| (relative to Ua's
+    change of that field), and max |Ub_f - Ua_f|."""
+    U0, Ua, Ub = (np.asarray(u, dtype=np.float64).reshape(-1, ndim + 2) for u in (U0, Ua, Ub))
+    groups = {"rho": [0], "momentum": list(range(1, 1 + ndim)), "energy": [1 + ndim]}
+    out = {}
+    for name, cols in groups.items():
+        d = np.linalg.norm(Ub[:, cols] - Ua[:, cols])
+        ch = np.linalg.norm(Ua[:, cols] - U0[:, cols])
+        out[name] = float(d / ch) if ch > 0 else None
+        out[name + "_max_abs"] = float(np.abs(Ub[:, cols] - Ua[:, cols]).max())
+    return out
+
+
 class FDVGalerkin:
     """``FDVGalerkin<Element*>`` (src/FDV/FDVGalerkin.h:52-154) over the whole
     element set: ``FDVGalerkin(ctx, dt)(all)`` builds every element block on the

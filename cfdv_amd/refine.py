@@ -147,3 +147,53 @@ def refine_pass(mesh: Mesh, U: np.ndarray, flags: np.ndarray, U0=None, refine_le
     r.apply(el, rl, mode, tol)
     m, U1, U01, lv = r.get()
     return m, U1, U01, lv, el, rl
+
+
+def sensor_refine(mesh: Mesh, thermo, U: np.ndarray, levels: int, mode: int = MODE_3D, tol: float = 1e-9,
+                  keep_passes: bool = False):
+    """``levels`` --adap passes as a run of the reference would make them
+    (main.cpp:143-157), each on the mesh and state the previous one left:
+    the s1 > 0.8 flags of the current state on the GPU
+    (FDVContext.refine_flags, RefineListAdaptive.h:28-38), then
+    AdaptiveRefineList and MeshRefine2D on the host.  ``tol`` is the
+    repeated-node distance (the reference's absolute 1e-3 merges distinct
+    nodes once cells are thinner than that; 1e-9 still catches the nodes
+    that neighbouring refinements interpolate to the same point).
+
+    Returns (mesh, U, info): info holds per pass the flagged and refined
+    element counts, the element and node counts after it, the GPU flag-pass
+    time and the host list/apply times; with keep_passes also each pass's
+    input (mesh, U) and flags under info["passes"]."""
+    import time
+    from .fdv import FDVContext
+    cur, Uc, U0c, lv = mesh, U, None, None
+    info = {"elements": [int(mesh.n_elem)], "nodes": [int(mesh.n_node)], "flagged": [], "refined": [],
+            "flags_ms": [], "host_list_s": [], "host_apply_s": []}
+    passes = []
+    for _ in range(levels):
+        ctx = FDVContext(cur, thermo, Uc, blocks=("fp32",))
+        t0 = time.perf_counter()
+        flags, _, _ = ctx.refine_flags()
+        info["flags_ms"].append((time.perf_counter() - t0) * 1e3)
+        ctx.close()
+        if keep_passes:
+            passes.append((cur, Uc, flags))
+        r = RefinableMesh(cur, Uc, U0c, lv)
+        t0 = time.perf_counter()
+        el, rl = r.refine_list(flags)
+        t1 = time.perf_counter()
+        r.apply(el, rl, mode, tol)
+        t2 = time.perf_counter()
+        cur, Uc, U0c, lv = r.get()
+        del r
+        info["flagged"].append(int(flags.sum()))
+        info["refined"].append(int(len(el)))
+        info["host_list_s"].append(t1 - t0)
+        info["host_apply_s"].append(t2 - t1)
+        info["elements"].append(int(cur.n_elem))
+        info["nodes"].append(int(cur.n_node))
+    info["adap_elements"] = int(cur.adap.sum()) if cur.adap is not None else 0
+    info["levels"] = np.bincount(lv).tolist() if lv is not None else [int(mesh.n_elem)]
+    if keep_passes:
+        info["passes"] = passes
+    return cur, Uc, info
