@@ -202,6 +202,16 @@ typedef struct {
 int orc_step(const orc_thermo *th, const orc_mesh *m, double *U, double *U0,
              const orc_step_opts *o, orc_step_stats *st);
 
+/* Multithreaded timing variant of orc_step with mode 1 (cfdv_oracle_par.c):
+   the build over elements, the product one element color at a time (cptr
+   [ncolor + 1] offsets into celem, the elements of each color), GMRES vector
+   work over rows, on nthreads OpenMP threads.  t_phase [4] (may be NULL)
+   receives seconds for set-up, build, ApplyBC + rhs, GMRES + update;
+   verbose != 0 prints build and restart progress to stderr. */
+int orc_step_par(const orc_thermo *th, const orc_mesh *m, double *U, double *U0, const orc_step_opts *o,
+                 const int *cptr, const int *celem, int ncolor, int nthreads, orc_step_stats *st,
+                 double *t_phase, int verbose);
+
 #ifdef __cplusplus
 }
 #endif

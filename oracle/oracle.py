@@ -141,6 +141,8 @@ def _bind(L: C.CDLL) -> C.CDLL:
     L.orc_color_greedy_keys.argtypes = [C.POINTER(MeshS), C.POINTER(C.c_uint64), ip]
     L.orc_step.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), dp, dp,
                            C.POINTER(StepOpts), C.POINTER(StepStats)]
+    L.orc_step_par.argtypes = [C.POINTER(Thermo), C.POINTER(MeshS), dp, dp, C.POINTER(StepOpts), ip, ip,
+                               C.c_int, C.c_int, C.POINTER(StepStats), dp, C.c_int]
     L.orc_psi.argtypes = [C.c_int, C.c_int, C.POINTER(Flow), dp, dp, dp, dp]
     L.orc_eval1dboundary.argtypes = [C.c_int, C.POINTER(Flow), dp, dp, dp]
     return L
@@ -370,6 +372,30 @@ def step(th, mesh, U, *, cfl=0.5, restart=1, mdim=20, mode=0, rtol=1e-8, maxrest
     if rc:
         raise RuntimeError(f"orc_step failed: {rc}")
     return U, U0, st
+
+
+def step_par(th, mesh, U, *, threads, cfl=0.5, mdim=30, rtol=1e-8, maxrestart=50, colors=None, verbose=False):
+    """One FDV step (true GMRES(mdim) to rtol) on ``threads`` OpenMP threads,
+    the timing variant of step(mode=1); returns (U_new, U0, stats, phases)
+    with phases the seconds of set-up, build, ApplyBC + rhs, GMRES + update."""
+    om = OMesh(mesh)
+    if colors is None:
+        colors, _ = color_greedy(mesh)
+    colors = np.asarray(colors, dtype=np.int32)
+    nc = int(colors.max()) + 1
+    celem = np.argsort(colors, kind="stable").astype(np.int32)
+    cptr = np.zeros(nc + 1, dtype=np.int32)
+    cptr[1:] = np.cumsum(np.bincount(colors, minlength=nc))
+    U = np.array(U, dtype=np.float64, copy=True)
+    U0 = np.zeros_like(U)
+    o = StepOpts(cfl, 1, mdim, 1, rtol, maxrestart)
+    st = StepStats()
+    ph = np.zeros(4)
+    rc = lib().orc_step_par(C.byref(thermo(th)), om.ref, _p(U), _p(U0), C.byref(o), _p(cptr, ip),
+                            _p(celem, ip), nc, int(threads), C.byref(st), _p(ph), int(verbose))
+    if rc:
+        raise RuntimeError(f"orc_step_par failed: {rc}")
+    return U, U0, st, ph
 
 
 def flow_struct(E, T, v, x, U, dU, dV, dT, k, mu, tau=None) -> Flow:

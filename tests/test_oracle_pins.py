@@ -330,12 +330,13 @@ def test_each_pin_catches_its_planted_mistake(tmp_path):
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         pytest.skip("no C compiler")
-    src = os.path.join(os.path.dirname(O.__file__), "cfdv_oracle.c")
+    here = os.path.dirname(O.__file__)
+    srcs = [os.path.join(here, "cfdv_oracle.c"), os.path.join(here, "cfdv_oracle_par.c")]
     pins = _pins()
     for k, (what, caught_by) in MUTATIONS.items():
         so = str(tmp_path / f"liboracle_m{k}.so")
-        subprocess.run([cc, "-O2", "-std=c99", "-fPIC", "-ffp-contract=off", "-shared", f"-DORC_MUTATE={k}",
-                        src, "-o", so, "-lm"], check=True)
+        subprocess.run([cc, "-O2", "-std=c99", "-fPIC", "-ffp-contract=off", "-fopenmp", "-shared",
+                        f"-DORC_MUTATE={k}", *srcs, "-o", so, "-lm"], check=True)
         with O.using_library(so):
             for name in caught_by:
                 try:
