@@ -84,7 +84,10 @@ __device__ void fdvparam_wave(const DevThermo& th, const double (*nd)[ND_STRIDE]
     else if (r < 0.01 && lo >= 10E-10) sv = 0.0;
     else if (re_half ? (lo <= 10E-10) : (lo < 10E-10)) sv = 1.0;
     else t = 1;
-    const double pw = 0.5 * (1 + pow(sv, 0.2));
+    // pow(0, 0.2) = 0 and pow(1, 0.2) = 1 exactly: the library pow runs only
+    // on lanes that need it (none, for most elements)
+    double pw = 0.5 * (1 + sv);
+    if (sv != 0.0 && sv != 1.0) pw = 0.5 * (1 + pow(sv, 0.2));
     s[0] = readlane_f64(sv, 0);
     s[1] = readlane_f64(pw, 0);
     s[2] = readlane_f64(sv, 32);
@@ -226,6 +229,44 @@ __device__ __forceinline__ size_t r64_at(const BuildArgs& A, long e, int q) {
     return A.r64_pairs ? pair_offset<NNOD, NEQ>(e, q) : blk_offset<double>(e, q, K * K);
 }
 
+// Lane (N, M)'s NEQ x NEQ block R of element e to / from the FP64 blocks:
+// with the node-pair layout one contiguous run (no per-entry index
+// arithmetic), otherwise entry by entry in the chunked layout.
+template <int NNOD, int NEQ>
+__device__ __forceinline__ void r64_store(const BuildArgs& A, long e, int N, int M, const double R[NEQ][NEQ]) {
+    constexpr int K = NNOD * NEQ;
+    if (A.r64_pairs) {
+        double* dst = A.R64 + (size_t)e * K * K + (size_t)(N * NNOD + M) * NEQ * NEQ;
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) dst[r * NEQ + s] = R[r][s];
+    } else {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++)
+                A.R64[blk_offset<double>(e, (NEQ * N + r) * K + NEQ * M + s, K * K)] = R[r][s];
+    }
+}
+template <int NNOD, int NEQ>
+__device__ __forceinline__ void r64_load(const BuildArgs& A, long e, int N, int M, double R[NEQ][NEQ]) {
+    constexpr int K = NNOD * NEQ;
+    if (A.r64_pairs) {
+        const double* src = A.R64 + (size_t)e * K * K + (size_t)(N * NNOD + M) * NEQ * NEQ;
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) R[r][s] = src[r * NEQ + s];
+    } else {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++)
+                R[r][s] = A.R64[blk_offset<double>(e, (NEQ * N + r) * K + NEQ * M + s, K * K)];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Interior hex8 elements (no face with a bc code, no constrained dof): the
 // Gauss-point state is the same for all 64 node pairs, so it is evaluated
@@ -243,16 +284,19 @@ struct GPShared {
 // The hex8 volume tables the interior kernel reads per point and node pair,
 // staged in LDS once per element (lane-dependent indices would otherwise be
 // vector loads from the constant buffer in every loop iteration)
+// Per point: Phi, dPhi/dxi and (from phase 1 on) w det J, 33 doubles, so
+// that the 8 lanes of phase 1 (one point each) read 8 different LDS banks.
 struct GaussHex {
-    double phi3[8][8];
-    double dpxi3[8][8][3];
-    double w3[8];
+    struct Point {
+        double phi[8];
+        double dpxi[8][3];
+        double wJ;
+    } p[8];
 };
 
-// dPhi_n/dx_j go to dphi_out[n][j] when dphi_out is given.  GT: GaussTables
-// or GaussHex.
-template <typename GT>
-CFDV_HD void gp_phase1(const DevThermo& th, const GT& G, const double (*nd)[ND_STRIDE], int gp,
+// Point P (Phi, dPhi/dxi) with weight w3; dPhi_n/dx_j go to dphi_out[n][j]
+// when dphi_out is given.
+CFDV_HD void gp_phase1(const DevThermo& th, const GaussHex::Point& P, double w3, const double (*nd)[ND_STRIDE],
                        double dt, GPShared& o, double (*dphi_out)[3]) {
     constexpr int NEQ = 5;
     const double reinv = 1.0 / th.creyn;
@@ -263,7 +307,7 @@ CFDV_HD void gp_phase1(const DevThermo& th, const GT& G, const double (*nd)[ND_S
         for (int j = 0; j < 3; j++) {
             double acc = 0.0;
 #pragma unroll
-            for (int n = 0; n < 8; n++) acc += G.dpxi3[gp][n][j] * nd[n][ND_X + i];
+            for (int n = 0; n < 8; n++) acc += P.dpxi[n][j] * nd[n][ND_X + i];
             jac[i][j] = acc;
         }
     const double J = jac[0][0] * jac[1][1] * jac[2][2] + jac[0][1] * jac[1][2] * jac[2][0] +
@@ -302,11 +346,11 @@ CFDV_HD void gp_phase1(const DevThermo& th, const GT& G, const double (*nd)[ND_S
         for (int i = 0; i < 3; i++) {
             double acc = 0.0;
 #pragma unroll
-            for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * G.dpxi3[gp][n][j]);
+            for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * P.dpxi[n][j]);
             dphi[i] = acc;
             if (dphi_out) dphi_out[n][i] = acc;
         }
-        const double ph = G.phi3[gp][n];
+        const double ph = P.phi[n];
         SPhi += ph;
         f.E += ph * nd[n][ND_E];
         const double Tnn = nd[n][ND_T];
@@ -331,7 +375,7 @@ CFDV_HD void gp_phase1(const DevThermo& th, const GT& G, const double (*nd)[ND_S
     f.mu = calc_mu(th, f.T);
     f.k = calc_k(th, f.mu);
     calc_tau<3>(f);
-    o.wJ = G.w3[gp] * J;
+    o.wJ = w3 * J;
     o.SPhi = SPhi;
     o.D[0] = 0.0;
 #pragma unroll
@@ -407,11 +451,12 @@ enum {
 CFDV_HD constexpr int ct_pair(int i, int j) { return 2 * i + (j > i ? j - 1 : j); }
 struct GPTab {
     double dphi[8][3];
-    double wJ;
     double M1[3][16];   // pattern slots 1..16
     double Af[3][25], Sf[3][25];
     double PQ[3][5];
     double C[CT_N];
+    // 271 doubles (odd): the tables of the 8 points, which 8 lanes write at
+    // once in phase 1, start in 8 different LDS banks
 };
 
 // Equation index x of direction 0 seen from direction I: momentum rows 1
@@ -489,7 +534,6 @@ CFDV_HD void gp_tab_dir_rot(const DevThermo& th, const GPShared& g, const double
 // Phase 1b, w det J and the c_ij / Re table of one Gauss point (one lane):
 // accum_c3's coefficients (NavierStokes3D.h:347-477), each distinct value once
 CFDV_HD void gp_tab_common(const DevThermo& th, const GPShared& g, GPTab& t) {
-    t.wJ = g.wJ;
     const GPState<3>& f = g.f;
     const double reinv = 1.0 / th.creyn;
     const double mu = f.mu;
@@ -571,8 +615,7 @@ CFDV_HD void accum_c3_ctab(const double* C, const double W[3][3], double out[5][
 // Phase 2 for lane (N, M): the domain contraction of every Gauss point
 // without the c_A A S product (mfma_as_product below): diagonal, direction
 // terms, c_ij, in contract3's order, and the rhs.
-template <typename GT>
-CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs, int lane,
+CFDV_HD void interior_phase2(const DevThermo& th, const GaussHex& G, const GPTab* tabs, int lane,
                              double dt, const double sp[4], double R[5][5], double rhs[5]) {
     constexpr int NEQ = 5;
     const int N = lane >> 3, M = lane & 7;
@@ -586,8 +629,8 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs
 #pragma unroll 1
     for (int gp = 0; gp < 8; gp++) {
         const GPTab& g = tabs[gp];
-        const double PN = G.phi3[gp][N], PM = G.phi3[gp][M];
-        const double wJ = g.wJ;
+        const double PN = G.p[gp].phi[N], PM = G.p[gp].phi[M];
+        const double wJ = G.p[gp].wJ;
         double DN[3], DM[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) { DN[i] = g.dphi[N][i]; DM[i] = g.dphi[M][i]; }
@@ -633,7 +676,8 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GT& G, const GPTab* tabs
 // j + 16 (i % 4).  Each operand entry is formed once, by the lane that feeds
 // it to the matrix core.  The 40 x 40 result goes to out[i * 40 + j].
 typedef double dvec4_t __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void mfma_as_product(const GPTab* tabs, double dt, int lane, double* out) {
+__device__ __forceinline__ void mfma_as_product(const GaussHex& G, const GPTab* tabs, double dt, int lane,
+                                                double* out) {
     const double dtsq_2 = (dt * dt) / 2.0;
     const int li = lane & 15, kk = lane >> 4;
     dvec4_t acc[3][3];
@@ -645,7 +689,7 @@ __device__ __forceinline__ void mfma_as_product(const GPTab* tabs, double dt, in
     for (int ks = 0; ks < 10; ks++) {
         const int k = 4 * ks + kk, g = k / 5, u = k - 5 * g;
         const GPTab& t = tabs[g];
-        const double cA = dtsq_2 * t.wJ;
+        const double cA = dtsq_2 * G.p[g].wJ;
         double av[3], bv[3];
 #pragma unroll
         for (int mt = 0; mt < 3; mt++) {
@@ -725,9 +769,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     }
     if (lane < NNOD) load_node3d(A, cn[lane], s_nd[lane]);
     if (!hang) {
-        s_g.phi3[lane >> 3][lane & 7] = c_gauss.phi3[lane >> 3][lane & 7];
+        s_g.p[lane >> 3].phi[lane & 7] = c_gauss.phi3[lane >> 3][lane & 7];
 #pragma unroll
-        for (int j = 0; j < 3; j++) s_g.dpxi3[lane >> 3][lane & 7][j] = c_gauss.dpxi3[lane >> 3][lane & 7][j];
+        for (int j = 0; j < 3; j++) s_g.p[lane >> 3].dpxi[lane & 7][j] = c_gauss.dpxi3[lane >> 3][lane & 7][j];
     } else {
         // adap() (MeshRefine2D.h:877-900): Phi <- Phi Hnm, dPxi <- dPxi Hnm,
         // lane (gp, M) one column, summed over N in order
@@ -740,11 +784,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 #pragma unroll
             for (int j = 0; j < 3; j++) dp[j] += c_gauss.dpxi3[gp][n][j] * h;
         }
-        s_g.phi3[gp][m] = ph;
+        s_g.p[gp].phi[m] = ph;
 #pragma unroll
-        for (int j = 0; j < 3; j++) s_g.dpxi3[gp][m][j] = dp[j];
+        for (int j = 0; j < 3; j++) s_g.p[gp].dpxi[m][j] = dp[j];
     }
-    if (lane < 8) s_g.w3[lane] = c_gauss.w3[lane];
     __syncthreads();
     double sp[4];
     int tie = 0;
@@ -755,9 +798,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     if (lane < 32) {
         const int gp = lane & 7, part = lane >> 3;
         GPShared g;
-        gp_phase1(th, s_g, s_nd, gp, A.dt, g, part == 3 ? s_tab[gp].dphi : nullptr);
-        if (part < 3) gp_tab_dir_rot(th, g, sp, A.dt, part, s_tab[gp]);
-        else gp_tab_common(th, g, s_tab[gp]);
+        gp_phase1(th, s_g.p[gp], c_gauss.w3[gp], s_nd, A.dt, g, part == 3 ? s_tab[gp].dphi : nullptr);
+        if (part < 3) {
+            gp_tab_dir_rot(th, g, sp, A.dt, part, s_tab[gp]);
+        } else {
+            gp_tab_common(th, g, s_tab[gp]);
+            s_g.p[gp].wJ = g.wJ;
+        }
     }
     __syncthreads();
     // phase 2: lane (N, M) contracts every point for its node pair; the
@@ -768,19 +815,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     interior_phase2(th, s_g, s_tab, lane, A.dt, sp, R, rhs);
     double* s_as = reinterpret_cast<double*>(s_tab);
     static_assert(sizeof(s_tab) >= 1600 * sizeof(double), "point tables hold the 40 x 40 product");
-    mfma_as_product(s_tab, A.dt, lane, s_as);
+    mfma_as_product(s_g, s_tab, A.dt, lane, s_as);
 #pragma unroll
     for (int r = 0; r < NEQ; r++)
 #pragma unroll
         for (int s = 0; s < NEQ; s++) R[r][s] += s_as[(NEQ * N + r) * K + NEQ * M + s];
+    if (W64) r64_store<8, 5>(A, e, N, M, R);
+    if (W32 && !boundary) {
 #pragma unroll
-    for (int r = 0; r < NEQ; r++)
+        for (int r = 0; r < NEQ; r++)
 #pragma unroll
-        for (int s = 0; s < NEQ; s++) {
-            const int q = (NEQ * N + r) * K + NEQ * M + s;
-            if (W64) A.R64[r64_at<8, 5>(A, e, q)] = R[r][s];
-            if (W32 && !boundary) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
-        }
+            for (int s = 0; s < NEQ; s++)
+                A.R32[blk_index<float>(e, (NEQ * N + r) * K + NEQ * M + s, KK)] = (float)R[r][s];
+    }
     if (M == 0) {
 #pragma unroll
         for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
@@ -1154,24 +1201,21 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
         __syncthreads();
         double R[NEQ][NEQ], rhs[NEQ];
 #pragma unroll
-        for (int r = 0; r < NEQ; r++) {
-            rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
-#pragma unroll
-            for (int s = 0; s < NEQ; s++) R[r][s] = A.R64[r64_at<8, 5>(A, e, (NEQ * N + r) * K + NEQ * M + s)];
-        }
+        for (int r = 0; r < NEQ; r++) rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
+        r64_load<8, 5>(A, e, N, M, R);
 #pragma unroll 1
         for (int q = 0; q < nb; q++)
             if (s_ft[q].ok) face_contract(s_ft[q], s_fm[q], N, M, A.dt, sp[2], R, rhs);
         const bool last = b0 + PB >= npts;
         if (last) apply_bc3(A, e, lane, R, rhs);
+        r64_store<8, 5>(A, e, N, M, R);
+        if (W32 && last && A.R32) {
 #pragma unroll
-        for (int r = 0; r < NEQ; r++)
+            for (int r = 0; r < NEQ; r++)
 #pragma unroll
-            for (int s = 0; s < NEQ; s++) {
-                const int q = (NEQ * N + r) * K + NEQ * M + s;
-                A.R64[r64_at<8, 5>(A, e, q)] = R[r][s];
-                if (W32 && last) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
-            }
+                for (int s = 0; s < NEQ; s++)
+                    A.R32[blk_index<float>(e, (NEQ * N + r) * K + NEQ * M + s, KK)] = (float)R[r][s];
+        }
         if (M == 0) {
 #pragma unroll
             for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
@@ -1203,7 +1247,7 @@ __global__ __launch_bounds__(256) void build3d_kernel(BuildArgs A, const int* li
     if (!real) {   // padding slot of a color range: zero block
         if (W64)
             for (int q = lane; q < KK; q += 64) A.R64[r64_at<8, 5>(A, e, q)] = 0.0;
-        if (W32)
+        if (W32 && A.R32)
             for (int q = lane; q < KK; q += 64) A.R32[blk_index<float>(e, q, KK)] = 0.0f;
         if (lane < K) A.rhs[e * K + lane] = 0.0;
         return;
@@ -1214,14 +1258,14 @@ __global__ __launch_bounds__(256) void build3d_kernel(BuildArgs A, const int* li
     build3d_elem(A, A.th, c_gauss, nd, e, lane, R, rhs, sp, &tie, &status);
     if (status && lane == 0) atomicOr(A.status, status);
     // ---------------- write ----------------
+    if (W64) r64_store<8, 5>(A, e, N, M, R);
+    if (W32 && A.R32) {
 #pragma unroll
-    for (int r = 0; r < NEQ; r++)
+        for (int r = 0; r < NEQ; r++)
 #pragma unroll
-        for (int s = 0; s < NEQ; s++) {
-            const int q = (NEQ * N + r) * K + NEQ * M + s;
-            if (W64) A.R64[r64_at<8, 5>(A, e, q)] = R[r][s];
-            if (W32) A.R32[blk_index<float>(e, q, KK)] = (float)R[r][s];
-        }
+            for (int s = 0; s < NEQ; s++)
+                A.R32[blk_index<float>(e, (NEQ * N + r) * K + NEQ * M + s, KK)] = (float)R[r][s];
+    }
     if (M == 0) {
 #pragma unroll
         for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
@@ -1689,9 +1733,13 @@ void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st) {
         else hipLaunchKernelGGL((build3d_interior_kernel<false, true>), dim3(nb), dim3(64), 0, st, a);
         // elements with faces or constrained dofs: faces + ApplyBC on top of
         // the FP64 domain part, or (FP32 blocks only) the full per-lane kernel
+        // (with FP64 blocks one instantiation whatever w32, the FP32 copy
+        // gated on R32 at run time, so that contexts with and without FP32
+        // blocks run the same element arithmetic, bit for bit)
+        BuildArgs b = a;
+        if (!w32) b.R32 = nullptr;
         if (a.n_bslots > 0) {
-            if (w64 && w32) hipLaunchKernelGGL((build3d_face_kernel<true>), dim3((int)a.n_bslots), dim3(64), 0, st, a);
-            else if (w64) hipLaunchKernelGGL((build3d_face_kernel<false>), dim3((int)a.n_bslots), dim3(64), 0, st, a);
+            if (w64) hipLaunchKernelGGL((build3d_face_kernel<true>), dim3((int)a.n_bslots), dim3(64), 0, st, b);
             else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3((int)((a.n_bslots + 3) / 4)), dim3(256), 0,
                                     st, a, a.bslots, a.n_bslots);
         }
@@ -1699,8 +1747,7 @@ void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st) {
         // element on the per-lane kernel
         if (a.n_aslots > 0) {
             const int nba = (int)((a.n_aslots + 3) / 4);
-            if (w64 && w32) hipLaunchKernelGGL((build3d_kernel<true, true>), dim3(nba), dim3(256), 0, st, a, a.aslots, a.n_aslots);
-            else if (w64) hipLaunchKernelGGL((build3d_kernel<true, false>), dim3(nba), dim3(256), 0, st, a, a.aslots, a.n_aslots);
+            if (w64) hipLaunchKernelGGL((build3d_kernel<true, true>), dim3(nba), dim3(256), 0, st, b, a.aslots, a.n_aslots);
             else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3(nba), dim3(256), 0, st, a, a.aslots, a.n_aslots);
         }
     } else {
