@@ -236,7 +236,7 @@ template <int NNOD, int NEQ>
 __device__ __forceinline__ void r64_store(const BuildArgs& A, long e, int N, int M, const double R[NEQ][NEQ]) {
     constexpr int K = NNOD * NEQ;
     if (A.r64_pairs) {
-        double* dst = A.R64 + (size_t)e * K * K + (size_t)(N * NNOD + M) * NEQ * NEQ;
+        double* dst = A.R64 + pair_base<NNOD, NEQ>(e, N * NNOD + M);
 #pragma unroll
         for (int r = 0; r < NEQ; r++)
 #pragma unroll
@@ -253,7 +253,7 @@ template <int NNOD, int NEQ>
 __device__ __forceinline__ void r64_load(const BuildArgs& A, long e, int N, int M, double R[NEQ][NEQ]) {
     constexpr int K = NNOD * NEQ;
     if (A.r64_pairs) {
-        const double* src = A.R64 + (size_t)e * K * K + (size_t)(N * NNOD + M) * NEQ * NEQ;
+        const double* src = A.R64 + pair_base<NNOD, NEQ>(e, N * NNOD + M);
 #pragma unroll
         for (int r = 0; r < NEQ; r++)
 #pragma unroll
@@ -1160,8 +1160,13 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
 #pragma unroll
     for (int fc = 0; fc < 6; fc++) nfa += (bcs[fc] != 0 && bcs[fc] < 100);
     const int npts = 4 * nfa;
-    // R and rhs are read and written per batch of points, so that the point
-    // tables are formed with no node-pair block live
+    // the domain part of R and rhs (written by the interior kernel) is read
+    // before the point tables are formed, so that its HBM latency overlaps
+    // them, and written once after the last batch of points
+    double R[NEQ][NEQ], rhs[NEQ];
+#pragma unroll
+    for (int r = 0; r < NEQ; r++) rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
+    r64_load<8, 5>(A, e, N, M, R);
     for (int b0 = 0; b0 < npts || b0 == 0; b0 += PB) {
         const int nb = (npts - b0 < PB) ? npts - b0 : PB;
         const int p = lane & 7, part = lane >> 3;
@@ -1199,29 +1204,24 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
         __syncthreads();
         if (p < nb && part < 5 && s_ft[p].ok) face_mats(s_ft[p], part, sp[0], s_fm[p]);
         __syncthreads();
-        double R[NEQ][NEQ], rhs[NEQ];
-#pragma unroll
-        for (int r = 0; r < NEQ; r++) rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
-        r64_load<8, 5>(A, e, N, M, R);
 #pragma unroll 1
         for (int q = 0; q < nb; q++)
             if (s_ft[q].ok) face_contract(s_ft[q], s_fm[q], N, M, A.dt, sp[2], R, rhs);
-        const bool last = b0 + PB >= npts;
-        if (last) apply_bc3(A, e, lane, R, rhs);
-        r64_store<8, 5>(A, e, N, M, R);
-        if (W32 && last && A.R32) {
-#pragma unroll
-            for (int r = 0; r < NEQ; r++)
-#pragma unroll
-                for (int s = 0; s < NEQ; s++)
-                    A.R32[blk_index<float>(e, (NEQ * N + r) * K + NEQ * M + s, KK)] = (float)R[r][s];
-        }
-        if (M == 0) {
-#pragma unroll
-            for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
-        }
-        if (last) break;
+        if (b0 + PB >= npts) break;
         __syncthreads();   // the tables are rewritten by the next batch
+    }
+    apply_bc3(A, e, lane, R, rhs);
+    r64_store<8, 5>(A, e, N, M, R);
+    if (W32 && A.R32) {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++)
+                A.R32[blk_index<float>(e, (NEQ * N + r) * K + NEQ * M + s, KK)] = (float)R[r][s];
+    }
+    if (M == 0) {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
     }
     if (lane == 0 && s_status) atomicOr(A.status, s_status);
 }

@@ -60,16 +60,22 @@ CFDV_HD size_t blk_offset(long e, int q, int KK) {
     return (size_t)(e >> 6) * 64 * KK + (size_t)(q - r) * 64 + (size_t)(e & 63) * V + r;
 }
 
-// FP64 element blocks of contexts with assembled node rows: element e's
-// k x k block as NNOD^2 contiguous NEQ x NEQ node-pair blocks, pair (N, M)
-// at (N NNOD + M) NEQ^2, row-major inside (R(i, j), i = N NEQ + r,
-// j = M NEQ + s): the row assembly reads each pair block as one contiguous
-// run, whatever the element order around a node.
+// FP64 element blocks of contexts with assembled node rows: chunks of 64
+// element slots, node-pair major inside a chunk.  Pair (N, M) of slot e is
+// one NEQ x NEQ block, row-major (R(i, j), i = N NEQ + r, j = M NEQ + s),
+// and the 64 slots of a chunk hold pair (N, M) in 64 consecutive blocks: the
+// row assembly reads a node-pair block as one contiguous run, and the same
+// pair of consecutive slots of a color as one longer run.
+template <int NNOD, int NEQ>
+CFDV_HD size_t pair_base(long e, int pair) {
+    constexpr int BB = NEQ * NEQ, KK = NNOD * NNOD * BB;
+    return (size_t)(e >> 6) * 64 * KK + (size_t)pair * 64 * BB + (size_t)(e & 63) * BB;
+}
 template <int NNOD, int NEQ>
 CFDV_HD size_t pair_offset(long e, int q) {
     constexpr int K = NNOD * NEQ;
     const int i = q / K, j = q % K;
-    return (size_t)e * K * K + (size_t)(((i / NEQ) * NNOD + j / NEQ) * NEQ * NEQ + (i % NEQ) * NEQ + j % NEQ);
+    return pair_base<NNOD, NEQ>(e, (i / NEQ) * NNOD + j / NEQ) + (size_t)((i % NEQ) * NEQ + j % NEQ);
 }
 
 template <int NDIM> struct Dims;

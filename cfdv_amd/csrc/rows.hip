@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 }
 
 // rows from the FP64 element blocks.  PAIRS (contexts with rows): each
-// node-pair block is NEQ^2 contiguous doubles (pair_offset), read whole.
+// node-pair block is NEQ^2 contiguous doubles (pair_base), read whole.
 // Otherwise the chunked layout of blk_offset (the entries of an element row
 // come in 16-byte pairs).  A block row of NEQ
 // entries starting at column j0 = M NEQ is read as whole pairs: NEQ / 2 of
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void rows_assemble_kernel(long n_chunks, const
                     const long s = code / NN2;
                     const int nm = code % NN2;
                     const int N = TR ? nm % NNOD : nm / NNOD, M = TR ? nm / NNOD : nm % NNOD;
-                    const double* Rp = R + (size_t)s * KK + (size_t)(N * NNOD + M) * BB;
+                    const double* Rp = R + pair_base<NNOD, NEQ>(s, N * NNOD + M);
                     double w[BB];
 #pragma unroll
                     for (int i = 0; i < BB; i++) w[i] = Rp[i];
@@ -298,6 +298,76 @@ void launch_nodeblock_rows(const RowsArgs& a, const long* diag, const double* B6
 #undef L
 }
 
+// Rows from the node-pair FP64 element blocks, one wavefront (workgroup) per
+// chunk of 64 nodes.  For slot k of the chunk the wave gathers the 64 node
+// blocks two at a time: each half-wave reads one contributing pair block
+// (NEQ^2 contiguous doubles, lane = entry) in one coalesced access and sums
+// its contributors in list order (the order of rows_assemble_kernel, so the
+// rows are the same bit for bit); the sums go through LDS to be written in
+// the row layout (lane = node).  TR: the rows of K^T, as in
+// rows_assemble_kernel.
+template <int NNOD, int NEQ, bool TR>
+__global__ __launch_bounds__(64) void rows_assemble_pairs_kernel(long n_chunks, const long* __restrict__ goff,
+                                                                 const long* __restrict__ coff,
+                                                                 const int* __restrict__ contrib,
+                                                                 const double* __restrict__ R,
+                                                                 double* __restrict__ B64, float* __restrict__ B32) {
+    constexpr int BB = NEQ * NEQ, GV = 4 * BB, K = NNOD * NEQ, KK = K * K, NN2 = NNOD * NNOD;
+    __shared__ double tile[64][BB];
+    const long per = (gridDim.x + 7) / 8;
+    const long c = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (c >= n_chunks) return;   // whole workgroup leaves together
+    const int lane = threadIdx.x;
+    const int half = lane >> 5, sub = lane & 31;
+    // entry of the (transposed) block this lane sums
+    const int dst = TR ? (sub % NEQ) * NEQ + sub / NEQ : sub;
+    const long g0 = goff[c], g1 = goff[c + 1];
+    for (long g = g0; g < g1; g++) {
+        for (int t = 0; t < 4; t++) {
+            const long k = g * 4 + t;
+            const long pb = coff[k];
+            const int np = (int)(coff[k + 1] - pb);
+            // contributor i of all 64 node blocks at once (32 loads in flight
+            // per lane), each block's sum in list order
+            double acc[32];
+#pragma unroll
+            for (int j = 0; j < 32; j++) acc[j] = 0.0;
+            for (int i = 0; i < np; i++) {
+                const int my = contrib[(pb + i) * 64 + lane];
+                // every load issued before the first sum (no branch around a
+                // load: lanes without a block read entry 0 and drop it)
+                double v[32];
+                bool ok[32];
+#pragma unroll
+                for (int j = 0; j < 32; j++) {
+                    const int cd = __shfl(my, 2 * j + half, 64);
+                    ok[j] = cd >= 0 && sub < BB;
+                    const long e = ok[j] ? cd / NN2 : 0;
+                    const int nm = ok[j] ? cd % NN2 : 0;
+                    const int N = TR ? nm % NNOD : nm / NNOD, M = TR ? nm / NNOD : nm % NNOD;
+                    v[j] = R[ok[j] ? pair_base<NNOD, NEQ>(e, N * NNOD + M) + sub : 0];
+                }
+#pragma unroll
+                for (int j = 0; j < 32; j++)
+                    if (ok[j]) acc[j] += v[j];
+            }
+            if (sub < BB) {
+#pragma unroll
+                for (int j = 0; j < 32; j++) tile[2 * j + half][dst] = acc[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < BB; i++) {
+                const double v = tile[lane][i];
+                const int f = t * BB + i;
+                if (B64) B64[row_offset<double>(g, lane, f, GV)] = v;
+                if (B32) B32[row_offset<float>(g, lane, f, GV)] = (float)v;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 static int row_grid(long n_chunks) {
     const long nb = (n_chunks + 3) / 4;
     return (int)(((nb + 7) / 8) * 8);   // whole rounds of 8 XCDs
@@ -337,12 +407,21 @@ void launch_rows_assemble(const RowsArgs& a, const double* R64, bool tr, double*
 #define L(NN, NQ, T, P)                                                                                        \
     hipLaunchKernelGGL((rows_assemble_kernel<NN, NQ, T, P>), grid, blk, 0, st, a.n_chunks, a.goff, a.coff, a.contrib, \
                        R64, B64, B32)
+    if (a.r64_pairs) {
+        // one 64-lane workgroup per chunk, whole rounds of the 8 XCDs
+        const dim3 pgrid((unsigned)(((a.n_chunks + 7) / 8) * 8)), pblk(64);
+#define P(NN, NQ, T)                                                                                               \
+    hipLaunchKernelGGL((rows_assemble_pairs_kernel<NN, NQ, T>), pgrid, pblk, 0, st, a.n_chunks, a.goff, a.coff, \
+                       a.contrib, R64, B64, B32)
+        if (a.nnod == 8) { if (tr) P(8, 5, true); else P(8, 5, false); }
+        else { if (tr) P(4, 4, true); else P(4, 4, false); }
+#undef P
+        return;
+    }
     if (a.nnod == 8) {
-        if (a.r64_pairs) { if (tr) L(8, 5, true, true); else L(8, 5, false, true); }
-        else { if (tr) L(8, 5, true, false); else L(8, 5, false, false); }
+        if (tr) L(8, 5, true, false); else L(8, 5, false, false);
     } else {
-        if (a.r64_pairs) { if (tr) L(4, 4, true, true); else L(4, 4, false, true); }
-        else { if (tr) L(4, 4, true, false); else L(4, 4, false, false); }
+        if (tr) L(4, 4, true, false); else L(4, 4, false, false);
     }
 #undef L
 }

@@ -35,9 +35,9 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     const int* ce = conn + e * NNOD;
     if (ce[0] < 0) return;
     if constexpr (PAIRS) {
-        // blocks in node-pair order (pair_offset): scalar loads, the same
+        // blocks in node-pair order (pair_base): scalar loads, the same
         // products summed in the same order as the chunked path
-        const TB* Re = R + (size_t)e * KK;
+        // pair (n, m) of this slot at pair_base(e, n NNOD + m)
         if (!TR) {
             TA xv[K];
 #pragma unroll
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
                     for (int m = 0; m < NNOD; m++)
 #pragma unroll
                         for (int s = 0; s < NEQ; s++)
-                            acc += (TA)Re[(n * NNOD + m) * NEQ * NEQ + r * NEQ + s] * xv[m * NEQ + s];
+                            acc += (TA)R[pair_base<NNOD, NEQ>(e, n * NNOD + m) + r * NEQ + s] * xv[m * NEQ + s];
                     TV* yp = y + (size_t)gn * NEQ + r;
                     *yp = (TV)((TA)*yp + acc);
                 }
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
                     for (int m = 0; m < NNOD; m++)
 #pragma unroll
                         for (int s = 0; s < NEQ; s++)
-                            yv[m * NEQ + s] += (TA)Re[(n * NNOD + m) * NEQ * NEQ + r * NEQ + s] * xi[r];
+                            yv[m * NEQ + s] += (TA)R[pair_base<NNOD, NEQ>(e, n * NNOD + m) + r * NEQ + s] * xi[r];
             }
 #pragma unroll
             for (int m = 0; m < NNOD; m++) {
