@@ -393,10 +393,11 @@ DevThermo make_thermo(const cfdv_thermo* t) {
     return th;
 }
 
-// Structure of the assembled node rows (see rows.hip): per node the sorted
-// neighbour nodes (nodes sharing an element, itself included), chunks of 64
-// rows with groups of 4 slots, and per slot the element blocks that sum
-// into it, listed in element order.  ptr/elem: node -> element CSR.
+// Structure of the assembled node rows (see rows.hip): per node the
+// neighbour nodes (nodes sharing an element, itself included) as slots, most
+// contributing elements first, chunks of 64 rows with groups of 4 slots, and
+// per slot the element blocks that sum into it, listed in element order.
+// ptr/elem: node -> element CSR.
 struct RowsHost {
     std::vector<long> goff, coff, diag;
     std::vector<int> cols, contrib;
@@ -449,6 +450,18 @@ RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* 
                 }
         }
     }
+    // slot order within a row: most contributors first (ties by column), so
+    // that slot k of the 64 rows of a chunk tends to have the same number of
+    // contributors where refinement makes the rows irregular (the gather
+    // then pads less); sperm[aptr[r] + k] is the adjacency entry of slot k
+    std::vector<long> sperm(adj.size());
+    for (long r = 0; r < nnode; r++) {
+        const long b = aptr[r], e = aptr[r + 1];
+        for (long a = b; a < e; a++) sperm[a] = a;
+        std::stable_sort(sperm.begin() + b, sperm.begin() + e, [&](long x, long y) {
+            return eptr[x + 1] - eptr[x] > eptr[y + 1] - eptr[y];
+        });
+    }
     // chunks and groups
     const long nch = (nnode + 63) / 64;
     RowsHost h;
@@ -473,8 +486,13 @@ RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* 
                 const long r = ch * 64 + lane;
                 if (r >= nnode) continue;
                 const long deg = aptr[r + 1] - aptr[r];
-                cols[(size_t)k * 64 + lane] = kl < deg ? adj[aptr[r] + kl] : (int)r;
-                if (kl < deg) cmax = std::max(cmax, eptr[aptr[r] + kl + 1] - eptr[aptr[r] + kl]);
+                if (kl < deg) {
+                    const long a = sperm[aptr[r] + kl];
+                    cols[(size_t)k * 64 + lane] = adj[a];
+                    cmax = std::max(cmax, eptr[a + 1] - eptr[a]);
+                } else {
+                    cols[(size_t)k * 64 + lane] = (int)r;
+                }
             }
             coff[k + 1] = coff[k] + cmax;
         }
@@ -483,9 +501,8 @@ RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* 
     // global slot of each row's own column (its diagonal block)
     h.diag.assign(nnode, 0);
     for (long r = 0; r < nnode; r++) {
-        const int* b = adj.data() + aptr[r];
-        const int* e = adj.data() + aptr[r + 1];
-        const long kl = std::lower_bound(b, e, (int)r) - b;
+        long kl = 0;
+        while (adj[sperm[aptr[r] + kl]] != (int)r) kl++;
         h.diag[r] = goff[r / 64] * 4 + kl;
     }
     for (long ch = 0; ch < nch; ch++)
@@ -494,7 +511,7 @@ RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* 
             for (int lane = 0; lane < 64; lane++) {
                 const long r = ch * 64 + lane;
                 if (r >= nnode || kl >= aptr[r + 1] - aptr[r]) continue;
-                const long a = aptr[r] + kl;
+                const long a = sperm[aptr[r] + kl];
                 for (long i = eptr[a]; i < eptr[a + 1]; i++)
                     contrib[(size_t)(coff[k] + (i - eptr[a])) * 64 + lane] = codes[i];
             }

@@ -12,8 +12,9 @@
 // vectors, and the node-block inverses from the diagonal slots.
 //
 // Layout.  Rows come in chunks of 64 nodes (lane = node % 64).  Chunk c has
-// G_c groups of 4 slots (slot = one neighbour node of the row, ascending
-// node number, padded with the row itself and a zero block); goff[c] is the
+// G_c groups of 4 slots (slot = one neighbour node of the row, the ones with
+// most contributing elements first, padded with the row itself and a zero
+// block); goff[c] is the
 // first group of chunk c.  A group's 4 blocks are 4 * NEQ^2 values per lane,
 // row-major per block, stored 16 bytes per lane at a time: value f of group
 // g for a lane sits at g * GV * 64 + (f - f % V) * 64 + lane * V + f % V
