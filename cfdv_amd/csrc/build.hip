@@ -613,10 +613,14 @@ CFDV_HD void accum_c3_ctab(const double* C, const double W[3][3], double out[5][
 }
 
 // Phase 2 for lane (N, M): the domain contraction of every Gauss point
-// without the c_A A S product (mfma_as_product below): diagonal, direction
+// without the c_A A S product (mfma_as_kstep below): diagonal, direction
 // terms, c_ij, in contract3's order, and the rhs.
+// before: called at the start of point gp's iteration (the kernel issues
+// the matrix-core steps of the A S product there, so that they execute while
+// the lanes contract the point on the VALU).
+template <typename Before>
 CFDV_HD void interior_phase2(const DevThermo& th, const GaussHex& G, const GPTab* tabs, int lane,
-                             double dt, const double sp[4], double R[5][5], double rhs[5]) {
+                             double dt, const double sp[4], double R[5][5], double rhs[5], Before&& before) {
     constexpr int NEQ = 5;
     const int N = lane >> 3, M = lane & 7;
     const double s3 = sp[2];
@@ -628,6 +632,7 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GaussHex& G, const GPTab
     }
 #pragma unroll 1
     for (int gp = 0; gp < 8; gp++) {
+        before(gp);
         const GPTab& g = tabs[gp];
         const double PN = G.p[gp].phi[N], PM = G.p[gp].phi[M];
         const double wJ = G.p[gp].wJ;
@@ -676,50 +681,47 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GaussHex& G, const GPTab
 // j + 16 (i % 4).  Each operand entry is formed once, by the lane that feeds
 // it to the matrix core.  The 40 x 40 result goes to out[i * 40 + j].
 typedef double dvec4_t __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void mfma_as_product(const GaussHex& G, const GPTab* tabs, double dt, int lane,
-                                                double* out) {
-    const double dtsq_2 = (dt * dt) / 2.0;
+// Inner step ks (inner indices 4 ks .. 4 ks + 3) of the product into acc
+__device__ __forceinline__ void mfma_as_kstep(const GaussHex& G, const GPTab* tabs, double dtsq_2, int ks, int lane,
+                                              dvec4_t acc[3][3]) {
     const int li = lane & 15, kk = lane >> 4;
-    dvec4_t acc[3][3];
+    const int k = 4 * ks + kk, g = k / 5, u = k - 5 * g;
+    const GPTab& t = tabs[g];
+    const double cA = dtsq_2 * G.p[g].wJ;
+    double av[3], bv[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) acc[a][b] = dvec4_t{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
-    for (int ks = 0; ks < 10; ks++) {
-        const int k = 4 * ks + kk, g = k / 5, u = k - 5 * g;
-        const GPTab& t = tabs[g];
-        const double cA = dtsq_2 * G.p[g].wJ;
-        double av[3], bv[3];
-#pragma unroll
-        for (int mt = 0; mt < 3; mt++) {
-            const int i = 16 * mt + li;       // row (N, r) of the element block
-            double v = 0.0;
-            if (i < 40) {
-                const int N = i / 5, r = i - 5 * (i / 5);
-                v = cA * (t.dphi[N][0] * t.Af[0][r * 5 + u] + t.dphi[N][1] * t.Af[1][r * 5 + u] +
-                          t.dphi[N][2] * t.Af[2][r * 5 + u]);
-            }
-            av[mt] = v;
+    for (int mt = 0; mt < 3; mt++) {
+        const int i = 16 * mt + li;       // row (N, r) of the element block
+        double v = 0.0;
+        if (i < 40) {
+            const int N = i / 5, r = i - 5 * (i / 5);
+            v = cA * (t.dphi[N][0] * t.Af[0][r * 5 + u] + t.dphi[N][1] * t.Af[1][r * 5 + u] +
+                      t.dphi[N][2] * t.Af[2][r * 5 + u]);
         }
-#pragma unroll
-        for (int nt = 0; nt < 3; nt++) {
-            const int j = 16 * nt + li;       // column (M, s)
-            double v = 0.0;
-            if (j < 40) {
-                const int M = j / 5, sc = j - 5 * (j / 5);
-                v = t.dphi[M][0] * t.Sf[0][u * 5 + sc] + t.dphi[M][1] * t.Sf[1][u * 5 + sc] +
-                    t.dphi[M][2] * t.Sf[2][u * 5 + sc];
-            }
-            bv[nt] = v;
-        }
-#pragma unroll
-        for (int mt = 0; mt < 3; mt++)
-#pragma unroll
-            for (int nt = 0; nt < 3; nt++)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv[nt], acc[mt][nt], 0, 0, 0);
+        av[mt] = v;
     }
-    __syncthreads();   // every lane is done reading tabs; out may alias them
+#pragma unroll
+    for (int nt = 0; nt < 3; nt++) {
+        const int j = 16 * nt + li;       // column (M, s)
+        double v = 0.0;
+        if (j < 40) {
+            const int M = j / 5, sc = j - 5 * (j / 5);
+            v = t.dphi[M][0] * t.Sf[0][u * 5 + sc] + t.dphi[M][1] * t.Sf[1][u * 5 + sc] +
+                t.dphi[M][2] * t.Sf[2][u * 5 + sc];
+        }
+        bv[nt] = v;
+    }
+#pragma unroll
+    for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 3; nt++)
+            acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv[nt], acc[mt][nt], 0, 0, 0);
+}
+// The 40 x 40 result to out[i * 40 + j] (out may alias the point tables:
+// every lane is done reading them first)
+__device__ __forceinline__ void mfma_as_store(const dvec4_t acc[3][3], int lane, double* out) {
+    const int li = lane & 15, kk = lane >> 4;
+    __syncthreads();
 #pragma unroll
     for (int mt = 0; mt < 3; mt++)
 #pragma unroll
@@ -812,10 +814,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     // lands in LDS over the (then consumed) point tables
     const int N = lane >> 3, M = lane & 7;
     double R[NEQ][NEQ], rhs[NEQ];
-    interior_phase2(th, s_g, s_tab, lane, A.dt, sp, R, rhs);
+    // the A S product's 10 inner steps are spread over the 8 points of the
+    // VALU contraction (steps 8 gp / 10 .. 8 (gp + 1) / 10 at point gp)
+    const double dtsq_2 = (A.dt * A.dt) / 2.0;
+    dvec4_t acc[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) acc[a][b] = dvec4_t{0.0, 0.0, 0.0, 0.0};
+    interior_phase2(th, s_g, s_tab, lane, A.dt, sp, R, rhs, [&](int gp) {
+        for (int ks = (gp * 10) >> 3; ks < ((gp + 1) * 10) >> 3; ks++) mfma_as_kstep(s_g, s_tab, dtsq_2, ks, lane, acc);
+    });
     double* s_as = reinterpret_cast<double*>(s_tab);
     static_assert(sizeof(s_tab) >= 1600 * sizeof(double), "point tables hold the 40 x 40 product");
-    mfma_as_product(s_g, s_tab, A.dt, lane, s_as);
+    mfma_as_store(acc, lane, s_as);
 #pragma unroll
     for (int r = 0; r < NEQ; r++)
 #pragma unroll
