@@ -299,21 +299,69 @@ void launch_nodeblock_rows(const RowsArgs& a, const long* diag, const double* B6
 #undef L
 }
 
+// Slot T of group g from the row values val[i] (value f = T NEQ^2 + i of
+// the group), stored as whole 16-byte vectors of the row layout (FP64 pairs,
+// FP32 quads): a vector whose last values belong to the next slot keeps its
+// first ones in c64 / c32 and is written by that slot's call.  The four
+// slots of a group are stored in order, and the last vector of a group ends
+// with it.
+template <int NEQ, int T>
+__device__ __forceinline__ void store_row_slot(long g, int lane, const double* val,
+                                               double* __restrict__ B64, float* __restrict__ B32, double& c64,
+                                               float (&c32)[3]) {
+    constexpr int BB = NEQ * NEQ, GV = 4 * BB, F0 = T * BB, F1 = F0 + BB;
+    typedef typename V16<double>::type d2_t;
+    typedef typename V16<float>::type f4_t;
+    if (B64) {
+        d2_t* p = reinterpret_cast<d2_t*>(B64 + (size_t)g * GV * 64) + lane;   // pair q at p + 64 q
+#pragma unroll
+        for (int q = F0 / 2; 2 * q < F1; q++) {
+            if (2 * q + 1 >= F1) {   // completed by the next slot
+                c64 = val[2 * q - F0];
+                continue;
+            }
+            const double a = 2 * q < F0 ? c64 : val[2 * q - F0];
+            p[(size_t)q * 64] = d2_t{a, val[2 * q + 1 - F0]};
+        }
+    }
+    if (B32) {
+        f4_t* p = reinterpret_cast<f4_t*>(B32 + (size_t)g * GV * 64) + lane;   // quad q at p + 64 q
+#pragma unroll
+        for (int q = F0 / 4; 4 * q < F1; q++) {
+            float w[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int f = 4 * q + u;
+                w[u] = f < F0 ? c32[u] : (f < F1 ? (float)val[f - F0] : 0.0f);
+            }
+            if (4 * q + 3 >= F1) {   // completed by the next slot
+#pragma unroll
+                for (int u = 0; u < 3; u++) c32[u] = w[u];
+                continue;
+            }
+            p[(size_t)q * 64] = f4_t{w[0], w[1], w[2], w[3]};
+        }
+    }
+}
+
 // Rows from the node-pair FP64 element blocks, one wavefront (workgroup) per
-// chunk of 64 nodes.  For slot k of the chunk the wave gathers the 64 node
-// blocks two at a time: each half-wave reads one contributing pair block
-// (NEQ^2 contiguous doubles, lane = entry) in one coalesced access and sums
-// its contributors in list order (the order of rows_assemble_kernel, so the
-// rows are the same bit for bit); the sums go through LDS to be written in
-// the row layout (lane = node).  TR: the rows of K^T, as in
-// rows_assemble_kernel.
+// chunk of 64 rows.  Contributor i of the 64 row blocks of slot k is read in
+// two half rounds of 32 rows, two blocks per load: half-wave h reads the
+// block of row 32 r + 2 j + h in half round r (lane = entry, one 200-byte
+// run).  The half rounds of all the chunk's slots run as one sequence, and
+// the 16 loads of the next half round are issued before the current one is
+// summed, so that one is in flight while the wave adds, transposes and
+// stores.  The block addresses are formed on the scalar unit from the
+// contributor codes (read lane by lane), each lane selecting its half's.
+// Each block sums its contributors in list order (the order of
+// rows_assemble_kernel, so the rows are the same bit for bit); the sums go
+// through LDS to be written in the row layout (lane = row) as whole 16-byte
+// vectors.  TR: the rows of K^T, as in rows_assemble_kernel.
 template <int NNOD, int NEQ, bool TR>
-__global__ __launch_bounds__(64) void rows_assemble_pairs_kernel(long n_chunks, const long* __restrict__ goff,
-                                                                 const long* __restrict__ coff,
-                                                                 const int* __restrict__ contrib,
-                                                                 const double* __restrict__ R,
-                                                                 double* __restrict__ B64, float* __restrict__ B32) {
-    constexpr int BB = NEQ * NEQ, GV = 4 * BB, K = NNOD * NEQ, KK = K * K, NN2 = NNOD * NNOD;
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void rows_assemble_pairs_kernel(
+    long n_chunks, const long* __restrict__ goff, const long* __restrict__ coff, const int* __restrict__ contrib,
+    const double* __restrict__ R, double* __restrict__ B64, float* __restrict__ B32) {
+    constexpr int BB = NEQ * NEQ, NN2 = NNOD * NNOD;
     __shared__ double tile[64][BB];
     const long per = (gridDim.x + 7) / 8;
     const long c = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
@@ -322,50 +370,75 @@ __global__ __launch_bounds__(64) void rows_assemble_pairs_kernel(long n_chunks, 
     const int half = lane >> 5, sub = lane & 31;
     // entry of the (transposed) block this lane sums
     const int dst = TR ? (sub % NEQ) * NEQ + sub / NEQ : sub;
-    const long g0 = goff[c], g1 = goff[c + 1];
-    for (long g = g0; g < g1; g++) {
-        for (int t = 0; t < 4; t++) {
-            const long k = g * 4 + t;
-            const long pb = coff[k];
-            const int np = (int)(coff[k + 1] - pb);
-            // contributor i of all 64 node blocks at once (32 loads in flight
-            // per lane), each block's sum in list order
-            double acc[32];
+    const long k0 = goff[c] * 4, k1 = goff[c + 1] * 4;
+    const long p0 = coff[k0], p1 = coff[k1];
+    // element-block offset of contributor code cd >= 0 (uniform)
+    auto block_of = [](unsigned cd) -> size_t {
+        const unsigned s = cd / NN2, nm = cd % NN2;
+        const unsigned N = TR ? nm % NNOD : nm / NNOD, M = TR ? nm / NNOD : nm % NNOD;
+        return pair_base<NNOD, NEQ>((long)s, (int)(N * NNOD + M));
+    };
+    const bool entry = sub < BB;
+    // the 16 loads of half round r with codes `my` (lane l: the code of row
+    // l, -1 for none); lanes without a block read R[0] and drop it
+    auto issue = [&](int my, int r, double (&v)[16], unsigned& ok) {
+        ok = 0;
 #pragma unroll
-            for (int j = 0; j < 32; j++) acc[j] = 0.0;
-            for (int i = 0; i < np; i++) {
-                const int my = contrib[(pb + i) * 64 + lane];
-                // every load issued before the first sum (no branch around a
-                // load: lanes without a block read entry 0 and drop it)
-                double v[32];
-                bool ok[32];
+        for (int j = 0; j < 16; j++) {
+            const int c0 = __builtin_amdgcn_readlane(my, 32 * r + 2 * j);
+            const int c1 = __builtin_amdgcn_readlane(my, 32 * r + 2 * j + 1);
+            const size_t b0 = block_of((unsigned)max(c0, 0)), b1 = block_of((unsigned)max(c1, 0));
+            const bool good = ((half ? c1 : c0) >= 0) & entry;
+            ok |= (unsigned)good << j;
+            v[j] = R[good ? (half ? b1 : b0) + sub : 0];
+        }
+    };
+    double acc[32];   // acc[16 r + j]: row 32 r + 2 j + half
 #pragma unroll
-                for (int j = 0; j < 32; j++) {
-                    const int cd = __shfl(my, 2 * j + half, 64);
-                    ok[j] = cd >= 0 && sub < BB;
-                    const long e = ok[j] ? cd / NN2 : 0;
-                    const int nm = ok[j] ? cd % NN2 : 0;
-                    const int N = TR ? nm % NNOD : nm / NNOD, M = TR ? nm / NNOD : nm % NNOD;
-                    v[j] = R[ok[j] ? pair_base<NNOD, NEQ>(e, N * NNOD + M) + sub : 0];
-                }
-#pragma unroll
-                for (int j = 0; j < 32; j++)
-                    if (ok[j]) acc[j] += v[j];
-            }
+    for (int j = 0; j < 32; j++) acc[j] = 0.0;
+    long k = k0;
+    double c64 = 0.0;
+    float c32[3] = {0.0f, 0.0f, 0.0f};
+    // every slot whose contributors end at or before pe: transposed through
+    // LDS, stored, the sums cleared
+    auto flush_to = [&](long pe) {
+        while (k < k1 && coff[k + 1] <= pe) {
             if (sub < BB) {
 #pragma unroll
-                for (int j = 0; j < 32; j++) tile[2 * j + half][dst] = acc[j];
+                for (int j = 0; j < 32; j++) tile[32 * (j >> 4) + 2 * (j & 15) + half][dst] = acc[j];
+            }
+            __syncthreads();
+            const double* val = tile[lane];
+            const long g = k >> 2;
+            switch ((int)(k & 3)) {
+                case 0: store_row_slot<NEQ, 0>(g, lane, val, B64, B32, c64, c32); break;
+                case 1: store_row_slot<NEQ, 1>(g, lane, val, B64, B32, c64, c32); break;
+                case 2: store_row_slot<NEQ, 2>(g, lane, val, B64, B32, c64, c32); break;
+                default: store_row_slot<NEQ, 3>(g, lane, val, B64, B32, c64, c32); break;
             }
             __syncthreads();
 #pragma unroll
-            for (int i = 0; i < BB; i++) {
-                const double v = tile[lane][i];
-                const int f = t * BB + i;
-                if (B64) B64[row_offset<double>(g, lane, f, GV)] = v;
-                if (B32) B32[row_offset<float>(g, lane, f, GV)] = (float)v;
-            }
-            __syncthreads();
+            for (int j = 0; j < 32; j++) acc[j] = 0.0;
+            k++;
         }
+    };
+    flush_to(p0);   // slots without contributors before the first one
+    double va[16], vb[16];
+    unsigned oka = 0, okb = 0;
+    int code = p0 < p1 ? contrib[p0 * 64 + lane] : -1;
+    if (p0 < p1) issue(code, 0, va, oka);
+    for (long p = p0; p < p1; p++) {
+        const int code_next = p + 1 < p1 ? contrib[(p + 1) * 64 + lane] : -1;
+        issue(code, 1, vb, okb);
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if ((oka >> j) & 1u) acc[j] += va[j];
+        if (p + 1 < p1) issue(code_next, 0, va, oka);
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if ((okb >> j) & 1u) acc[16 + j] += vb[j];
+        flush_to(p + 1);
+        code = code_next;
     }
 }
 

@@ -1,8 +1,9 @@
 """Stand-alone run of the element build and the products K x, for profilers.
 
-    python -m cfdv_amd.profile_matvec --size 256x128x64 --reps 20 [--storage rows]
+    python -m cfdv_amd.profile_matvec --size 256x128x64 --reps 20 [--storage rows] [--refine-levels 2]
 
-Builds the compression-ramp mesh of the given size, runs one element build
+Builds the compression-ramp mesh of the given size (with ``--refine-levels``
+refined by that many sensor passes, as the benchmark does), runs one element build
 (with ``--storage rows`` followed by the assembly of the node rows), then
 ``--reps`` full products in FP32 and in FP64 (element by element, or over
 the assembled rows) and ``--reps`` 8-vector FP32 EBE products, and prints
@@ -31,6 +32,8 @@ def main(argv=None):
     p.add_argument("--copy-mib", type=int, default=0)
     p.add_argument("--precision", choices=["32", "64", "both"], default="both")
     p.add_argument("--storage", choices=["ebe", "rows"], default="ebe")
+    p.add_argument("--refine-levels", type=int, default=0)
+    p.add_argument("--no-multi", action="store_true", help="skip the 8-vector product")
     a = p.parse_args(argv)
     nx, ny, nz = (int(v) for v in a.size.split("x"))
     out = {"size": a.size}
@@ -45,6 +48,9 @@ def main(argv=None):
         out["copy_bytes_written"] = n * 4
         del src, dst
     mesh, th, U = M.compression_ramp(nx, ny, nz)
+    if a.refine_levels > 0:
+        from .refine import sensor_refine
+        mesh, U, _ = sensor_refine(mesh, th, U, a.refine_levels)
     rows = a.storage == "rows"
     ctx = FDVContext(mesh, th, U, blocks=("fp64", "fp32", "rows") if rows else ("fp64", "fp32"))
     dt = ctx.cfl(a.cfl)
@@ -54,19 +60,26 @@ def main(argv=None):
     out["build_wall_ms"] = (time.perf_counter() - t0) * 1e3
     ne, nn = mesh.n_elem, mesh.n_node
     k = mesh.k
-    nnzb = (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1)
+    # node blocks of the assembled rows (on the structured grid
+    # (3nx+1)(3ny+1)(3nz+1)): 25 values and a column index each
+    nnzb = ctx.rows_info()[0] if rows else 0
+    if rows:
+        out["row_blocks"], out["row_slots_stored"] = ctx.rows_info()
     for prec in ((32, 64) if a.precision == "both" else (int(a.precision),)):
         ms = ctx.profile_matvec(prec, a.reps)
         w = prec // 8
         if rows:
-            # node blocks of the structured node grid: (3nx+1)(3ny+1)(3nz+1),
-            # 25 values and a column index each; x read and y written once
+            # x read and y written once
             nbytes = nnzb * (mesh.neqn ** 2 * w + 4) + nn * mesh.neqn * 2 * w
         else:
             nbytes = ne * k * k * w + nn * mesh.neqn * 2 * w
         out[f"matvec{prec}_ms"] = ms
         out[f"matvec{prec}_bytes"] = nbytes
         out[f"matvec{prec}_GBps"] = nbytes / (ms * 1e-3) / 1e9
+    if a.no_multi:
+        out.update(elements=ne, nodes=nn, colors=ctx.info()["ncolors"], storage=a.storage)
+        print(json.dumps(out), flush=True)
+        return 0
     ms8 = ctx.profile_matvec_multi(a.reps)
     if rows:
         b8 = nnzb * (mesh.neqn ** 2 * 4 + 4) + nn * mesh.neqn * 8 * 2 * 4
