@@ -186,90 +186,98 @@ void launch_matvec_color(const MatvecArgs& a, const double* R64, const float* R3
 }
 
 // ---------------------------------------------------------------------------
-// EBE product with NV right-hand sides at once (FP32 blocks and vectors,
-// FP32 accumulation): Y[n][NV] = K X[n][NV] (or K^T X).  NV lanes per element,
-// one vector each: the NV lanes of an element issue the same 16-byte block
-// loads (one transaction for all of them) and the NV/... elements of a
-// wavefront read consecutive pieces of the chunked layout, so one pass over
-// the blocks serves NV products.  The vectors are node-major with the NV
-// components contiguous, so an element's lanes gather and scatter 4*NV
-// contiguous bytes per dof.  This is HBM-bound like the single product
-// (25.6 kflop per 6.4 KB block at NV = 8): plain FP32 FMAs keep up with the
-// block stream, so no matrix-core tiling is needed.
+// Y = K X (or K^T X) for 8 vectors over the FP32 element blocks of one color
+// (the EBE form, for contexts without node rows), on the FP32 matrix cores.
+// Per element it is the dense product of the 40 x 40 block with the 40 x 8
+// gathered vectors, computed transposed, Y_e^T = X_e^T R_e^T (or X_e^T R_e),
+// as rank-1 updates of 4 x 4 tiles: v_mfma_f32_4x4x1f32 runs 16 independent
+// 4 x 4 x 1 products, one per element, so a wavefront holds 16 elements of
+// the color, 4 lanes each (block b = lane / 4; lane 4 b + q supplies A row q
+// = vector 4 vb + q of X_e^T, B column q = block row 4 cb + q, and receives
+// D column q = the 4 vectors 4 vb .. 4 vb + 3 of block row 4 cb + q).  Every
+// 4 x 4 tile is full (40 and 8 are multiples of 4), 20 tiles per element,
+// accumulated over the 40 inner indices.  K x reads each lane's block row as
+// ten 16-byte pieces of the chunked layout (four inner indices per load); K^T
+// x reads block column 4 cb + q, one entry per load.  Same-color elements
+// share no node, so the Y rows are read, added to and written without
+// atomics.  The summation order differs from the single product's (tiles of
+// four vectors accumulate over the inner index in order).
 // ---------------------------------------------------------------------------
-template <int NV, bool TR>
-__global__ __launch_bounds__(256) void matvec_multi_kernel(long e0, long e1, const int* __restrict__ conn,
+template <bool TR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void matvec_multi_kernel(long e0, long e1, const int* __restrict__ conn,
                                                            const float* __restrict__ R,
                                                            const float* __restrict__ X, float* __restrict__ Y) {
-    constexpr int NNOD = 8, NEQ = 5, K = NNOD * NEQ, KK = K * K, V = 4, GPR = K / V;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long e = e0 + t / NV;
-    const int v = (int)(t % NV);
-    if (e >= e1) return;
-    const int* ce = conn + e * NNOD;
-    if (ce[0] < 0) return;
-    const v4f_t* Rb = reinterpret_cast<const v4f_t*>(R + (size_t)(e >> 6) * 64 * KK) + (e & 63);
-    if (!TR) {
-        float xv[K];
+    constexpr int NNOD = 8, NEQ = 5, K = NNOD * NEQ, KK = K * K, NV = 8;
+    const int lane = threadIdx.x & 63;
+    const int b = lane >> 2, q = lane & 3;
+    const long e = e0 + ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + b;   // 16-aligned within a chunk
+    if (e0 + ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 >= e1) return;   // whole wavefront
+    const bool live = e < e1 && conn[e * NNOD] >= 0;   // padding slots hold zero blocks
+    int cn[NNOD];
 #pragma unroll
-        for (int m = 0; m < NNOD; m++) {
-            const int gm = ce[m];
+    for (int m = 0; m < NNOD; m++) cn[m] = live ? conn[e * NNOD + m] : 0;
+    const float* Re = R + (size_t)(e >> 6) * 64 * KK + (size_t)(e & 63) * 4;   // piece p at Re + 256 p
+    v4f_t acc[10][2];
 #pragma unroll
-            for (int q = 0; q < NEQ; q++) xv[m * NEQ + q] = X[((size_t)gm * NEQ + q) * NV + v];
+    for (int cb = 0; cb < 10; cb++)
+#pragma unroll
+        for (int vb = 0; vb < 2; vb++) acc[cb][vb] = v4f_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+    for (int kp = 0; kp < 10; kp++) {
+        // B: entries (4 cb + q, 4 kp + kk) of R_e (K x) or (4 kp + kk, 4 cb + q) (K^T x)
+        v4f_t bv[10];
+        if (!TR) {
+#pragma unroll
+            for (int cb = 0; cb < 10; cb++)
+                bv[cb] = __builtin_nontemporal_load(reinterpret_cast<const v4f_t*>(Re + (size_t)256 * ((4 * cb + q) * 10 + kp)));
+        } else {
+#pragma unroll
+            for (int cb = 0; cb < 10; cb++)
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++)
+                    bv[cb][kk] = __builtin_nontemporal_load(Re + (size_t)256 * ((4 * kp + kk) * 10 + cb) + q);
         }
-        for (int n = 0; n < NNOD; n++) {
-            const int gn = ce[n];
+        // A: vectors 4 vb + q of the gathered X_e at inner index 4 kp + kk
+        float av[4][2];
 #pragma unroll
-            for (int r = 0; r < NEQ; r++) {
-                v4f_t rv[GPR];
+        for (int kk = 0; kk < 4; kk++) {
+            const int k = 4 * kp + kk;
+            const int m = k / NEQ, d = k - NEQ * (k / NEQ);
+            const float* xp = X + ((size_t)cn[m] * NEQ + d) * NV + q;
 #pragma unroll
-                for (int c = 0; c < GPR; c++) rv[c] = __builtin_nontemporal_load(Rb + (size_t)((n * NEQ + r) * GPR + c) * 64);
-                float acc = 0.0f;
-#pragma unroll
-                for (int c = 0; c < GPR; c++)
-#pragma unroll
-                    for (int w = 0; w < V; w++) acc += rv[c][w] * xv[c * V + w];
-                float* yp = Y + ((size_t)gn * NEQ + r) * NV + v;
-                *yp += acc;
-            }
-        }
-    } else {
-        float yv[K];
-#pragma unroll
-        for (int j = 0; j < K; j++) yv[j] = 0.0f;
-        for (int n = 0; n < NNOD; n++) {
-            const int gn = ce[n];
-#pragma unroll
-            for (int r = 0; r < NEQ; r++) {
-                const float xi = X[((size_t)gn * NEQ + r) * NV + v];
-#pragma unroll
-                for (int c = 0; c < GPR; c++) {
-                    const v4f_t rv = __builtin_nontemporal_load(Rb + (size_t)((n * NEQ + r) * GPR + c) * 64);
-#pragma unroll
-                    for (int w = 0; w < V; w++) yv[c * V + w] += rv[w] * xi;
-                }
-            }
+            for (int vb = 0; vb < 2; vb++) av[kk][vb] = live ? xp[4 * vb] : 0.0f;
         }
 #pragma unroll
-        for (int m = 0; m < NNOD; m++) {
-            const int gm = ce[m];
+        for (int kk = 0; kk < 4; kk++)
 #pragma unroll
-            for (int q = 0; q < NEQ; q++) Y[((size_t)gm * NEQ + q) * NV + v] += yv[m * NEQ + q];
-        }
+            for (int cb = 0; cb < 10; cb++)
+#pragma unroll
+                for (int vb = 0; vb < 2; vb++)
+                    acc[cb][vb] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[kk][vb], bv[cb][kk], acc[cb][vb], 0, 0, 0);
+    }
+    if (!live) return;
+    // lane q holds row 4 cb + q of Y_e, vectors 4 vb .. 4 vb + 3
+#pragma unroll
+    for (int cb = 0; cb < 10; cb++) {
+        const int r = 4 * cb + q;
+        const int m = r / NEQ, d = r - NEQ * (r / NEQ);
+        v4f_t* yp = reinterpret_cast<v4f_t*>(Y + ((size_t)cn[m] * NEQ + d) * NV);
+#pragma unroll
+        for (int vb = 0; vb < 2; vb++) yp[vb] += acc[cb][vb];
     }
 }
 
 void launch_matvec_multi_color(const MatvecArgs& a, const float* R32, const float* X, float* Y, bool tr,
                                hipStream_t st) {
-    constexpr int NV = 8;
-    const long n = (a.e_end - a.e_begin) * NV;
+    // 64 elements (four wavefronts of 16) per workgroup; color ranges start
+    // on a multiple of 64
+    const long n = a.e_end - a.e_begin;
     if (n <= 0) return;
-    const int bs = 256;
-    const int nb = (int)((n + bs - 1) / bs);
+    const int nb = (int)((n + 63) / 64);
     if (tr)
-        hipLaunchKernelGGL((matvec_multi_kernel<NV, true>), dim3(nb), dim3(bs), 0, st, a.e_begin, a.e_end, a.conn, R32, X, Y);
+        hipLaunchKernelGGL((matvec_multi_kernel<true>), dim3(nb), dim3(256), 0, st, a.e_begin, a.e_end, a.conn, R32, X, Y);
     else
-        hipLaunchKernelGGL((matvec_multi_kernel<NV, false>), dim3(nb), dim3(bs), 0, st, a.e_begin, a.e_end, a.conn, R32, X, Y);
+        hipLaunchKernelGGL((matvec_multi_kernel<false>), dim3(nb), dim3(256), 0, st, a.e_begin, a.e_end, a.conn, R32, X, Y);
 }
 
 // ---------------------------------------------------------------------------
