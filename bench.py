@@ -52,10 +52,13 @@ REFINE_NODE_TOL = 1e-9
 # on MI355X, each scaled by the counter's reading for a plain 1 GiB copy
 # kernel (fetch x2.0, write x1.0); summaries and commands in
 # profiles/ebe_matvec_ramp_256x128x64_hbm_counters.txt (EBE, 8 color
-# launches) and profiles/rows_matvec_ramp_256x128x64_hbm_counters.txt (node
-# rows, one launch).  Only the default workload size has a measurement.
-MEASURED_TRAFFIC = {("ebe", "256x128x64"): {"fp32": 14.607e9, "fp64": 29.121e9},
-                    ("rows", "256x128x64"): {"fp32": 6.342e9, "fp64": 12.502e9}}
+# launches), profiles/rows_matvec_ramp_256x128x64_hbm_counters.txt (node
+# rows, one launch) and profiles/rows_matvec_ramp_refined_4390912_hbm_counters.txt
+# (node rows of the ramp after two sensor refinement passes).  Keyed by
+# (storage, base size, refinement levels); other workloads have none.
+MEASURED_TRAFFIC = {("ebe", "256x128x64", 0): {"fp32": 14.607e9, "fp64": 29.121e9},
+                    ("rows", "256x128x64", 0): {"fp32": 6.342e9, "fp64": 12.502e9},
+                    ("rows", "256x128x64", 2): {"fp32": 13.900e9, "fp64": 27.664e9}}
 
 
 def log(*a):
@@ -308,7 +311,7 @@ def main():
                              f"{ne} elements with {mv_per_step:.0f} FP64 products per step (the FP64 GPU "
                              f"path's count), at {main_run['threads']} threads")}}
 
-    traffic = MEASURED_TRAFFIC.get((args.storage, args.size), {}) if refine_info is None else {}
+    traffic = MEASURED_TRAFFIC.get((args.storage, args.size, args.refine_levels), {})
     wl = (f"3D hypersonic 15-deg compression ramp hex8 {nx}x{ny}x{nz} ({base_elem} elements, "
           f"{base_node} nodes)")
     if refine_info is not None:
