@@ -41,7 +41,8 @@ EXPORTS = ["cfdv_create", "cfdv_destroy", "cfdv_last_error", "cfdv_mesh_info", "
            "cfdv_get_state", "cfdv_cfl", "cfdv_build", "cfdv_get_blocks", "cfdv_get_blocks32",
            "cfdv_get_rhs_vector", "cfdv_solve", "cfdv_get_du", "cfdv_update", "cfdv_rms",
            "cfdv_refine_flags", "cfdv_color", "cfdv_matvec", "cfdv_profile_matvec", "cfdv_step",
-           "cfdv_set_thermo", "cfdv_adjoint_solve", "cfdv_build_vjp", "cfdv_matvec_multi",
+           "cfdv_set_thermo", "cfdv_adjoint_solve", "cfdv_adjoint_solve_multi", "cfdv_build_vjp",
+           "cfdv_matvec_multi",
            "cfdv_profile_matvec_multi", "cfdv_refine_create", "cfdv_refine_destroy",
            "cfdv_refine_last_error", "cfdv_refine_list", "cfdv_refine_apply", "cfdv_refine_size",
            "cfdv_refine_get", "cfdv_reference_trace", "cfdv_refine_flags_noalloc", "cfdv_rows_info"]
@@ -126,6 +127,7 @@ def load() -> C.CDLL:
     L.cfdv_step.argtypes = [vp, C.c_double, C.POINTER(SolveOpts), C.POINTER(StepStats)]
     L.cfdv_set_thermo.argtypes = [vp, C.POINTER(ThermoS)]
     L.cfdv_adjoint_solve.argtypes = [vp, C.POINTER(SolveOpts), dp, dp, C.POINTER(SolveStats)]
+    L.cfdv_adjoint_solve_multi.argtypes = [vp, C.POINTER(SolveOpts), dp, dp, C.c_int, dp, C.POINTER(SolveStats)]
     L.cfdv_build_vjp.argtypes = [vp, dp, dp, dp, dp]
     fp = C.POINTER(C.c_float)
     L.cfdv_matvec_multi.argtypes = [vp, fp, fp, C.c_int, C.c_int]

@@ -89,6 +89,47 @@ def multi_step_gradient(ctx: FDVContext, U0, dts, g=None, mode="fp64", rtol=1e-1
     return gk.reshape(n, -1), gM, U
 
 
+def multi_step_gradient_multi(ctx: FDVContext, U0, dts, functionals, rtol=1e-12, precond="nodeblock",
+                              inner_rtol=1e-4, progress=None):
+    """Gradients of up to 8 functionals after N = len(dts) steps at once:
+    the forward sweep as multi_step_gradient (FP64 steps, states kept), the
+    reverse sweep with one 8-right-hand-side adjoint solve per step
+    (cfdv_adjoint_solve_multi: FP64 refinement per functional around one
+    FP32 global GMRES whose Arnoldi steps are 8-vector products K^T X) and
+    one build tangent per functional.  Returns (list of dJ_s/dU0
+    [n_node][neqn], list of dJ_s/dM, U_N, list of J_s)."""
+    nf = len(functionals)
+    if not 1 <= nf <= 8:
+        raise ValueError("1 to 8 functionals")
+    n = ctx.mesh.n_node
+    states = []
+    U = np.asarray(U0, dtype=np.float64).reshape(n, -1)
+    for k, dt in enumerate(dts):
+        states.append(U)
+        U1, _ = step_fixed_dt(ctx, U, dt, mode="fp64", rtol=rtol, precond=precond)
+        U = U1.reshape(n, -1)
+        if progress:
+            progress("forward", k)
+    G = np.zeros((8, ctx.ndof))
+    gM = [0.0] * nf
+    Js = []
+    for s, F in enumerate(functionals):
+        J, gU_N, gMs = F.value_and_grad(U, ctx.thermo.mach)
+        Js.append(J)
+        G[s] = gU_N.ravel()
+        gM[s] = gMs
+    for k, (Un, dt) in enumerate(zip(reversed(states), reversed(list(dts)))):
+        _, du = step_fixed_dt(ctx, Un, dt, mode="fp64", rtol=rtol, precond=precond)
+        lam, _, _ = ctx.adjoint_solve_multi(G, rtol=rtol, inner_rtol=inner_rtol, precond=precond)
+        for s in range(nf):
+            gU, gMn = ctx.build_vjp(lam[s], du)
+            G[s] = G[s] + gU.ravel()
+            gM[s] += gMn
+        if progress:
+            progress("reverse", len(dts) - 1 - k)
+    return [G[s].reshape(n, -1) for s in range(nf)], gM, U, Js
+
+
 def functional_after_steps(ctx: FDVContext, U0, dts, functional, mach=None, mode="fp64", rtol=1e-12,
                            precond="nodeblock", progress=None):
     """functional(U_N, M) after N steps at fixed dt's from U0, with the Thermo

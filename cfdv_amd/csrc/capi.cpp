@@ -164,6 +164,12 @@ struct cfdv_ctx {
     bool rows_tr = false;     // the rows hold K^T (assembled for K^T x)
     // discrete adjoint: per-slot tangents [n_pad][41], host-given vectors
     double *d_gelem = nullptr, *d_lam = nullptr, *d_duv = nullptr, *d_gU = nullptr, *d_g = nullptr;
+    // eight adjoint right-hand sides at once: FP32 basis of side-by-side
+    // vectors [8 ndof] (vcap8 slots), FP64 correction and work, the
+    // right-hand sides, solutions and residuals vector-major [8][ndof]
+    float *d_V8 = nullptr, *d_z8 = nullptr;
+    int vcap8 = 0;
+    double *d_x8 = nullptr, *d_t8 = nullptr, *d_G8 = nullptr, *d_L8 = nullptr, *d_R8 = nullptr;
     double last_dt = 0.0;
     // warm start: d_du keeps the last update's increment, built with prev_dt;
     // with warm_start 2 or 3, d_old[k] the k+1-th before it (old_dt[k];
@@ -342,6 +348,25 @@ struct cfdv_ctx {
         }
         d_V32 = alloc<float>((size_t)(m + 1) * ldv());
         vcap32 = m + 1;
+    }
+    long ldv8() const { return (8 * ndof + 63) & ~63L; }
+    void ensure_multi(int m) {
+        if (!d_x8) {
+            d_x8 = alloc<double>((size_t)8 * ndof);
+            d_t8 = alloc<double>((size_t)8 * ndof);
+            d_z8 = alloc<float>((size_t)8 * ndof);
+            d_G8 = alloc<double>((size_t)8 * ndof);
+            d_L8 = alloc<double>((size_t)8 * ndof);
+            d_R8 = alloc<double>((size_t)8 * ndof);
+        }
+        if (vcap8 >= m + 1) return;
+        if (d_V8) {
+            auto it = std::find(allocs.begin(), allocs.end(), (void*)d_V8);
+            if (it != allocs.end()) allocs.erase(it);
+            (void)hipFree(d_V8);
+        }
+        d_V8 = alloc<float>((size_t)(m + 1) * ldv8());
+        vcap8 = m + 1;
     }
     // node-block inverses from the current blocks (FP64 if held, else FP32)
     void ensure_precond() {
@@ -1162,6 +1187,170 @@ int solve_mixed(cfdv_ctx* c, const cfdv_solve_opts* o, cfdv_solve_stats* s) {
     return rc;
 }
 
+// The 8-vector product K X / K^T X on side-by-side FP32 vectors
+void matvec8(cfdv_ctx* c, const float* X, float* Y, bool tr) {
+    if (c->d_B32) {
+        c->orient_rows(tr);
+        launch_rows_matvec8(c->ra, c->d_B32, X, Y, c->st);
+    } else {
+        vec_fill_f(Y, 0.0f, c->ndof * 8, c->st);
+        for (int k = 0; k < c->ncolor; k++) {
+            MatvecArgs a{c->ndim, c->nnod, c->neqn, c->cstart[k], c->cstart[k + 1], c->d_conn, 0};
+            launch_matvec_multi_color(a, c->d_R32, X, Y, tr, c->st);
+        }
+    }
+    c->check_launch();
+}
+
+// One cycle of global GMRES (Jbilou, Messaoudi and Sadok, 1999) for the 8
+// systems K^T u_s = v_s at once: the Krylov vectors are the 8 systems'
+// columns side by side, inner products and norms are Frobenius ones over
+// all of them, so every Arnoldi step is one 8-vector product and one
+// Hessenberg column (one polynomial in K^T for all 8).  rhs8 in V_0 on
+// entry ([8 ndof] FP32, Frobenius norm beta); x8 += Pinv^T V y in FP64.
+// Right-preconditioned by node-block Jacobi (Pinv) like the single solve;
+// stops when the Givens estimate is at or below tol_abs.  The Hessenberg
+// column, its rotations and the back substitution run on the device; the
+// host reads each step's record before the next.
+int gmres_global8(cfdv_ctx* c, bool tr, double* x8, int m, double tol_abs, double beta, cfdv_solve_stats* s,
+                  bool* conv, const float* Pinv) {
+    const long n = 8 * c->ndof;
+    const long ldv = c->ldv8();
+    float* V = c->d_V8;
+    float* w = V + (size_t)m * ldv;   // the last slot is the Arnoldi work vector
+    const int ldh = m + 1;
+    double* H = c->d_hess;
+    double* g = H + (size_t)ldh * m;
+    double* cs = g + ldh;
+    double* sn = cs + m;
+    double* y = c->d_small + 16;
+    *conv = false;
+    launch_gmres_start(g, m, beta, c->d_gate, c->st);
+    int jdone = 0;
+    for (int j = 0; j < m - 1; j++) {
+        float* vj = V + (size_t)j * ldv;
+        if (Pinv) {
+            vec_nodeblock_apply8<float, float>(Pinv, vj, c->d_z8, c->neqn, c->n_node, false, tr, c->st);
+            matvec8(c, c->d_z8, w, tr);
+        } else {
+            matvec8(c, vj, w, tr);
+        }
+        s->matvecs32++;
+        const int nv = j + 1;
+        // classical Gram-Schmidt, one pass (h = V^T w; w -= V h, |w|^2):
+        // the FP64 refinement around it checks and corrects
+        if (nv <= 32) {
+            vec_cgs<float>(0, w, V, ldv, nv, nullptr, n, c->d_part, NBLK_RED, c->d_hcol, c->st);
+            vec_cgs<float>(2, w, V, ldv, nv, c->d_hcol, n, c->d_part, NBLK_RED, c->d_hcol + 2040, c->st);
+        } else {
+            vec_multidot<float>(V, ldv, nv, w, n, c->d_part, c->d_hcol, NBLK_RED, c->st);
+            vec_multi_axpy<float>(w, V, ldv, nv, c->d_hcol, 1.0, n, c->st);
+            vec_multidot<float>(w, ldv, 1, w, n, c->d_part, c->d_hcol + 2040, NBLK_RED, c->st);
+        }
+        launch_givens_step(H, ldh, g, cs, sn, c->d_hcol, nullptr, c->d_hcol + 2040, j, tol_abs, c->d_small + 9,
+                           c->d_gate, c->d_steps, c->st);
+        vec_scale_inv<float>(V + (size_t)(j + 1) * ldv, w, c->d_small + 9, n, c->st, c->d_gate);
+        c->check_launch();
+        HIP_CHECK_THROW(hipEventRecord(c->ev_step[0], c->st));
+        cfdv_ctx::wait_event(c->ev_step[0]);
+        s->iters++;
+        jdone = j + 1;
+        const GmresStep st = c->h_steps[j];
+        s->est_resid = st.est;
+        if (st.fail) s->breakdown = 1;
+        if (st.stop) break;
+    }
+    launch_backsub(H, ldh, g, jdone, y, c->st);
+    if (Pinv) {
+        vec_fill(c->d_t8, 0.0, n, c->st);
+        vec_combine<float>(c->d_t8, V, ldv, jdone, y, n, c->st);
+        vec_nodeblock_apply8<double, float>(Pinv, c->d_t8, x8, c->neqn, c->n_node, true, tr, c->st);
+    } else {
+        vec_combine<float>(x8, V, ldv, jdone, y, n, c->st);
+    }
+    c->check_launch();
+    *conv = s->est_resid <= tol_abs;
+    return jdone;
+}
+
+// K^T lambda_s = g_s (or K lambda_s = g_s) for 8 right-hand sides at once:
+// per right-hand side FP64 iterative refinement (r_s = g_s - K64^T lambda_s,
+// lambda_s += ||r_s|| d_s) around one FP32 global GMRES cycle per sweep on
+// the normalised residuals r_s / ||r_s|| of the systems not yet converged
+// (converged ones enter as zero columns).  Ends when every relative residual
+// is at or below o->rtol, or after o->restarts sweeps.  G, L: device [8][ndof].
+int solve_multi8(cfdv_ctx* c, const cfdv_solve_opts* o, bool tr, const double* G, double* L, double rel[8],
+                 cfdv_solve_stats* s) {
+    const long n = c->ndof;
+    if (!c->have_blocks(32, tr))
+        return fail(c, CFDV_ERR_ARG, "the 8-vector solve needs CFDV_BLOCKS_FP32 or CFDV_BLOCKS_ROWS");
+    const int outer_prec = c->have_blocks(64, tr) ? 64 : 3;
+    const int m = std::min(o->inner_m, MAX_M1 - 1);
+    c->ensure_multi(m);
+    const bool pc = o->precond == CFDV_PRECOND_NODE_BLOCK;
+    if (pc) c->ensure_precond();
+    double gn[8], rn[8];
+    for (int k = 0; k < 8; k++) {
+        gn[k] = c->norm(G + (size_t)k * n);
+        rn[k] = gn[k];
+    }
+    s->bnorm = 0.0;
+    for (int k = 0; k < 8; k++) s->bnorm = std::max(s->bnorm, gn[k]);
+    vec_fill(L, 0.0, 8 * n, c->st);
+    vec_copy(c->d_R8, G, 8 * n, c->st);
+    int sweeps = 0;
+    for (;;) {
+        Scale8 inv{}, amp{};
+        int active = 0;
+        bool finite = true;
+        for (int k = 0; k < 8; k++) {
+            finite = finite && std::isfinite(rn[k]);
+            if (gn[k] > 0 && rn[k] > o->rtol * gn[k]) {
+                inv.v[k] = 1.0 / rn[k];
+                amp.v[k] = rn[k];
+                active++;
+            }
+        }
+        if (!finite || active == 0 || sweeps >= o->restarts) break;
+        // the inner right-hand side: the active residuals, each of unit norm,
+        // side by side (Frobenius norm beta); V_0 is it over beta
+        const double beta = std::sqrt((double)active);
+        for (int k = 0; k < 8; k++) inv.v[k] /= beta;
+        vec_multivec_pack(c->d_V8, c->d_R8, inv, n, c->st);
+        vec_fill(c->d_x8, 0.0, 8 * n, c->st);
+        // reduce the Frobenius residual by inner_rtol (each column then by at
+        // most sqrt(active) inner_rtol)
+        bool conv = false;
+        cfdv_solve_stats si{};
+        gmres_global8(c, tr, c->d_x8, m, o->inner_rtol * beta, beta, &si, &conv, pc ? c->d_P32 : nullptr);
+        s->iters += si.iters;
+        s->matvecs32 += si.matvecs32;
+        if (si.breakdown) s->breakdown = 1;
+        // lambda_s += ||r_s|| d_s, r_s = g_s - K lambda_s in FP64
+        vec_multivec_unpack_axpy(L, c->d_x8, amp, n, c->st);
+        for (int k = 0; k < 8; k++) {
+            if (amp.v[k] == 0.0) continue;
+            c->matvec(L + (size_t)k * n, c->d_y, outer_prec, tr);
+            if (outer_prec == 64) s->matvecs64++; else s->matvecs32++;
+            vec_sub(c->d_R8 + (size_t)k * n, G + (size_t)k * n, c->d_y, n, c->st);
+            rn[k] = c->norm(c->d_R8 + (size_t)k * n);
+        }
+        sweeps++;
+    }
+    s->restarts = sweeps;
+    double worst = 0.0;
+    for (int k = 0; k < 8; k++) {
+        rel[k] = gn[k] > 0 ? rn[k] / gn[k] : 0.0;
+        if (!std::isfinite(rel[k])) return fail(c, CFDV_ERR_NAN, "NaN in the 8-vector solve");
+        worst = std::max(worst, rel[k]);
+    }
+    s->rel_resid = worst;
+    s->resid = 0.0;
+    for (int k = 0; k < 8; k++) s->resid = std::max(s->resid, rn[k]);
+    if (worst > o->rtol) return fail(c, CFDV_ERR_NOCONV, "8-vector refinement did not converge");
+    return CFDV_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1645,6 +1834,29 @@ int cfdv_adjoint_solve(cfdv_ctx* ctx, const cfdv_solve_opts* o, const double* g,
     const int rc = (o->mode == CFDV_SOLVE_FP64) ? solve_fp64_rhs(ctx, &ot, s, ctx->d_g, ctx->d_lam)
                                                 : solve_mixed_rhs(ctx, &ot, s, ctx->d_g, ctx->d_lam);
     HIP_CHECK_THROW(hipMemcpyAsync(lambda, ctx->d_lam, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    ctx->sync();
+    return rc;
+    GUARD_END
+}
+
+int cfdv_adjoint_solve_multi(cfdv_ctx* ctx, const cfdv_solve_opts* o, const double* G, double* Lambda, int nvec,
+                             double* rel_resid, cfdv_solve_stats* s) {
+    if (!ctx || !o || !G || !Lambda || !rel_resid || !s) return CFDV_ERR_ARG;
+    if (nvec != 8) return fail(ctx, CFDV_ERR_ARG, "cfdv_adjoint_solve_multi takes 8 right-hand sides");
+    if (!ctx->built) return fail(ctx, CFDV_ERR_ARG, "cfdv_adjoint_solve_multi before cfdv_build");
+    if (ctx->ndim != 3) return fail(ctx, CFDV_ERR_ARG, "the 8-vector product is implemented for hex8 meshes");
+    if (o->mode != CFDV_SOLVE_MIXED) return fail(ctx, CFDV_ERR_ARG, "the 8-vector solve is the mixed solver");
+    if (o->restarts < 1 || o->inner_m < 2 || o->inner_m > MAX_M1 - 1 || !(o->rtol > 0) || !(o->inner_rtol > 0))
+        return fail(ctx, CFDV_ERR_ARG, "bad sweeps, inner Krylov dimension or tolerances");
+    if (o->precond != CFDV_PRECOND_NONE && o->precond != CFDV_PRECOND_NODE_BLOCK)
+        return fail(ctx, CFDV_ERR_ARG, "unknown preconditioner");
+    std::memset(s, 0, sizeof(*s));
+    GUARD_BEGIN
+    ctx->ensure_multi(std::min(o->inner_m, MAX_M1 - 1));
+    const size_t n8 = (size_t)8 * ctx->ndof;
+    HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_G8, G, sizeof(double) * n8, hipMemcpyHostToDevice, ctx->st));
+    const int rc = solve_multi8(ctx, o, true, ctx->d_G8, ctx->d_L8, rel_resid, s);
+    HIP_CHECK_THROW(hipMemcpyAsync(Lambda, ctx->d_L8, sizeof(double) * n8, hipMemcpyDeviceToHost, ctx->st));
     ctx->sync();
     return rc;
     GUARD_END

@@ -204,6 +204,17 @@ void launch_rows_assemble(const RowsArgs& a, const double* R64, bool tr, double*
 void launch_nodeblock_rows(const RowsArgs& a, const long* diag, const double* B64, bool tr, double* P64, float* P32,
                            int* status, hipStream_t st);
 
+// ---- eight vectors side by side ([n][8]) (solve.hip) ----
+struct Scale8 { double v[8]; };
+// V[i][s] = (float)(R[s][i] scale[s]), R vector-major [8][n]
+void vec_multivec_pack(float* V, const double* R, const Scale8& scale, long n, hipStream_t st);
+// L[s][i] += a[s] X[i][s]
+void vec_multivec_unpack_axpy(double* L, const double* X, const Scale8& a, long n, hipStream_t st);
+// node-block Jacobi on each of the 8 vectors of v [n_node * neqn][8]
+template <typename TV, typename TP>
+void vec_nodeblock_apply8(const TP* P, const TV* v, TV* z, int neqn, long n_node, bool accumulate, bool trans,
+                          hipStream_t st);
+
 // ---- coloring (solve.hip) ----
 void launch_color_round(const int* conn, int nnod, const int* node_ptr, const int* node_elem, long n_elem,
                         const unsigned long long* prio, const int* color_in, int* color_out, int* changed,

@@ -928,6 +928,77 @@ template void vec_nodeblock_apply<float, float>(const float*, const float*, floa
 template void vec_nodeblock_apply<double, float>(const float*, const double*, double*, int, long, bool, bool,
                                                  hipStream_t, const int*);
 
+// ---------------------------------------------------------------------------
+// Eight vectors side by side ([n][8], the 8 values of entry i contiguous), as
+// the 8-vector product reads and writes them, and the node-block
+// preconditioner applied to each of them.
+// ---------------------------------------------------------------------------
+// V[i][s] = (float)(R[s][i] * scale[s]) for the vector-major FP64 R [8][n]
+__global__ void multivec_pack_kernel(float* __restrict__ V, const double* __restrict__ R, Scale8 scale, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        float v[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) v[s] = (float)(R[(size_t)s * n + i] * scale.v[s]);
+        v4f_t* o = reinterpret_cast<v4f_t*>(V + (size_t)i * 8);
+        o[0] = v4f_t{v[0], v[1], v[2], v[3]};
+        o[1] = v4f_t{v[4], v[5], v[6], v[7]};
+    }
+}
+void vec_multivec_pack(float* V, const double* R, const Scale8& scale, long n, hipStream_t st) {
+    hipLaunchKernelGGL(multivec_pack_kernel, dim3(grid_for(n)), dim3(256), 0, st, V, R, scale, n);
+}
+
+// L[s][i] += a[s] X[i][s] (X FP64, side by side; L vector-major)
+__global__ void multivec_unpack_axpy_kernel(double* __restrict__ L, const double* __restrict__ X, Scale8 a, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int s = 0; s < 8; s++) L[(size_t)s * n + i] += a.v[s] * X[(size_t)i * 8 + s];
+    }
+}
+void vec_multivec_unpack_axpy(double* L, const double* X, const Scale8& a, long n, hipStream_t st) {
+    hipLaunchKernelGGL(multivec_unpack_axpy_kernel, dim3(grid_for(n)), dim3(256), 0, st, L, X, a, n);
+}
+
+// z[n][r][s] = sum_c Pinv_n(r, c) v[n][c][s] (Pinv_n^T with trans; += with
+// accumulate), one thread per (node, vector); arithmetic in the type of z
+template <typename TV, typename TP, int NEQ>
+__global__ __launch_bounds__(256) void nodeblock_apply8_kernel(const TP* __restrict__ P, const TV* __restrict__ v,
+                                                               TV* __restrict__ z, long n_node, int accumulate,
+                                                               int trans) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n_node * 8; t += (long)gridDim.x * blockDim.x) {
+        const long n = t >> 3;
+        const int s = (int)(t & 7);
+        TV x[NEQ];
+#pragma unroll
+        for (int c = 0; c < NEQ; c++) x[c] = v[((size_t)n * NEQ + c) * 8 + s];
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) {
+            TV acc = accumulate ? z[((size_t)n * NEQ + r) * 8 + s] : (TV)0;
+#pragma unroll
+            for (int c = 0; c < NEQ; c++)
+                acc += (TV)P[pblk_offset<NEQ>(n, trans ? c * NEQ + r : r * NEQ + c)] * x[c];
+            z[((size_t)n * NEQ + r) * 8 + s] = acc;
+        }
+    }
+}
+template <typename TV, typename TP>
+void vec_nodeblock_apply8(const TP* P, const TV* v, TV* z, int neqn, long n_node, bool accumulate, bool trans,
+                          hipStream_t st) {
+    const int nb = grid_for(n_node * 8);
+    if (neqn == 5)
+        hipLaunchKernelGGL((nodeblock_apply8_kernel<TV, TP, 5>), dim3(nb), dim3(256), 0, st, P, v, z, n_node,
+                           (int)accumulate, (int)trans);
+    else
+        hipLaunchKernelGGL((nodeblock_apply8_kernel<TV, TP, 4>), dim3(nb), dim3(256), 0, st, P, v, z, n_node,
+                           (int)accumulate, (int)trans);
+}
+template void vec_nodeblock_apply8<float, float>(const float*, const float*, float*, int, long, bool, bool,
+                                                 hipStream_t);
+template void vec_nodeblock_apply8<double, float>(const float*, const double*, double*, int, long, bool, bool,
+                                                  hipStream_t);
+template void vec_nodeblock_apply8<double, double>(const double*, const double*, double*, int, long, bool, bool,
+                                                   hipStream_t);
+
 // per-equation sums of squares of a node array (MPI_RMSError.h:19-26)
 __global__ void sumsq_eqn_kernel(const double* __restrict__ U, int neqn, long n_node, double* __restrict__ part) {
     __shared__ double sm[5][256];

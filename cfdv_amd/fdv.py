@@ -269,6 +269,22 @@ class FDVContext:
         self._check(self.L.cfdv_adjoint_solve(self.h, C.byref(o), _dp(g), _dp(lam), C.byref(st)))
         return lam, st
 
+    def adjoint_solve_multi(self, G, restarts=20, rtol=1e-12, inner_m=30, inner_rtol=1e-4,
+                            precond="nodeblock"):
+        """Lambda [8][ndof] with K^T Lambda[s] = G[s] for 8 right-hand sides
+        at once (G [8][ndof]): FP64 refinement per right-hand side around a
+        FP32 global GMRES whose every Arnoldi step is one 8-vector product.
+        Returns (Lambda, rel_resid[8], stats)."""
+        G = np.ascontiguousarray(G, dtype=np.float64).reshape(8, self.ndof)
+        o = _lib.SolveOpts(_lib.CFDV_SOLVE_MIXED, restarts, 30, rtol, inner_m, inner_rtol, 1, 1,
+                           _PRECOND[precond])
+        lam = np.zeros((8, self.ndof))
+        rel = np.zeros(8)
+        st = _lib.SolveStats()
+        self._check(self.L.cfdv_adjoint_solve_multi(self.h, C.byref(o), _dp(G), _dp(lam), 8, _dp(rel),
+                                                    C.byref(st)))
+        return lam, rel, st
+
     def build_vjp(self, lam, du=None):
         """Gradient of Phi = lam^T (b - K du) with respect to the node state U
         and the Mach number, at the state and dt of the last build."""

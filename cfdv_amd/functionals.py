@@ -56,10 +56,11 @@ def _face_tables():
 
 class WallFunctional:
     """J(U, M) over the faces with bc code ``code``; ``kind`` "drag" or
-    "heat".  ``value_and_grad(U, mach)`` returns (J, dJ/dU [n_node][5],
-    dJ/dM)."""
+    "heat"; ``x_range`` = (x0, x1) keeps the faces whose element centroid
+    has x0 <= x < x1 (a streamwise patch of the wall).
+    ``value_and_grad(U, mach)`` returns (J, dJ/dU [n_node][5], dJ/dM)."""
 
-    def __init__(self, mesh, thermo, code, kind="drag"):
+    def __init__(self, mesh, thermo, code, kind="drag", x_range=None):
         import torch
         if mesh.ndim != 3:
             raise ValueError("wall functionals are hex8 only")
@@ -68,6 +69,10 @@ class WallFunctional:
         self.kind = kind
         self.th = thermo
         e, f = np.nonzero(np.asarray(mesh.bc) == code)
+        if x_range is not None:
+            xc = np.asarray(mesh.x)[np.asarray(mesh.conn)[e], 0].mean(axis=1)
+            keep = (xc >= x_range[0]) & (xc < x_range[1])
+            e, f = e[keep], f[keep]
         if len(e) == 0:
             raise ValueError(f"no face with bc code {code}")
         self.elems, self.faces = e, f

@@ -34,6 +34,8 @@ def main(argv=None):
     p.add_argument("--storage", choices=["ebe", "rows"], default="ebe")
     p.add_argument("--refine-levels", type=int, default=0)
     p.add_argument("--no-multi", action="store_true", help="skip the 8-vector product")
+    p.add_argument("--adjoint8", action="store_true",
+                   help="also time 8 adjoint solves at once against 8 single ones (rtol 1e-10)")
     a = p.parse_args(argv)
     nx, ny, nz = (int(v) for v in a.size.split("x"))
     out = {"size": a.size}
@@ -89,6 +91,24 @@ def main(argv=None):
     out["matvec32x8_bytes"] = b8
     out["matvec32x8_GBps"] = b8 / (ms8 * 1e-3) / 1e9
     out["matvec32x8_speedup_vs_8_single"] = 8 * out.get("matvec32_ms", ms8) / ms8
+    if a.adjoint8:
+        import numpy as np
+        G = np.random.default_rng(3).standard_normal((8, ctx.ndof))
+        ctx.adjoint_solve_multi(G[:, :], rtol=1e-10)   # warm
+        t0 = time.perf_counter()
+        _, rel, st = ctx.adjoint_solve_multi(G, rtol=1e-10)
+        out["adjoint8_s"] = time.perf_counter() - t0
+        out["adjoint8_products"] = {"x8_fp32": st.matvecs32, "fp64": st.matvecs64, "sweeps": st.restarts}
+        out["adjoint8_max_rel"] = float(rel.max())
+        for mode in ("mixed", "fp64"):
+            t0 = time.perf_counter()
+            n32 = n64 = 0
+            for s in range(8):
+                _, st1 = ctx.adjoint_solve(G[s], mode=mode, rtol=1e-10, restarts=200 if mode == "fp64" else 20)
+                n32 += st1.matvecs32
+                n64 += st1.matvecs64
+            out[f"adjoint_single_{mode}_x8_s"] = time.perf_counter() - t0
+            out[f"adjoint_single_{mode}_x8_products"] = {"fp32": n32, "fp64": n64}
     out["elements"] = ne
     out["nodes"] = nn
     out["colors"] = ctx.info()["ncolors"]

@@ -44,7 +44,8 @@
  *                        (main.cpp:143-151); cfdv_refine_create/_get stand for
  *                        the element and node vectors those mutate
  *   cfdv_set_thermo      Thermo::set (FDV/Thermo.h:22-33) after create
- *   cfdv_adjoint_solve, cfdv_build_vjp: no reference counterpart (discrete
+ *   cfdv_adjoint_solve, cfdv_adjoint_solve_multi,
+ *   cfdv_build_vjp:      no reference counterpart (discrete
  *                        adjoint of one step: transpose solve, and the
  *                        forward-mode tangents of the element build).
  */
@@ -261,6 +262,22 @@ int cfdv_set_thermo(cfdv_ctx *ctx, const cfdv_thermo *thermo);
    host arrays [n_node*neqn] */
 int cfdv_adjoint_solve(cfdv_ctx *ctx, const cfdv_solve_opts *opts, const double *g, double *lambda,
                        cfdv_solve_stats *st);
+/* K^T lambda_s = g_s for 8 right-hand sides at once (no reference
+   counterpart; the adjoints of 8 functionals of one step).  Mixed
+   precision: per right-hand side FP64 refinement r_s = g_s - K64^T lambda_s,
+   lambda_s += ||r_s|| d_s, around one FP32 global GMRES cycle per sweep
+   (Jbilou, Messaoudi and Sadok, 1999) whose Krylov vectors hold the 8
+   normalised residuals side by side, so that each Arnoldi step is one
+   8-vector product K^T X (cfdv_matvec_multi).  G, Lambda: host arrays
+   [8][n_node*neqn] (right-hand side s contiguous at s*n_node*neqn);
+   rel_resid[8] receives ||g_s - K^T lambda_s|| / ||g_s||.  opts: mode
+   CFDV_SOLVE_MIXED, rtol per right-hand side, inner_m (2 to 500),
+   inner_rtol for the Frobenius norm of the inner residual, restarts = the
+   most refinement sweeps, precond; st: summed counts, rel_resid the
+   largest.  CFDV_ERR_NOCONV when a right-hand side stays above rtol.
+   hex8 meshes. */
+int cfdv_adjoint_solve_multi(cfdv_ctx *ctx, const cfdv_solve_opts *opts, const double *G, double *Lambda,
+                             int nvec, double *rel_resid, cfdv_solve_stats *st);
 /* gradient of Phi = lambda^T (b - K du) with respect to the node state U
    (gU [n_node][neqn]) and the Thermo Mach number (*gmach), at the state and
    dt of the last cfdv_build, lambda and du held fixed (du NULL: the last

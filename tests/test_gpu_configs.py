@@ -178,6 +178,27 @@ def test_c4_cylinder_drag_gradient(gpu_available):
               f"{' / '.join(f'{fd:.10e}' for fd in fds)}, rel err {err:.2e}", flush=True)
     ctx.set_thermo(th)
     assert worst <= 1e-5
+    # the adjoints of 8 functionals of the last state at once -- the wall
+    # heat flux and drag over four streamwise patches of the wall -- through
+    # the global GMRES on 8-vector products, against single FP64 adjoints
+    e, _ = np.nonzero(np.asarray(mesh.bc) == 1)
+    xc = np.asarray(mesh.x)[np.asarray(mesh.conn)[e], 0].mean(axis=1)
+    edges = np.linspace(xc.min(), xc.max() + 1e-9, 5)
+    funcs = [WallFunctional(mesh, th, 1, kind, x_range=(edges[p], edges[p + 1]))
+             for p in range(4) for kind in ("heat", "drag")]
+    ctx.set_state(U0)
+    ctx.build(dt)
+    G = np.stack([f.value_and_grad(U0, th.mach)[1].ravel() for f in funcs])
+    t1 = time.time()
+    lam, rel, st = ctx.adjoint_solve_multi(G, rtol=1e-10)
+    print(f"C5: 8 adjoints at once in {time.time() - t1:.1f} s, {st.matvecs32} 8-vector products, "
+          f"{st.matvecs64} FP64 products; rel residuals {np.array2string(rel, precision=1)}", flush=True)
+    assert np.all(rel <= 1e-10)
+    for k in (0, 5):
+        one, _ = ctx.adjoint_solve(G[k], mode="fp64", rtol=1e-10)
+        d = np.linalg.norm(lam[k] - one) / np.linalg.norm(one)
+        print(f"C5: functional {k} adjoint vs the single FP64 solve: rel diff {d:.1e}", flush=True)
+        assert d <= 1e-7
 
 
 def test_c5_plate_eight_vectors_and_heat_flux_adjoint(gpu_available):
@@ -226,3 +247,24 @@ def test_c5_plate_eight_vectors_and_heat_flux_adjoint(gpu_available):
               f"{' / '.join(f'{fd:.10e}' for fd in fds)}, rel err {err:.2e}", flush=True)
     ctx.set_thermo(th)
     assert worst <= 1e-5
+    # the adjoints of 8 functionals of the last state at once -- the wall
+    # heat flux and drag over four streamwise patches of the wall -- through
+    # the global GMRES on 8-vector products, against single FP64 adjoints
+    e, _ = np.nonzero(np.asarray(mesh.bc) == 1)
+    xc = np.asarray(mesh.x)[np.asarray(mesh.conn)[e], 0].mean(axis=1)
+    edges = np.linspace(xc.min(), xc.max() + 1e-9, 5)
+    funcs = [WallFunctional(mesh, th, 1, kind, x_range=(edges[p], edges[p + 1]))
+             for p in range(4) for kind in ("heat", "drag")]
+    ctx.set_state(U0)
+    ctx.build(dt)
+    G = np.stack([f.value_and_grad(U0, th.mach)[1].ravel() for f in funcs])
+    t1 = time.time()
+    lam, rel, st = ctx.adjoint_solve_multi(G, rtol=1e-10)
+    print(f"C5: 8 adjoints at once in {time.time() - t1:.1f} s, {st.matvecs32} 8-vector products, "
+          f"{st.matvecs64} FP64 products; rel residuals {np.array2string(rel, precision=1)}", flush=True)
+    assert np.all(rel <= 1e-10)
+    for k in (0, 5):
+        one, _ = ctx.adjoint_solve(G[k], mode="fp64", rtol=1e-10)
+        d = np.linalg.norm(lam[k] - one) / np.linalg.norm(one)
+        print(f"C5: functional {k} adjoint vs the single FP64 solve: rel diff {d:.1e}", flush=True)
+        assert d <= 1e-7
