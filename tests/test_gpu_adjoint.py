@@ -177,6 +177,24 @@ def test_torch_fdv_step_backward(gpu_available):
     assert abs(Mt.grad.item() - gM) <= 1e-10 * max(abs(gM), 1.0)
 
 
+def test_torch_gradcheck(gpu_available):
+    """torch.autograd.gradcheck of fdv_step in float64 on a tiny mesh: the
+    full Jacobian of U1 with respect to U0 and the Mach number from the
+    adjoint (one backward per output) against central differences of GPU
+    steps (one pair per input)."""
+    torch = pytest.importorskip("torch")
+    from cfdv_amd.adjoint import fdv_step
+    mesh, th, U0 = M.mixed_bc_box(2, 2, 2, seed=3)
+    ctx = FDVContext(mesh, th, U0)
+    dt = ctx.cfl(0.5)
+    Ut = torch.tensor(U0, dtype=torch.float64, requires_grad=True)
+    Mt = torch.tensor(th.mach, dtype=torch.float64, requires_grad=True)
+    ok = torch.autograd.gradcheck(lambda U, m: fdv_step(U, m, ctx, dt), (Ut, Mt), eps=1e-6, atol=1e-7,
+                                  rtol=1e-5, fast_mode=False)
+    ctx.set_thermo(th)
+    assert ok
+
+
 def test_build_vjp_on_hanging_node_elements(gpu_available):
     rng = np.random.default_rng(8)
     m, th, U = M.mixed_bc_box(4, 3, 3, seed=6)
