@@ -227,6 +227,13 @@ struct cfdv_ctx {
         allocs.push_back(p);
         return p;
     }
+    // frees a buffer from alloc() before the context ends
+    void release(void* p) {
+        if (!p) return;
+        auto it = std::find(allocs.begin(), allocs.end(), p);
+        if (it != allocs.end()) allocs.erase(it);
+        (void)hipFree(p);
+    }
     // wait for the stream by polling an event: the solvers wait once per
     // Arnoldi step for a few scalars, and a blocking wait adds its wake-up
     // latency to every step
@@ -348,6 +355,62 @@ struct cfdv_ctx {
         }
         d_V32 = alloc<float>((size_t)(m + 1) * ldv());
         vcap32 = m + 1;
+    }
+    // internal node order (contexts with node rows): internal node i is the
+    // caller's node norder[i] (d_norder), caller node j is internal node
+    // d_npos[j]; node arrays cross the C ABI in the caller's order
+    bool renum = false;
+    int *d_norder = nullptr, *d_npos = nullptr;
+    double* d_ptmp = nullptr;    // [ndof] staging of a permuted node array
+    float* d_ptmp32 = nullptr;   // [8 ndof] staging of a permuted 8-vector array
+    // host node array with q values per node into device memory (internal order)
+    void upload_nodes(double* dev, const double* host, int q) {
+        const size_t n = (size_t)n_node * q;
+        if (!renum) {
+            HIP_CHECK_THROW(hipMemcpyAsync(dev, host, sizeof(double) * n, hipMemcpyHostToDevice, st));
+            return;
+        }
+        if (!d_ptmp) d_ptmp = alloc<double>((size_t)ndof);
+        if (n > (size_t)ndof) throw std::invalid_argument("node array wider than the staging buffer");
+        HIP_CHECK_THROW(hipMemcpyAsync(d_ptmp, host, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        vec_gather_nodes<double>(dev, d_ptmp, d_norder, n_node, q, st);
+        check_launch();
+    }
+    void download_nodes(double* host, const double* dev, int q) {
+        const size_t n = (size_t)n_node * q;
+        if (!renum) {
+            HIP_CHECK_THROW(hipMemcpyAsync(host, dev, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            return;
+        }
+        if (!d_ptmp) d_ptmp = alloc<double>((size_t)ndof);
+        if (n > (size_t)ndof) throw std::invalid_argument("node array wider than the staging buffer");
+        vec_gather_nodes<double>(d_ptmp, dev, d_npos, n_node, q, st);
+        check_launch();
+        HIP_CHECK_THROW(hipMemcpyAsync(host, d_ptmp, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    }
+    void upload_nodes32(float* dev, const float* host, int q) {
+        const size_t n = (size_t)n_node * q;
+        if (!renum) {
+            HIP_CHECK_THROW(hipMemcpyAsync(dev, host, sizeof(float) * n, hipMemcpyHostToDevice, st));
+            return;
+        }
+        if (!d_ptmp32) d_ptmp32 = alloc<float>((size_t)8 * ndof);
+        if (n > (size_t)8 * ndof) throw std::invalid_argument("node array wider than the staging buffer");
+        HIP_CHECK_THROW(hipMemcpyAsync(d_ptmp32, host, sizeof(float) * n, hipMemcpyHostToDevice, st));
+        vec_gather_nodes<float>(dev, d_ptmp32, d_norder, n_node, q, st);
+        check_launch();
+    }
+    void download_nodes32(float* host, const float* dev, int q) {
+        const size_t n = (size_t)n_node * q;
+        if (!renum) {
+            HIP_CHECK_THROW(hipMemcpyAsync(host, dev, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+            return;
+        }
+        if (!d_ptmp32) d_ptmp32 = alloc<float>((size_t)8 * ndof);
+        if (n > (size_t)8 * ndof) throw std::invalid_argument("node array wider than the staging buffer");
+        vec_gather_nodes<float>(d_ptmp32, dev, d_npos, n_node, q, st);
+        check_launch();
+        HIP_CHECK_THROW(hipMemcpyAsync(host, d_ptmp32, sizeof(float) * n, hipMemcpyDeviceToHost, st));
     }
     long ldv8() const { return (8 * ndof + 63) & ~63L; }
     void ensure_multi(int m) {
@@ -542,6 +605,93 @@ RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* 
             }
         }
     return h;
+}
+
+// The internal node order of contexts with node rows.  A candidate order:
+// the nodes along a Morton curve of their coordinates (each axis quantized
+// by rank to 2^21 levels, so that stretched directions keep their
+// resolution), then within each window of 1024 nodes the ones with most
+// neighbours first, so that the 64 rows of a chunk have about as many slots
+// each.  It replaces the caller's order when it shrinks the vector footprint
+// of the rows that run at the same time: for windows of 128 chunks (8192
+// rows, what one XCD's 32 CUs hold at four waves each), the number of
+// distinct 64-node blocks of x that the rows' columns touch, summed over
+// every 8th window, by 5% or more.  A structured grid in its own x-fastest
+// numbering keeps it (2.1M hex8 ramp: 408 blocks per window against 446;
+// the FP32 product 0.72 of HBM in its order, 0.69 in the Morton one); the
+// 2-level refined ramp, whose refined nodes are numbered in the order the
+// refinement appends them, is renumbered (698 against 486 blocks; FP32
+// product 0.65 -> 0.75 of HBM, assembly 25.5 -> 22.9 ms).  Returns
+// order[i] = caller node of internal node i, or an empty vector to keep the
+// caller's order.
+std::vector<int> choose_node_order(int ndim, int nn, long nnode, long ne, const double* x, const int32_t* conn) {
+    const int bits = 21;
+    std::vector<uint64_t> key((size_t)nnode, 0);
+    std::vector<int> idx((size_t)nnode);
+    for (int a = 0; a < ndim; a++) {
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int p, int q) {
+            return x[(size_t)p * ndim + a] < x[(size_t)q * ndim + a];
+        });
+        for (long r = 0; r < nnode; r++) {
+            const uint64_t qv = (uint64_t)((double)r / (double)nnode * (double)(1u << bits));
+            uint64_t k = 0;
+            for (int b = 0; b < bits; b++) k |= ((qv >> b) & 1ULL) << (ndim * b + a);
+            key[idx[r]] |= k;
+        }
+    }
+    std::vector<int> order((size_t)nnode);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return key[p] < key[q]; });
+    // node -> element lists and the neighbours of a node (sorted, unique)
+    std::vector<int> ptr((size_t)nnode + 1, 0);
+    for (long i = 0; i < ne * nn; i++) ptr[conn[i] + 1]++;
+    for (long i = 0; i < nnode; i++) ptr[i + 1] += ptr[i];
+    std::vector<int> elem((size_t)ne * nn);
+    {
+        std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+        for (long e = 0; e < ne; e++)
+            for (int l = 0; l < nn; l++) elem[pos[conn[e * nn + l]]++] = (int)e;
+    }
+    std::vector<int> tmp;
+    auto neighbours = [&](long n) {
+        tmp.clear();
+        for (int p = ptr[n]; p < ptr[n + 1]; p++)
+            for (int l = 0; l < nn; l++) tmp.push_back(conn[(long)elem[p] * nn + l]);
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    };
+    std::vector<int> deg((size_t)nnode, 1);
+    for (long n = 0; n < nnode; n++) {
+        neighbours(n);
+        deg[n] = std::max<int>(1, (int)tmp.size());
+    }
+    for (long w0 = 0; w0 < nnode; w0 += 1024) {
+        const long w1 = std::min(nnode, w0 + 1024);
+        std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int p, int q) { return deg[p] > deg[q]; });
+    }
+    std::vector<int> pos((size_t)nnode);
+    for (long i = 0; i < nnode; i++) pos[order[i]] = (int)i;
+    // footprint of both orders: distinct 64-node blocks per window of 8192 rows
+    const long W = 8192;
+    std::vector<int> blk;
+    long fc = 0, fo = 0;
+    for (long w0 = 0; w0 < nnode; w0 += 8 * W) {
+        const long w1 = std::min(nnode, w0 + W);
+        for (int which = 0; which < 2; which++) {
+            blk.clear();
+            for (long r = w0; r < w1; r++) {
+                const long n = which ? order[r] : r;
+                neighbours(n);
+                for (int c : tmp) blk.push_back((which ? pos[c] : c) >> 6);
+            }
+            std::sort(blk.begin(), blk.end());
+            const long u = std::unique(blk.begin(), blk.end()) - blk.begin();
+            (which ? fo : fc) += u;
+        }
+    }
+    if (!((double)fo < 0.95 * (double)fc)) return {};
+    return order;
 }
 
 void setup_rows(cfdv_ctx* c, const cfdv_mesh* m, const std::vector<int>& ptr, const std::vector<int>& elem) {
@@ -1380,6 +1530,9 @@ int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flag
     cfdv_ctx* ctx = new (std::nothrow) cfdv_ctx();
     if (!ctx) return CFDV_ERR_HIP;
     try {
+        // an error an earlier, unrelated HIP call left on this thread is not
+        // this context's (its own launches are checked as they are made)
+        (void)hipGetLastError();
         ctx->ndim = mesh->ndim;
         ctx->nnod = mesh->nnod;
         ctx->neqn = mesh->neqn;
@@ -1401,7 +1554,44 @@ int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flag
         } catch (const std::exception& ex) {
             ctx->flag_scratch_err = ex.what();   // reported by cfdv_refine_flags_noalloc
         }
-        setup(ctx, mesh);
+        if (block_flags & CFDV_BLOCKS_ROWS) {
+            // the context works in its own node order (locality_order); the
+            // mesh's node arrays and connectivity are renumbered here and node
+            // vectors are permuted at every call that passes one
+            for (long i = 0; i < mesh->n_elem * mesh->nnod; i++)
+                if (mesh->conn[i] < 0 || mesh->conn[i] >= mesh->n_node)
+                    throw std::invalid_argument("connectivity out of range");
+            const long nnode = mesh->n_node, nd = mesh->ndim, nq = mesh->neqn;
+            std::vector<int> order = choose_node_order(mesh->ndim, mesh->nnod, nnode, mesh->n_elem, mesh->x, mesh->conn);
+            if (order.empty()) {
+                order.resize((size_t)nnode);
+                std::iota(order.begin(), order.end(), 0);
+            }
+            std::vector<int> pos((size_t)nnode);
+            for (long i = 0; i < nnode; i++) pos[order[i]] = (int)i;
+            std::vector<int32_t> conn((size_t)mesh->n_elem * mesh->nnod);
+            for (size_t i = 0; i < conn.size(); i++) conn[i] = pos[mesh->conn[i]];
+            std::vector<double> xs((size_t)nnode * nd);
+            std::vector<uint8_t> ds((size_t)nnode * nq);
+            for (long i = 0; i < nnode; i++) {
+                for (int a = 0; a < nd; a++) xs[(size_t)i * nd + a] = mesh->x[(size_t)order[i] * nd + a];
+                for (int q = 0; q < nq; q++) ds[(size_t)i * nq + q] = mesh->dirichlet[(size_t)order[i] * nq + q];
+            }
+            cfdv_mesh mi = *mesh;
+            mi.conn = conn.data();
+            mi.x = xs.data();
+            mi.dirichlet = ds.data();
+            setup(ctx, &mi);
+            ctx->d_norder = ctx->alloc<int>((size_t)nnode);
+            ctx->d_npos = ctx->alloc<int>((size_t)nnode);
+            HIP_CHECK_THROW(hipMemcpy(ctx->d_norder, order.data(), sizeof(int) * nnode, hipMemcpyHostToDevice));
+            HIP_CHECK_THROW(hipMemcpy(ctx->d_npos, pos.data(), sizeof(int) * nnode, hipMemcpyHostToDevice));
+            bool identity = true;
+            for (long i = 0; i < nnode && identity; i++) identity = order[i] == (int)i;
+            ctx->renum = !identity;
+        } else {
+            setup(ctx, mesh);
+        }
     } catch (const std::invalid_argument& ex) {
         delete ctx;
         return CFDV_ERR_ARG;
@@ -1441,7 +1631,7 @@ int cfdv_rows_info(const cfdv_ctx* ctx, int64_t* n_blocks, int64_t* n_slots) {
 int cfdv_set_state(cfdv_ctx* ctx, const double* U) {
     if (!ctx || !U) return CFDV_ERR_ARG;
     GUARD_BEGIN
-    HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_U, U, sizeof(double) * ctx->ndof, hipMemcpyHostToDevice, ctx->st));
+    ctx->upload_nodes(ctx->d_U, U, ctx->neqn);
     ctx->sync();
     ctx->built = false;
     ctx->pinv_valid = false;
@@ -1454,9 +1644,14 @@ int cfdv_set_state(cfdv_ctx* ctx, const double* U) {
 int cfdv_get_state(cfdv_ctx* ctx, double* U, double* U0) {
     if (!ctx) return CFDV_ERR_ARG;
     GUARD_BEGIN
-    if (U) HIP_CHECK_THROW(hipMemcpyAsync(U, ctx->d_U, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
-    if (U0) HIP_CHECK_THROW(hipMemcpyAsync(U0, ctx->d_U0, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
-    ctx->sync();
+    if (U) {
+        ctx->download_nodes(U, ctx->d_U, ctx->neqn);
+        ctx->sync();
+    }
+    if (U0) {
+        ctx->download_nodes(U0, ctx->d_U0, ctx->neqn);
+        ctx->sync();
+    }
     return CFDV_OK;
     GUARD_END
 }
@@ -1580,7 +1775,7 @@ int cfdv_get_rhs_vector(cfdv_ctx* ctx, double* b) {
     GUARD_BEGIN
     launch_compose(ctx->d_node_ptr, ctx->d_node_ent, ctx->d_rhs, ctx->k, ctx->neqn, ctx->n_node, ctx->d_b, ctx->st);
     ctx->check_launch();
-    HIP_CHECK_THROW(hipMemcpyAsync(b, ctx->d_b, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    ctx->download_nodes(b, ctx->d_b, ctx->neqn);
     ctx->sync();
     return CFDV_OK;
     GUARD_END
@@ -1637,7 +1832,7 @@ int cfdv_reference_trace(cfdv_ctx* ctx, int32_t* m, int32_t* n_restarts, double*
 int cfdv_get_du(cfdv_ctx* ctx, double* du) {
     if (!ctx || !du || !ctx->have_du) return fail(ctx, CFDV_ERR_ARG, "no solution");
     GUARD_BEGIN
-    HIP_CHECK_THROW(hipMemcpyAsync(du, ctx->d_du, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    ctx->download_nodes(du, ctx->d_du, ctx->neqn);
     ctx->sync();
     return CFDV_OK;
     GUARD_END
@@ -1689,9 +1884,8 @@ int cfdv_refine_flags(cfdv_ctx* ctx, uint8_t* flags, double* s1, int32_t* nflagg
     if (s1) HIP_CHECK_THROW(hipMemcpyAsync(s1, d_s1, sizeof(double) * ctx->n_elem, hipMemcpyDeviceToHost, ctx->st));
     HIP_CHECK_THROW(hipMemcpyAsync(&ties, ctx->d_counters, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
     ctx->sync();
-    if (d_s1) { ctx->allocs.pop_back(); (void)hipFree(d_s1); }
-    ctx->allocs.pop_back();
-    (void)hipFree(d_f);
+    ctx->release(d_s1);
+    ctx->release(d_f);
     if (nflagged) {
         long cnt = 0;
         for (long e = 0; e < ctx->n_elem; e++) cnt += flags[e];
@@ -1748,7 +1942,7 @@ int cfdv_matvec(cfdv_ctx* ctx, const double* x, double* y, int precision, int tr
         return fail(ctx, CFDV_ERR_ARG, precision == 64 ? "no FP64 blocks" : "no FP32 blocks");
     GUARD_BEGIN
     const long n = ctx->ndof;
-    HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_r, x, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    ctx->upload_nodes(ctx->d_r, x, ctx->neqn);
     if (precision == 32) {
         vec_d2f_scaled(ctx->d_w32, ctx->d_r, 1.0, n, ctx->st);
         float* yf = ctx->alloc<float>(n);
@@ -1756,12 +1950,11 @@ int cfdv_matvec(cfdv_ctx* ctx, const double* x, double* y, int precision, int tr
         vec_fill(ctx->d_y, 0.0, n, ctx->st);
         vec_add_f2d(ctx->d_y, yf, 1.0, n, ctx->st);
         ctx->sync();
-        ctx->allocs.pop_back();
-        (void)hipFree(yf);
+        ctx->release(yf);
     } else {
         ctx->matvec(ctx->d_r, ctx->d_y, precision, transpose != 0);
     }
-    HIP_CHECK_THROW(hipMemcpyAsync(y, ctx->d_y, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    ctx->download_nodes(y, ctx->d_y, ctx->neqn);
     ctx->sync();
     return CFDV_OK;
     GUARD_END
@@ -1790,7 +1983,7 @@ int cfdv_profile_matvec(cfdv_ctx* ctx, int precision, int reps, double* ms) {
     float t = 0;
     HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
     *ms = t / reps;
-    if (ytmp) { ctx->allocs.pop_back(); (void)hipFree(ytmp); }
+    ctx->release(ytmp);
     return CFDV_OK;
     GUARD_END
 }
@@ -1828,12 +2021,12 @@ int cfdv_adjoint_solve(cfdv_ctx* ctx, const cfdv_solve_opts* o, const double* g,
         ctx->d_g = ctx->alloc<double>(ctx->ndof);
         ctx->d_lam = ctx->alloc<double>(ctx->ndof);
     }
-    HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_g, g, sizeof(double) * ctx->ndof, hipMemcpyHostToDevice, ctx->st));
+    ctx->upload_nodes(ctx->d_g, g, ctx->neqn);
     cfdv_solve_opts ot = *o;
     ot.transpose = 1;
     const int rc = (o->mode == CFDV_SOLVE_FP64) ? solve_fp64_rhs(ctx, &ot, s, ctx->d_g, ctx->d_lam)
                                                 : solve_mixed_rhs(ctx, &ot, s, ctx->d_g, ctx->d_lam);
-    HIP_CHECK_THROW(hipMemcpyAsync(lambda, ctx->d_lam, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    ctx->download_nodes(lambda, ctx->d_lam, ctx->neqn);
     ctx->sync();
     return rc;
     GUARD_END
@@ -1853,11 +2046,13 @@ int cfdv_adjoint_solve_multi(cfdv_ctx* ctx, const cfdv_solve_opts* o, const doub
     std::memset(s, 0, sizeof(*s));
     GUARD_BEGIN
     ctx->ensure_multi(std::min(o->inner_m, MAX_M1 - 1));
-    const size_t n8 = (size_t)8 * ctx->ndof;
-    HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_G8, G, sizeof(double) * n8, hipMemcpyHostToDevice, ctx->st));
+    const size_t n = (size_t)ctx->ndof;
+    for (int k = 0; k < 8; k++) ctx->upload_nodes(ctx->d_G8 + k * n, G + k * n, ctx->neqn);
     const int rc = solve_multi8(ctx, o, true, ctx->d_G8, ctx->d_L8, rel_resid, s);
-    HIP_CHECK_THROW(hipMemcpyAsync(Lambda, ctx->d_L8, sizeof(double) * n8, hipMemcpyDeviceToHost, ctx->st));
-    ctx->sync();
+    for (int k = 0; k < 8; k++) {
+        ctx->download_nodes(Lambda + k * n, ctx->d_L8 + k * n, ctx->neqn);
+        ctx->sync();
+    }
     return rc;
     GUARD_END
 }
@@ -1877,9 +2072,9 @@ int cfdv_build_vjp(cfdv_ctx* ctx, const double* lambda, const double* du, double
             ctx->d_lam = ctx->alloc<double>(ctx->ndof);
         }
     }
-    HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_lam, lambda, sizeof(double) * ctx->ndof, hipMemcpyHostToDevice, ctx->st));
+    ctx->upload_nodes(ctx->d_lam, lambda, ctx->neqn);
     if (du)
-        HIP_CHECK_THROW(hipMemcpyAsync(ctx->d_duv, du, sizeof(double) * ctx->ndof, hipMemcpyHostToDevice, ctx->st));
+        ctx->upload_nodes(ctx->d_duv, du, ctx->neqn);
     else
         vec_copy(ctx->d_duv, ctx->d_du, ctx->ndof, ctx->st);
     BuildArgs a{};
@@ -1898,7 +2093,7 @@ int cfdv_build_vjp(cfdv_ctx* ctx, const double* lambda, const double* du, double
     launch_vjp_gather(ctx->d_node_ptr, ctx->d_node_ent, ctx->d_gelem, ctx->n_node, ctx->d_gU, ctx->st);
     launch_column_sum(ctx->d_gelem, ctx->n_pad, 41, 40, ctx->d_part, NBLK_RED, ctx->d_small + 1000, ctx->st);
     ctx->check_launch();
-    HIP_CHECK_THROW(hipMemcpyAsync(gU, ctx->d_gU, sizeof(double) * ctx->ndof, hipMemcpyDeviceToHost, ctx->st));
+    ctx->download_nodes(gU, ctx->d_gU, ctx->neqn);
     HIP_CHECK_THROW(hipMemcpyAsync(gmach, ctx->d_small + 1000, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     ctx->sync();
     return CFDV_OK;
@@ -1930,14 +2125,12 @@ int cfdv_matvec_multi(cfdv_ctx* ctx, const float* X, float* Y, int nvec, int tra
     const size_t n = (size_t)ctx->ndof * 8;
     float* dX = ctx->alloc<float>(n);
     float* dY = ctx->alloc<float>(n);
-    HIP_CHECK_THROW(hipMemcpyAsync(dX, X, sizeof(float) * n, hipMemcpyHostToDevice, ctx->st));
+    ctx->upload_nodes32(dX, X, 8 * ctx->neqn);
     matvec_multi_dev(ctx, dX, dY, transpose != 0);
-    HIP_CHECK_THROW(hipMemcpyAsync(Y, dY, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->st));
+    ctx->download_nodes32(Y, dY, 8 * ctx->neqn);
     ctx->sync();
-    ctx->allocs.pop_back();
-    ctx->allocs.pop_back();
-    (void)hipFree(dX);
-    (void)hipFree(dY);
+    ctx->release(dX);
+    ctx->release(dY);
     return CFDV_OK;
     GUARD_END
 }
@@ -1959,10 +2152,8 @@ int cfdv_profile_matvec_multi(cfdv_ctx* ctx, int reps, double* ms) {
     float t = 0;
     HIP_CHECK_THROW(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
     *ms = t / reps;
-    ctx->allocs.pop_back();
-    ctx->allocs.pop_back();
-    (void)hipFree(dX);
-    (void)hipFree(dY);
+    ctx->release(dX);
+    ctx->release(dY);
     return CFDV_OK;
     GUARD_END
 }

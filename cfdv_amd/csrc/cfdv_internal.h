@@ -215,6 +215,10 @@ template <typename TV, typename TP>
 void vec_nodeblock_apply8(const TP* P, const TV* v, TV* z, int neqn, long n_node, bool accumulate, bool trans,
                           hipStream_t st);
 
+// out[i q + c] = in[map[i] q + c] (node permutation, q values per node)
+template <typename T>
+void vec_gather_nodes(T* out, const T* in, const int* map, long n_node, int q, hipStream_t st);
+
 // ---- coloring (solve.hip) ----
 void launch_color_round(const int* conn, int nnod, const int* node_ptr, const int* node_elem, long n_elem,
                         const unsigned long long* prio, const int* color_in, int* color_out, int* changed,

@@ -53,7 +53,7 @@ __device__ __forceinline__ long chunk_of_wave(long n_chunks) {
     const long per = (nb + 7) / 8;
     const long xcd = b % 8, idx = b / 8;
     const long bb = xcd * per + idx;   // < 8 * per
-    return bb * 4 + threadIdx.x / 64;
+    return bb * (blockDim.x / 64) + threadIdx.x / 64;
 }
 
 }  // namespace
@@ -442,15 +442,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     }
 }
 
-static int row_grid(long n_chunks) {
-    const long nb = (n_chunks + 3) / 4;
+static int row_grid(long n_chunks, int waves_per_block = 4) {
+    const long nb = (n_chunks + waves_per_block - 1) / waves_per_block;
     return (int)(((nb + 7) / 8) * 8);   // whole rounds of 8 XCDs
 }
 
 void launch_rows_matvec(const RowsArgs& a, const double* B64, const float* B32, const void* x, void* y, int prec,
                         hipStream_t st, const int* gate) {
     if (a.n_chunks <= 0) return;
-    const dim3 grid(row_grid(a.n_chunks)), blk(256);
+    // FP32 blocks: one wavefront per workgroup, so that a chunk with more
+    // slots (refined rows) holds no other wave's SIMD slot while it
+    // finishes (2.48 -> 2.46 ms on the refined ramp); FP64: four (measured
+    // the faster there)
+    const int wpb = prec == 64 ? 4 : 1;
+    const dim3 grid(row_grid(a.n_chunks, wpb)), blk(64 * wpb);
 #define L(TB, TV, TA, NQ, BP)                                                                              \
     hipLaunchKernelGGL((rows_matvec_kernel<TB, TV, TA, NQ>), grid, blk, 0, st, a.n_node, a.n_chunks, a.goff, \
                        a.cols, BP, (const TV*)x, (TV*)y, gate)

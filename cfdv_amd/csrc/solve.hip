@@ -999,6 +999,25 @@ template void vec_nodeblock_apply8<double, float>(const float*, const double*, d
 template void vec_nodeblock_apply8<double, double>(const double*, const double*, double*, int, long, bool, bool,
                                                    hipStream_t);
 
+// out[i q + c] = in[map[i] q + c] for node i < n_node, c < q (a node
+// permutation of an array with q values per node)
+template <typename T>
+__global__ void gather_nodes_kernel(T* __restrict__ out, const T* __restrict__ in, const int* __restrict__ map,
+                                    long n_node, int q) {
+    const long n = n_node * q;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const long i = t / q;
+        const int c = (int)(t - i * q);
+        out[t] = in[(size_t)map[i] * q + c];
+    }
+}
+template <typename T>
+void vec_gather_nodes(T* out, const T* in, const int* map, long n_node, int q, hipStream_t st) {
+    hipLaunchKernelGGL((gather_nodes_kernel<T>), dim3(grid_for(n_node * q)), dim3(256), 0, st, out, in, map, n_node, q);
+}
+template void vec_gather_nodes<double>(double*, const double*, const int*, long, int, hipStream_t);
+template void vec_gather_nodes<float>(float*, const float*, const int*, long, int, hipStream_t);
+
 // per-equation sums of squares of a node array (MPI_RMSError.h:19-26)
 __global__ void sumsq_eqn_kernel(const double* __restrict__ U, int neqn, long n_node, double* __restrict__ part) {
     __shared__ double sm[5][256];

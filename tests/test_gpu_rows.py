@@ -152,6 +152,11 @@ def test_rows_solves(gpu_available, case):
 
 
 def test_reference_solve_stays_element_by_element(gpu_available):
+    """The reference solve of a context with node rows multiplies element by
+    element like a context without: the same iterations and the same step.
+    The rows context numbers its nodes in its own (locality) order, so the
+    solve's dot products sum in another order: the steps agree to rounding,
+    not bit for bit."""
     m, th, U = _case("box3d")
     a = FDVContext(m, th, U, blocks=("fp64", "rows"))
     b = FDVContext(m, th, U)
@@ -160,7 +165,9 @@ def test_reference_solve_stays_element_by_element(gpu_available):
     assert sa.solve.iters == sb.solve.iters
     Ua, _ = a.get_state()
     Ub, _ = b.get_state()
-    assert np.array_equal(Ua, Ub)
+    d = np.linalg.norm(Ua - Ub) / np.linalg.norm(Ub - U)
+    print(f"rows context vs EBE context, reference step: |dU| rel diff {d:.1e}")
+    assert d <= 1e-10
 
 
 def test_mixed_step_on_rows(gpu_available):
