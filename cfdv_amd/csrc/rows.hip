@@ -450,12 +450,10 @@ static int row_grid(long n_chunks, int waves_per_block = 4) {
 void launch_rows_matvec(const RowsArgs& a, const double* B64, const float* B32, const void* x, void* y, int prec,
                         hipStream_t st, const int* gate) {
     if (a.n_chunks <= 0) return;
-    // FP32 blocks: one wavefront per workgroup, so that a chunk with more
-    // slots (refined rows) holds no other wave's SIMD slot while it
-    // finishes (2.48 -> 2.46 ms on the refined ramp); FP64: four (measured
-    // the faster there)
-    const int wpb = prec == 64 ? 4 : 1;
-    const dim3 grid(row_grid(a.n_chunks, wpb)), blk(64 * wpb);
+    // one wavefront per workgroup, so that a chunk with more slots (refined
+    // rows) holds no other wave's SIMD slot while it finishes (refined ramp:
+    // FP32 2.48 -> 2.46 ms, FP64 4.27 -> 4.16 ms against four per workgroup)
+    const dim3 grid(row_grid(a.n_chunks, 1)), blk(64);
 #define L(TB, TV, TA, NQ, BP)                                                                              \
     hipLaunchKernelGGL((rows_matvec_kernel<TB, TV, TA, NQ>), grid, blk, 0, st, a.n_node, a.n_chunks, a.goff, \
                        a.cols, BP, (const TV*)x, (TV*)y, gate)
