@@ -294,12 +294,23 @@ struct GaussHex {
     } p[8];
 };
 
-// Point P (Phi, dPhi/dxi) with weight w3; dPhi_n/dx_j go to dphi_out[n][j]
-// when dphi_out is given.
-CFDV_HD void gp_phase1(const DevThermo& th, const GaussHex::Point& P, double w3, const double (*nd)[ND_STRIDE],
-                       double dt, GPShared& o, double (*dphi_out)[3]) {
-    constexpr int NEQ = 5;
-    const double reinv = 1.0 / th.creyn;
+// Phase 1 of the interior kernel is spread over the 64 lanes in three
+// steps, each quantity formed once per point:
+//   gp_geom  (lane = 8 gp + n): Jacobian of point gp (its 8 lanes alike),
+//            dPhi_n/dx of node n and w det J;
+//   gp_sums  (lane = 8 gp + k): for node column src(k) (U, v, E, T, or the
+//            constant 1), u(gp) = sum_n Phi_n u_n and du/dx_j(gp) =
+//            sum_n dPhi_n/dx_j u_n -- every sum over n = 0..7 in order, one
+//            fused multiply-add per node, as a serial loop forms it;
+//   gp_state (lane = 8 part + gp): the state and D, P of the point from
+//            those sums.
+// GP_NSRC node columns feed sums: ND_U .. ND_T (U, v, E, T) and the
+// constant 1, whose sums are SPhi and sum_n dPhi_n/dx_j.
+constexpr int GP_NSRC = 11;
+CFDV_HD constexpr int gp_src_col(int k) { return k < 10 ? ND_U + k : -1; }
+
+CFDV_HD void gp_geom(const GaussHex::Point& P, double w3, const double (*nd)[ND_STRIDE], int n, double dphi_n[3],
+                     double* wJ) {
     double jac[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -307,7 +318,7 @@ CFDV_HD void gp_phase1(const DevThermo& th, const GaussHex::Point& P, double w3,
         for (int j = 0; j < 3; j++) {
             double acc = 0.0;
 #pragma unroll
-            for (int n = 0; n < 8; n++) acc += P.dpxi[n][j] * nd[n][ND_X + i];
+            for (int m = 0; m < 8; m++) acc += P.dpxi[m][j] * nd[m][ND_X + i];
             jac[i][j] = acc;
         }
     const double J = jac[0][0] * jac[1][1] * jac[2][2] + jac[0][1] * jac[1][2] * jac[2][0] +
@@ -324,58 +335,64 @@ CFDV_HD void gp_phase1(const DevThermo& th, const GaussHex::Point& P, double w3,
     ji[2][1] = -(jac[0][0] * jac[1][2] - jac[0][2] * jac[1][0]);
     ji[2][2] = jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0];
     const double Jinv = 1.0 / J;
-    GPState<3> f;
-    f.E = 0; f.T = 0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        f.v[i] = 0; f.dT[i] = 0;
+        double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < 3; j++) f.dV[i][j] = 0;
+        for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * P.dpxi[n][j]);
+        dphi_n[i] = acc;
     }
+    *wJ = w3 * J;
+}
+
+// The four sums of node column col (-1: the constant 1) at point P, whose
+// dPhi_n/dx are dphi[n]: out = {sum Phi_n u_n, sum dPhi_n/dx_j u_n (j = 0..2)}
+CFDV_HD void gp_sums(const GaussHex::Point& P, const double (*dphi)[3], const double (*nd)[ND_STRIDE], int col,
+                     double out[4]) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        const double u = col >= 0 ? nd[n][col] : 1.0;
+        a0 += P.phi[n] * u;
+        a1 += dphi[n][0] * u;
+        a2 += dphi[n][1] * u;
+        a3 += dphi[n][2] * u;
+    }
+    out[0] = a0;
+    out[1] = a1;
+    out[2] = a2;
+    out[3] = a3;
+}
+
+// The point's state from the sums S[4 k + c] of source k (gp_sums), its
+// viscosity, conductivity and tau, and the viscous vectors D and P
+CFDV_HD void gp_state(const DevThermo& th, const double* S, double dt, GPShared& o) {
+    constexpr int NEQ = 5;
+    const double reinv = 1.0 / th.creyn;
+    GPState<3> f;
 #pragma unroll
     for (int q = 0; q < NEQ; q++) {
-        f.U[q] = 0;
+        f.U[q] = S[4 * q];
 #pragma unroll
-        for (int j = 0; j < 3; j++) f.dU[q][j] = 0;
+        for (int j = 0; j < 3; j++) f.dU[q][j] = S[4 * q + 1 + j];
     }
-    double SPhi = 0.0, SD[3] = {0, 0, 0};
-#pragma unroll 2
-    for (int n = 0; n < 8; n++) {
-        double dphi[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            double acc = 0.0;
+    for (int i = 0; i < 3; i++) {
+        f.v[i] = S[4 * (ND_V - ND_U + i)];
 #pragma unroll
-            for (int j = 0; j < 3; j++) acc += Jinv * (ji[i][j] * P.dpxi[n][j]);
-            dphi[i] = acc;
-            if (dphi_out) dphi_out[n][i] = acc;
-        }
-        const double ph = P.phi[n];
-        SPhi += ph;
-        f.E += ph * nd[n][ND_E];
-        const double Tnn = nd[n][ND_T];
-        f.T += ph * Tnn;
-#pragma unroll
-        for (int q = 0; q < NEQ; q++) {
-            const double u_ = nd[n][ND_U + q];
-            f.U[q] += ph * u_;
-#pragma unroll
-            for (int j = 0; j < 3; j++) f.dU[q][j] += dphi[j] * u_;
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const double v_ = nd[n][ND_V + i];
-            SD[i] += dphi[i];
-            f.v[i] += ph * v_;
-            f.dT[i] += dphi[i] * Tnn;
-#pragma unroll
-            for (int j = 0; j < 3; j++) f.dV[i][j] += dphi[j] * v_;
-        }
+        for (int j = 0; j < 3; j++) f.dV[i][j] = S[4 * (ND_V - ND_U + i) + 1 + j];
     }
+    f.E = S[4 * (ND_E - ND_U)];
+    f.T = S[4 * (ND_T - ND_U)];
+#pragma unroll
+    for (int j = 0; j < 3; j++) f.dT[j] = S[4 * (ND_T - ND_U) + 1 + j];
+    const double SPhi = S[4 * 10];
+    double SD[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) SD[j] = S[4 * 10 + 1 + j];
     f.mu = calc_mu(th, f.T);
     f.k = calc_k(th, f.mu);
     calc_tau<3>(f);
-    o.wJ = w3 * J;
     o.SPhi = SPhi;
     o.D[0] = 0.0;
 #pragma unroll
@@ -794,19 +811,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     double sp[4];
     int tie = 0;
     fdvparam_wave(th, s_nd, lane, sp, &tie);
-    // phase 1: lanes (part, gp) = (lane / 8, lane % 8), part < 4: the
-    // Gauss-point state (evaluated by the four lanes of a point alike), then
-    // direction 0, 1, 2 of the Jacobian combinations or the common part
-    if (lane < 32) {
-        const int gp = lane & 7, part = lane >> 3;
-        GPShared g;
-        gp_phase1(th, s_g.p[gp], c_gauss.w3[gp], s_nd, A.dt, g, part == 3 ? s_tab[gp].dphi : nullptr);
-        if (part < 3) {
-            gp_tab_dir_rot(th, g, sp, A.dt, part, s_tab[gp]);
-        } else {
-            gp_tab_common(th, g, s_tab[gp]);
-            s_g.p[gp].wJ = g.wJ;
+    // phase 1: (a) lane (gp, n) = (lane / 8, lane % 8): geometry of point
+    // gp and dPhi_n/dx; (b) lane (gp, k): the interpolation sums of sources k
+    // and k + 8, into the point's A/S table area (written only after every
+    // lane has read them back); (c) lanes (part, gp) = (lane / 8, lane % 8),
+    // part < 4: the point's state from the sums (the four lanes of a point
+    // alike), then direction 0, 1, 2 of the Jacobian combinations or the
+    // common part
+    {
+        const int gp = lane >> 3, n = lane & 7;
+        double wJ;
+        gp_geom(s_g.p[gp], c_gauss.w3[gp], s_nd, n, s_tab[gp].dphi[n], &wJ);
+        if (n == 0) s_g.p[gp].wJ = wJ;
+    }
+    __syncthreads();
+    static_assert(sizeof(GPTab::Af) >= 4 * GP_NSRC * sizeof(double), "sums fit the A table");
+    {
+        const int gp = lane >> 3, k = lane & 7;
+        double* S = &s_tab[gp].Af[0][0];
+        double o4[4];
+        gp_sums(s_g.p[gp], s_tab[gp].dphi, s_nd, gp_src_col(k), o4);
+#pragma unroll
+        for (int c = 0; c < 4; c++) S[4 * k + c] = o4[c];
+        if (k + 8 < GP_NSRC) {
+            gp_sums(s_g.p[gp], s_tab[gp].dphi, s_nd, gp_src_col(k + 8), o4);
+#pragma unroll
+            for (int c = 0; c < 4; c++) S[4 * (k + 8) + c] = o4[c];
         }
+    }
+    __syncthreads();
+    GPShared g;
+    const int gp1 = lane & 7, part = lane >> 3;
+    if (lane < 32) gp_state(th, &s_tab[gp1].Af[0][0], A.dt, g);
+    __syncthreads();
+    if (lane < 32) {
+        if (part < 3) gp_tab_dir_rot(th, g, sp, A.dt, part, s_tab[gp1]);
+        else gp_tab_common(th, g, s_tab[gp1]);
     }
     __syncthreads();
     // phase 2: lane (N, M) contracts every point for its node pair; the
