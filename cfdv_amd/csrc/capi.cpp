@@ -1086,9 +1086,12 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
         launch_gmres_start(g, m, beta, gate, c->st);
         int jdone = 0;
         bool early = false;   // the end of the cycle was read one step late
+        bool z_ready = false;   // z = P v_j formed with v_j by the previous step
         for (int j = 0; j < m; j++) {
             if (Pinv) {
-                vec_nodeblock_apply<T, T>(Pinv, V + (size_t)j * ldv, z, c->neqn, c->n_node, false, tr, c->st, gate);
+                if (!z_ready)
+                    vec_nodeblock_apply<T, T>(Pinv, V + (size_t)j * ldv, z, c->neqn, c->n_node, false, tr, c->st,
+                                              gate);
                 c->matvec(z, w, prec, tr, gate);
             } else {
                 c->matvec(V + (size_t)j * ldv, w, prec, tr, gate);
@@ -1117,7 +1120,13 @@ int gmres_core(cfdv_ctx* c, int prec, bool tr, const T* rhs, double* x, int m, i
             }
             launch_givens_step(H, ldh, g, cs, sn, c->d_hcol, cgs2 ? c->d_hcol + 1024 : nullptr, c->d_hcol + 2040, j,
                                tol_abs, c->d_small + 9, gate, c->d_steps, c->st);
-            vec_scale_inv<T>(V + (size_t)(j + 1) * ldv, w, c->d_small + 9, n, c->st, gate);
+            if (Pinv && j + 1 < m) {
+                vec_nodeblock_scale_apply<T>(Pinv, w, c->d_small + 9, V + (size_t)(j + 1) * ldv, z, c->neqn,
+                                             c->n_node, tr, c->st, gate);
+                z_ready = true;
+            } else {
+                vec_scale_inv<T>(V + (size_t)(j + 1) * ldv, w, c->d_small + 9, n, c->st, gate);
+            }
             c->check_launch();
             HIP_CHECK_THROW(hipEventRecord(c->ev_step[j & 1], c->st));
             if (prec == 64) s->matvecs64++; else s->matvecs32++;

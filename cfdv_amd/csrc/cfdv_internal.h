@@ -141,6 +141,10 @@ void launch_nodeblock(const int* node_ptr, const int* node_ent, const double* R6
 template <typename TV, typename TP>
 void vec_nodeblock_apply(const TP* P, const TV* v, TV* z, int neqn, long n_node, bool accumulate, bool trans,
                          hipStream_t st, const int* gate = nullptr);
+// v = w / *d and z = P v (node blocks) in one pass
+template <typename T>
+void vec_nodeblock_scale_apply(const T* P, const T* w, const double* d, T* v, T* z, int neqn, long n_node,
+                               bool trans, hipStream_t st, const int* gate = nullptr);
 void vec_sumsq_eqn(const double* U, int neqn, long n_node, double* part, double* out, int nblk, hipStream_t st);
 
 // ---- GMRES least squares on the device (solve.hip) ----

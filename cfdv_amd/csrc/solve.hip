@@ -928,6 +928,50 @@ template void vec_nodeblock_apply<float, float>(const float*, const float*, floa
 template void vec_nodeblock_apply<double, float>(const float*, const double*, double*, int, long, bool, bool,
                                                  hipStream_t, const int*);
 
+// The next Arnoldi vector and its preconditioned image in one pass:
+// v = w / d (as scale_inv_kernel) and z = P v (as nodeblock_apply_kernel,
+// same order), so the right-preconditioned step reads w once instead of w
+// and then v.
+template <typename T, int NEQ>
+__global__ __launch_bounds__(256) void nodeblock_scale_apply_kernel(const T* __restrict__ P, const T* __restrict__ w,
+                                                                    const double* __restrict__ d, T* __restrict__ v,
+                                                                    T* __restrict__ z, long n_node, int trans,
+                                                                    const int* __restrict__ gate) {
+    if (gate && *gate) return;
+    const double dd = *d;
+    for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < n_node; n += (long)gridDim.x * blockDim.x) {
+        T x[NEQ];
+#pragma unroll
+        for (int c = 0; c < NEQ; c++) {
+            x[c] = (T)((double)w[n * NEQ + c] / dd);
+            v[n * NEQ + c] = x[c];
+        }
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) {
+            T acc = (T)0;
+#pragma unroll
+            for (int c = 0; c < NEQ; c++) acc += P[pblk_offset<NEQ>(n, trans ? c * NEQ + r : r * NEQ + c)] * x[c];
+            z[n * NEQ + r] = acc;
+        }
+    }
+}
+
+template <typename T>
+void vec_nodeblock_scale_apply(const T* P, const T* w, const double* d, T* v, T* z, int neqn, long n_node,
+                               bool trans, hipStream_t st, const int* gate) {
+    const int nb = grid_for(n_node);
+    if (neqn == 5)
+        hipLaunchKernelGGL((nodeblock_scale_apply_kernel<T, 5>), dim3(nb), dim3(256), 0, st, P, w, d, v, z, n_node,
+                           (int)trans, gate);
+    else
+        hipLaunchKernelGGL((nodeblock_scale_apply_kernel<T, 4>), dim3(nb), dim3(256), 0, st, P, w, d, v, z, n_node,
+                           (int)trans, gate);
+}
+template void vec_nodeblock_scale_apply<double>(const double*, const double*, const double*, double*, double*, int,
+                                                long, bool, hipStream_t, const int*);
+template void vec_nodeblock_scale_apply<float>(const float*, const float*, const double*, float*, float*, int, long,
+                                               bool, hipStream_t, const int*);
+
 // ---------------------------------------------------------------------------
 // Eight vectors side by side ([n][8], the 8 values of entry i contiguous), as
 // the 8-vector product reads and writes them, and the node-block
