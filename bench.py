@@ -82,7 +82,10 @@ def parse():
     # sweeps, 1e-5: more inner iterations)
     p.add_argument("--inner-rtol", type=float, default=1e-4)
     p.add_argument("--precond", choices=["nodeblock", "none"], default="nodeblock")
-    p.add_argument("--storage", choices=["rows", "ebe"], default="rows")
+    # rows: FP64 element blocks, node rows summed from them after the build;
+    # rows-inplace: node rows built in place by the build (no element
+    # blocks); ebe: element blocks only, products element by element
+    p.add_argument("--storage", choices=["rows", "rows-inplace", "ebe"], default="rows")
     # 2: each solve starts from the increment rate of the two previous steps
     # extrapolated (the first step after the initial state starts from 0,
     # the second from du_1 dt / dt_1); 3: quadratic in the rate of the last
@@ -226,7 +229,8 @@ def main():
                 f"{refine_info['elements'][p + 1]} elements, {refine_info['nodes'][p + 1]} nodes; host list "
                 f"{refine_info['host_list_s'][p]:.1f} s, apply {refine_info['host_apply_s'][p]:.1f} s")
     t0 = time.time()
-    ctx = FDVContext(mesh, th, U0, blocks=("fp64", "rows") if args.storage == "rows" else ("fp64", "fp32"))
+    blocks = {"rows": ("fp64", "rows"), "rows-inplace": ("rows",), "ebe": ("fp64", "fp32")}[args.storage]
+    ctx = FDVContext(mesh, th, U0, blocks=blocks)
     info = ctx.info()
     log(f"context: {info} ({time.time() - t0:.1f}s)")
 
@@ -262,7 +266,7 @@ def main():
     fp64_sps = args.steps / el_64
 
     ne, nn = mesh.n_elem, mesh.n_node
-    if args.storage == "rows":
+    if args.storage.startswith("rows"):
         # one block per node pair sharing an element (on the unrefined
         # structured grid (3nx+1)(3ny+1)(3nz+1) of them); each block's 25
         # values plus its column index, x read and y written once
@@ -311,7 +315,8 @@ def main():
                              f"{ne} elements with {mv_per_step:.0f} FP64 products per step (the FP64 GPU "
                              f"path's count), at {main_run['threads']} threads")}}
 
-    traffic = MEASURED_TRAFFIC.get((args.storage, args.size, args.refine_levels), {})
+    traffic = MEASURED_TRAFFIC.get(("rows" if args.storage.startswith("rows") else args.storage, args.size,
+                                    args.refine_levels), {})
     wl = (f"3D hypersonic 15-deg compression ramp hex8 {nx}x{ny}x{nz} ({base_elem} elements, "
           f"{base_node} nodes)")
     if refine_info is not None:

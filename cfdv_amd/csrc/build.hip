@@ -267,6 +267,34 @@ __device__ __forceinline__ void r64_load(const BuildArgs& A, long e, int N, int 
     }
 }
 
+// Rows built in place: lane (N, M)'s block of element e added into its
+// staged FP64 row slot, or stored there when the element is the slot's first
+// contributor (the lowest element slot; colors run in slot order, so every
+// slot sums its blocks in one fixed order).  No two elements of one color
+// share a node, so no two lanes of a color's launches meet in one slot.
+__device__ __forceinline__ void rows_put(const BuildArgs& A, long e, int N, int M, const double R[5][5]) {
+    const int pr = N * 8 + M;
+    double* dst = A.S64 + (size_t)A.pslot[e * 64 + pr] * 25;
+    if ((A.pfirst[e] >> pr) & 1ULL) {
+#pragma unroll
+        for (int r = 0; r < 5; r++)
+#pragma unroll
+            for (int s = 0; s < 5; s++) dst[r * 5 + s] = R[r][s];
+    } else {
+        double old[25];
+#pragma unroll
+        for (int q = 0; q < 25; q++) old[q] = dst[q];
+#pragma unroll
+        for (int r = 0; r < 5; r++)
+#pragma unroll
+            for (int s = 0; s < 5; s++) dst[r * 5 + s] = old[r * 5 + s] + R[r][s];
+    }
+}
+// the domain part of a face-kernel element between the two kernels
+__device__ __forceinline__ double* rb_block(const BuildArgs& A, long e, int N, int M) {
+    return A.Rb + ((size_t)A.bidx[e] * 64 + N * 8 + M) * 25;
+}
+
 // ---------------------------------------------------------------------------
 // Interior hex8 elements (no face with a bc code, no constrained dof): the
 // Gauss-point state is the same for all 64 node pairs, so it is evaluated
@@ -759,9 +787,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     __shared__ GPTab s_tab[8];
     __shared__ GaussHex s_g;
     const int lane = threadIdx.x;
-    const long e = (long)xcd_remap(blockIdx.x, gridDim.x);
+    const long e = A.e_lo + (long)xcd_remap(blockIdx.x, gridDim.x);
     const DevThermo th = A.th;
-    if (e >= A.n_elem_pad) return;   // whole workgroup leaves together
+    if (e >= A.e_hi) return;   // whole workgroup leaves together
     const int* cn = A.conn + e * NNOD;
     const bool real = cn[0] >= 0;
     bool boundary = false, hang = false;
@@ -779,7 +807,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         if (hang && boundary) return;
         if (boundary && !W64) return;
     } else {   // padding slot of a color range: zero block
-        if (W64)
+        if (W64 && A.R64)
             for (int q = lane; q < KK; q += 64) A.R64[r64_at<8, 5>(A, e, q)] = 0.0;
         if (W32)
             for (int q = lane; q < KK; q += 64) A.R32[blk_index<float>(e, q, KK)] = 0.0f;
@@ -872,7 +900,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     for (int r = 0; r < NEQ; r++)
 #pragma unroll
         for (int s = 0; s < NEQ; s++) R[r][s] += s_as[(NEQ * N + r) * K + NEQ * M + s];
-    if (W64) r64_store<8, 5>(A, e, N, M, R);
+    if (A.S64) {
+        if (boundary) {
+            double* dst = rb_block(A, e, N, M);
+#pragma unroll
+            for (int r = 0; r < NEQ; r++)
+#pragma unroll
+                for (int s = 0; s < NEQ; s++) dst[r * NEQ + s] = R[r][s];
+        } else {
+            rows_put(A, e, N, M, R);
+        }
+    } else if (W64) {
+        r64_store<8, 5>(A, e, N, M, R);
+    }
     if (W32 && !boundary) {
 #pragma unroll
         for (int r = 0; r < NEQ; r++)
@@ -1218,7 +1258,15 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
     double R[NEQ][NEQ], rhs[NEQ];
 #pragma unroll
     for (int r = 0; r < NEQ; r++) rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
-    r64_load<8, 5>(A, e, N, M, R);
+    if (A.S64) {
+        const double* src = rb_block(A, e, N, M);
+#pragma unroll
+        for (int r = 0; r < NEQ; r++)
+#pragma unroll
+            for (int s = 0; s < NEQ; s++) R[r][s] = src[r * NEQ + s];
+    } else {
+        r64_load<8, 5>(A, e, N, M, R);
+    }
     for (int b0 = 0; b0 < npts || b0 == 0; b0 += PB) {
         const int nb = (npts - b0 < PB) ? npts - b0 : PB;
         const int p = lane & 7, part = lane >> 3;
@@ -1263,7 +1311,8 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
         __syncthreads();   // the tables are rewritten by the next batch
     }
     apply_bc3(A, e, lane, R, rhs);
-    r64_store<8, 5>(A, e, N, M, R);
+    if (A.S64) rows_put(A, e, N, M, R);
+    else r64_store<8, 5>(A, e, N, M, R);
     if (W32 && A.R32) {
 #pragma unroll
         for (int r = 0; r < NEQ; r++)
@@ -1297,7 +1346,7 @@ __global__ __launch_bounds__(256) void build3d_kernel(BuildArgs A, const int* li
     __syncthreads();
     if (!in_range) return;
     if (!real) {   // padding slot of a color range: zero block
-        if (W64)
+        if (W64 && A.R64)
             for (int q = lane; q < KK; q += 64) A.R64[r64_at<8, 5>(A, e, q)] = 0.0;
         if (W32 && A.R32)
             for (int q = lane; q < KK; q += 64) A.R32[blk_index<float>(e, q, KK)] = 0.0f;
@@ -1310,7 +1359,8 @@ __global__ __launch_bounds__(256) void build3d_kernel(BuildArgs A, const int* li
     build3d_elem(A, A.th, c_gauss, nd, e, lane, R, rhs, sp, &tie, &status);
     if (status && lane == 0) atomicOr(A.status, status);
     // ---------------- write ----------------
-    if (W64) r64_store<8, 5>(A, e, N, M, R);
+    if (A.S64) rows_put(A, e, N, M, R);
+    else if (W64) r64_store<8, 5>(A, e, N, M, R);
     if (W32 && A.R32) {
 #pragma unroll
         for (int r = 0; r < NEQ; r++)
@@ -1807,6 +1857,32 @@ void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st) {
         if (w64 && w32) hipLaunchKernelGGL((build2d_kernel<true, true>), dim3(nb), dim3(256), 0, st, a);
         else if (w64) hipLaunchKernelGGL((build2d_kernel<true, false>), dim3(nb), dim3(256), 0, st, a);
         else hipLaunchKernelGGL((build2d_kernel<false, true>), dim3(nb), dim3(256), 0, st, a);
+    }
+}
+
+void launch_build_direct(const BuildArgs& a, int ncolor, const long* cstart, const long* bstart, const long* astart,
+                         hipStream_t st) {
+    // per color: the interior kernel over its slots (the face kernel's
+    // elements leave their domain part in Rb), the face kernel over its
+    // boundary elements, the per-lane kernel over its hanging-node elements
+    // with faces; each adds into the staged rows
+    for (int c = 0; c < ncolor; c++) {
+        BuildArgs b = a;
+        b.e_lo = cstart[c];
+        b.e_hi = cstart[c + 1];
+        const long nb = b.e_hi - b.e_lo;
+        if (nb > 0) hipLaunchKernelGGL((build3d_interior_kernel<true, false>), dim3((unsigned)nb), dim3(64), 0, st, b);
+        const long nbs = bstart[c + 1] - bstart[c];
+        if (nbs > 0) {
+            BuildArgs f = a;
+            f.bslots = a.bslots + bstart[c];
+            f.n_bslots = nbs;
+            hipLaunchKernelGGL((build3d_face_kernel<true>), dim3((unsigned)nbs), dim3(64), 0, st, f);
+        }
+        const long nas = astart[c + 1] - astart[c];
+        if (nas > 0)
+            hipLaunchKernelGGL((build3d_kernel<true, true>), dim3((unsigned)((nas + 3) / 4)), dim3(256), 0, st, a,
+                               a.aslots + astart[c], nas);
     }
 }
 

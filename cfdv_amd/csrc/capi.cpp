@@ -162,6 +162,18 @@ struct cfdv_ctx {
     float* d_B32 = nullptr;
     bool rows_off = false;    // set while the reference solve runs (EBE only)
     bool rows_tr = false;     // the rows hold K^T (assembled for K^T x)
+    // rows built in place (CFDV_BLOCKS_ROWS alone, hex8): the build adds
+    // each element block into a staged copy of the FP64 rows, one color
+    // after the other, and the rows (or those of K^T) are laid out from it
+    bool rows_direct = false;
+    long n_stage = 0;
+    double* d_S64 = nullptr;                 // [n_stage][neqn^2] staged FP64 rows
+    int* d_pslot = nullptr;                  // [n_pad][nnod^2] staged slot of each pair
+    unsigned long long* d_pfirst = nullptr;  // [n_pad] pairs first into their slot
+    int* d_mirror = nullptr;                 // [n_stage] slot of the mirrored pair
+    double* d_Rb = nullptr;                  // [n_bslots][nnod^2][neqn^2] domain parts
+    int* d_bidx = nullptr;                   // [n_pad] position in the face-kernel list
+    std::vector<long> bstart, astart;        // per color: ranges of bslots, aslots
     // discrete adjoint: per-slot tangents [n_pad][41], host-given vectors
     double *d_gelem = nullptr, *d_lam = nullptr, *d_duv = nullptr, *d_gU = nullptr, *d_g = nullptr;
     // eight adjoint right-hand sides at once: FP32 basis of side-by-side
@@ -261,7 +273,8 @@ struct cfdv_ctx {
     // element blocks when the orientation changes
     void orient_rows(bool tr) {
         if (!d_B64 || rows_tr == tr) return;
-        launch_rows_assemble(ra, d_R64, tr, d_B64, d_B32, st);
+        if (rows_direct) launch_rows_from_stage(ra, n_stage, d_S64, d_mirror, tr, d_B64, d_B32, st);
+        else launch_rows_assemble(ra, d_R64, tr, d_B64, d_B32, st);
         check_launch();
         rows_tr = tr;
     }
@@ -490,10 +503,17 @@ struct RowsHost {
     std::vector<long> goff, coff, diag;
     std::vector<int> cols, contrib;
     long n_blocks = 0;   // node pairs sharing an element (the real blocks)
+    // rows built in place (contexts without element blocks): per element
+    // slot and node pair the staged row slot it adds into (-1: none), per
+    // element slot the pairs that come first into their row slot (element
+    // slots ascend with the color, so the first is the lowest), and per
+    // staged row slot the slot of the mirrored pair (-1: padding)
+    std::vector<int> pslot, mirror;
+    std::vector<unsigned long long> pfirst;
 };
 
 RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* conn, const std::vector<int>& perm_inv,
-                        const std::vector<int>& ptr, const std::vector<int>& elem) {
+                        const std::vector<int>& ptr, const std::vector<int>& elem, bool direct) {
     const int nn2 = nn * nn;
     if ((double)n_pad * nn2 >= 2147483647.0) throw std::invalid_argument("mesh too large for assembled rows");
     // neighbour lists
@@ -584,8 +604,6 @@ RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* 
             }
             coff[k + 1] = coff[k] + cmax;
         }
-    std::vector<int>& contrib = h.contrib;
-    contrib.assign((size_t)coff[ng * 4] * 64, -1);
     // global slot of each row's own column (its diagonal block)
     h.diag.assign(nnode, 0);
     for (long r = 0; r < nnode; r++) {
@@ -593,6 +611,38 @@ RowsHost rows_structure(int nn, long ne, long nnode, long n_pad, const int32_t* 
         while (adj[sperm[aptr[r] + kl]] != (int)r) kl++;
         h.diag[r] = goff[r / 64] * 4 + kl;
     }
+    if (direct) {
+        // staged row slot of row r's slot kl: (4 goff[r / 64] + kl) * 64 + r % 64
+        const long nstage = ng * 4 * 64;
+        std::vector<int> kl_of(adj.size());
+        for (long r = 0; r < nnode; r++)
+            for (long kl = 0; kl < aptr[r + 1] - aptr[r]; kl++) kl_of[sperm[aptr[r] + kl]] = (int)kl;
+        h.pslot.assign((size_t)n_pad * nn2, -1);
+        h.pfirst.assign((size_t)n_pad, 0ULL);
+        h.mirror.assign((size_t)nstage, -1);
+        for (long r = 0; r < nnode; r++)
+            for (long kl = 0; kl < aptr[r + 1] - aptr[r]; kl++) {
+                const long a = sperm[aptr[r] + kl];
+                const long stg = (goff[r / 64] * 4 + kl) * 64 + r % 64;
+                long best = -1;
+                for (long i = eptr[a]; i < eptr[a + 1]; i++) {
+                    h.pslot[(size_t)codes[i]] = (int)stg;
+                    if (best < 0 || codes[i] / nn2 < codes[best] / nn2) best = i;
+                }
+                if (best >= 0) h.pfirst[(size_t)(codes[best] / nn2)] |= 1ULL << (codes[best] % nn2);
+                const int col = adj[a];
+                const long a2 = slot_of(col, (int)r);
+                h.mirror[(size_t)stg] = (int)((goff[col / 64] * 4 + kl_of[a2]) * 64 + col % 64);
+            }
+        for (long e = 0; e < ne; e++)
+            for (int q = 0; q < nn2; q++) {
+                const int v = h.pslot[(size_t)perm_inv[e] * nn2 + q];
+                if (v < 0 || v >= nstage) throw std::runtime_error("row structure: element pair without a row slot");
+            }
+        return h;   // the rows are not summed from element blocks: no lists
+    }
+    std::vector<int>& contrib = h.contrib;
+    contrib.assign((size_t)coff[ng * 4] * 64, -1);
     for (long ch = 0; ch < nch; ch++)
         for (long k = goff[ch] * 4; k < goff[ch + 1] * 4; k++) {
             const long kl = k - goff[ch] * 4;
@@ -695,7 +745,8 @@ std::vector<int> choose_node_order(int ndim, int nn, long nnode, long ne, const 
 }
 
 void setup_rows(cfdv_ctx* c, const cfdv_mesh* m, const std::vector<int>& ptr, const std::vector<int>& elem) {
-    RowsHost h = rows_structure(c->nnod, c->n_elem, c->n_node, c->n_pad, m->conn, c->perm_inv, ptr, elem);
+    RowsHost h = rows_structure(c->nnod, c->n_elem, c->n_node, c->n_pad, m->conn, c->perm_inv, ptr, elem,
+                                c->rows_direct);
     const long nch = (long)h.goff.size() - 1, ng = h.goff[nch], nnode = c->n_node;
     const auto &goff = h.goff, &coff = h.coff;
     const auto &cols = h.cols, &contrib = h.contrib;
@@ -703,17 +754,33 @@ void setup_rows(cfdv_ctx* c, const cfdv_mesh* m, const std::vector<int>& ptr, co
     c->n_row_blocks = h.n_blocks;
     c->d_goff = c->alloc<long>(nch + 1);
     c->d_cols = c->alloc<int>(cols.size());
-    c->d_coff = c->alloc<long>(coff.size());
-    c->d_contrib = c->alloc<int>(contrib.size());
     c->d_diag = c->alloc<long>(h.diag.size());
     HIP_CHECK_THROW(hipMemcpy(c->d_diag, h.diag.data(), sizeof(long) * h.diag.size(), hipMemcpyHostToDevice));
     HIP_CHECK_THROW(hipMemcpy(c->d_goff, goff.data(), sizeof(long) * goff.size(), hipMemcpyHostToDevice));
     HIP_CHECK_THROW(hipMemcpy(c->d_cols, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice));
-    HIP_CHECK_THROW(hipMemcpy(c->d_coff, coff.data(), sizeof(long) * coff.size(), hipMemcpyHostToDevice));
-    HIP_CHECK_THROW(hipMemcpy(c->d_contrib, contrib.data(), sizeof(int) * contrib.size(), hipMemcpyHostToDevice));
+    if (!c->rows_direct) {
+        c->d_coff = c->alloc<long>(coff.size());
+        c->d_contrib = c->alloc<int>(contrib.size());
+        HIP_CHECK_THROW(hipMemcpy(c->d_coff, coff.data(), sizeof(long) * coff.size(), hipMemcpyHostToDevice));
+        HIP_CHECK_THROW(hipMemcpy(c->d_contrib, contrib.data(), sizeof(int) * contrib.size(), hipMemcpyHostToDevice));
+    }
     const size_t nval = (size_t)ng * 4 * c->neqn * c->neqn * 64;
     c->d_B64 = c->alloc<double>(nval);
     c->d_B32 = c->alloc<float>(nval);
+    if (c->rows_direct) {
+        const size_t nstage = (size_t)ng * 4 * 64;
+        c->n_stage = (long)nstage;
+        c->d_S64 = c->alloc<double>(nstage * c->neqn * c->neqn);
+        // row slots no element adds into (padding, rows of no element) stay zero
+        HIP_CHECK_THROW(hipMemset(c->d_S64, 0, sizeof(double) * nstage * c->neqn * c->neqn));
+        c->d_pslot = c->alloc<int>(h.pslot.size());
+        c->d_pfirst = c->alloc<unsigned long long>(h.pfirst.size());
+        c->d_mirror = c->alloc<int>(h.mirror.size());
+        HIP_CHECK_THROW(hipMemcpy(c->d_pslot, h.pslot.data(), sizeof(int) * h.pslot.size(), hipMemcpyHostToDevice));
+        HIP_CHECK_THROW(hipMemcpy(c->d_pfirst, h.pfirst.data(), sizeof(unsigned long long) * h.pfirst.size(),
+                                  hipMemcpyHostToDevice));
+        HIP_CHECK_THROW(hipMemcpy(c->d_mirror, h.mirror.data(), sizeof(int) * h.mirror.size(), hipMemcpyHostToDevice));
+    }
     c->ra = RowsArgs{nnode, nch, c->nnod, c->neqn, c->d_goff, c->d_cols, c->d_coff, c->d_contrib, c->r64_pairs};
 }
 
@@ -912,6 +979,22 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
     if (c->block_flags & CFDV_BLOCKS_FP64) c->d_R64 = c->alloc<double>((size_t)np * kk);
     if (c->block_flags & CFDV_BLOCKS_FP32) c->d_R32 = c->alloc<float>((size_t)np * kk);
     if (c->block_flags & CFDV_BLOCKS_ROWS) setup_rows(c, m, ptr, elem);
+    if (c->rows_direct) {
+        // per color the sub-ranges of the face-kernel and per-lane lists
+        // (both ascend by slot, the colors are slot ranges), and the domain
+        // parts of the face kernel's elements between the two kernels
+        c->bstart.assign(c->ncolor + 1, 0);
+        c->astart.assign(c->ncolor + 1, 0);
+        for (int k = 0; k <= c->ncolor; k++) {
+            c->bstart[k] = std::lower_bound(bslots.begin(), bslots.end(), (int)c->cstart[k]) - bslots.begin();
+            c->astart[k] = std::lower_bound(aslots.begin(), aslots.end(), (int)c->cstart[k]) - aslots.begin();
+        }
+        std::vector<int> bidx((size_t)np, -1);
+        for (size_t i = 0; i < bslots.size(); i++) bidx[bslots[i]] = (int)i;
+        c->d_bidx = c->alloc<int>((size_t)np);
+        HIP_CHECK_THROW(hipMemcpy(c->d_bidx, bidx.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+        if (c->n_bslots > 0) c->d_Rb = c->alloc<double>((size_t)c->n_bslots * kk);
+    }
     c->d_rhs = c->alloc<double>((size_t)np * c->k);
     c->d_s = c->alloc<double>((size_t)np * 4);
     c->d_b = c->alloc<double>(c->ndof);
@@ -1551,9 +1634,13 @@ int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flag
     if (mesh->n_elem <= 0 || mesh->n_node <= 0 || !mesh->conn || !mesh->x || !mesh->bc || !mesh->dirichlet)
         return CFDV_ERR_ARG;
     if (mesh->adap && !mesh->hnm) return CFDV_ERR_ARG;
-    if (!(block_flags & (CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32))) return CFDV_ERR_ARG;
+    if (!(block_flags & (CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32 | CFDV_BLOCKS_ROWS))) return CFDV_ERR_ARG;
     if (block_flags & ~(CFDV_BLOCKS_FP64 | CFDV_BLOCKS_FP32 | CFDV_BLOCKS_ROWS)) return CFDV_ERR_ARG;
-    if ((block_flags & CFDV_BLOCKS_ROWS) && !(block_flags & CFDV_BLOCKS_FP64)) return CFDV_ERR_ARG;
+    // rows with FP32 element blocks need the FP64 ones; rows alone (built in
+    // place, no element blocks) are implemented for hex8
+    if ((block_flags & CFDV_BLOCKS_ROWS) && (block_flags & CFDV_BLOCKS_FP32) && !(block_flags & CFDV_BLOCKS_FP64))
+        return CFDV_ERR_ARG;
+    if (block_flags == CFDV_BLOCKS_ROWS && mesh->ndim != 3) return CFDV_ERR_ARG;
     cfdv_ctx* ctx = new (std::nothrow) cfdv_ctx();
     if (!ctx) return CFDV_ERR_HIP;
     try {
@@ -1571,6 +1658,7 @@ int cfdv_create(const cfdv_mesh* mesh, const cfdv_thermo* thermo, int block_flag
         ctx->ndof = mesh->n_node * mesh->neqn;
         ctx->block_flags = block_flags;
         ctx->r64_pairs = (block_flags & CFDV_BLOCKS_ROWS) ? 1 : 0;
+        ctx->rows_direct = block_flags == CFDV_BLOCKS_ROWS;
         ctx->th = make_thermo(thermo);
         HIP_CHECK_THROW(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
         for (auto& e : ctx->ev) HIP_CHECK_THROW(hipEventCreate(&e));
@@ -1727,9 +1815,23 @@ int cfdv_build(cfdv_ctx* ctx, double dt) {
     a.aslots = ctx->d_aslots;
     a.n_aslots = ctx->n_aslots;
     a.r64_pairs = ctx->r64_pairs;
-    launch_build(a, ctx->d_R64 != nullptr, ctx->d_R32 != nullptr, ctx->st);
+    a.e_lo = 0;
+    a.e_hi = ctx->n_pad;
+    if (ctx->rows_direct) {
+        a.S64 = ctx->d_S64;
+        a.pslot = ctx->d_pslot;
+        a.pfirst = ctx->d_pfirst;
+        a.Rb = ctx->d_Rb;
+        a.bidx = ctx->d_bidx;
+        launch_build_direct(a, ctx->ncolor, ctx->cstart.data(), ctx->bstart.data(), ctx->astart.data(), ctx->st);
+    } else {
+        launch_build(a, ctx->d_R64 != nullptr, ctx->d_R32 != nullptr, ctx->st);
+    }
     if (ctx->timing) HIP_CHECK_THROW(hipEventRecord(ctx->ev[5], ctx->st));
-    if (ctx->d_B64) launch_rows_assemble(ctx->ra, ctx->d_R64, false, ctx->d_B64, ctx->d_B32, ctx->st);
+    if (ctx->rows_direct)
+        launch_rows_from_stage(ctx->ra, ctx->n_stage, ctx->d_S64, ctx->d_mirror, false, ctx->d_B64, ctx->d_B32, ctx->st);
+    else if (ctx->d_B64)
+        launch_rows_assemble(ctx->ra, ctx->d_R64, false, ctx->d_B64, ctx->d_B32, ctx->st);
     ctx->rows_tr = false;
     ctx->pinv_valid = false;   // the node-block inverses belong to the previous blocks
     ctx->warm_consumed = false;

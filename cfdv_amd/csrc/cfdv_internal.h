@@ -43,6 +43,16 @@ struct BuildArgs {
     const int* aslots;        // 3D adapted slots (full per-lane kernel), ascending
     long n_aslots;
     int r64_pairs;            // R64 in pair_offset order (contexts with rows)
+    long e_lo, e_hi;          // interior kernel: the slot range it builds
+    // rows built in place (no element blocks; R64 and R32 null): each
+    // block is added into the staged FP64 row slot S64 + pslot[e * 64 + pair]
+    // * 25 (stored when bit pair of pfirst[e] is set), and the face kernel's
+    // elements keep their domain part in Rb at bidx[e]
+    double* S64;
+    const int* pslot;
+    const unsigned long long* pfirst;
+    double* Rb;
+    const int* bidx;
 };
 
 void upload_gauss_tables(const GaussTables& g);
@@ -56,6 +66,10 @@ void launch_vjp_gather(const int* node_ptr, const int* node_ent, const double* g
 void launch_column_sum(const double* a, long n, int stride, int offset, double* part, int nblk, double* out,
                        hipStream_t st);
 void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st);
+// hex8 build into the staged rows, color after color (cstart: ncolor + 1
+// slot bounds; bstart, astart: the colors' ranges of a.bslots, a.aslots)
+void launch_build_direct(const BuildArgs& a, int ncolor, const long* cstart, const long* bstart, const long* astart,
+                         hipStream_t st);
 void launch_flags(int ndim, const DevThermo& th, const int* conn, const double* U, const int* perm_inv,
                   long n_elem, uint8_t* flags, double* s1, int* ties, hipStream_t st);
 // The allocation-free flag pass works in a fixed device array (build.hip):
@@ -201,6 +215,10 @@ void launch_rows_matvec8(const RowsArgs& a, const float* B32, const float* X, fl
 // those of K, or with tr those of K^T (the same slots and sums, each block
 // transposed), over which launch_rows_matvec then forms K^T x
 void launch_rows_assemble(const RowsArgs& a, const double* R64, bool tr, double* B64, float* B32, hipStream_t st);
+// the rows (tr: those of K^T, through the mirrored slots) laid out from the
+// staged FP64 rows [n_stage][neqn^2]
+void launch_rows_from_stage(const RowsArgs& a, long n_stage, const double* S64, const int* mirror, bool tr,
+                            double* B64, float* B32, hipStream_t st);
 // node-block inverses (as launch_nodeblock) from the diagonal slots of the
 // FP64 rows (diag[n]: global slot index of node n's own column; tr: the
 // rows hold K^T); the slot sums are the element blocks' in element order,

@@ -442,6 +442,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     }
 }
 
+// The rows of K (TR: of K^T) in the product layout from the staged FP64
+// rows S [n_stage][NEQ^2] (staged slot (4 g + t) * 64 + lane = slot t of
+// group g of the lane's row, one contiguous row-major block each): lane
+// `lane` of the wavefront of group g gathers its 4 blocks (for K^T the
+// mirrored slots' blocks, transposed; mirror -1: a padding slot, zero) and
+// writes the group's 16-byte vectors, each wavefront store one contiguous
+// kilobyte.
+template <int NEQ, bool TR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void rows_from_stage_kernel(
+    long n_groups, const double* __restrict__ S, const int* __restrict__ mirror, double* __restrict__ B64,
+    float* __restrict__ B32) {
+    constexpr int BB = NEQ * NEQ, GV = 4 * BB;
+    const long g = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (g >= n_groups) return;
+    double v[GV];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const long i = (g * 4 + t) * 64 + lane;
+        const long src = TR ? (long)mirror[i] : i;
+        const double* b = S + (size_t)(src >= 0 ? src : 0) * BB;
+#pragma unroll
+        for (int q = 0; q < BB; q++) v[t * BB + q] = src >= 0 ? b[TR ? (q % NEQ) * NEQ + q / NEQ : q] : 0.0;
+    }
+    typedef double __attribute__((ext_vector_type(2))) d2_t;
+    typedef float __attribute__((ext_vector_type(4))) f4_t;
+    d2_t* d = reinterpret_cast<d2_t*>(B64 + (size_t)g * GV * 64) + lane;
+    f4_t* f = reinterpret_cast<f4_t*>(B32 + (size_t)g * GV * 64) + lane;
+#pragma unroll
+    for (int j = 0; j < GV / 2; j++) d[(size_t)j * 64] = d2_t{v[2 * j], v[2 * j + 1]};
+#pragma unroll
+    for (int j = 0; j < GV / 4; j++)
+        f[(size_t)j * 64] = f4_t{(float)v[4 * j], (float)v[4 * j + 1], (float)v[4 * j + 2], (float)v[4 * j + 3]};
+}
+
 static int row_grid(long n_chunks, int waves_per_block = 4) {
     const long nb = (n_chunks + waves_per_block - 1) / waves_per_block;
     return (int)(((nb + 7) / 8) * 8);   // whole rounds of 8 XCDs
@@ -501,6 +536,20 @@ void launch_rows_assemble(const RowsArgs& a, const double* R64, bool tr, double*
         if (tr) L(4, 4, true, false); else L(4, 4, false, false);
     }
 #undef L
+}
+
+void launch_rows_from_stage(const RowsArgs& a, long n_stage, const double* S64, const int* mirror, bool tr,
+                            double* B64, float* B32, hipStream_t st) {
+    if (n_stage <= 0) return;
+    const long ng = n_stage / 256;   // groups of 4 slots of 64 rows
+    const dim3 grid((unsigned)((ng + 3) / 4)), blk(256);
+    if (a.neqn == 5) {
+        if (tr) hipLaunchKernelGGL((rows_from_stage_kernel<5, true>), grid, blk, 0, st, ng, S64, mirror, B64, B32);
+        else hipLaunchKernelGGL((rows_from_stage_kernel<5, false>), grid, blk, 0, st, ng, S64, mirror, B64, B32);
+    } else {
+        if (tr) hipLaunchKernelGGL((rows_from_stage_kernel<4, true>), grid, blk, 0, st, ng, S64, mirror, B64, B32);
+        else hipLaunchKernelGGL((rows_from_stage_kernel<4, false>), grid, blk, 0, st, ng, S64, mirror, B64, B32);
+    }
 }
 
 }  // namespace cfdv

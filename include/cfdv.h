@@ -110,7 +110,16 @@ typedef struct {          /* the aero deck keys (MPIBinaryConfiguration.h:54-63)
    element).  K^T x (transposed solves, the adjoint) uses them too: the
    first K^T x after a K x re-sums the rows as those of K^T (one assembly)
    and the first K x after a K^T x back.  The reference solve stays element
-   by element. */
+   by element.
+   CFDV_BLOCKS_ROWS alone (hex8 meshes): the rows without element blocks.
+   The build adds each element block into a staged copy of the FP64 rows,
+   one color after the other (each slot sums its blocks in color order), and
+   lays the FP64 and FP32 rows out from it; K^T's rows are laid out from the
+   same copy through the mirrored slots.  The FP64 and mixed solves, the
+   adjoint solves, the build tangents and the products work as above;
+   cfdv_get_blocks(R), cfdv_get_blocks32 and the reference solve, which need
+   element blocks, return CFDV_ERR_ARG.  CFDV_BLOCKS_ROWS with
+   CFDV_BLOCKS_FP32 needs CFDV_BLOCKS_FP64. */
 #define CFDV_BLOCKS_ROWS 4
 
 /* solver modes */
@@ -180,7 +189,8 @@ typedef struct {
     int n_matvec64, n_matvec32;
     cfdv_solve_stats solve;
     double ms_assemble;   /* node rows from the element blocks (CFDV_BLOCKS_ROWS),
-                             not part of ms_build                           */
+                             or laid out from the staged rows (CFDV_BLOCKS_ROWS
+                             alone), not part of ms_build                   */
 } cfdv_step_stats;
 
 int cfdv_create(const cfdv_mesh *mesh, const cfdv_thermo *thermo, int block_flags,

@@ -72,7 +72,7 @@ def test_create_rejects_bad_arguments_before_device_use(lib):
     h = C.c_void_p()
     for ms, flags in [(mesh_struct(ndim=2), 1), (mesh_struct(nnod=4), 1),
                       (mesh_struct(n_elem=0), 1), (mesh_struct(), 0),
-                      (mesh_struct(), _lib.CFDV_BLOCKS_ROWS),                       # rows need FP64
+                      (mesh_struct(ndim=2, nnod=4, neqn=4), _lib.CFDV_BLOCKS_ROWS),  # rows alone: hex8 only
                       (mesh_struct(), _lib.CFDV_BLOCKS_FP32 | _lib.CFDV_BLOCKS_ROWS),
                       (mesh_struct(), 1 | 8)]:                                      # unknown flag
         assert lib.cfdv_create(C.byref(ms), C.byref(t), flags, C.byref(h)) == _lib.CFDV_ERR_ARG

@@ -185,8 +185,10 @@ def test_mixed_step_on_rows(gpu_available):
     R1, _ = ref.get_state()
     du, dr = U1 - U0, R1 - U0
     assert np.linalg.norm(du - dr) <= 1e-8 * np.linalg.norm(dr)
+    # rows with FP32 element blocks need the FP64 ones (rows alone build in
+    # place: tests/test_gpu_rows_direct.py)
     with pytest.raises(CFDVError):
-        FDVContext(m, th, U, blocks=("rows",))
+        FDVContext(m, th, U, blocks=("fp32", "rows"))
 
 
 @pytest.mark.parametrize("ws", [1, 2, 3])
