@@ -1228,7 +1228,11 @@ __device__ __forceinline__ void face_contract(const FaceTab& F, const FaceMat& M
 // 0..2 one direction's Jacobian combinations each, part 3 the c_ij scalars
 // and the geometry), then lanes (point, part), part < 5, the node-pair
 // matrices, then lane (N, M) adds them to its block.
-template <bool W32>
+// ONE: every listed element has at most one face with a bc code (one batch
+// of points): the domain block is read after the point tables are formed,
+// so that it holds no registers while they are (the loop over batches keeps
+// it live across them, which costs the general kernel half its occupancy).
+template <bool W32, bool ONE>
 __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
     constexpr int NNOD = 8, NEQ = 5, K = 40, KK = 1600, PB = 4;
     __shared__ double s_nd[NNOD][ND_STRIDE];
@@ -1252,21 +1256,28 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
 #pragma unroll
     for (int fc = 0; fc < 6; fc++) nfa += (bcs[fc] != 0 && bcs[fc] < 100);
     const int npts = 4 * nfa;
-    // the domain part of R and rhs (written by the interior kernel) is read
-    // before the point tables are formed, so that its HBM latency overlaps
-    // them, and written once after the last batch of points
-    double R[NEQ][NEQ], rhs[NEQ];
-#pragma unroll
-    for (int r = 0; r < NEQ; r++) rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
-    if (A.S64) {
-        const double* src = rb_block(A, e, N, M);
-#pragma unroll
-        for (int r = 0; r < NEQ; r++)
-#pragma unroll
-            for (int s = 0; s < NEQ; s++) R[r][s] = src[r * NEQ + s];
-    } else {
-        r64_load<8, 5>(A, e, N, M, R);
+    if (ONE && npts > PB) {   // not an element of this list (the host sorts them)
+        if (lane == 0) atomicOr(A.status, 4);
+        return;
     }
+    // the domain part of R and rhs (written by the interior kernel), read
+    // before the point tables are formed (so that its HBM latency overlaps
+    // them) or, ONE, after them; written once after the last batch of points
+    double R[NEQ][NEQ], rhs[NEQ];
+    auto load_domain = [&]() {
+#pragma unroll
+        for (int r = 0; r < NEQ; r++) rhs[r] = (M == 0) ? A.rhs[e * K + NEQ * N + r] : 0.0;
+        if (A.S64) {
+            const double* src = rb_block(A, e, N, M);
+#pragma unroll
+            for (int r = 0; r < NEQ; r++)
+#pragma unroll
+                for (int s = 0; s < NEQ; s++) R[r][s] = src[r * NEQ + s];
+        } else {
+            r64_load<8, 5>(A, e, N, M, R);
+        }
+    };
+    if (!ONE) load_domain();
     for (int b0 = 0; b0 < npts || b0 == 0; b0 += PB) {
         const int nb = (npts - b0 < PB) ? npts - b0 : PB;
         const int p = lane & 7, part = lane >> 3;
@@ -1303,11 +1314,12 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
         }
         __syncthreads();
         if (p < nb && part < 5 && s_ft[p].ok) face_mats(s_ft[p], part, sp[0], s_fm[p]);
+        if (ONE) load_domain();
         __syncthreads();
 #pragma unroll 1
         for (int q = 0; q < nb; q++)
             if (s_ft[q].ok) face_contract(s_ft[q], s_fm[q], N, M, A.dt, sp[2], R, rhs);
-        if (b0 + PB >= npts) break;
+        if (ONE || b0 + PB >= npts) break;
         __syncthreads();   // the tables are rewritten by the next batch
     }
     apply_bc3(A, e, lane, R, rhs);
@@ -1841,9 +1853,25 @@ void launch_build(const BuildArgs& a, bool w64, bool w32, hipStream_t st) {
         BuildArgs b = a;
         if (!w32) b.R32 = nullptr;
         if (a.n_bslots > 0) {
-            if (w64) hipLaunchKernelGGL((build3d_face_kernel<true>), dim3((int)a.n_bslots), dim3(64), 0, st, b);
-            else hipLaunchKernelGGL((build3d_kernel<false, true>), dim3((int)((a.n_bslots + 3) / 4)), dim3(256), 0,
-                                    st, a, a.bslots, a.n_bslots);
+            if (w64) {
+                // elements with one face (most) on the two-waves-per-SIMD
+                // instance, the others on the general one
+                if (a.n_bslots1 > 0) {
+                    BuildArgs b1 = b;
+                    b1.bslots = a.bslots1;
+                    b1.n_bslots = a.n_bslots1;
+                    hipLaunchKernelGGL((build3d_face_kernel<true, true>), dim3((int)a.n_bslots1), dim3(64), 0, st, b1);
+                }
+                if (a.n_bslotsm > 0) {
+                    BuildArgs bm = b;
+                    bm.bslots = a.bslotsm;
+                    bm.n_bslots = a.n_bslotsm;
+                    hipLaunchKernelGGL((build3d_face_kernel<true, false>), dim3((int)a.n_bslotsm), dim3(64), 0, st, bm);
+                }
+            } else {
+                hipLaunchKernelGGL((build3d_kernel<false, true>), dim3((int)((a.n_bslots + 3) / 4)), dim3(256), 0, st,
+                                   a, a.bslots, a.n_bslots);
+            }
         }
         // hanging-node elements (Hnm-transformed test functions): whole
         // element on the per-lane kernel
@@ -1877,7 +1905,7 @@ void launch_build_direct(const BuildArgs& a, int ncolor, const long* cstart, con
             BuildArgs f = a;
             f.bslots = a.bslots + bstart[c];
             f.n_bslots = nbs;
-            hipLaunchKernelGGL((build3d_face_kernel<true>), dim3((unsigned)nbs), dim3(64), 0, st, f);
+            hipLaunchKernelGGL((build3d_face_kernel<true, false>), dim3((unsigned)nbs), dim3(64), 0, st, f);
         }
         const long nas = astart[c + 1] - astart[c];
         if (nas > 0)

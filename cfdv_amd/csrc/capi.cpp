@@ -134,6 +134,8 @@ struct cfdv_ctx {
     unsigned long long* d_cmask = nullptr;
     int* d_bslots = nullptr;        // 3D slots with faces / constrained dofs
     long n_bslots = 0;
+    int *d_bslots1 = nullptr, *d_bslotsm = nullptr;   // those with <= 1 face with a bc code, and more
+    long n_bslots1 = 0, n_bslotsm = 0;
     unsigned char* d_adap = nullptr;  // hanging-node flags per slot (or none)
     double* d_hnm = nullptr;          // [n_pad][64] Hnm(N, M) at N + 8 M
     int* d_aslots = nullptr;
@@ -958,6 +960,24 @@ void setup(cfdv_ctx* c, const cfdv_mesh* m) {
     if (c->n_bslots > 0) {
         c->d_bslots = c->alloc<int>((size_t)c->n_bslots);
         HIP_CHECK_THROW(hipMemcpy(c->d_bslots, bslots.data(), sizeof(int) * c->n_bslots, hipMemcpyHostToDevice));
+        // split by the number of faces with a bc code 0 < bc < 100 (the face
+        // kernel's points): one batch of points or more
+        std::vector<int> b1, bm;
+        for (const int sl : bslots) {
+            int nfa = 0;
+            for (int f = 0; f < nf; f++) nfa += (bc_s[(size_t)sl * nf + f] != 0 && bc_s[(size_t)sl * nf + f] < 100);
+            (nfa <= 1 ? b1 : bm).push_back(sl);
+        }
+        c->n_bslots1 = (long)b1.size();
+        c->n_bslotsm = (long)bm.size();
+        if (!b1.empty()) {
+            c->d_bslots1 = c->alloc<int>(b1.size());
+            HIP_CHECK_THROW(hipMemcpy(c->d_bslots1, b1.data(), sizeof(int) * b1.size(), hipMemcpyHostToDevice));
+        }
+        if (!bm.empty()) {
+            c->d_bslotsm = c->alloc<int>(bm.size());
+            HIP_CHECK_THROW(hipMemcpy(c->d_bslotsm, bm.data(), sizeof(int) * bm.size(), hipMemcpyHostToDevice));
+        }
     }
     if (!adap_s.empty()) {
         c->d_adap = c->alloc<unsigned char>((size_t)np);
@@ -1810,6 +1830,10 @@ int cfdv_build(cfdv_ctx* ctx, double dt) {
     a.status = ctx->d_counters + 1;
     a.bslots = ctx->d_bslots;
     a.n_bslots = ctx->n_bslots;
+    a.bslots1 = ctx->d_bslots1;
+    a.n_bslots1 = ctx->n_bslots1;
+    a.bslotsm = ctx->d_bslotsm;
+    a.n_bslotsm = ctx->n_bslotsm;
     a.adap = ctx->d_adap;
     a.hnm = ctx->d_hnm;
     a.aslots = ctx->d_aslots;
@@ -1843,6 +1867,7 @@ int cfdv_build(cfdv_ctx* ctx, double dt) {
     ctx->built = true;
     if (cnt[1] & 1) return fail(ctx, CFDV_ERR_BC, "unknown boundary-condition code");
     if (cnt[1] & 2) return fail(ctx, CFDV_ERR_GEOM, "face surface Jacobian below 1e-12");
+    if (cnt[1] & 4) return fail(ctx, CFDV_ERR_HIP, "internal: an element of the one-face list has more faces");
     return CFDV_OK;
     GUARD_END
 }

@@ -38,6 +38,10 @@ struct BuildArgs {
     int* status;              // bit 0: unknown bc code, bit 1: bad face detjac
     const int* bslots;        // 3D: slots with faces or constrained dofs and no
     long n_bslots;            //     Hnm (face kernel), ascending
+    const int* bslots1;       // the same with at most one face with a bc code
+    long n_bslots1;
+    const int* bslotsm;       // and with more
+    long n_bslotsm;
     const unsigned char* adap;  // [n_elem_pad] hanging-node element flags or nullptr
     const double* hnm;        // [n_elem_pad][64] Hnm(N, M) at N + 8 M (adapted slots)
     const int* aslots;        // 3D adapted slots (full per-lane kernel), ascending
