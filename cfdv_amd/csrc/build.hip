@@ -658,23 +658,21 @@ CFDV_HD void accum_c3_ctab(const double* C, const double W[3][3], double out[5][
 }
 
 // Phase 2 for lane (N, M): the domain contraction of every Gauss point
-// without the c_A A S product (mfma_as_kstep below): diagonal, direction
-// terms, c_ij, in contract3's order, and the rhs.
+// without the c_A A S product (mfma_as_kstep below) and the rhs
+// (interior_rhs): diagonal, direction terms, c_ij, in contract3's order.
 // before: called at the start of point gp's iteration (the kernel issues
 // the matrix-core steps of the A S product there, so that they execute while
 // the lanes contract the point on the VALU).
 template <typename Before>
 CFDV_HD void interior_phase2(const DevThermo& th, const GaussHex& G, const GPTab* tabs, int lane,
-                             double dt, const double sp[4], double R[5][5], double rhs[5], Before&& before) {
+                             double dt, const double sp[4], double R[5][5], Before&& before) {
     constexpr int NEQ = 5;
     const int N = lane >> 3, M = lane & 7;
     const double s3 = sp[2];
 #pragma unroll
-    for (int r = 0; r < NEQ; r++) {
-        rhs[r] = 0.0;
+    for (int r = 0; r < NEQ; r++)
 #pragma unroll
         for (int s = 0; s < NEQ; s++) R[r][s] = 0.0;
-    }
 #pragma unroll 1
     for (int gp = 0; gp < 8; gp++) {
         before(gp);
@@ -684,11 +682,10 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GaussHex& G, const GPTab
         double DN[3], DM[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) { DN[i] = g.dphi[N][i]; DM[i] = g.dphi[M][i]; }
-        double wg[3], wr[3], W[3][3];
+        double wg[3], W[3][3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             wg[i] = -dt * wJ * PM * DN[i];
-            wr[i] = wJ * DN[i];
 #pragma unroll
             for (int j = 0; j < 3; j++) W[i][j] = -dt * s3 * wJ * DN[i] * DM[j];
         }
@@ -700,16 +697,26 @@ CFDV_HD void interior_phase2(const DevThermo& th, const GaussHex& G, const GPTab
 #pragma unroll
             for (int p = 0; p < 17; p++)
                 R[sig_eq(I, JPAT_R[0][p])][sig_eq(I, JPAT_S[0][p])] += wg[I] * (p == 0 ? sp[0] : g.M1[I][p - 1]);
-        // rhs: P and the A D product (contract3's AQ), as PQ = P - dt^2/2 A D
-#pragma unroll
-        for (int r = 0; r < NEQ; r++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int i = 0; i < 3; i++) acc += wr[i] * g.PQ[i][sig_eq(i, r)];
-            rhs[r] += acc;
-        }
         accum_c3_ctab(g.C, W, R);
     }
+}
+
+// The rhs entry (N, r) = (lane / 5, lane % 5) of lane < 40: P and the A D
+// product (contract3's AQ) of every point, as PQ = P - dt^2/2 A D, summed
+// point by point in the contraction's order
+CFDV_HD double interior_rhs(const GaussHex& G, const GPTab* tabs, int lane) {
+    const int N = lane / 5, r = lane - 5 * (lane / 5);
+    double rhs = 0.0;
+#pragma unroll 1
+    for (int gp = 0; gp < 8; gp++) {
+        const GPTab& g = tabs[gp];
+        const double wJ = G.p[gp].wJ;
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) acc += (wJ * g.dphi[N][i]) * g.PQ[i][sig_eq(i, r)];
+        rhs += acc;
+    }
+    return rhs;
 }
 
 // The c_A (sum_i dPhi_N,i A_i)(sum_j dPhi_M,j S_j) term of every node pair
@@ -881,7 +888,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     // c_A A S term of all pairs runs on the matrix cores, its 40 x 40 result
     // lands in LDS over the (then consumed) point tables
     const int N = lane >> 3, M = lane & 7;
-    double R[NEQ][NEQ], rhs[NEQ];
+    double R[NEQ][NEQ];
     // the A S product's 10 inner steps are spread over the 8 points of the
     // VALU contraction (steps 8 gp / 10 .. 8 (gp + 1) / 10 at point gp)
     const double dtsq_2 = (A.dt * A.dt) / 2.0;
@@ -890,9 +897,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) acc[a][b] = dvec4_t{0.0, 0.0, 0.0, 0.0};
-    interior_phase2(th, s_g, s_tab, lane, A.dt, sp, R, rhs, [&](int gp) {
+    interior_phase2(th, s_g, s_tab, lane, A.dt, sp, R, [&](int gp) {
         for (int ks = (gp * 10) >> 3; ks < ((gp + 1) * 10) >> 3; ks++) mfma_as_kstep(s_g, s_tab, dtsq_2, ks, lane, acc);
     });
+    // the rhs, entry (N, r) per lane < 40, before the product overwrites
+    // the point tables
+    const double rhs_l = lane < K ? interior_rhs(s_g, s_tab, lane) : 0.0;
     double* s_as = reinterpret_cast<double*>(s_tab);
     static_assert(sizeof(s_tab) >= 1600 * sizeof(double), "point tables hold the 40 x 40 product");
     mfma_as_store(acc, lane, s_as);
@@ -920,10 +930,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
             for (int s = 0; s < NEQ; s++)
                 A.R32[blk_index<float>(e, (NEQ * N + r) * K + NEQ * M + s, KK)] = (float)R[r][s];
     }
-    if (M == 0) {
-#pragma unroll
-        for (int r = 0; r < NEQ; r++) A.rhs[e * K + NEQ * N + r] = rhs[r];
-    }
+    if (lane < K) A.rhs[e * K + lane] = rhs_l;
     if (lane == 0) {
         if (A.svals) {
 #pragma unroll
