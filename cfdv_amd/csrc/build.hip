@@ -523,6 +523,9 @@ CFDV_HD T pick3(int I, T x0, T x1, T x2) { return I == 1 ? x1 : (I == 2 ? x2 : x
 // of PQ_I.
 // Af and Sf, which the matrix-core product reads at lane-dependent indices,
 // are stored in their own orientation.
+// The lane also forms direction I's share of the point's c_ij / Re table
+// (gp_tab_c_dir), so that no lane group runs a branch of its own.
+CFDV_HD void gp_tab_c_dir(const DevThermo& th, const GPState<3>& f0, const GPState<3>& f, int I, GPTab& t);
 CFDV_HD void gp_tab_dir_rot(const DevThermo& th, const GPShared& g, const double sp[4], double dt, int I,
                             GPTab& t) {
     const GPState<3>& f0 = g.f;
@@ -545,6 +548,7 @@ CFDV_HD void gp_tab_dir_rot(const DevThermo& th, const GPShared& g, const double
                                f0.dU[sig_eq(2, q)][pi_ax(2, k)]);
         }
     }
+    gp_tab_c_dir(th, f0, f, I, t);
     double a[5][5], b[5][5];
     calc_a3_dir<0>(th, f, a);
     calc_b3_dir<0>(th, f, b);
@@ -576,50 +580,56 @@ CFDV_HD void gp_tab_dir_rot(const DevThermo& th, const GPShared& g, const double
     }
 }
 
-// Phase 1b, w det J and the c_ij / Re table of one Gauss point (one lane):
-// accum_c3's coefficients (NavierStokes3D.h:347-477), each distinct value once
-CFDV_HD void gp_tab_common(const DevThermo& th, const GPShared& g, GPTab& t) {
-    const GPState<3>& f = g.f;
+// Phase 1b, the c_ij / Re table of one Gauss point: accum_c3's coefficients
+// (NavierStokes3D.h:347-477), each distinct value once, formed by the
+// point's three direction lanes.  Lane I forms the entries indexed by I --
+// velocity component I's (CT_R0, CT_M0, CT_L0), the diagonal block c_II's
+// row 4 (CT_40, CT_4R, CT_4M), the pairs (I, j) (CT_O40, CT_O4I, CT_O4J) --
+// and each lane the four scalars (CT_RR, CT_MR, CT_LR, CT_44: the same value
+// to the same address).  Every entry is the same expression of the same
+// values whichever lane forms it.  f0 is the point's state, f the state
+// rotated for direction I (gp_tab_dir_rot), whose momentum components 1, 2,
+// 3 are U_{1+I} and the other two, U_{1+j} for j = (I == 1 ? 0 : 1) and
+// j = (I == 2 ? 0 : 2).
+CFDV_HD void gp_tab_c_dir(const DevThermo& th, const GPState<3>& f0, const GPState<3>& f, int I, GPTab& t) {
     const double reinv = 1.0 / th.creyn;
-    const double mu = f.mu;
+    const double mu = f0.mu;
     const double lambda = -2. * mu / 3.;
     const double mu_R = 2. * mu + lambda;
-    const double rho = f.U[0];
+    const double rho = f0.U[0];
     const double rinv = 1.0 / rho;
     const double denom = 1. / (rho * rho);
-    const double Uc[3] = {f.U[1], f.U[2], f.U[3]};
-    const double re = f.U[4];
-    const double hkocv = f.k / th.Cv;
-    const double TERM1 = f.k / th.Cv * denom;
+    const double Uc[3] = {f0.U[1], f0.U[2], f0.U[3]};
+    const double re = f0.U[4];
+    const double hkocv = f0.k / th.Cv;
+    const double TERM1 = f0.k / th.Cv * denom;
     const double en = TERM1 * (-re + (Uc[0] * Uc[0] + Uc[1] * Uc[1] + Uc[2] * Uc[2]) * rinv);
-    const double Ur[3] = {Uc[0] * rinv, Uc[1] * rinv, Uc[2] * rinv};
     double* C = t.C;
-    double cR0[3], cM0[3];
+    const double UcI = f.U[1], UrI = UcI * rinv;
+    const double cR0I = mu_R * UcI * denom, cM0I = mu * UcI * denom;
+    double cM0[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        cR0[k] = mu_R * Uc[k] * denom;
-        cM0[k] = mu * Uc[k] * denom;
-        C[CT_R0 + k] = cR0[k] * reinv;
-        C[CT_M0 + k] = cM0[k] * reinv;
-        C[CT_L0 + k] = lambda * Uc[k] * denom * reinv;
-    }
+    for (int k = 0; k < 3; k++) cM0[k] = mu * Uc[k] * denom;
+    C[CT_R0 + I] = cR0I * reinv;
+    C[CT_M0 + I] = cM0I * reinv;
+    C[CT_L0 + I] = lambda * UcI * denom * reinv;
     const double cRR = -mu_R * rinv, cMR = -mu * rinv, cLR = -lambda * rinv;
     C[CT_RR] = cRR * reinv;
     C[CT_MR] = cMR * reinv;
     C[CT_LR] = cLR * reinv;
+    double acc = 0.0;
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        double acc = 0.0;
+    for (int k = 0; k < 3; k++) acc += Uc[k] * (k == I ? cR0I : cM0[k]);
+    C[CT_40 + I] = (acc * rinv - en) * reinv;
+    C[CT_4R + I] = (UrI * cRR + TERM1 * UcI) * reinv;
+    C[CT_4M + I] = (UrI * cMR + TERM1 * UcI) * reinv;
+    C[CT_O4I + I] = UrI * cMR * reinv;
+    C[CT_O4J + I] = UrI * cLR * reinv;
 #pragma unroll
-        for (int k = 0; k < 3; k++) acc += Uc[k] * (k == i ? cR0[k] : cM0[k]);
-        C[CT_40 + i] = (acc * rinv - en) * reinv;
-        C[CT_4R + i] = (Ur[i] * cRR + TERM1 * Uc[i]) * reinv;
-        C[CT_4M + i] = (Ur[i] * cMR + TERM1 * Uc[i]) * reinv;
-        C[CT_O4I + i] = Ur[i] * cMR * reinv;
-        C[CT_O4J + i] = Ur[i] * cLR * reinv;
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-            if (j != i) C[CT_O40 + ct_pair(i, j)] = (Ur[i] * (lambda * Uc[j] * denom) + Ur[j] * cM0[i]) * reinv;
+    for (int o = 0; o < 2; o++) {
+        const int j = o == 0 ? (I == 1 ? 0 : 1) : (I == 2 ? 0 : 2);
+        const double Ucj = f.U[2 + o];
+        C[CT_O40 + ct_pair(I, j)] = (UrI * (lambda * Ucj * denom) + (Ucj * rinv) * cM0I) * reinv;
     }
     C[CT_44] = -hkocv * rinv * reinv;
 }
@@ -850,9 +860,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     // gp and dPhi_n/dx; (b) lane (gp, k): the interpolation sums of sources k
     // and k + 8, into the point's A/S table area (written only after every
     // lane has read them back); (c) lanes (part, gp) = (lane / 8, lane % 8),
-    // part < 4: the point's state from the sums (the four lanes of a point
-    // alike), then direction 0, 1, 2 of the Jacobian combinations or the
-    // common part
+    // part < 3: the point's state from the sums (the three lanes of a point
+    // alike), then direction part's Jacobian combinations and share of the
+    // c_ij table
     {
         const int gp = lane >> 3, n = lane & 7;
         double wJ;
@@ -877,12 +887,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     __syncthreads();
     GPShared g;
     const int gp1 = lane & 7, part = lane >> 3;
-    if (lane < 32) gp_state(th, &s_tab[gp1].Af[0][0], A.dt, g);
+    if (lane < 24) gp_state(th, &s_tab[gp1].Af[0][0], A.dt, g);
     __syncthreads();
-    if (lane < 32) {
-        if (part < 3) gp_tab_dir_rot(th, g, sp, A.dt, part, s_tab[gp1]);
-        else gp_tab_common(th, g, s_tab[gp1]);
-    }
+    if (lane < 24) gp_tab_dir_rot(th, g, sp, A.dt, part, s_tab[gp1]);
     __syncthreads();
     // phase 2: lane (N, M) contracts every point for its node pair; the
     // c_A A S term of all pairs runs on the matrix cores, its 40 x 40 result
@@ -1232,8 +1239,8 @@ __device__ __forceinline__ void face_contract(const FaceTab& F, const FaceMat& M
 // elements: those are built whole by build3d_kernel).  Face points come in
 // build3d_faces' order (point-major over the faces with 0 < bc < 100), four
 // at a time: lanes (point, part), part < 4, form the point tables (part
-// 0..2 one direction's Jacobian combinations each, part 3 the c_ij scalars
-// and the geometry), then lanes (point, part), part < 5, the node-pair
+// 0..2 one direction's Jacobian combinations and share of the c_ij table
+// each, part 3 the geometry), then lanes (point, part), part < 5, the node-pair
 // matrices, then lane (N, M) adds them to its block.
 // ONE: every listed element has at most one face with a bc code (one batch
 // of points): the domain block is read after the point tables are formed,
@@ -1314,10 +1321,7 @@ __global__ __launch_bounds__(64) void build3d_face_kernel(BuildArgs A) {
                 for (int j = 0; j < 3; j++) F.en[j] = en[j];
                 if (st) atomicOr(&s_status, st);
             }
-            if (ok) {
-                if (part < 3) gp_tab_dir_rot(th, g, sp, A.dt, part, F.t);
-                else gp_tab_common(th, g, F.t);
-            }
+            if (ok && part < 3) gp_tab_dir_rot(th, g, sp, A.dt, part, F.t);
         }
         __syncthreads();
         if (p < nb && part < 5 && s_ft[p].ok) face_mats(s_ft[p], part, sp[0], s_fm[p]);

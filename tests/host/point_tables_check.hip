@@ -5,10 +5,13 @@
 //     state -- equals calc_a3_dir<I> / calc_b3_dir<I> entry by entry, its
 //     rotated M1 slots cover exactly direction I's 17-entry pattern (slot 0
 //     is s1), and PQ = P - dt^2/2 (a + b) D;
-//   * accum_c3_ctab over gp_tab_common's coefficient table equals
-//     accum_c3 (NavierStokes3D.h:347-477) for random weights W.
+//   * accum_c3_ctab over the coefficient table that the three direction
+//     calls fill between them (each its share, gp_tab_c_dir) equals
+//     accum_c3 (NavierStokes3D.h:347-477) for random weights W; the table
+//     starts as NaNs, and one that no direction writes fails.
 // Prints the worst relative difference of each and exits 1 above 1e-14.
 #include "../../cfdv_amd/csrc/build.hip"
+#include <cmath>
 #include <cstdio>
 #include <random>
 using namespace cfdv;
@@ -42,8 +45,13 @@ int main() {
         const double sp[4] = {0.5 + 0.5 * U(rng), 0.7 + 0.2 * U(rng), 0.5 + 0.5 * U(rng), 0.7 + 0.2 * U(rng)};
         const double dt = 0.05 * (1.5 + U(rng));
         GPTab t;
+        for (int c = 0; c < CT_N; c++) t.C[c] = NAN;
         for (int I = 0; I < 3; I++) gp_tab_dir_rot(th, g, sp, dt, I, t);
-        gp_tab_common(th, g, t);
+        for (int c = 0; c < CT_N; c++)
+            if (std::isnan(t.C[c])) {
+                printf("c_ij table entry %d written by no direction\n", c);
+                return 1;
+            }
         for (int I = 0; I < 3; I++) {
             double a[5][5], b[5][5];
             if (I == 0) { calc_a3_dir<0>(th, f, a); calc_b3_dir<0>(th, f, b); }
