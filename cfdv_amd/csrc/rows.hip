@@ -476,11 +476,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     typedef float __attribute__((ext_vector_type(4))) f4_t;
     d2_t* d = reinterpret_cast<d2_t*>(B64 + (size_t)g * GV * 64) + lane;
     f4_t* f = reinterpret_cast<f4_t*>(B32 + (size_t)g * GV * 64) + lane;
+    // stored nontemporal, as the assembly's rows (refined ramp: 12.1 ->
+    // 11.3 ms); the staged rows are read with ordinary loads, whose lines
+    // the wavefront's next loads share
 #pragma unroll
-    for (int j = 0; j < GV / 2; j++) d[(size_t)j * 64] = d2_t{v[2 * j], v[2 * j + 1]};
+    for (int j = 0; j < GV / 2; j++) __builtin_nontemporal_store(d2_t{v[2 * j], v[2 * j + 1]}, d + (size_t)j * 64);
 #pragma unroll
     for (int j = 0; j < GV / 4; j++)
-        f[(size_t)j * 64] = f4_t{(float)v[4 * j], (float)v[4 * j + 1], (float)v[4 * j + 2], (float)v[4 * j + 3]};
+        __builtin_nontemporal_store(f4_t{(float)v[4 * j], (float)v[4 * j + 1], (float)v[4 * j + 2], (float)v[4 * j + 3]},
+                                    f + (size_t)j * 64);
 }
 
 static int row_grid(long n_chunks, int waves_per_block = 4) {
